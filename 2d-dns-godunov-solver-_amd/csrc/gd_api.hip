@@ -1,0 +1,860 @@
+// C ABI of libgodunov: mesh upload and reordering, state management, the
+// reference operators and the device-resident time loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "gd_internal.hpp"
+
+using namespace gd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define GD_HIP(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) return fail(GD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dev_alloc(std::vector<void*>& allocs, T** p, size_t n, int64_t* bytes = nullptr) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(GD_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  allocs.push_back((void*)*p);
+  if (bytes) *bytes += (int64_t)(n * sizeof(T));
+  return GD_OK;
+}
+
+template <class T>
+int upload(std::vector<void*>& allocs, const T** dst, const std::vector<T>& src, int64_t* bytes) {
+  T* p = nullptr;
+  int rc = dev_alloc(allocs, &p, src.size(), bytes);
+  if (rc) return rc;
+  if (!src.empty()) {
+    hipError_t e = hipMemcpy(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(GD_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  }
+  *dst = p;
+  return GD_OK;
+}
+
+// 1-based column-major reads of the caller's Julia arrays
+inline int32_t CI(const int32_t* a, int64_t n, int64_t i0, int k1) { return a[(int64_t)(k1 - 1) * n + i0]; }
+inline double CD(const double* a, int64_t n, int64_t i0, int k1) { return a[(int64_t)(k1 - 1) * n + i0]; }
+
+// Reverse Cuthill-McKee on the cell face graph (Cuthill & McKee 1969; George &
+// Liu): BFS from a low-degree peripheral cell, neighbours visited in order of
+// increasing degree, final order reversed.
+std::vector<int64_t> rcm_order(int64_t n, const std::vector<int64_t>& xadj, const std::vector<int64_t>& adj) {
+  std::vector<int64_t> deg(n), order;
+  order.reserve(n);
+  for (int64_t i = 0; i < n; ++i) deg[i] = xadj[i + 1] - xadj[i];
+  std::vector<char> seen(n, 0);
+  std::vector<int64_t> level(n, -1);
+  auto bfs_last = [&](int64_t s) {  // farthest low-degree cell from s
+    std::deque<int64_t> q{s};
+    std::vector<int64_t> touched{s};
+    level[s] = 0;
+    int64_t best = s;
+    while (!q.empty()) {
+      int64_t c = q.front(); q.pop_front();
+      if (level[c] > level[best] || (level[c] == level[best] && deg[c] < deg[best])) best = c;
+      for (int64_t e = xadj[c]; e < xadj[c + 1]; ++e) {
+        int64_t d = adj[e];
+        if (level[d] < 0) { level[d] = level[c] + 1; q.push_back(d); touched.push_back(d); }
+      }
+    }
+    for (int64_t t : touched) level[t] = -1;
+    return best;
+  };
+  std::vector<int64_t> by_deg(n);
+  std::iota(by_deg.begin(), by_deg.end(), 0);
+  std::stable_sort(by_deg.begin(), by_deg.end(), [&](int64_t a, int64_t b) { return deg[a] < deg[b]; });
+  std::vector<int64_t> nbrs;
+  for (int64_t s0 : by_deg) {
+    if (seen[s0]) continue;
+    const int64_t s = bfs_last(bfs_last(s0));
+    std::deque<int64_t> q{s};
+    seen[s] = 1;
+    while (!q.empty()) {
+      int64_t c = q.front(); q.pop_front();
+      order.push_back(c);
+      nbrs.clear();
+      for (int64_t e = xadj[c]; e < xadj[c + 1]; ++e)
+        if (!seen[adj[e]]) { nbrs.push_back(adj[e]); seen[adj[e]] = 1; }
+      std::stable_sort(nbrs.begin(), nbrs.end(), [&](int64_t a, int64_t b) { return deg[a] < deg[b]; });
+      for (int64_t d : nbrs) q.push_back(d);
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+}  // namespace
+
+struct gd_mesh : gd::Mesh {};
+struct gd_state : gd::State {
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<double> host_tmp;
+};
+
+// The definitions below take C linkage from their declarations in godunov.h.
+
+const char* gd_version(void) { return "libgodunov 0.1 (gfx950)"; }
+
+int gd_last_error(char* buf, size_t n) {
+  if (!buf || n == 0) return GD_ERR_INVALID;
+  std::snprintf(buf, n, "%s", g_err.c_str());
+  return GD_OK;
+}
+
+int gd_device_count(int32_t* n) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  if (n) *n = c;
+  return GD_OK;
+}
+
+void gd_params_default(gd_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->scheme = GD_SCHEME_REF;
+  p->precision = GD_FP64;
+  p->flux = GD_FLUX_AUSM_PLUS;
+  p->limiter = GD_LIM_REF_HARMONIC;
+  p->limiter_eps = 1.0e-12;
+  p->venkat_K = 5.0;
+  p->viscous = 0;
+  p->energy_law = GD_ENERGY_AIR_POLY;
+  p->mu_const = 1.0e-2;
+  p->k_const = 0.0;
+  p->RGAS = 287.058;
+  p->Gamma = 1.4;
+  p->mu_ref = 1.8476e-5;
+  p->T_ref = 300.0;
+  p->S_suth = 116.0;
+  p->Pr = 0.72;
+  for (int i = 0; i <= GD_MAX_BC; ++i) p->bc[i].kind = GD_BC_EXTRAPOLATE;
+}
+
+int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd_mesh** out) {
+  if (!d || !out) return fail(GD_ERR_INVALID, "null argument");
+  *out = nullptr;
+  const int64_t nc = d->n_cells, nn = d->n_nodes;
+  const int32_t ncol = d->n_neib_cells;
+  if (nc <= 0 || nn <= 0 || ncol <= 0 || nc >= (int64_t)INT32_MAX || nn >= (int64_t)INT32_MAX)
+    return fail(GD_ERR_INVALID, "bad mesh sizes");
+  if (!d->mesh_connectivity || !d->cell_stiffness || !d->cell_edges_Nx || !d->cell_edges_Ny ||
+      !d->cell_edges_length || !d->cell_areas || !d->cell_clusters || !d->node_stencils || !d->node2cellsL2up ||
+      !d->node2cellsL2down || !d->cells2nodes || !d->cell_mid_points || !d->cell_nodes_X || !d->cell_nodes_Y ||
+      !d->xNodes || !d->yNodes)
+    return fail(GD_ERR_INVALID, "missing mesh array");
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) return fail(GD_ERR_NO_DEVICE, "no HIP device");
+
+  // --- validate and collect topology in the caller's numbering (0-based)
+  std::vector<int32_t> nv(nc);
+  int32_t has_quads = 0;
+  for (int64_t i = 0; i < nc; ++i) {
+    nv[i] = CI(d->mesh_connectivity, nc, i, 3);
+    if (nv[i] != 3 && nv[i] != 4) return fail(GD_ERR_INVALID, "cell with neither 3 nor 4 vertices");
+    if (nv[i] == 4) has_quads = 1;
+  }
+  const int nf = has_quads ? 4 : 3;
+  std::vector<int64_t> xadj(nc + 1, 0), adj;
+  adj.reserve(nc * 3);
+  int64_t nbf = 0;
+  for (int64_t i = 0; i < nc; ++i) {
+    for (int k = 1; k <= nv[i]; ++k) {
+      const int32_t e = CI(d->cell_stiffness, nc, i, k);
+      if (e >= 1 && e <= nc) adj.push_back(e - 1);
+      else if (e > nc) return fail(GD_ERR_INVALID, "neighbour id out of range");
+      else ++nbf;
+    }
+    xadj[i + 1] = (int64_t)adj.size();
+  }
+  for (int64_t i = 0; i < nc; ++i)
+    for (int k = 1; k <= nv[i]; ++k) {
+      const int64_t vv[4] = {CI(d->node2cellsL2down, nc, i, 2 * k - 1), CI(d->node2cellsL2up, nc, i, 2 * k - 1),
+                             CI(d->cells2nodes, nc, i, 2 * k - 1), CI(d->cells2nodes, nc, i, 2 * k)};
+      const bool interior = CI(d->cell_stiffness, nc, i, k) >= 1;
+      if (vv[2] < 1 || vv[2] > nn || vv[3] < 1 || vv[3] > nn) return fail(GD_ERR_INVALID, "cells2nodes out of range");
+      if (interior && (vv[0] < 1 || vv[0] > nn || vv[1] < 1 || vv[1] > nn))
+        return fail(GD_ERR_INVALID, "node2cellsL2up/down out of range");
+      if (interior && nv[i] == 4) {
+        const int64_t a = CI(d->node2cellsL2down, nc, i, 2 * k), b = CI(d->node2cellsL2up, nc, i, 2 * k);
+        if (a < 1 || a > nn || b < 1 || b > nn) return fail(GD_ERR_INVALID, "quad stencil node out of range");
+      }
+    }
+
+  gd_mesh* M = new gd_mesh();
+  (void)hipGetDevice(&M->device);
+  // --- cell order
+  std::vector<int64_t>& cp = M->cperm;
+  if (order == GD_ORDER_RCM) {
+    cp = rcm_order(nc, xadj, adj);
+  } else if (order == GD_ORDER_PERMUTATION) {
+    if (!perm) { delete M; return fail(GD_ERR_INVALID, "GD_ORDER_PERMUTATION without a permutation"); }
+    cp.assign(perm, perm + nc);
+  } else {
+    cp.resize(nc);
+    std::iota(cp.begin(), cp.end(), 0);
+  }
+  M->ciperm.assign(nc, -1);
+  for (int64_t c = 0; c < nc; ++c) {
+    if (cp[c] < 0 || cp[c] >= nc || M->ciperm[cp[c]] != -1) { delete M; return fail(GD_ERR_INVALID, "not a permutation"); }
+    M->ciperm[cp[c]] = c;
+  }
+  // node order: as given, or first touch in the new cell order
+  std::vector<int64_t>& np = M->nperm;
+  M->niperm.assign(nn, -1);
+  if (order == GD_ORDER_AS_GIVEN) {
+    np.resize(nn);
+    std::iota(np.begin(), np.end(), 0);
+    for (int64_t j = 0; j < nn; ++j) M->niperm[j] = j;
+  } else {
+    np.reserve(nn);
+    for (int64_t c = 0; c < nc; ++c) {
+      const int64_t o = cp[c];
+      for (int v = 0; v < nv[o]; ++v) {
+        const int64_t j = CI(d->mesh_connectivity, nc, o, 4 + v) - 1;
+        if (j >= 0 && j < nn && M->niperm[j] < 0) { M->niperm[j] = (int64_t)np.size(); np.push_back(j); }
+      }
+    }
+    for (int64_t j = 0; j < nn; ++j)
+      if (M->niperm[j] < 0) { M->niperm[j] = (int64_t)np.size(); np.push_back(j); }
+  }
+  const auto& ci = M->ciperm;
+  const auto& ni = M->niperm;
+  auto new_node = [&](int64_t id1) -> int32_t { return (id1 >= 1 && id1 <= nn) ? (int32_t)ni[id1 - 1] : 0; };
+
+  // --- per-cell arrays in device order
+  std::vector<int32_t> h_nv(nc), h_nb(nf * nc), h_dn1(nf * nc, 0), h_dn2(nf * nc, 0), h_up1(nf * nc, 0),
+      h_up2(nf * nc, 0), h_ea(nf * nc, 0), h_eb(nf * nc, 0), h_wid(nf * nc, -1);
+  std::vector<double> h_nx(nf * nc, 0.0), h_ny(nf * nc, 0.0), h_len(nf * nc, 0.0), h_area(nc), h_rarea(nc), h_hx(nc),
+      h_hmin(nc), h_xc(nc), h_yc(nc);
+  std::vector<double> h_lsq(nf * 2 * nc, 0.0), h_rf(nf * 2 * nc, 0.0), h_dc(nf * 2 * nc, 0.0);
+  double max_hx = 0.0;
+  int64_t bandwidth = 0;
+  int32_t n_wall = 0;
+  // face midpoint relative to the centroid, from the cell's own vertex coordinates
+  auto rface = [&](int64_t o, int k, double* r) {
+    const int32_t a = CI(d->cells2nodes, nc, o, 2 * k - 1), b = CI(d->cells2nodes, nc, o, 2 * k);
+    double xa = 0, ya = 0, xb = 0, yb = 0;
+    for (int v = 0; v < nv[o]; ++v) {
+      const int32_t id = CI(d->mesh_connectivity, nc, o, 4 + v);
+      if (id == a) { xa = CD(d->cell_nodes_X, nc, o, v + 1); ya = CD(d->cell_nodes_Y, nc, o, v + 1); }
+      if (id == b) { xb = CD(d->cell_nodes_X, nc, o, v + 1); yb = CD(d->cell_nodes_Y, nc, o, v + 1); }
+    }
+    r[0] = 0.5 * (xa + xb) - CD(d->cell_mid_points, nc, o, 1);
+    r[1] = 0.5 * (ya + yb) - CD(d->cell_mid_points, nc, o, 2);
+  };
+  for (int64_t c = 0; c < nc; ++c) {
+    const int64_t o = cp[c];
+    h_nv[c] = nv[o];
+    const double A = d->cell_areas[o];
+    if (!(A > 0.0)) { delete M; return fail(GD_ERR_INVALID, "non-positive cell area"); }
+    h_area[c] = A;
+    h_rarea[c] = 1.0 / A;
+    h_hx[c] = d->HX ? d->HX[o] : 0.0;
+    max_hx = std::max(max_hx, h_hx[c]);
+    h_xc[c] = CD(d->cell_mid_points, nc, o, 1);
+    h_yc[c] = CD(d->cell_mid_points, nc, o, 2);
+    double perim = 0.0;
+    double M11 = 0, M12 = 0, M22 = 0;
+    double dcx[4] = {0, 0, 0, 0}, dcy[4] = {0, 0, 0, 0}, wk[4] = {0, 0, 0, 0};
+    for (int k = 1; k <= nf; ++k) {
+      const int64_t s = (int64_t)(k - 1) * nc + c;
+      if (k > nv[o]) { h_nb[s] = -1; continue; }
+      const int32_t e = CI(d->cell_stiffness, nc, o, k);
+      h_nx[s] = CD(d->cell_edges_Nx, nc, o, k);
+      h_ny[s] = CD(d->cell_edges_Ny, nc, o, k);
+      h_len[s] = CD(d->cell_edges_length, nc, o, k);
+      perim += h_len[s];
+      h_ea[s] = new_node(CI(d->cells2nodes, nc, o, 2 * k - 1));
+      h_eb[s] = new_node(CI(d->cells2nodes, nc, o, 2 * k));
+      double r[2];
+      rface(o, k, r);
+      h_rf[((int64_t)(k - 1) * 2 + 0) * nc + c] = r[0];
+      h_rf[((int64_t)(k - 1) * 2 + 1) * nc + c] = r[1];
+      if (e >= 1) {
+        h_nb[s] = (int32_t)ci[e - 1];
+        bandwidth = std::max<int64_t>(bandwidth, (int64_t)std::llabs((long long)(ci[e - 1] - c)));
+        h_dn1[s] = new_node(CI(d->node2cellsL2down, nc, o, 2 * k - 1));
+        h_up1[s] = new_node(CI(d->node2cellsL2up, nc, o, 2 * k - 1));
+        if (nv[o] == 4) {
+          h_dn2[s] = new_node(CI(d->node2cellsL2down, nc, o, 2 * k));
+          h_up2[s] = new_node(CI(d->node2cellsL2up, nc, o, 2 * k));
+        }
+        // neighbour centroid offset through the shared face (periodic safe)
+        const int64_t oj = e - 1;
+        int kj = 0;
+        for (int q = 1; q <= nv[oj]; ++q)
+          if (CI(d->cell_stiffness, nc, oj, q) == o + 1) kj = q;
+        if (kj == 0) { delete M; return fail(GD_ERR_INVALID, "neighbour relation is not symmetric"); }
+        double rj[2];
+        rface(oj, kj, rj);
+        dcx[k - 1] = r[0] - rj[0];
+        dcy[k - 1] = r[1] - rj[1];
+      } else {
+        h_nb[s] = e - 1;  // code - 1 < 0
+        h_wid[s] = n_wall++;
+        // ghost centroid: the own centroid mirrored across the face
+        const double rn = r[0] * h_nx[s] + r[1] * h_ny[s];
+        dcx[k - 1] = 2.0 * rn * h_nx[s];
+        dcy[k - 1] = 2.0 * rn * h_ny[s];
+      }
+      const double w = 1.0 / (dcx[k - 1] * dcx[k - 1] + dcy[k - 1] * dcy[k - 1]);
+      wk[k - 1] = w;
+      M11 += w * dcx[k - 1] * dcx[k - 1];
+      M12 += w * dcx[k - 1] * dcy[k - 1];
+      M22 += w * dcy[k - 1] * dcy[k - 1];
+      h_dc[((int64_t)(k - 1) * 2 + 0) * nc + c] = dcx[k - 1];
+      h_dc[((int64_t)(k - 1) * 2 + 1) * nc + c] = dcy[k - 1];
+    }
+    h_hmin[c] = 2.0 * A / perim;
+    // inverse-distance-squared weighted least squares: grad = M^-1 sum_k w_k d_k dq_k
+    const double det = M11 * M22 - M12 * M12;
+    for (int k = 1; k <= nv[o]; ++k) {
+      const double ix = (M22 * dcx[k - 1] - M12 * dcy[k - 1]) / det;
+      const double iy = (-M12 * dcx[k - 1] + M11 * dcy[k - 1]) / det;
+      h_lsq[((int64_t)(k - 1) * 2 + 0) * nc + c] = wk[k - 1] * ix;
+      h_lsq[((int64_t)(k - 1) * 2 + 1) * nc + c] = wk[k - 1] * iy;
+    }
+  }
+  // --- clusters: keep slot order, drop empty slots, width = widest node
+  int32_t ncl = 0;
+  for (int64_t j = 0; j < nn; ++j) {
+    int32_t cnt = 0;
+    for (int q = 1; q <= ncol; ++q) {
+      const int32_t c = CI(d->cell_clusters, nn, j, q);
+      if (c != 0) {
+        if (c < 0 || c > nc) { delete M; return fail(GD_ERR_INVALID, "cell_clusters id out of range"); }
+        ++cnt;
+      }
+    }
+    ncl = std::max(ncl, cnt);
+  }
+  if (ncl == 0) ncl = 1;
+  std::vector<int32_t> h_cl((int64_t)ncl * nn, -1);
+  std::vector<double> h_w((int64_t)ncl * nn, 0.0), h_xn(nn), h_yn(nn);
+  for (int64_t jn = 0; jn < nn; ++jn) {
+    const int64_t j = np[jn];
+    int32_t s = 0;
+    for (int q = 1; q <= ncol; ++q) {
+      const int32_t c = CI(d->cell_clusters, nn, j, q);
+      if (c != 0) {
+        h_cl[(int64_t)s * nn + jn] = (int32_t)ci[c - 1];
+        h_w[(int64_t)s * nn + jn] = CD(d->node_stencils, nn, j, q);
+        ++s;
+      }
+    }
+    h_xn[jn] = d->xNodes[j];
+    h_yn[jn] = d->yNodes[j];
+  }
+
+  MeshView& v = M->v;
+  v.nc = nc; v.nn = nn; v.nf = nf; v.ncl = ncl; v.has_quads = has_quads; v.n_wall = n_wall; v.max_hx = max_hx;
+  int64_t* B = &M->device_bytes;
+  int rc = 0;
+  auto& al = M->allocs;
+  rc |= upload(al, &v.nv, h_nv, B);
+  rc |= upload(al, &v.nb, h_nb, B);
+  rc |= upload(al, &v.fnx, h_nx, B);
+  rc |= upload(al, &v.fny, h_ny, B);
+  rc |= upload(al, &v.flen, h_len, B);
+  rc |= upload(al, &v.dn1, h_dn1, B);
+  rc |= upload(al, &v.dn2, h_dn2, B);
+  rc |= upload(al, &v.up1, h_up1, B);
+  rc |= upload(al, &v.up2, h_up2, B);
+  rc |= upload(al, &v.ea, h_ea, B);
+  rc |= upload(al, &v.eb, h_eb, B);
+  rc |= upload(al, &v.area, h_area, B);
+  rc |= upload(al, &v.rarea, h_rarea, B);
+  rc |= upload(al, &v.hx, h_hx, B);
+  rc |= upload(al, &v.hmin, h_hmin, B);
+  rc |= upload(al, &v.xc, h_xc, B);
+  rc |= upload(al, &v.yc, h_yc, B);
+  rc |= upload(al, &v.lsq, h_lsq, B);
+  rc |= upload(al, &v.rf, h_rf, B);
+  rc |= upload(al, &v.dc, h_dc, B);
+  rc |= upload(al, &v.wall_id, h_wid, B);
+  rc |= upload(al, &v.cl, h_cl, B);
+  rc |= upload(al, &v.w, h_w, B);
+  rc |= upload(al, &v.xn, h_xn, B);
+  rc |= upload(al, &v.yn, h_yn, B);
+  if (rc) {
+    std::string msg = g_err;
+    gd_mesh_destroy(M);
+    return fail(rc, msg);
+  }
+  M->n_boundary_faces = nbf;
+  M->bandwidth = bandwidth;
+  M->max_cluster = ncl;
+  *out = M;
+  return GD_OK;
+}
+
+int gd_mesh_info_get(const gd_mesh* m, gd_mesh_info* info) {
+  if (!m || !info) return fail(GD_ERR_INVALID, "null argument");
+  info->n_cells = m->v.nc;
+  info->n_nodes = m->v.nn;
+  info->n_boundary_faces = m->n_boundary_faces;
+  info->has_quads = m->v.has_quads;
+  info->max_cluster = m->max_cluster;
+  info->device_bytes = m->device_bytes;
+  info->bandwidth = m->bandwidth;
+  return GD_OK;
+}
+
+int gd_mesh_destroy(gd_mesh* m) {
+  if (!m) return GD_OK;
+  for (void* p : m->allocs) (void)hipFree(p);
+  delete m;
+  return GD_OK;
+}
+
+static int check_params(const gd::Mesh* m, const gd_params* p) {
+  if (p->scheme != GD_SCHEME_REF && p->scheme != GD_SCHEME_NS) return fail(GD_ERR_INVALID, "unknown scheme");
+  if (p->precision != GD_FP64 && p->precision != GD_FP32_MIXED) return fail(GD_ERR_INVALID, "unknown precision");
+  if (p->scheme == GD_SCHEME_REF && p->precision != GD_FP64)
+    return fail(GD_ERR_UNSUPPORTED, "the reference scheme runs in FP64 only");
+  if (p->flux < GD_FLUX_AUSM_PLUS || p->flux > GD_FLUX_AUSM_PLUS_M2020) return fail(GD_ERR_INVALID, "unknown flux");
+  if (p->scheme == GD_SCHEME_NS && p->flux == GD_FLUX_AUSM_PLUS_M2020)
+    return fail(GD_ERR_UNSUPPORTED, "AUSM+M2020 is available in the reference scheme only");
+  if (p->scheme == GD_SCHEME_REF && (p->limiter < GD_LIM_REF_HARMONIC || p->limiter > GD_LIM_MINMOD))
+    return fail(GD_ERR_INVALID, "reference scheme limiter must be one of the reference limiters");
+  if (p->scheme == GD_SCHEME_NS && p->limiter != GD_LIM_VENKATAKRISHNAN)
+    return fail(GD_ERR_UNSUPPORTED, "the ns scheme uses the Venkatakrishnan limiter");
+  if (!(p->Gamma > 1.0) || !(p->RGAS > 0.0)) return fail(GD_ERR_INVALID, "bad gas constants");
+  for (int i = 0; i <= GD_MAX_BC; ++i)
+    if (p->bc[i].kind < GD_BC_NONE || p->bc[i].kind > GD_BC_ISOTHERMAL_WALL) return fail(GD_ERR_INVALID, "bad BC kind");
+  (void)m;
+  return GD_OK;
+}
+
+int gd_state_create(gd_mesh* m, const gd_params* p, gd_state** out) {
+  if (!m || !p || !out) return fail(GD_ERR_INVALID, "null argument");
+  *out = nullptr;
+  int rc = check_params(m, p);
+  if (rc) return rc;
+  gd_state* S = new gd_state();
+  S->mesh = m;
+  S->prm = *p;
+  const int64_t nc = m->v.nc, nn = m->v.nn;
+  hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete S; return fail(GD_ERR_HIP, "hipStreamCreate failed"); }
+  auto& al = S->allocs;
+  rc |= dev_alloc(al, &S->U[0], 4 * nc);
+  rc |= dev_alloc(al, &S->U[1], 4 * nc);
+  rc |= dev_alloc(al, &S->Q[0], 4 * nc);
+  rc |= dev_alloc(al, &S->Q[1], 4 * nc);
+  rc |= dev_alloc(al, &S->aS, nc);
+  rc |= dev_alloc(al, &S->vmax, nc);
+  rc |= dev_alloc(al, &S->Qn, 4 * nn);
+  rc |= dev_alloc(al, &S->Un, 4 * nn);
+  rc |= dev_alloc(al, &S->Ud, 4 * nc);
+  rc |= dev_alloc(al, &S->F, 4 * nc);
+  rc |= dev_alloc(al, &S->Dl, 4 * nc);
+  rc |= dev_alloc(al, &S->phiN, 3 * nn);
+  rc |= dev_alloc(al, &S->g6, 6 * nc);
+  rc |= dev_alloc(al, &S->g6N, 6 * nn);
+  double* gbuf = nullptr;
+  rc |= dev_alloc(al, &gbuf, 8 * nc);
+  S->grad = gbuf;
+  double* pbuf = nullptr;
+  rc |= dev_alloc(al, &pbuf, 4 * nc);
+  S->phi = pbuf;
+  rc |= dev_alloc(al, &S->wall_f, 2 * (size_t)std::max<int32_t>(m->v.n_wall, 1));
+  rc |= dev_alloc(al, &S->drag, 4);
+  rc |= dev_alloc(al, &S->acc, 3);
+  rc |= dev_alloc(al, &S->red, 8 + 4 * ((nc + 255) / 256) + 8);
+  if (rc) {
+    std::string msg = g_err;
+    gd_state_destroy(S);
+    return fail(rc, msg);
+  }
+  // zero everything once so that reads of never-written arrays are defined
+  (void)hipMemsetAsync(S->U[0], 0, 4 * nc * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->U[1], 0, 4 * nc * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->Q[0], 0, 4 * nc * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->Q[1], 0, 4 * nc * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->aS, 0, nc * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->vmax, 0, nc * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->Qn, 0, 4 * nn * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->Un, 0, 4 * nn * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->Ud, 0, 4 * nc * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->F, 0, 4 * nc * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->Dl, 0, 4 * nc * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->wall_f, 0, 2 * std::max<int32_t>(m->v.n_wall, 1) * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->drag, 0, 4 * sizeof(double), S->stream);
+  e = hipStreamSynchronize(S->stream);
+  if (e != hipSuccess) { gd_state_destroy(S); return fail(GD_ERR_HIP, "state init failed"); }
+  *out = S;
+  return GD_OK;
+}
+
+int gd_state_destroy(gd_state* S) {
+  if (!S) return GD_OK;
+  if (S->stream) (void)hipStreamSynchronize(S->stream);
+  for (hipEvent_t ev : S->ev_pool) (void)hipEventDestroy(ev);
+  for (void* p : S->allocs) (void)hipFree(p);
+  if (S->stream) (void)hipStreamDestroy(S->stream);
+  delete S;
+  return GD_OK;
+}
+
+int gd_state_set_params(gd_state* S, const gd_params* p) {
+  if (!S || !p) return fail(GD_ERR_INVALID, "null argument");
+  int rc = check_params(S->mesh, p);
+  if (rc) return rc;
+  S->prm = *p;
+  return GD_OK;
+}
+
+// host (caller order) <-> device (device order) column transfers
+static int put_cols(gd_state* S, double* dev, const double* host, int ncols, bool nodes) {
+  const Mesh& M = *S->mesh;
+  const int64_t n = nodes ? M.v.nn : M.v.nc;
+  const std::vector<int64_t>& perm = nodes ? M.nperm : M.cperm;
+  std::vector<double>& t = S->host_tmp;
+  t.resize((size_t)n * ncols);
+  for (int q = 0; q < ncols; ++q)
+    for (int64_t i = 0; i < n; ++i) t[(size_t)q * n + i] = host[(size_t)q * n + perm[i]];
+  GD_HIP(hipMemcpyAsync(dev, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, S->stream));
+  GD_HIP(hipStreamSynchronize(S->stream));
+  return GD_OK;
+}
+
+static int get_cols(gd_state* S, const double* dev, double* host, int ncols, bool nodes) {
+  const Mesh& M = *S->mesh;
+  const int64_t n = nodes ? M.v.nn : M.v.nc;
+  const std::vector<int64_t>& perm = nodes ? M.nperm : M.cperm;
+  std::vector<double>& t = S->host_tmp;
+  t.resize((size_t)n * ncols);
+  GD_HIP(hipMemcpyAsync(t.data(), dev, t.size() * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  GD_HIP(hipStreamSynchronize(S->stream));
+  for (int q = 0; q < ncols; ++q)
+    for (int64_t i = 0; i < n; ++i) host[(size_t)q * n + perm[i]] = t[(size_t)q * n + i];
+  return GD_OK;
+}
+
+int gd_state_set_fields(gd_state* S, const double* rC, const double* uC, const double* vC, const double* pC,
+                        const double* rN, const double* uN, const double* vN, const double* pN) {
+  if (!S) return fail(GD_ERR_INVALID, "null state");
+  const int64_t nc = S->mesh->v.nc, nn = S->mesh->v.nn;
+  const double* cs[4] = {rC, uC, vC, pC};
+  const double* ns[4] = {rN, uN, vN, pN};
+  int rc = 0;
+  for (int q = 0; q < 4; ++q)
+    if (cs[q] && (rc = put_cols(S, S->Q[S->qc] + q * nc, cs[q], 1, false))) return rc;
+  for (int q = 0; q < 4; ++q)
+    if (ns[q] && (rc = put_cols(S, S->Qn + q * nn, ns[q], 1, true))) return rc;
+  // aSoundCells / VMAXCells as createFields2d computes them (initfields_TGV2d.jl:47-48)
+  GD_HIP(launch_fields_derived(nc, S->prm.Gamma, S->Q[S->qc], S->aS, S->vmax, S->stream));
+  GD_HIP(hipStreamSynchronize(S->stream));
+  return GD_OK;
+}
+
+int gd_state_get_fields(gd_state* S, double* rC, double* uC, double* vC, double* pC, double* aC, double* vmC,
+                        double* rN, double* uN, double* vN, double* pN) {
+  if (!S) return fail(GD_ERR_INVALID, "null state");
+  const int64_t nc = S->mesh->v.nc, nn = S->mesh->v.nn;
+  double* cs[4] = {rC, uC, vC, pC};
+  double* ns[4] = {rN, uN, vN, pN};
+  int rc = 0;
+  for (int q = 0; q < 4; ++q)
+    if (cs[q] && (rc = get_cols(S, S->Q[S->qc] + q * nc, cs[q], 1, false))) return rc;
+  if (aC || vmC) {
+    GD_HIP(launch_fields_derived(nc, S->prm.Gamma, S->Q[S->qc], S->aS, S->vmax, S->stream));
+    if (aC && (rc = get_cols(S, S->aS, aC, 1, false))) return rc;
+    if (vmC && (rc = get_cols(S, S->vmax, vmC, 1, false))) return rc;
+  }
+  for (int q = 0; q < 4; ++q)
+    if (ns[q] && (rc = get_cols(S, S->Qn + q * nn, ns[q], 1, true))) return rc;
+  return GD_OK;
+}
+
+static double* array_ptr(gd_state* S, int32_t which, bool* nodes) {
+  *nodes = false;
+  switch (which) {
+    case GD_ARR_U_OLD: return S->U[S->cur];
+    case GD_ARR_U_NEW: return S->U[1 - S->cur];
+    case GD_ARR_U_DIFF: return S->Ud;
+    case GD_ARR_FLUXES: return S->F;
+    case GD_ARR_DELTA: return S->Dl;
+    case GD_ARR_U_NODES: *nodes = true; return S->Un;
+    default: return nullptr;
+  }
+}
+
+int gd_state_set_array(gd_state* S, int32_t which, const double* host) {
+  if (!S || !host) return fail(GD_ERR_INVALID, "null argument");
+  bool nodes;
+  double* p = array_ptr(S, which, &nodes);
+  if (!p) return fail(GD_ERR_INVALID, "unknown array");
+  return put_cols(S, p, host, 4, nodes);
+}
+
+int gd_state_get_array(gd_state* S, int32_t which, double* host) {
+  if (!S || !host) return fail(GD_ERR_INVALID, "null argument");
+  bool nodes;
+  double* p = array_ptr(S, which, &nodes);
+  if (!p) return fail(GD_ERR_INVALID, "unknown array");
+  return get_cols(S, p, host, 4, nodes);
+}
+
+int gd_state_device_u(gd_state* S, double** dev) {
+  if (!S || !dev) return fail(GD_ERR_INVALID, "null argument");
+  *dev = S->U[S->cur];
+  return GD_OK;
+}
+
+int gd_sync(gd_state* S) {
+  if (!S) return fail(GD_ERR_INVALID, "null state");
+  GD_HIP(hipStreamSynchronize(S->stream));
+  return GD_OK;
+}
+
+static int need_ref(gd_state* S) {
+  if (S->prm.scheme != GD_SCHEME_REF) return fail(GD_ERR_UNSUPPORTED, "operator belongs to the reference scheme");
+  return GD_OK;
+}
+
+int gd_calc_one_stage(gd_state* S, int32_t variant, double betta, double dt, double flow_time) {
+  if (!S) return fail(GD_ERR_INVALID, "null state");
+  int rc = need_ref(S);
+  if (rc) return rc;
+  if (variant != GD_STAGE_MUSCL_FAST && variant != GD_STAGE_MUSCL) return fail(GD_ERR_INVALID, "unknown variant");
+  RefStepArgs a{};
+  a.m = S->mesh->v;
+  a.prm = S->prm;
+  a.Q = S->Q[S->qc];
+  a.Qn = S->Qn;
+  a.Uo = S->U[S->cur];
+  a.Ud = variant == GD_STAGE_MUSCL_FAST ? S->Ud : nullptr;
+  a.include_udiff = variant == GD_STAGE_MUSCL_FAST;
+  a.Un = S->U[1 - S->cur];
+  a.F = S->F;
+  a.betta = betta;
+  a.dt = dt;
+  a.flow_time = flow_time;
+  GD_HIP(launch_ref_step(a, S->stream));
+  GD_HIP(hipStreamSynchronize(S->stream));
+  return GD_OK;
+}
+
+int gd_calc_diff_term(gd_state* S) {
+  if (!S) return fail(GD_ERR_INVALID, "null state");
+  int rc = need_ref(S);
+  if (rc) return rc;
+  const MeshView& v = S->mesh->v;
+  GD_HIP(launch_ref_phi(v, S->U[S->cur], S->phiN, S->stream));
+  GD_HIP(launch_ref_gg1(v, S->phiN, S->g6, S->stream));
+  GD_HIP(launch_ref_gg_nodes(v, S->g6, S->g6N, S->stream));
+  GD_HIP(launch_ref_diff_term(v, S->prm, S->Q[S->qc], S->g6N, S->Ud, S->stream));
+  GD_HIP(hipStreamSynchronize(S->stream));
+  return GD_OK;
+}
+
+int gd_update_variables(gd_state* S) {
+  if (!S) return fail(GD_ERR_INVALID, "null state");
+  const MeshView& v = S->mesh->v;
+  GD_HIP(launch_update_variables(v, S->prm.Gamma, S->U[1 - S->cur], S->U[S->cur], S->Q[S->qc], S->aS, S->vmax, S->Dl,
+                                 S->stream));
+  GD_HIP(hipStreamSynchronize(S->stream));
+  return GD_OK;
+}
+
+int gd_cells2nodes(gd_state* S) {
+  if (!S) return fail(GD_ERR_INVALID, "null state");
+  const MeshView& v = S->mesh->v;
+  GD_HIP(launch_ref_nodes(v, S->Q[S->qc], S->U[S->cur], S->Qn, S->Un, nullptr, S->stream));
+  GD_HIP(hipStreamSynchronize(S->stream));
+  return GD_OK;
+}
+
+// ---------------------------------------------------------------- time loop
+
+namespace {
+struct Timer {
+  gd_state* S;
+  bool on;
+  gd_run_stats* st;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> recs;
+  size_t next = 0;
+  std::vector<std::string> names;
+  std::vector<double> bytes;
+  hipEvent_t get() {
+    if (next < S->ev_pool.size()) return S->ev_pool[next++];
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    S->ev_pool.push_back(e);
+    ++next;
+    return e;
+  }
+  int kid(const char* name, double b) {
+    for (size_t i = 0; i < names.size(); ++i)
+      if (names[i] == name) return (int)i;
+    names.push_back(name);
+    bytes.push_back(b);
+    return (int)names.size() - 1;
+  }
+  template <class F>
+  hipError_t run(const char* name, double b, F&& f) {
+    if (!on) return f();
+    const int k = kid(name, b);
+    hipEvent_t e0 = get(), e1 = get();
+    (void)hipEventRecord(e0, S->stream);
+    hipError_t e = f();
+    (void)hipEventRecord(e1, S->stream);
+    recs.push_back({k, {e0, e1}});
+    return e;
+  }
+  void finish() {
+    if (!on || !st) return;
+    st->n_kernels = (int32_t)std::min<size_t>(names.size(), GD_MAX_KERNELS);
+    for (int k = 0; k < st->n_kernels; ++k) {
+      std::snprintf(st->kernel_names[k], 32, "%s", names[k].c_str());
+      st->kernel_ms_total[k] = 0.0;
+      st->kernel_launches[k] = 0;
+      st->kernel_bytes[k] = bytes[k];
+    }
+    for (auto& r : recs) {
+      if (r.first >= GD_MAX_KERNELS) continue;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, r.second.first, r.second.second);
+      st->kernel_ms_total[r.first] += ms;
+      st->kernel_launches[r.first] += 1;
+    }
+  }
+};
+}  // namespace
+
+// algorithmic HBM bytes per launch (compulsory arrays once per launch at their
+// stored width; gathered neighbour/node data counted once at unique size)
+static double bytes_ref_step(const MeshView& v, bool visc) {
+  const double nc = (double)v.nc, nn = (double)v.nn;
+  double b = nc * (32 /*Q own*/ + 12 /*nb*/ + 72 /*nx,ny,len*/ + 24 /*dn1,up1*/ + 8 /*rarea*/ + 8 /*hmin*/ +
+                   32 /*U old*/ + 32 /*U new*/ + 32 /*Q new*/) +
+             nn * 32 /*node primitives*/;
+  if (visc) b += nc * (24 /*ea,eb*/ + 8 /*area*/) + nn * 48 /*g6N*/;
+  return b;
+}
+static double bytes_ref_nodes(const MeshView& v, bool visc) {
+  const double nc = (double)v.nc, nn = (double)v.nn;
+  const double inc = 3.0 * nc;  // cell-node incidences of a triangle mesh
+  return inc * (4 + 8) /*ids + weights*/ + nc * 32 /*Q*/ + nn * 32 /*Qn*/ + (visc ? nc * 24 + nn * 24 : 0.0);
+}
+static double bytes_ref_gg1(const MeshView& v) {
+  return (double)v.nc * (24 + 72 + 8 + 48) + (double)v.nn * 24;
+}
+static double bytes_ref_gg_nodes(const MeshView& v) {
+  return 3.0 * v.nc * 4 + (double)v.nc * 48 + (double)v.nn * 48;
+}
+
+int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats) {
+  if (!S || !rp) return fail(GD_ERR_INVALID, "null argument");
+  if (n_steps < 0) return fail(GD_ERR_INVALID, "negative step count");
+  if (rp->dt_policy < GD_DT_FIXED || rp->dt_policy > GD_DT_LOCAL) return fail(GD_ERR_INVALID, "unknown dt policy");
+  const MeshView& v = S->mesh->v;
+  const gd_params& P = S->prm;
+  Timer T{S, rp->time_kernels != 0, stats};
+  const bool wall = rp->wall_code != 0 && v.n_wall > 0;
+  const bool ns = P.scheme == GD_SCHEME_NS;
+  GD_HIP(launch_fields_derived(v.nc, P.Gamma, S->Q[S->qc], S->aS, S->vmax, S->stream));
+  GD_HIP(launch_prime_acc(v, P, S->Q[S->qc], S->vmax, S->acc, rp->flow_time, rp->dt_policy, ns && P.viscous,
+                          S->stream));
+  if (wall) {
+    GD_HIP(launch_fill(S->wall_f, 2 * (int64_t)v.n_wall, 0.0, S->stream));
+    GD_HIP(launch_fill(S->drag, 4, 0.0, S->stream));
+  }
+  if (!ns) {
+    const bool visc = P.viscous != 0;
+    if (visc) GD_HIP(T.run("ref_phi", 0.0, [&] { return launch_ref_phi(v, S->U[S->cur], S->phiN, S->stream); }));
+    for (int32_t s = 0; s < n_steps; ++s) {
+      if (visc) {
+        GD_HIP(T.run("ref_gg1", bytes_ref_gg1(v), [&] { return launch_ref_gg1(v, S->phiN, S->g6, S->stream); }));
+        GD_HIP(T.run("ref_gg_nodes", bytes_ref_gg_nodes(v),
+                     [&] { return launch_ref_gg_nodes(v, S->g6, S->g6N, S->stream); }));
+      }
+      RefStepArgs a{};
+      a.m = v;
+      a.prm = P;
+      a.Q = S->Q[S->qc];
+      a.Qn = S->Qn;
+      a.Uo = S->U[S->cur];
+      a.g6N = visc ? S->g6N : nullptr;
+      a.Un = S->U[1 - S->cur];
+      a.Qw = S->Q[1 - S->qc];
+      a.acc = S->acc;
+      a.step = s;
+      a.dt_policy = rp->dt_policy;
+      a.dt = rp->dt;
+      a.cfl = rp->cfl;
+      a.betta = rp->betta;
+      a.include_udiff = 1;
+      a.wall_f = wall ? S->wall_f : nullptr;
+      a.wall_code = rp->wall_code;
+      GD_HIP(T.run("ref_step", bytes_ref_step(v, visc), [&] { return launch_ref_step(a, S->stream); }));
+      if (wall) GD_HIP(launch_wall_sum(S->wall_f, v.n_wall, S->drag, S->stream));
+      S->cur ^= 1;
+      S->qc ^= 1;
+      GD_HIP(T.run("ref_nodes", bytes_ref_nodes(v, visc), [&] {
+        return launch_ref_nodes(v, S->Q[S->qc], S->U[S->cur], S->Qn, nullptr, visc ? S->phiN : nullptr, S->stream);
+      }));
+    }
+  } else {
+    return fail(GD_ERR_UNSUPPORTED, "ns scheme not built");
+  }
+  // reference bookkeeping after the loop: Delta of the last step, U_NEW == U_OLD
+  if (n_steps > 0) {
+    GD_HIP(launch_delta(v.nc, S->U[S->cur], S->U[1 - S->cur], S->Dl, S->stream));
+    if (rp->residuals) GD_HIP(launch_residuals(v, S->U[S->cur], S->U[1 - S->cur], S->red, S->stream));
+    GD_HIP(launch_copy(S->U[1 - S->cur], S->U[S->cur], 4 * v.nc, S->stream));
+  }
+  Acc acc[3];
+  GD_HIP(hipMemcpyAsync(acc, S->acc, sizeof(acc), hipMemcpyDeviceToHost, S->stream));
+  double red[4] = {0, 0, 0, 0}, drag[4] = {0, 0, 0, 0};
+  if (rp->residuals && n_steps > 0) GD_HIP(hipMemcpyAsync(red, S->red, sizeof(red), hipMemcpyDeviceToHost, S->stream));
+  if (wall) GD_HIP(hipMemcpyAsync(drag, S->drag, sizeof(drag), hipMemcpyDeviceToHost, S->stream));
+  GD_HIP(hipStreamSynchronize(S->stream));
+  const Acc& last = acc[n_steps % 3];
+  rp->flow_time = last.time;
+  if (stats) {
+    stats->flow_time = last.time;
+    stats->last_dt = last.dt;
+    for (int q = 0; q < 4; ++q) stats->residuals[q] = red[q];
+    stats->wall_force[0] = drag[0];
+    stats->wall_force[1] = drag[1];
+    stats->wall_force_sum[0] = drag[2];
+    stats->wall_force_sum[1] = drag[3];
+    stats->nonphysical = (int32_t)last.nonphysical;
+    if (!T.on) stats->n_kernels = 0;
+  }
+  T.finish();
+  if (last.nonphysical) {
+    char buf[128];
+    std::snprintf(buf, sizeof(buf), "%u cells with non-positive density/pressure or NaN", last.nonphysical);
+    g_err = buf;
+    return GD_ERR_NONPHYSICAL;
+  }
+  return GD_OK;
+}
+

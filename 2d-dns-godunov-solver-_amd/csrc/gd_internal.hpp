@@ -1,0 +1,168 @@
+// Internal device data model of libgodunov (not part of the C ABI).
+//
+// HBM layout: structure-of-arrays, 0-based, in device cell order.  Per-face
+// arrays are stored face-slot-major, [slot][cell], so that thread i of a
+// wavefront reads element i of each slot: every load of the cell-centric
+// kernels is a unit-stride 64-lane access except the neighbour gathers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/godunov.h"
+
+namespace gd {
+
+// Everything a kernel needs about the mesh, passed by value as a kernel argument.
+struct MeshView {
+  int64_t nc, nn;
+  int32_t nf;          // face slots stored: 3 (triangles only) or 4
+  int32_t ncl;         // cluster width stored
+  int32_t has_quads;
+  int32_t n_wall;      // faces in the wall list
+  // per cell (nc) unless noted
+  const int32_t* nv;   // vertices per cell (3/4)
+  const int32_t* nb;   // [nf][nc] neighbour (>= 0) or (code - 1) (< 0) for a boundary face
+  const double* fnx;   // [nf][nc] inward unit normal
+  const double* fny;
+  const double* flen;  // [nf][nc] edge length
+  const int32_t* dn1;  // [nf][nc] own far-side node(s) of the face (node2cellsL2down)
+  const int32_t* dn2;
+  const int32_t* up1;  // [nf][nc] neighbour far-side node(s) (node2cellsL2up)
+  const int32_t* up2;
+  const int32_t* ea;   // [nf][nc] edge end nodes (cells2nodes)
+  const int32_t* eb;
+  const double* area;
+  const double* rarea; // 1/A
+  const double* hx;    // longest edge (HX)
+  const double* hmin;  // 2A / perimeter
+  const double* xc;    // centroid (cell_mid_points)
+  const double* yc;
+  // ns geometry
+  const double* lsq;   // [nf][2][nc] least-squares weights: grad = sum_k w_k (q_nb - q_i)
+  const double* rf;    // [nf][2][nc] face midpoint - own centroid
+  const double* dc;    // [nf][2][nc] neighbour centroid - own centroid (periodic safe)
+  const int32_t* wall_id; // [nf][nc] index into the wall list or -1
+  // per node
+  const int32_t* cl;   // [ncl][nn] cluster cell ids, -1 padded
+  const double* w;     // [ncl][nn] interpolation weights
+  const double* xn;
+  const double* yn;
+  double max_hx;
+};
+
+struct Mesh {
+  MeshView v{};
+  std::vector<void*> allocs;
+  std::vector<int64_t> cperm, ciperm;  // cells new->old, old->new
+  std::vector<int64_t> nperm, niperm;  // nodes new->old, old->new
+  int64_t n_boundary_faces = 0;
+  int64_t device_bytes = 0;
+  int64_t bandwidth = 0;
+  int32_t max_cluster = 0;
+  int device = 0;
+};
+
+// Run-time accumulator slot, rotated over three buffers (see gd_run).
+struct Acc {
+  unsigned long long vmax_bits;   // max over cells of |u|+a (positive doubles compare as integers)
+  unsigned long long hdt_bits;    // min over cells of local dt / CFL
+  double time;
+  double dt;
+  unsigned int nonphysical;
+  unsigned int pad;
+  double force[2];
+};
+
+struct State {
+  Mesh* mesh = nullptr;
+  gd_params prm{};
+  hipStream_t stream = nullptr;
+  double* U[2] = {nullptr, nullptr};   // [4][nc]
+  double* Q[2] = {nullptr, nullptr};   // [4][nc] primitives rho,u,v,p
+  int cur = 0;                          // U[cur] is U_OLD, U[1-cur] is U_NEW
+  int qc = 0;                           // Q[qc] are the current cell primitives
+  double* aS = nullptr;                 // [nc]
+  double* vmax = nullptr;               // [nc]
+  double* Qn = nullptr;                 // [4][nn]
+  double* Un = nullptr;                 // [4][nn]
+  double* Ud = nullptr;                 // [4][nc]
+  double* F = nullptr;                  // [4][nc]
+  double* Dl = nullptr;                 // [4][nc]
+  double* phiN = nullptr;               // [3][nn]
+  double* g6 = nullptr;                 // [6][nc]
+  double* g6N = nullptr;                // [6][nn]
+  void* grad = nullptr;                 // ns: [8][nc] (double or float)
+  void* phi = nullptr;                  // ns: [4][nc]
+  double* wall_f = nullptr;             // [n_wall][2]
+  double* drag = nullptr;               // [4]: last step force x,y; running sums
+  Acc* acc = nullptr;                   // [3]
+  double* red = nullptr;                // reduction scratch
+  std::vector<void*> allocs;
+};
+
+// ---- kernel launchers (gd_ref_kernels.hip) ----
+struct RefStepArgs {
+  MeshView m;
+  gd_params prm;
+  const double* Q;      // cell primitives (read)
+  const double* Qn;     // node primitives
+  const double* Uo;     // U_OLD
+  const double* Ud;     // U_DIFF (stage variant) or nullptr
+  const double* g6N;    // node means of gradients (fused viscous) or nullptr
+  double* Un;           // U_NEW
+  double* F;            // FLUXES or nullptr
+  double* Qw;           // fused update: new primitives, or nullptr
+  double* aS;
+  double* vmax;
+  Acc* acc;             // fused run: accumulators (3 slots) or nullptr
+  int32_t step;         // rotates the accumulator slots
+  int32_t dt_policy;
+  double dt, cfl, betta, flow_time;
+  int32_t include_udiff;
+  double* wall_f;
+  int32_t wall_code;
+};
+
+hipError_t launch_ref_step(const RefStepArgs& a, hipStream_t s);
+hipError_t launch_ref_nodes(const MeshView& m, const double* Q, const double* U, double* Qn, double* Un,
+                            double* phiN, hipStream_t s);
+hipError_t launch_ref_phi(const MeshView& m, const double* U, double* phiN, hipStream_t s);
+hipError_t launch_ref_gg1(const MeshView& m, const double* phiN, double* g6, hipStream_t s);
+hipError_t launch_ref_gg_nodes(const MeshView& m, const double* g6, double* g6N, hipStream_t s);
+hipError_t launch_ref_diff_term(const MeshView& m, const gd_params& prm, const double* Q, const double* g6N,
+                                double* Ud, hipStream_t s);
+hipError_t launch_update_variables(const MeshView& m, double gamma, const double* Unew, double* Uold, double* Q,
+                                   double* aS, double* vmax, double* Delta, hipStream_t s);
+hipError_t launch_prime_acc(const MeshView& m, const gd_params& prm, const double* Q, const double* vmax, Acc* acc,
+                            double flow_time, int32_t dt_policy, int32_t viscous_ns, hipStream_t s);
+hipError_t launch_residuals(const MeshView& m, const double* Unew, const double* Uold, double* red, hipStream_t s);
+hipError_t launch_wall_sum(const double* wall_f, int32_t n, double* out4, hipStream_t s);
+hipError_t launch_fields_derived(int64_t nc, double gamma, const double* Q, double* aS, double* vmax, hipStream_t s);
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
+hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
+hipError_t launch_delta(int64_t nc, const double* Un, const double* Uo, double* D, hipStream_t s);
+
+// ---- ns scheme (gd_ns_kernels.hip) ----
+struct NsArgs {
+  MeshView m;
+  gd_params prm;
+  const double* Uo;
+  double* Un;
+  double* Qo;           // current primitives (read) as doubles
+  double* Qw;           // new primitives (write)
+  void* grad;           // [8][nc]
+  void* phi;            // [4][nc]
+  Acc* acc;
+  int32_t step;
+  int32_t dt_policy;
+  double dt, cfl;
+  double* wall_f;
+  int32_t wall_code;
+};
+hipError_t launch_ns_grad(const NsArgs& a, hipStream_t s);
+hipError_t launch_ns_flux(const NsArgs& a, hipStream_t s);
+
+}  // namespace gd

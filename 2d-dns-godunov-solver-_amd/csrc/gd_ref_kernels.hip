@@ -1,0 +1,565 @@
+// Reference-scheme kernels (FP64), compiled with -ffp-contract=off.
+//
+// Cell-centric and deterministic: one work-item per cell gathers its
+// neighbours' states and sums its own face fluxes in face order, so there are
+// no atomics on the state and results are bit-reproducible run to run and
+// independent of the cell numbering.  Per step of the device-resident loop:
+//   k_ref_step    flux sum + explicit Euler update + primitive recovery
+//                 (+ the calcDiffTerm Laplacian from node means of gradients)
+//   k_ref_nodes   cell -> node weighted means (next step's MUSCL stencil)
+//                 and unweighted cluster means of rho*u, rho*v, E
+//   k_ref_gg1     Green-Gauss gradients of rho*u, rho*v, E   (viscous only)
+//   k_ref_gg_nodes  cluster means of those gradients         (viscous only)
+#include "gd_internal.hpp"
+#include "gd_ref_math.hpp"
+
+namespace gd {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-reduce the step's dt inputs and non-physical count into the next
+// accumulator slot (max/min of positive doubles as integer atomics: the
+// result does not depend on the order of arrival).
+__device__ __forceinline__ void reduce_acc(double vmax, double hdt, unsigned bad, Acc* w) {
+  __shared__ double s_max[kBlock / 64], s_min[kBlock / 64];
+  __shared__ unsigned s_bad[kBlock / 64];
+  vmax = wave_max(vmax);
+  hdt = wave_min(hdt);
+  bad = wave_sum_u(bad);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { s_max[wid] = vmax; s_min[wid] = hdt; s_bad[wid] = bad; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < kBlock / 64; ++q) {
+      vmax = fmax(vmax, s_max[q]); hdt = fmin(hdt, s_min[q]); bad += s_bad[q];
+    }
+    atomicMax(&w->vmax_bits, (unsigned long long)__double_as_longlong(vmax));
+    atomicMin(&w->hdt_bits, (unsigned long long)__double_as_longlong(hdt));
+    if (bad) atomicAdd(&w->nonphysical, bad);
+  }
+}
+
+__device__ __forceinline__ double step_dt(const Acc& r, int policy, double dt_fixed, double cfl, double max_hx) {
+  if (policy == GD_DT_REF_TAU) return cfl * max_hx / refm::jmax(__longlong_as_double((long long)r.vmax_bits), 1.0e-6);
+  if (policy == GD_DT_LOCAL) return cfl * __longlong_as_double((long long)r.hdt_bits);
+  return dt_fixed;
+}
+
+__device__ __forceinline__ void rotate_acc(Acc* acc, int step, double t, double dt) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Acc* w = &acc[(step + 1) % 3];
+    w->time = t + dt;
+    w->dt = dt;
+    Acc* z = &acc[(step + 2) % 3];
+    z->vmax_bits = 0ull;
+    z->hdt_bits = 0x7ff0000000000000ull;  // +inf
+    z->nonphysical = 0u;
+  }
+}
+
+// GG face sum of one node-mean field: sum_k ((0.5 (phi_a + phi_b)) (-n_k)) len_k,
+// calcGradients.jl:116-129 (the x/y swap of the source is applied by the caller)
+template <bool QUADS>
+__device__ __forceinline__ void gg_sums(const MeshView& m, int64_t i, int nf, const double* fld, double* sx,
+                                        double* sy) {
+  const int64_t nc = m.nc;
+  double ax = 0.0, ay = 0.0;
+#pragma unroll
+  for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
+    if (k < nf) {
+      const double pa = fld[m.ea[k * nc + i]], pb = fld[m.eb[k * nc + i]];
+      const double hm = 0.5 * (pa + pb);
+      const double fx = hm * -m.fny[k * nc + i] * m.flen[k * nc + i];
+      const double fy = hm * -m.fnx[k * nc + i] * m.flen[k * nc + i];
+      if (k == 0) { ax = fx; ay = fy; } else { ax = ax + fx; ay = ay + fy; }
+    }
+  }
+  *sx = ax;
+  *sy = ay;
+}
+
+// calcDiffTerm's per-cell tail (calcDiffTerm.jl:98-119): Laplacians from the
+// second Green-Gauss level, mu = mu_const, energy row scaled by k(T).
+template <bool QUADS>
+__device__ __forceinline__ void ref_udiff(const MeshView& m, const gd_params& p, const double* g6N, int64_t i, int nf,
+                                          const double* q, double* ud) {
+  const int64_t nn = m.nn;
+  const double A = m.area[i];
+  double lap[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double sx, sy, tx, ty;
+    gg_sums<QUADS>(m, i, nf, g6N + (2 * c) * nn, &sx, &sy);      // "gradX" of the "gradX" field
+    gg_sums<QUADS>(m, i, nf, g6N + (2 * c + 1) * nn, &tx, &ty);  // "gradY" of the "gradY" field
+    lap[c] = sx / A + ty / A;
+  }
+  const double T = q[3] / q[0] / p.RGAS;
+  const double mu = p.mu_const;
+  const double k = p.energy_law == GD_ENERGY_CONST ? p.k_const : refm::air_k(T);
+  ud[1] = mu * lap[0];
+  ud[2] = mu * lap[1];
+  ud[3] = mu * k * lap[2];
+}
+
+// One cell of computeInterfaceSlope (computeslope2d.jl:3-136) summed over its
+// faces as in SecondOrderUpwindM2 (SOUscheme.jl:56-92).
+template <bool QUADS>
+__device__ __forceinline__ void ref_cell_flux(const RefStepArgs& a, int64_t i, int nf, const double* qL, double t,
+                                              double* F, double* wallF) {
+  const MeshView& m = a.m;
+  const int64_t nc = m.nc;
+  const double g = a.prm.Gamma;
+#pragma unroll
+  for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
+    if (k < nf) {
+      const int64_t s = k * nc + i;
+      const int32_t j = m.nb[s];
+      const double nx = m.fnx[s], ny = m.fny[s], len = m.flen[s];
+      double up[4], dn[4], qR[4];
+      if (j >= 0) {
+        const int32_t d1 = m.dn1[s], u1 = m.up1[s];
+        if (QUADS && nf == 4) {
+          const int32_t d2 = m.dn2[s], u2 = m.up2[s];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            up[v] = 0.5 * (a.Qn[v * m.nn + u1] + a.Qn[v * m.nn + u2]);
+            dn[v] = 0.5 * (a.Qn[v * m.nn + d1] + a.Qn[v * m.nn + d2]);
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) { up[v] = a.Qn[v * m.nn + u1]; dn[v] = a.Qn[v * m.nn + d1]; }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) qR[v] = a.Q[v * nc + j];
+      } else {
+        const int32_t na = m.ea[s], nbn = m.eb[s];
+        refm::bc_ghost(a.prm.bc, j + 1, qL, nx, ny, m.xc[i], m.yc[i], m.xn[na], m.yn[na], m.xn[nbn], m.yn[nbn], t,
+                       a.prm.RGAS, qR);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { dn[v] = qL[v]; up[v] = qR[v]; }
+      }
+      const double eps = a.prm.limiter_eps;
+      double fL[4], fR[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        fL[v] = qL[v] + 0.5 * refm::limiter(a.prm.limiter, qL[v] - dn[v], qR[v] - qL[v], eps);
+        fR[v] = qR[v] - 0.5 * refm::limiter(a.prm.limiter, qR[v] - qL[v], up[v] - qR[v], eps);
+      }
+      double e[4];
+      refm::face_flux(a.prm.flux, fR, fL, nx, ny, len, g, e);  // neighbour side is "L"
+      if (j < 0 && wallF && m.wall_id && a.wall_code == j + 1) {
+        const int32_t wid = m.wall_id[s];
+        if (wid >= 0) { wallF[2 * wid] = e[1]; wallF[2 * wid + 1] = e[2]; }
+      }
+      if (k == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) F[v] = e[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) F[v] = F[v] + e[v];
+      }
+    }
+  }
+}
+
+template <bool QUADS, bool VISC, bool FUSED>
+__global__ __launch_bounds__(kBlock) void k_ref_step(RefStepArgs a) {
+  const MeshView& m = a.m;
+  const int64_t nc = m.nc;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double dt = a.dt, t = a.flow_time;
+  if (FUSED) {
+    const Acc r = a.acc[a.step % 3];
+    t = r.time;
+    dt = step_dt(r, a.dt_policy, a.dt, a.cfl, m.max_hx);
+  }
+  double vmx = 0.0, hdt = __longlong_as_double(0x7ff0000000000000ll);
+  unsigned bad = 0;
+  if (i < nc) {
+    const double qL[4] = {a.Q[i], a.Q[nc + i], a.Q[2 * nc + i], a.Q[3 * nc + i]};
+    const int nf = QUADS ? m.nv[i] : 3;
+    double F[4];
+    ref_cell_flux<QUADS>(a, i, nf, qL, t, F, FUSED ? a.wall_f : nullptr);
+    double ud[4] = {0.0, 0.0, 0.0, 0.0};
+    bool with_ud = FUSED ? true : (a.include_udiff && a.Ud);
+    if (VISC) ref_udiff<QUADS>(m, a.prm, a.g6N, i, nf, qL, ud);
+    else if (!FUSED && with_ud) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) ud[v] = a.Ud[v * nc + i];
+    }
+    const double Rarea = m.rarea[i];
+    const double b = a.betta;
+    double Un[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double uo = a.Uo[v * nc + i];
+      Un[v] = with_ud ? (uo - F[v] * b * dt * Rarea + b * dt * ud[v]) : (uo - F[v] * b * dt * Rarea);
+      a.Un[v * nc + i] = Un[v];
+      if (a.F) a.F[v * nc + i] = F[v];
+    }
+    if (FUSED) {
+      // updateVariablesSA, evaluate2d.jl:87-93
+      const double G = a.prm.Gamma;
+      const double rho = Un[0];
+      const double ux = Un[1] / Un[0];
+      const double uy = Un[2] / Un[0];
+      const double p = (G - 1.0) * (Un[3] - 0.5 * (Un[1] * Un[1] + Un[2] * Un[2]) / Un[0]);
+      const double as = sqrt(G * p / rho);
+      const double vm = sqrt(ux * ux + uy * uy) + as;
+      a.Qw[i] = rho; a.Qw[nc + i] = ux; a.Qw[2 * nc + i] = uy; a.Qw[3 * nc + i] = p;
+      if (a.aS) a.aS[i] = as;
+      if (a.vmax) a.vmax[i] = vm;
+      vmx = vm;
+      hdt = m.hmin[i] / vm;
+      bad = (!(rho > 0.0) || !(p > 0.0) || vm != vm) ? 1u : 0u;
+      if (vm != vm) { vmx = 0.0; hdt = __longlong_as_double(0x7ff0000000000000ll); }
+    }
+  }
+  if (FUSED) {
+    reduce_acc(vmx, hdt, bad, &a.acc[(a.step + 1) % 3]);
+    rotate_acc(a.acc, a.step, t, dt);
+  }
+}
+
+// cells2nodesSolutionReconstructionWithStencils (utilsFVM2dp.jl:299-365) plus,
+// when phiN is given, the unweighted cluster means of calcPhiInCellCluster
+// (calcGradients.jl:3-18) for rho*u, rho*v, E.
+__global__ __launch_bounds__(kBlock) void k_ref_nodes(MeshView m, const double* Q, const double* U, double* Qn,
+                                                      double* Un, double* phiN) {
+  const int64_t J = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (J >= m.nn) return;
+  const int64_t nc = m.nc, nn = m.nn;
+  double det = 0.0, r = 0.0, u = 0.0, v = 0.0, p = 0.0;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+  int cnt = 0;
+  for (int q = 0; q < m.ncl; ++q) {
+    const int32_t c = m.cl[q * nn + J];
+    if (c >= 0) {
+      const double wi = m.w[q * nn + J];
+      if (Qn) {
+        r = r + Q[c] * wi; u = u + Q[nc + c] * wi; v = v + Q[2 * nc + c] * wi; p = p + Q[3 * nc + c] * wi;
+      }
+      if (Un) {
+        s0 = s0 + U[c] * wi; s1 = s1 + U[nc + c] * wi; s2 = s2 + U[2 * nc + c] * wi; s3 = s3 + U[3 * nc + c] * wi;
+      }
+      if (phiN) { f1 = f1 + U[nc + c]; f2 = f2 + U[2 * nc + c]; f3 = f3 + U[3 * nc + c]; }
+      det = det + wi;
+      ++cnt;
+    }
+  }
+  if (det != 0) {
+    r = r / det; u = u / det; v = v / det; p = p / det;
+    s0 = s0 / det; s1 = s1 / det; s2 = s2 / det; s3 = s3 / det;
+  }
+  if (Qn) { Qn[J] = r; Qn[nn + J] = u; Qn[2 * nn + J] = v; Qn[3 * nn + J] = p; }
+  if (Un) { Un[J] = s0; Un[nn + J] = s1; Un[2 * nn + J] = s2; Un[3 * nn + J] = s3; }
+  if (phiN) {
+    const double dc = (double)cnt;
+    phiN[J] = cnt ? f1 / dc : 0.0;
+    phiN[nn + J] = cnt ? f2 / dc : 0.0;
+    phiN[2 * nn + J] = cnt ? f3 / dc : 0.0;
+  }
+}
+
+// First Green-Gauss level of calcDiffTerm (calcDiffTerm.jl:65-67): for
+// rho*u, rho*v, E the pair ("gradX", "gradY") = (d/dy, d/dx).
+template <bool QUADS>
+__global__ __launch_bounds__(kBlock) void k_ref_gg1(MeshView m, const double* phiN, double* g6) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= m.nc) return;
+  const int nf = QUADS ? m.nv[i] : 3;
+  const double A = m.area[i];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double sx, sy;
+    gg_sums<QUADS>(m, i, nf, phiN + c * m.nn, &sx, &sy);
+    g6[(2 * c) * m.nc + i] = sx / A;
+    g6[(2 * c + 1) * m.nc + i] = sy / A;
+  }
+}
+
+// Cluster means of the six gradient fields (calcPhiInCellCluster applied
+// inside the second gradient level, calcDiffTerm.jl:88-95).
+__global__ __launch_bounds__(kBlock) void k_ref_gg_nodes(MeshView m, const double* g6, double* g6N) {
+  const int64_t J = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (J >= m.nn) return;
+  const int64_t nc = m.nc, nn = m.nn;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  int cnt = 0;
+  for (int q = 0; q < m.ncl; ++q) {
+    const int32_t c = m.cl[q * nn + J];
+    if (c >= 0) {
+#pragma unroll
+      for (int r = 0; r < 6; ++r) s[r] = s[r] + g6[r * nc + c];
+      ++cnt;
+    }
+  }
+  const double dc = (double)cnt;
+#pragma unroll
+  for (int r = 0; r < 6; ++r) g6N[r * nn + J] = cnt ? s[r] / dc : 0.0;
+}
+
+template <bool QUADS>
+__global__ __launch_bounds__(kBlock) void k_ref_diff_term(MeshView m, gd_params p, const double* Q, const double* g6N,
+                                                          double* Ud) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= m.nc) return;
+  const int64_t nc = m.nc;
+  const double q[4] = {Q[i], Q[nc + i], Q[2 * nc + i], Q[3 * nc + i]};
+  double ud[4] = {0, 0, 0, 0};
+  ref_udiff<QUADS>(m, p, g6N, i, QUADS ? m.nv[i] : 3, q, ud);
+  Ud[nc + i] = ud[1];
+  Ud[2 * nc + i] = ud[2];
+  Ud[3 * nc + i] = ud[3];
+}
+
+// updateVariablesSA, evaluate2d.jl:83-108
+__global__ __launch_bounds__(kBlock) void k_update_variables(int64_t nc, double G, const double* Un, double* Uo,
+                                                             double* Q, double* aS, double* vmax, double* Delta) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nc) return;
+  const double u0 = Un[i], u1 = Un[nc + i], u2 = Un[2 * nc + i], u3 = Un[3 * nc + i];
+  const double rho = u0, ux = u1 / u0, uy = u2 / u0;
+  const double p = (G - 1.0) * (u3 - 0.5 * (u1 * u1 + u2 * u2) / u0);
+  const double as = sqrt(G * p / rho);
+  Q[i] = rho; Q[nc + i] = ux; Q[2 * nc + i] = uy; Q[3 * nc + i] = p;
+  aS[i] = as;
+  vmax[i] = sqrt(ux * ux + uy * uy) + as;
+  const double un[4] = {u0, u1, u2, u3};
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    if (Delta) Delta[v * nc + i] = un[v] - Uo[v * nc + i];
+    Uo[v * nc + i] = un[v];
+  }
+}
+
+__global__ void k_acc_reset(Acc* acc, double t) {
+  if (threadIdx.x < 3) {
+    Acc* z = &acc[threadIdx.x];
+    z->vmax_bits = 0ull;
+    z->hdt_bits = 0x7ff0000000000000ull;
+    z->nonphysical = 0u;
+    z->time = t;
+    z->dt = 0.0;
+    z->force[0] = z->force[1] = 0.0;
+  }
+}
+
+// Fills accumulator slot 0 from the current state (the reference takes
+// max(VMAXCells) at the top of each step, testViscousFluxesTGV2d.jl:420).
+__global__ __launch_bounds__(kBlock) void k_prime_acc(MeshView m, gd_params p, const double* Q, const double* vmax,
+                                                      Acc* acc, int32_t viscous_ns) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double vm = 0.0, hdt = __longlong_as_double(0x7ff0000000000000ll);
+  unsigned bad = 0;
+  if (i < m.nc) {
+    const int64_t nc = m.nc;
+    const double rho = Q[i], pr = Q[3 * nc + i];
+    vm = vmax[i];
+    double lam = vm;
+    if (viscous_ns) {
+      const double T = pr / (rho * p.RGAS);
+      const double mu = p.mu_ref * pow(T / p.T_ref, 1.5) * (p.T_ref + p.S_suth) / (T + p.S_suth);
+      lam = vm + 2.0 * p.Gamma * mu / (p.Pr * rho * m.hmin[i]);
+    }
+    hdt = m.hmin[i] / lam;
+    bad = (!(rho > 0.0) || !(pr > 0.0) || vm != vm) ? 1u : 0u;
+    if (vm != vm) { vm = 0.0; hdt = __longlong_as_double(0x7ff0000000000000ll); }
+  }
+  reduce_acc(vm, hdt, bad, &acc[0]);
+}
+
+// Deterministic sums of (Unew - Uold)^2 per variable: per-block partials, then
+// one block folds them in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_resid_partial(int64_t nc, const double* Un, const double* Uo,
+                                                          double* part) {
+  __shared__ double s[4][kBlock];
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    double d = 0.0;
+    if (i < nc) { d = Un[v * nc + i] - Uo[v * nc + i]; d = d * d; }
+    s[v][threadIdx.x] = d;
+  }
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) s[v][threadIdx.x] += s[v][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) part[(int64_t)blockIdx.x * 4 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fold(const double* part, int64_t nparts, int width, double* out) {
+  __shared__ double s[kBlock];
+  for (int v = 0; v < width; ++v) {
+    double a = 0.0;
+    for (int64_t b = threadIdx.x; b < nparts; b += kBlock) a += part[b * width + v];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[v] = s[0];
+    __syncthreads();
+  }
+}
+
+// wall force of one step: fold the per-face contributions in a fixed order;
+// out = {fx, fy, running sum fx, running sum fy}
+__global__ __launch_bounds__(kBlock) void k_wall_sum(const double* wf, int32_t n, double* out) {
+  __shared__ double sx[kBlock], sy[kBlock];
+  double ax = 0.0, ay = 0.0;
+  for (int b = threadIdx.x; b < n; b += kBlock) { ax += wf[2 * b]; ay += wf[2 * b + 1]; }
+  sx[threadIdx.x] = ax; sy[threadIdx.x] = ay;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) { sx[threadIdx.x] += sx[threadIdx.x + o]; sy[threadIdx.x] += sy[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[0] = sx[0]; out[1] = sy[0]; out[2] += sx[0]; out[3] += sy[0]; }
+}
+
+// aSoundCells and VMAXCells from the primitives (initfields_TGV2d.jl:47-48,
+// evaluate2d.jl:92-93)
+__global__ __launch_bounds__(kBlock) void k_fields_derived(int64_t nc, double G, const double* Q, double* aS,
+                                                           double* vmax) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nc) return;
+  const double rho = Q[i], ux = Q[nc + i], uy = Q[2 * nc + i], p = Q[3 * nc + i];
+  const double as = sqrt(G * p / rho);
+  aS[i] = as;
+  vmax[i] = sqrt(ux * ux + uy * uy) + as;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(double* p, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(double* d, const double* s, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) d[i] = s[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_delta(int64_t nc, const double* Un, const double* Uo, double* D) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < 4 * nc) D[i] = Un[i] - Uo[i];
+}
+
+inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+hipError_t launch_fields_derived(int64_t nc, double gamma, const double* Q, double* aS, double* vmax, hipStream_t s) {
+  hipLaunchKernelGGL(k_fields_derived, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, gamma, Q, aS, vmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy, dim3(nblocks(n)), dim3(kBlock), 0, s, dst, src, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_delta(int64_t nc, const double* Un, const double* Uo, double* D, hipStream_t s) {
+  hipLaunchKernelGGL(k_delta, dim3(nblocks(4 * nc)), dim3(kBlock), 0, s, nc, Un, Uo, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_step(const RefStepArgs& a, hipStream_t s) {
+  const bool q = a.m.has_quads != 0;
+  const bool fused = a.acc != nullptr;
+  const bool visc = a.g6N != nullptr;
+  const dim3 g(nblocks(a.m.nc)), b(kBlock);
+#define GD_LAUNCH(Q_, V_, F_) hipLaunchKernelGGL((k_ref_step<Q_, V_, F_>), g, b, 0, s, a)
+  if (fused) {
+    if (q) { if (visc) GD_LAUNCH(true, true, true); else GD_LAUNCH(true, false, true); }
+    else { if (visc) GD_LAUNCH(false, true, true); else GD_LAUNCH(false, false, true); }
+  } else {
+    if (q) GD_LAUNCH(true, false, false); else GD_LAUNCH(false, false, false);
+  }
+#undef GD_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_nodes(const MeshView& m, const double* Q, const double* U, double* Qn, double* Un, double* phiN,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_ref_nodes, dim3(nblocks(m.nn)), dim3(kBlock), 0, s, m, Q, U, Qn, Un, phiN);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_phi(const MeshView& m, const double* U, double* phiN, hipStream_t s) {
+  hipLaunchKernelGGL(k_ref_nodes, dim3(nblocks(m.nn)), dim3(kBlock), 0, s, m, nullptr, U, nullptr, nullptr, phiN);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_gg1(const MeshView& m, const double* phiN, double* g6, hipStream_t s) {
+  if (m.has_quads) hipLaunchKernelGGL(k_ref_gg1<true>, dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m, phiN, g6);
+  else hipLaunchKernelGGL(k_ref_gg1<false>, dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m, phiN, g6);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_gg_nodes(const MeshView& m, const double* g6, double* g6N, hipStream_t s) {
+  hipLaunchKernelGGL(k_ref_gg_nodes, dim3(nblocks(m.nn)), dim3(kBlock), 0, s, m, g6, g6N);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_diff_term(const MeshView& m, const gd_params& prm, const double* Q, const double* g6N,
+                                double* Ud, hipStream_t s) {
+  if (m.has_quads)
+    hipLaunchKernelGGL(k_ref_diff_term<true>, dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m, prm, Q, g6N, Ud);
+  else
+    hipLaunchKernelGGL(k_ref_diff_term<false>, dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m, prm, Q, g6N, Ud);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_variables(const MeshView& m, double gamma, const double* Unew, double* Uold, double* Q,
+                                   double* aS, double* vmax, double* Delta, hipStream_t s) {
+  hipLaunchKernelGGL(k_update_variables, dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m.nc, gamma, Unew, Uold, Q, aS,
+                     vmax, Delta);
+  return hipGetLastError();
+}
+
+hipError_t launch_prime_acc(const MeshView& m, const gd_params& prm, const double* Q, const double* vmax, Acc* acc,
+                            double flow_time, int32_t dt_policy, int32_t viscous_ns, hipStream_t s) {
+  (void)dt_policy;
+  hipLaunchKernelGGL(k_acc_reset, dim3(1), dim3(64), 0, s, acc, flow_time);
+  hipLaunchKernelGGL(k_prime_acc, dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m, prm, Q, vmax, acc, viscous_ns);
+  return hipGetLastError();
+}
+
+hipError_t launch_residuals(const MeshView& m, const double* Unew, const double* Uold, double* red, hipStream_t s) {
+  const unsigned nb = nblocks(m.nc);
+  hipLaunchKernelGGL(k_resid_partial, dim3(nb), dim3(kBlock), 0, s, m.nc, Unew, Uold, red + 8);
+  hipLaunchKernelGGL(k_fold, dim3(1), dim3(kBlock), 0, s, red + 8, (int64_t)nb, 4, red);
+  return hipGetLastError();
+}
+
+hipError_t launch_wall_sum(const double* wall_f, int32_t n, double* out4, hipStream_t s) {
+  hipLaunchKernelGGL(k_wall_sum, dim3(1), dim3(kBlock), 0, s, wall_f, n, out4);
+  return hipGetLastError();
+}
+
+}  // namespace gd

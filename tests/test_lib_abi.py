@@ -1,0 +1,44 @@
+"""The C-ABI library loads and exports every entry point include/godunov.h declares.
+
+No compute calls here: this runs without a GPU.
+"""
+import ctypes
+import os
+import re
+
+from conftest import ROOT, pkg
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "godunov.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gd_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_the_binding_list():
+    L = pkg()._lib
+    assert sorted(L.EXPORTS) == declared_functions()
+
+
+def test_library_exports_every_declared_symbol():
+    L = pkg()._lib
+    lib = L.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_version_and_errors_without_device():
+    L = pkg()._lib
+    lib = L.load()
+    assert b"gfx950" in lib.gd_version()
+    buf = ctypes.create_string_buffer(16)
+    assert lib.gd_last_error(buf, 16) == 0
+    assert lib.gd_last_error(None, 0) == L.GD_ERR_INVALID
+
+
+def test_struct_sizes_match_header_layout():
+    L = pkg()._lib
+    # 4 ints, 2 doubles, 2 ints, 8 doubles, 17 x (2 ints + 4 doubles)
+    assert ctypes.sizeof(L.gd_params) == 4 * 4 + 2 * 8 + 2 * 4 + 8 * 8 + 17 * (8 + 32)
+    assert ctypes.sizeof(L.gd_run_params) == 4 + 4 + 4 * 8 + 4 + 4
+    assert ctypes.sizeof(L.gd_mesh_desc) == 8 + 8 + 4 + 4 + 17 * 8
