@@ -60,7 +60,8 @@ class gd_bc(C.Structure):  # noqa: N801
 
 class gd_params(C.Structure):  # noqa: N801
     _fields_ = [("scheme", C.c_int32), ("precision", C.c_int32), ("flux", C.c_int32), ("limiter", C.c_int32),
-                ("limiter_eps", C.c_double), ("venkat_K", C.c_double), ("viscous", C.c_int32),
+                ("limiter_eps", C.c_double), ("venkat_K", C.c_double), ("venkat_scale", C.c_double * 4),
+                ("viscous", C.c_int32),
                 ("energy_law", C.c_int32), ("mu_const", C.c_double), ("k_const", C.c_double),
                 ("RGAS", C.c_double), ("Gamma", C.c_double), ("mu_ref", C.c_double), ("T_ref", C.c_double),
                 ("S_suth", C.c_double), ("Pr", C.c_double), ("bc", gd_bc * (GD_MAX_BC + 1))]

@@ -23,7 +23,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-t
 UNITS = {
     "gd_api.hip": [],
     "gd_ref_kernels.hip": ["-ffp-contract=off"],
-    "gd_ns_kernels.hip": [],
+    "gd_ns_fp64.hip": ["-ffp-contract=off"],
+    "gd_ns_fp32.hip": [],
 }
 
 

@@ -140,6 +140,7 @@ void gd_params_default(gd_params* p) {
   p->limiter = GD_LIM_REF_HARMONIC;
   p->limiter_eps = 1.0e-12;
   p->venkat_K = 5.0;
+  for (int v = 0; v < 4; ++v) p->venkat_scale[v] = 1.0;
   p->viscous = 0;
   p->energy_law = GD_ENERGY_AIR_POLY;
   p->mu_const = 1.0e-2;
@@ -481,6 +482,12 @@ int gd_state_create(gd_mesh* m, const gd_params* p, gd_state** out) {
   double* pbuf = nullptr;
   rc |= dev_alloc(al, &pbuf, 4 * nc);
   S->phi = pbuf;
+  double* qt0 = nullptr;
+  double* qt1 = nullptr;
+  rc |= dev_alloc(al, &qt0, 4 * nc);
+  rc |= dev_alloc(al, &qt1, 4 * nc);
+  S->Qt[0] = qt0;
+  S->Qt[1] = qt1;
   rc |= dev_alloc(al, &S->wall_f, 2 * (size_t)std::max<int32_t>(m->v.n_wall, 1));
   rc |= dev_alloc(al, &S->drag, 4);
   rc |= dev_alloc(al, &S->acc, 3);
@@ -768,6 +775,14 @@ static double bytes_ref_gg1(const MeshView& v) {
 static double bytes_ref_gg_nodes(const MeshView& v) {
   return 3.0 * v.nc * 4 + (double)v.nc * 48 + (double)v.nn * 48;
 }
+// ns: t = bytes of the compute type; gathered neighbour data at unique size
+static double bytes_ns_grad(const MeshView& v, double t) {
+  return (double)v.nc * (4 * t /*q*/ + 12 /*nb*/ + 48 /*lsq*/ + 48 /*rf*/ + 8 /*hmin*/ + 12 * t /*grad+phi*/);
+}
+static double bytes_ns_flux(const MeshView& v, double t) {
+  return (double)v.nc * (16 * t /*q, grad, phi*/ + 12 /*nb*/ + 72 /*nx,ny,len*/ + 48 /*rf*/ + 48 /*dc*/ +
+                         8 /*rarea*/ + 8 /*hmin*/ + 32 /*U old*/ + 32 /*U new*/ + 4 * t /*q new*/);
+}
 
 int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats) {
   if (!S || !rp) return fail(GD_ERR_INVALID, "null argument");
@@ -778,7 +793,11 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
   Timer T{S, rp->time_kernels != 0, stats};
   const bool wall = rp->wall_code != 0 && v.n_wall > 0;
   const bool ns = P.scheme == GD_SCHEME_NS;
-  GD_HIP(launch_fields_derived(v.nc, P.Gamma, S->Q[S->qc], S->aS, S->vmax, S->stream));
+  if (ns)  // the ns scheme advances U; its double primitives are recovered from U
+    GD_HIP(launch_update_variables(v, P.Gamma, S->U[S->cur], S->U[S->cur], S->Q[S->qc], S->aS, S->vmax, nullptr,
+                                   S->stream));
+  else
+    GD_HIP(launch_fields_derived(v.nc, P.Gamma, S->Q[S->qc], S->aS, S->vmax, S->stream));
   GD_HIP(launch_prime_acc(v, P, S->Q[S->qc], S->vmax, S->acc, rp->flow_time, rp->dt_policy, ns && P.viscous,
                           S->stream));
   if (wall) {
@@ -821,7 +840,38 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       }));
     }
   } else {
-    return fail(GD_ERR_UNSUPPORTED, "ns scheme not built");
+    const bool mixed = P.precision == GD_FP32_MIXED;
+    const double t = mixed ? 4.0 : 8.0;
+    auto prims = mixed ? launch_ns_prims_f32 : launch_ns_prims_f64;
+    auto grad = mixed ? launch_ns_grad_f32 : launch_ns_grad_f64;
+    auto flux = mixed ? launch_ns_flux_f32 : launch_ns_flux_f64;
+    int tq = 0;
+    GD_HIP(prims(v.nc, P.Gamma, S->U[S->cur], S->Qt[tq], S->stream));
+    for (int32_t s = 0; s < n_steps; ++s) {
+      NsStepArgs a{};
+      a.m = v;
+      a.prm = P;
+      a.Uo = S->U[S->cur];
+      a.Un = S->U[1 - S->cur];
+      a.Q = S->Qt[tq];
+      a.Qw = S->Qt[1 - tq];
+      a.grad = S->grad;
+      a.phi = S->phi;
+      a.acc = S->acc;
+      a.step = s;
+      a.dt_policy = rp->dt_policy;
+      a.dt = rp->dt;
+      a.cfl = rp->cfl;
+      a.wall_f = wall ? S->wall_f : nullptr;
+      a.wall_code = rp->wall_code;
+      GD_HIP(T.run(mixed ? "ns_grad_fp32" : "ns_grad_fp64", bytes_ns_grad(v, t), [&] { return grad(a, S->stream); }));
+      GD_HIP(T.run(mixed ? "ns_flux_fp32" : "ns_flux_fp64", bytes_ns_flux(v, t), [&] { return flux(a, S->stream); }));
+      if (wall) GD_HIP(launch_wall_sum(S->wall_f, v.n_wall, S->drag, S->stream));
+      S->cur ^= 1;
+      tq ^= 1;
+    }
+    GD_HIP(launch_update_variables(v, P.Gamma, S->U[S->cur], S->U[S->cur], S->Q[S->qc], S->aS, S->vmax, nullptr,
+                                   S->stream));
   }
   // reference bookkeeping after the loop: Delta of the last step, U_NEW == U_OLD
   if (n_steps > 0) {

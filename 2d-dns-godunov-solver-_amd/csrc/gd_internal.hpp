@@ -96,6 +96,7 @@ struct State {
   double* g6N = nullptr;                // [6][nn]
   void* grad = nullptr;                 // ns: [8][nc] (double or float)
   void* phi = nullptr;                  // ns: [4][nc]
+  void* Qt[2] = {nullptr, nullptr};     // ns: primitives in the compute type [4][nc]
   double* wall_f = nullptr;             // [n_wall][2]
   double* drag = nullptr;               // [4]: last step force x,y; running sums
   Acc* acc = nullptr;                   // [3]
@@ -145,14 +146,14 @@ hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
 hipError_t launch_delta(int64_t nc, const double* Un, const double* Uo, double* D, hipStream_t s);
 
-// ---- ns scheme (gd_ns_kernels.hip) ----
-struct NsArgs {
+// ---- ns scheme (gd_ns_fp64.hip, gd_ns_fp32.hip) ----
+struct NsStepArgs {
   MeshView m;
   gd_params prm;
   const double* Uo;
   double* Un;
-  double* Qo;           // current primitives (read) as doubles
-  double* Qw;           // new primitives (write)
+  const void* Q;        // current primitives, compute type
+  void* Qw;             // next primitives
   void* grad;           // [8][nc]
   void* phi;            // [4][nc]
   Acc* acc;
@@ -162,7 +163,11 @@ struct NsArgs {
   double* wall_f;
   int32_t wall_code;
 };
-hipError_t launch_ns_grad(const NsArgs& a, hipStream_t s);
-hipError_t launch_ns_flux(const NsArgs& a, hipStream_t s);
+hipError_t launch_ns_prims_f64(int64_t nc, double gamma, const double* U, void* Q, hipStream_t s);
+hipError_t launch_ns_grad_f64(const NsStepArgs& a, hipStream_t s);
+hipError_t launch_ns_flux_f64(const NsStepArgs& a, hipStream_t s);
+hipError_t launch_ns_prims_f32(int64_t nc, double gamma, const double* U, void* Q, hipStream_t s);
+hipError_t launch_ns_grad_f32(const NsStepArgs& a, hipStream_t s);
+hipError_t launch_ns_flux_f32(const NsStepArgs& a, hipStream_t s);
 
 }  // namespace gd

@@ -1,0 +1,438 @@
+// ns-scheme kernels, templated on the compute type T of slopes, limiter,
+// reconstruction and face fluxes (double: FP64 step; float: FP32-mixed step).
+// Geometry is read as FP64, the face-flux sum, the update and the p/T
+// recovery are FP64 in both forms.
+//
+// Two launches per step, both cell-centric gathers without atomics on the
+// state (deterministic, numbering independent):
+//   k_ns_grad  least-squares gradient of (rho,u,v,p) + Venkatakrishnan limiter
+//   k_ns_flux  reconstruction, AUSM+/Roe, viscous stress/heat flux, explicit
+//              Euler update, primitive recovery, dt and wall-force partials
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gd_internal.hpp"
+
+namespace gd {
+namespace ns {
+
+constexpr int kBlock = 256;
+
+template <class T>
+struct NsKArgs {
+  MeshView m;
+  gd_params prm;
+  const double* Uo;
+  double* Un;
+  const T* Q;       // current primitives (rho,u,v,p) [4][nc]
+  T* Qw;            // next primitives
+  T* grad;          // [8][nc]: d/dx, d/dy of rho, u, v, p
+  T* phi;           // [4][nc]
+  Acc* acc;
+  int32_t step;
+  int32_t dt_policy;
+  double dt, cfl;
+  double* wall_f;
+  int32_t wall_code;
+};
+
+template <class T>
+__device__ __forceinline__ T tmin(T a, T b) { return b < a ? b : a; }
+template <class T>
+__device__ __forceinline__ T tmax(T a, T b) { return b > a ? b : a; }
+
+template <class T>
+__device__ __forceinline__ T sutherland(T Tt, const gd_params& p) {
+  const T Tr = (T)p.T_ref, S = (T)p.S_suth;
+  return (T)p.mu_ref * pow(Tt / Tr, (T)1.5) * (Tr + S) / (Tt + S);
+}
+
+// ghost primitive state of a boundary face (first order)
+template <class T>
+__device__ __forceinline__ void ghost(const gd_params& p, int code, const T* q, T nx, T ny, T* g) {
+  const int idx = -code;
+  const int kind = (idx >= 0 && idx <= GD_MAX_BC) ? p.bc[idx].kind : GD_BC_EXTRAPOLATE;
+  if (kind == GD_BC_FIXED) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) g[v] = (T)p.bc[idx].data[v];
+  } else if (kind == GD_BC_SLIP_WALL || kind == GD_BC_ISOTHERMAL_WALL) {
+    const T un = q[1] * nx + q[2] * ny;
+    g[0] = q[0]; g[1] = q[1] - (T)2 * un * nx; g[2] = q[2] - (T)2 * un * ny; g[3] = q[3];
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) g[v] = q[v];
+  }
+}
+
+// AUSM+ per unit length (Liou 1996): neighbour side "L", inward normal; -(F.n)
+template <class T>
+__device__ __forceinline__ void ausm(const T* L, const T* R, T nx, T ny, T g, T* out) {
+  const T rL = L[0], uL = L[1], vL = L[2], pL = L[3];
+  const T rR = R[0], uR = R[1], vR = R[2], pR = R[3];
+  const T htL = (T)0.5 * (uL * uL + vL * vL) + g / (g - 1) * pL / rL;
+  const T htR = (T)0.5 * (uR * uR + vR * vR) + g / (g - 1) * pR / rR;
+  const T c = 2 * (g - 1) / (g + 1);
+  const T unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
+  const T asL = sqrt(c * htL), asR = sqrt(c * htR);
+  const T aL = asL * fmin((T)1, asL / fabs(unL));
+  const T aR = asR * fmin((T)1, asR / fabs(unR));
+  const T a12 = fmin(aL, aR);
+  const T ML = unL / a12, MR = unR / a12;
+  T mp, mm, pp, pm;
+  {
+    const T d = ML * ML - 1;
+    mp = fabs(ML) >= 1 ? (T)0.5 * (ML + fabs(ML)) : (T)0.25 * (ML + 1) * (ML + 1) + (T)0.125 * d * d;
+    pp = fabs(ML) >= 1 ? (ML > 0 ? (T)1 : (T)0) : (T)0.25 * (ML + 1) * (ML + 1) * (2 - ML) + (T)0.1875 * ML * d * d;
+  }
+  {
+    const T d = MR * MR - 1;
+    mm = fabs(MR) >= 1 ? (T)0.5 * (MR - fabs(MR)) : -(T)0.25 * (MR - 1) * (MR - 1) - (T)0.125 * d * d;
+    pm = fabs(MR) >= 1 ? (MR < 0 ? (T)1 : (T)0) : (T)0.25 * (MR - 1) * (MR - 1) * (2 + MR) - (T)0.1875 * MR * d * d;
+  }
+  const T m = mp + mm;
+  const T p12 = pp * pL + pm * pR;
+  const T mpl = (T)0.5 * (m + fabs(m)), mmi = (T)0.5 * (m - fabs(m));
+  out[0] = -(a12 * (mpl * rL + mmi * rR));
+  out[1] = -(a12 * (mpl * rL * uL + mmi * rR * uR) + p12 * nx);
+  out[2] = -(a12 * (mpl * rL * vL + mmi * rR * vR) + p12 * ny);
+  out[3] = -(a12 * (mpl * rL * htL + mmi * rR * htR));
+}
+
+// Roe per unit length, Harten fix of width 0.2 a on the acoustic waves
+template <class T>
+__device__ __forceinline__ void roe(const T* L, const T* R, T nx, T ny, T g, T* out) {
+  const T rL = L[0], uL = L[1], vL = L[2], pL = L[3];
+  const T rR = R[0], uR = R[1], vR = R[2], pR = R[3];
+  const T mx = -ny, my = nx;
+  const T unL = uL * nx + vL * ny, umL = uL * mx + vL * my;
+  const T unR = uR * nx + vR * ny, umR = uR * mx + vR * my;
+  const T HL = g / (g - 1) * pL / rL + (T)0.5 * (uL * uL + vL * vL);
+  const T HR = g / (g - 1) * pR / rR + (T)0.5 * (uR * uR + vR * vR);
+  const T RT = sqrt(rR / rL);
+  const T rho = RT * rL;
+  const T u = (uL + RT * uR) / (1 + RT), v = (vL + RT * vR) / (1 + RT), H = (HL + RT * HR) / (1 + RT);
+  const T a = sqrt((g - 1) * (H - (T)0.5 * (u * u + v * v)));
+  const T un = u * nx + v * ny, um = u * mx + v * my;
+  const T dp = pR - pL, dun = unR - unL, dum = umR - umL, drho = rR - rL;
+  const T w1 = (dp - rho * a * dun) / (2 * a * a), w2 = rho * dum, w3 = drho - dp / (a * a),
+          w4 = (dp + rho * a * dun) / (2 * a * a);
+  T s1 = fabs(un - a), s4 = fabs(un + a);
+  const T s2 = fabs(un);
+  const T dw = (T)0.2 * a;
+  if (s1 < dw) s1 = (T)0.5 * (s1 * s1 / dw + dw);
+  if (s4 < dw) s4 = (T)0.5 * (s4 * s4 / dw + dw);
+  const T d1 = s1 * w1 + s2 * w3 + s4 * w4;
+  const T d2 = s1 * w1 * (u - a * nx) + s2 * w2 * mx + s2 * w3 * u + s4 * w4 * (u + a * nx);
+  const T d3 = s1 * w1 * (v - a * ny) + s2 * w2 * my + s2 * w3 * v + s4 * w4 * (v + a * ny);
+  const T d4 = s1 * w1 * (H - un * a) + s2 * w2 * um + s2 * w3 * (T)0.5 * (u * u + v * v) + s4 * w4 * (H + un * a);
+  out[0] = -(T)0.5 * (rL * unL + rR * unR - d1);
+  out[1] = -(T)0.5 * (rL * unL * uL + pL * nx + rR * unR * uR + pR * nx - d2);
+  out[2] = -(T)0.5 * (rL * unL * vL + pL * ny + rR * unR * vR + pR * ny - d3);
+  out[3] = -(T)0.5 * (rL * unL * HL + rR * unR * HR - d4);
+}
+
+// primitives (rho, u, v, p) from conservative, in FP64
+__device__ __forceinline__ void prims64(const double* U, int64_t nc, int64_t i, double g, double* q) {
+  const double r = U[i], mx = U[nc + i], my = U[2 * nc + i], E = U[3 * nc + i];
+  q[0] = r;
+  q[1] = mx / r;
+  q[2] = my / r;
+  q[3] = (g - 1.0) * (E - 0.5 * (mx * mx + my * my) / r);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_ns_prims(int64_t nc, double g, const double* U, T* Q) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nc) return;
+  double q[4];
+  prims64(U, nc, i, g, q);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Q[v * nc + i] = (T)q[v];
+}
+
+// ---- pass G: gradients + limiter
+template <class T, bool QUADS>
+__global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
+  const MeshView& m = a.m;
+  const int64_t nc = m.nc;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nc) return;
+  const int nf = QUADS ? m.nv[i] : 3;
+  T qi[4], gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0}, qmx[4], qmn[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) { qi[v] = a.Q[v * nc + i]; qmx[v] = qi[v]; qmn[v] = qi[v]; }
+  T rx[4], ry[4];
+#pragma unroll
+  for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
+    if (k < nf) {
+      const int64_t s = k * nc + i;
+      const int32_t j = m.nb[s];
+      T qk[4];
+      if (j >= 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) qk[v] = a.Q[v * nc + j];
+      } else {
+        ghost<T>(a.prm, j + 1, qi, (T)m.fnx[s], (T)m.fny[s], qk);
+      }
+      const T wx = (T)m.lsq[(2 * k) * nc + i], wy = (T)m.lsq[(2 * k + 1) * nc + i];
+      rx[k] = (T)m.rf[(2 * k) * nc + i];
+      ry[k] = (T)m.rf[(2 * k + 1) * nc + i];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const T d = qk[v] - qi[v];
+        gx[v] += wx * d;
+        gy[v] += wy * d;
+        qmx[v] = fmax(qmx[v], qk[v]);
+        qmn[v] = fmin(qmn[v], qk[v]);
+      }
+    }
+  }
+  const T kh = (T)(a.prm.venkat_K * m.hmin[i]);
+  const T kh3 = kh * kh * kh;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const T e2 = kh3 * (T)(a.prm.venkat_scale[v] * a.prm.venkat_scale[v]);
+    T phi = 1;
+#pragma unroll
+    for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
+      if (k < nf) {
+        const T d2 = gx[v] * rx[k] + gy[v] * ry[k];
+        if (d2 != 0) {
+          const T d1 = d2 > 0 ? qmx[v] - qi[v] : qmn[v] - qi[v];
+          const T psi = ((d1 * d1 + e2) * d2 + 2 * d2 * d2 * d1) / (d1 * d1 + 2 * d2 * d2 + d1 * d2 + e2) / d2;
+          phi = fmin(phi, psi);
+        }
+      }
+    }
+    a.phi[v * nc + i] = fmax(phi, (T)0);
+    a.grad[(2 * v) * nc + i] = gx[v];
+    a.grad[(2 * v + 1) * nc + i] = gy[v];
+  }
+}
+
+__device__ __forceinline__ double wave_max64(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_min64(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ---- pass F: fluxes, update, primitives, dt inputs
+template <class T, bool QUADS, bool VISC>
+__global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
+  const MeshView& m = a.m;
+  const gd_params& P = a.prm;
+  const int64_t nc = m.nc;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const Acc r = a.acc[a.step % 3];
+  double dt = a.dt;
+  if (a.dt_policy == GD_DT_LOCAL) dt = a.cfl * __longlong_as_double((long long)r.hdt_bits);
+  else if (a.dt_policy == GD_DT_REF_TAU) {
+    const double vm = __longlong_as_double((long long)r.vmax_bits);
+    dt = a.cfl * m.max_hx / (vm > 1.0e-6 ? vm : 1.0e-6);
+  }
+  double vmx = 0.0, hdt = __longlong_as_double(0x7ff0000000000000ll);
+  unsigned bad = 0;
+  if (i < nc) {
+    const T g = (T)P.Gamma, Rg = (T)P.RGAS;
+    const T cp = g * Rg / (g - 1);
+    const int nf = QUADS ? m.nv[i] : 3;
+    T qi[4], phi_i[4], gi[8];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { qi[v] = a.Q[v * nc + i]; phi_i[v] = a.phi[v * nc + i]; }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) gi[c] = a.grad[c * nc + i];
+    double F[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
+      if (k < nf) {
+        const int64_t s = k * nc + i;
+        const int32_t j = m.nb[s];
+        const T nx = (T)m.fnx[s], ny = (T)m.fny[s];
+        const double len = m.flen[s];
+        const double rxd = m.rf[(2 * k) * nc + i], ryd = m.rf[(2 * k + 1) * nc + i];
+        const T rx = (T)rxd, ry = (T)ryd;
+        T qa[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) qa[v] = qi[v] + phi_i[v] * (gi[2 * v] * rx + gi[2 * v + 1] * ry);
+        if (!(qa[0] > 0) || !(qa[3] > 0)) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) qa[v] = qi[v];
+        }
+        T qb[4], qj[4], gj[8];
+        double dxd = 0.0, dyd = 0.0;
+        if (j >= 0) {
+          dxd = m.dc[(2 * k) * nc + i];
+          dyd = m.dc[(2 * k + 1) * nc + i];
+          const T rxj = (T)(rxd - dxd), ryj = (T)(ryd - dyd);
+#pragma unroll
+          for (int c = 0; c < 8; ++c) gj[c] = a.grad[c * nc + j];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            qj[v] = a.Q[v * nc + j];
+            qb[v] = qj[v] + a.phi[v * nc + j] * (gj[2 * v] * rxj + gj[2 * v + 1] * ryj);
+          }
+          if (!(qb[0] > 0) || !(qb[3] > 0)) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) qb[v] = qj[v];
+          }
+        } else {
+          ghost<T>(P, j + 1, qa, nx, ny, qb);
+        }
+        T e[4];
+        if (P.flux == GD_FLUX_ROE) roe<T>(qb, qa, nx, ny, g, e);
+        else ausm<T>(qb, qa, nx, ny, g, e);
+        T gv[4] = {0, 0, 0, 0};
+        if (VISC) {
+          if (j >= 0) {
+            const T Ti = qi[3] / (qi[0] * Rg), Tj = qj[3] / (qj[0] * Rg);
+            T g_i[3][2], g_j[3][2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+              g_i[0][c] = gi[2 + c]; g_i[1][c] = gi[4 + c];
+              g_j[0][c] = gj[2 + c]; g_j[1][c] = gj[4 + c];
+              g_i[2][c] = (gi[6 + c] - Ti * Rg * gi[c]) / (qi[0] * Rg);
+              g_j[2][c] = (gj[6 + c] - Tj * Rg * gj[c]) / (qj[0] * Rg);
+            }
+            const T ddx = (T)dxd, ddy = (T)dyd;
+            const T dl = sqrt(ddx * ddx + ddy * ddy);
+            const T tx = ddx / dl, ty = ddy / dl;
+            const T dphi[3] = {qj[1] - qi[1], qj[2] - qi[2], Tj - Ti};
+            T gf[3][2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const T ax = (T)0.5 * (g_i[c][0] + g_j[c][0]), ay = (T)0.5 * (g_i[c][1] + g_j[c][1]);
+              const T corr = dphi[c] / dl - (ax * tx + ay * ty);
+              gf[c][0] = ax + corr * tx; gf[c][1] = ay + corr * ty;
+            }
+            const T Tf = (T)0.5 * (Ti + Tj);
+            const T uf = (T)0.5 * (qi[1] + qj[1]), vf = (T)0.5 * (qi[2] + qj[2]);
+            const T mu = sutherland<T>(Tf, P);
+            const T kap = mu * cp / (T)P.Pr;
+            const T div = gf[0][0] + gf[1][1];
+            const T txx = mu * (2 * gf[0][0] - (T)(2.0 / 3.0) * div);
+            const T tyy = mu * (2 * gf[1][1] - (T)(2.0 / 3.0) * div);
+            const T txy = mu * (gf[0][1] + gf[1][0]);
+            gv[1] = txx * nx + txy * ny;
+            gv[2] = txy * nx + tyy * ny;
+            gv[3] = uf * gv[1] + vf * gv[2] + kap * (gf[2][0] * nx + gf[2][1] * ny);
+          } else {
+            const int idx = -(j + 1);
+            if (idx <= GD_MAX_BC && P.bc[idx].kind == GD_BC_ISOTHERMAL_WALL) {
+              const T Tw = (T)P.bc[idx].data[0];
+              const T Ti = qi[3] / (qi[0] * Rg);
+              const T dperp = -(rx * nx + ry * ny);
+              const T dudn = (0 - qi[1]) / dperp, dvdn = (0 - qi[2]) / dperp, dTdn = (Tw - Ti) / dperp;
+              const T ox = -nx, oy = -ny;
+              const T gu0 = dudn * ox, gu1 = dudn * oy, gv0 = dvdn * ox, gv1 = dvdn * oy;
+              const T gT0 = dTdn * ox, gT1 = dTdn * oy;
+              const T mu = sutherland<T>(Tw, P);
+              const T kap = mu * cp / (T)P.Pr;
+              const T div = gu0 + gv1;
+              const T txx = mu * (2 * gu0 - (T)(2.0 / 3.0) * div);
+              const T tyy = mu * (2 * gv1 - (T)(2.0 / 3.0) * div);
+              const T txy = mu * (gu1 + gv0);
+              gv[1] = txx * nx + txy * ny;
+              gv[2] = txy * nx + tyy * ny;
+              gv[3] = kap * (gT0 * nx + gT1 * ny);
+            }
+          }
+        }
+        double fk[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) fk[v] = ((double)e[v] + (double)gv[v]) * len;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) F[v] += fk[v];
+        if (j < 0 && a.wall_f && a.wall_code == j + 1) {
+          const int32_t wid = m.wall_id[s];
+          if (wid >= 0) { a.wall_f[2 * wid] = fk[1]; a.wall_f[2 * wid + 1] = fk[2]; }
+        }
+      }
+    }
+    const double ra = m.rarea[i];
+    double Un[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      Un[v] = a.Uo[v * nc + i] - F[v] * dt * ra;
+      a.Un[v * nc + i] = Un[v];
+    }
+    const double G = P.Gamma;
+    const double rho = Un[0], ux = Un[1] / rho, uy = Un[2] / rho;
+    const double p = (G - 1.0) * (Un[3] - 0.5 * (Un[1] * Un[1] + Un[2] * Un[2]) / rho);
+    a.Qw[i] = (T)rho; a.Qw[nc + i] = (T)ux; a.Qw[2 * nc + i] = (T)uy; a.Qw[3 * nc + i] = (T)p;
+    const double as = sqrt(G * p / rho);
+    const double vm = sqrt(ux * ux + uy * uy) + as;
+    double lam = vm;
+    if (VISC) {
+      const double Tc = p / (rho * P.RGAS);
+      const double mu = P.mu_ref * pow(Tc / P.T_ref, 1.5) * (P.T_ref + P.S_suth) / (Tc + P.S_suth);
+      lam = vm + 2.0 * G * mu / (P.Pr * rho * m.hmin[i]);
+    }
+    vmx = vm;
+    hdt = m.hmin[i] / lam;
+    bad = (!(rho > 0.0) || !(p > 0.0) || vm != vm) ? 1u : 0u;
+    if (vm != vm || lam != lam) { vmx = 0.0; hdt = __longlong_as_double(0x7ff0000000000000ll); }
+  }
+  // block reduction of the next step's dt inputs
+  __shared__ double s_max[kBlock / 64], s_min[kBlock / 64];
+  __shared__ unsigned s_bad[kBlock / 64];
+  vmx = wave_max64(vmx);
+  hdt = wave_min64(hdt);
+  bad = wave_sum_u32(bad);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { s_max[wid] = vmx; s_min[wid] = hdt; s_bad[wid] = bad; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < kBlock / 64; ++q) { vmx = fmax(vmx, s_max[q]); hdt = fmin(hdt, s_min[q]); bad += s_bad[q]; }
+    Acc* w = &a.acc[(a.step + 1) % 3];
+    atomicMax(&w->vmax_bits, (unsigned long long)__double_as_longlong(vmx));
+    atomicMin(&w->hdt_bits, (unsigned long long)__double_as_longlong(hdt));
+    if (bad) atomicAdd(&w->nonphysical, bad);
+    if (blockIdx.x == 0) {
+      w->time = r.time + dt;
+      w->dt = dt;
+      Acc* z = &a.acc[(a.step + 2) % 3];
+      z->vmax_bits = 0ull;
+      z->hdt_bits = 0x7ff0000000000000ull;
+      z->nonphysical = 0u;
+    }
+  }
+}
+
+inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <class T>
+hipError_t launch_grad(const NsKArgs<T>& a, hipStream_t s) {
+  if (a.m.has_quads) hipLaunchKernelGGL((k_ns_grad<T, true>), dim3(nblocks(a.m.nc)), dim3(kBlock), 0, s, a);
+  else hipLaunchKernelGGL((k_ns_grad<T, false>), dim3(nblocks(a.m.nc)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_flux(const NsKArgs<T>& a, hipStream_t s) {
+  const dim3 g(nblocks(a.m.nc)), b(kBlock);
+  const bool visc = a.prm.viscous != 0;
+  if (a.m.has_quads) {
+    if (visc) hipLaunchKernelGGL((k_ns_flux<T, true, true>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_ns_flux<T, true, false>), g, b, 0, s, a);
+  } else {
+    if (visc) hipLaunchKernelGGL((k_ns_flux<T, false, true>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_ns_flux<T, false, false>), g, b, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_prims(int64_t nc, double g, const double* U, T* Q, hipStream_t s) {
+  hipLaunchKernelGGL((k_ns_prims<T>), dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, g, U, Q);
+  return hipGetLastError();
+}
+
+}  // namespace ns
+}  // namespace gd

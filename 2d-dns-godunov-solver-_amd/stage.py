@@ -271,7 +271,7 @@ def phs2dcns2dcellsSA(ACons, testFields, gamma):  # noqa: N802
 # ---------------------------------------------------------------------------
 
 def make_params(scheme=L.GD_SCHEME_REF, precision=L.GD_FP64, flux=L.GD_FLUX_AUSM_PLUS, limiter=None,
-                limiter_eps=1e-12, venkat_K=5.0, viscous=True, energy_law=L.GD_ENERGY_AIR_POLY, mu_const=1e-2,
+                limiter_eps=1e-12, venkat_K=5.0, venkat_scale=(1.0, 1.0, 1.0, 1.0), viscous=True, energy_law=L.GD_ENERGY_AIR_POLY, mu_const=1e-2,
                 k_const=0.0, RGAS=287.058, Gamma=1.4, mu_ref=1.8476e-5, T_ref=300.0, S_suth=116.0, Pr=0.72,
                 bcs=None) -> L.gd_params:
     p = L.default_params()
@@ -282,6 +282,8 @@ def make_params(scheme=L.GD_SCHEME_REF, precision=L.GD_FP64, flux=L.GD_FLUX_AUSM
         if limiter is None else limiter
     p.limiter_eps = limiter_eps
     p.venkat_K = venkat_K
+    for q in range(4):
+        p.venkat_scale[q] = float(venkat_scale[q])
     p.viscous = int(bool(viscous))
     p.energy_law = energy_law
     p.mu_const = mu_const
@@ -355,3 +357,51 @@ class GodunovSolver:
     def close(self):
         self.state.close()
         self.dmesh.close()
+
+
+# ---------------------------------------------------------------------------
+# supersonic cylinder (north-star configurations): free stream and gas
+# ---------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class CylinderCase:
+    """M = 3.5 air flow over a cylinder of diameter 1 (mesh units) at Re_D.
+
+    Dimensional air at T = 300 K, p = 101325 Pa; the Sutherland law has the
+    reference's constants (sutherland.jl: As = 1.4792e-6, Ts = 116 K) with its
+    scale chosen so that mu(T_inf) gives Re_D on a diameter-1 cylinder."""
+    mach: float = 3.5
+    re_d: float = 1.0e5
+    T_inf: float = 300.0
+    p_inf: float = 101325.0
+    T_wall: float = 300.0
+    gamma: float = 1.4
+    R: float = 287.058
+    Pr: float = 0.72
+    S: float = 116.0
+
+    @property
+    def rho_inf(self):
+        return self.p_inf / (self.R * self.T_inf)
+
+    @property
+    def u_inf(self):
+        return self.mach * (self.gamma * self.R * self.T_inf) ** 0.5
+
+    @property
+    def mu_inf(self):
+        return self.rho_inf * self.u_inf * 1.0 / self.re_d
+
+    def bcs(self):
+        return {-1: (L.GD_BC_FIXED, (self.rho_inf, self.u_inf, 0.0, self.p_inf)),
+                -2: (L.GD_BC_EXTRAPOLATE, ()),
+                -3: (L.GD_BC_ISOTHERMAL_WALL, (self.T_wall,))}
+
+    def params(self, precision=L.GD_FP64, flux=L.GD_FLUX_AUSM_PLUS, viscous=True, venkat_K=5.0):
+        return make_params(scheme=L.GD_SCHEME_NS, precision=precision, flux=flux, viscous=viscous,
+                           venkat_K=venkat_K, venkat_scale=(self.rho_inf, self.u_inf, self.u_inf, self.p_inf),
+                           RGAS=self.R, Gamma=self.gamma, mu_ref=self.mu_inf, T_ref=self.T_inf, S_suth=self.S,
+                           Pr=self.Pr, bcs=self.bcs())
+
+    def free_stream(self, n):
+        return (np.full(n, self.rho_inf), np.full(n, self.u_inf), np.zeros(n), np.full(n, self.p_inf))

@@ -1,0 +1,223 @@
+"""Throughput of the explicit Godunov step on one MI355X.
+
+Workload (default): the supersonic viscous cylinder, M = 3.5, Re_D = 1e5,
+O-grid of 720 x 1445 nodes split into 2,079,360 triangles, cells in a seeded
+random order, ns scheme (least-squares gradients + Venkatakrishnan, AUSM+,
+Sutherland viscous fluxes, isothermal wall) at CFL 0.4, starting from the
+uniform free stream.  A "step" is one explicit Euler step over all cells,
+every array resident in HBM.  The FP64 step is the headline value; the
+FP32-mixed step (same mesh, same start) is reported beside it with its density,
+pressure and wall-drag deviation from the FP64 run after the same steps.
+
+Prints one JSON line.  Only one GPU is supported (--gpus N>1 exits).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+PKG = importlib.import_module("2d-dns-godunov-solver-_amd")
+L = PKG._lib
+M = PKG.mesh
+S = PKG.stage
+
+METRIC = "cell-updates/s @2.08M tris: FP32-mixed vs FP64 at stated drag err; adjoint/fwd cost"
+HBM_PEAK_GBS = 8000.0
+SEED = 20261203
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_mesh(n_theta, n_r, dr0, order):
+    t0 = time.perf_counter()
+    mesh = M.make_cylinder_ogrid_mesh(n_theta, n_r, 0.5, 10.0, dr0)
+    if order == "random":
+        mesh = M.permute_cells(mesh, M.random_permutation(mesh.nCells, SEED))
+    log(f"mesh {n_theta}x{n_r}: {mesh.nCells} cells, {mesh.nNodes} nodes ({time.perf_counter() - t0:.1f} s)")
+    return mesh
+
+
+def initial_U(case, n):
+    r, u, v, p = case.free_stream(n)
+    U = np.zeros((n, 4), order="F")
+    U[:, 0] = r; U[:, 1] = r * u; U[:, 2] = r * v
+    U[:, 3] = p / (case.gamma - 1.0) + 0.5 * r * (u * u + v * v)
+    return U
+
+
+def run_variant(mesh, case, precision, flux, order_dev, U0, warmup, steps, cfl):
+    prm = case.params(precision=precision, flux=flux)
+    t0 = time.perf_counter()
+    sol = S.GodunovSolver(mesh, prm, order=order_dev)
+    log(f"device mesh + state ({time.perf_counter() - t0:.1f} s), bandwidth {sol.dmesh.info().bandwidth}")
+    sol.state.set_array(L.GD_ARR_U_OLD, U0)
+    if warmup:
+        sol.run(warmup, cfl=cfl, dt_policy=L.GD_DT_LOCAL, wall_code=-3)
+    sol.state.sync()
+    t0 = time.perf_counter()
+    st = sol.run(steps, cfl=cfl, dt_policy=L.GD_DT_LOCAL, wall_code=-3, time_kernels=True)
+    sol.state.sync()
+    el = time.perf_counter() - t0
+    kt = L.kernel_table(st)
+    U = sol.conservative()
+    f = sol.fields()
+    sol.close()
+    return {"ms_per_step": 1e3 * el / steps, "seconds": el, "kernels": kt, "U": U, "rho": f[0], "p": f[3],
+            "drag": float(st.wall_force[0]), "lift": float(st.wall_force[1]), "flow_time": float(st.flow_time),
+            "nonphysical": int(st.nonphysical)}
+
+
+def roofline(kt, steps):
+    dom = max(kt, key=lambda k: k["ms_total"])
+    achieved = dom["bytes_per_launch"] / (dom["ms_avg"] * 1e-3) / 1e9
+    return dom, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+
+
+def measured_traffic(kernel_name):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        d = json.load(open(path))
+        return d.get(kernel_name, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(mesh, case, U0, sample_steps, threads):
+    """The oracle's FP64 ns step (C++/OpenMP restatement) on the same mesh."""
+    import oracle as O
+    O.set_threads(threads)
+    mv = O.MeshView(mesh)
+    bcs = {-1: (O.BC_FIXED, (case.rho_inf, case.u_inf, 0.0, case.p_inf)), -2: (O.BC_EXTRAPOLATE, ()),
+           -3: (O.BC_ISOTHERMAL_WALL, (case.T_wall,))}
+    prm = O.ns_params(gamma=case.gamma, R=case.R, Pr=case.Pr, mu_ref=case.mu_inf, T_ref=case.T_inf, S=case.S,
+                      venkat_K=5.0, venkat_scale=(case.rho_inf, case.u_inf, case.u_inf, case.p_inf), bcs=bcs)
+    U = U0.copy(order="F")
+    times = []
+    for _ in range(sample_steps):
+        t0 = time.perf_counter()
+        dt = O.ns_dt(mv, prm, U, 0.4)
+        U, _, _, _ = O.ns_step(mv, prm, dt, U)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": round(mesh.nCells / med, 1), "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{sample_steps} FP64 ns steps of the same 2.08M-cell workload from the free stream "
+                      f"(oracle C++/OpenMP restatement, median step {med:.3f} s)"}
+
+
+def copy_probe(nbytes):
+    """A plain device copy of known size, for scaling rocprofv3 FETCH/WRITE counters."""
+    import torch
+    n = nbytes // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n-theta", type=int, default=720)
+    ap.add_argument("--n-r", type=int, default=1445)
+    ap.add_argument("--dr0", type=float, default=2e-4)
+    ap.add_argument("--order", choices=["random", "given"], default="random")
+    ap.add_argument("--device-order", choices=["given", "rcm"], default="given")
+    ap.add_argument("--flux", choices=["ausm", "roe"], default="ausm")
+    ap.add_argument("--cfl", type=float, default=0.4)
+    ap.add_argument("--variants", default="fp64,mixed,mixed-rcm")
+    ap.add_argument("--cpu-sample-steps", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, help="bytes; run only a device copy of this size")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU path is not built",
+              file=sys.stderr)
+        sys.exit(2)
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
+    if L.device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+    case = S.CylinderCase()
+    mesh = build_mesh(a.n_theta, a.n_r, a.dr0, a.order)
+    U0 = initial_U(case, mesh.nCells)
+    flux = L.GD_FLUX_ROE if a.flux == "roe" else L.GD_FLUX_AUSM_PLUS
+    dev_order = L.GD_ORDER_RCM if a.device_order == "rcm" else L.GD_ORDER_AS_GIVEN
+    res = {}
+    for var in a.variants.split(","):
+        prec = L.GD_FP64 if var.startswith("fp64") else L.GD_FP32_MIXED
+        order = L.GD_ORDER_RCM if var.endswith("rcm") else dev_order
+        res[var] = run_variant(mesh, case, prec, flux, order, U0, a.warmup, a.steps, a.cfl)
+        r = res[var]
+        log(f"{var}: {r['ms_per_step']:.4f} ms/step, {mesh.nCells / (r['ms_per_step'] * 1e-3):.4e} cell-updates/s, "
+            f"drag {r['drag']:.6e}, nonphysical {r['nonphysical']}")
+        for k in r["kernels"]:
+            log(f"   {k['name']:16s} {k['ms_avg'] * 1e3:9.2f} us/launch  "
+                f"{k['bytes_per_launch'] / (k['ms_avg'] * 1e-3) / 1e9 if k['ms_avg'] > 0 else 0:8.1f} GB/s")
+    head = "fp64" if "fp64" in res else list(res)[0]
+    r64 = res[head]
+    value = mesh.nCells / (r64["ms_per_step"] * 1e-3)
+    dom, roof = roofline(r64["kernels"], a.steps)
+    roof["traffic"] = measured_traffic(dom["name"])
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(r64["ms_per_step"], 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if head == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"supersonic viscous cylinder M=3.5 Re_D=1e5, O-grid {a.n_theta}x{a.n_r} nodes, "
+                               f"{mesh.nCells} triangles, {a.order} cell order, ns scheme "
+                               f"({'Roe' if flux == L.GD_FLUX_ROE else 'AUSM+'}, LSQ+Venkatakrishnan, Sutherland), "
+                               f"CFL {a.cfl}, {head}",
+                   "cells": mesh.nCells, "nodes": mesh.nNodes, "precision": head,
+                   "device_order": a.device_order},
+        "roofline": {**roof, "kernel": dom["name"], "bytes_per_launch": dom["bytes_per_launch"],
+                     "avg_launch_us": round(dom["ms_avg"] * 1e3, 2)},
+        "kernels_us": {k["name"]: round(k["ms_avg"] * 1e3, 2) for k in r64["kernels"]},
+    }
+    for var, r in res.items():
+        if var == head:
+            continue
+        v = mesh.nCells / (r["ms_per_step"] * 1e-3)
+        d = {"value": round(v, 1), "ms_per_step": round(r["ms_per_step"], 5),
+             "speedup_vs_fp64": round(r64["ms_per_step"] / r["ms_per_step"], 3),
+             "kernels_us": {k["name"]: round(k["ms_avg"] * 1e3, 2) for k in r["kernels"]}}
+        if "fp64" in res:
+            d["rho_rel_linf_vs_fp64"] = float(np.max(np.abs(r["rho"] - r64["rho"])) / np.max(np.abs(r64["rho"])))
+            d["p_rel_linf_vs_fp64"] = float(np.max(np.abs(r["p"] - r64["p"])) / np.max(np.abs(r64["p"])))
+            d["rho_rel_l2_vs_fp64"] = float(np.linalg.norm(r["rho"] - r64["rho"]) / np.linalg.norm(r64["rho"]))
+            d["p_rel_l2_vs_fp64"] = float(np.linalg.norm(r["p"] - r64["p"]) / np.linalg.norm(r64["p"]))
+            d["drag_rel_err_vs_fp64"] = float(abs(r["drag"] - r64["drag"]) / max(abs(r64["drag"]), 1e-300))
+            d["steps_compared"] = a.warmup + a.steps
+        _, rf = roofline(r["kernels"], a.steps)
+        d["roofline_frac"] = rf["frac"]
+        out[var.replace("-", "_")] = d
+    out["drag_fp64"] = r64["drag"]
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
+        try:
+            out["cpu_baseline"] = cpu_baseline(mesh, case, U0, a.cpu_sample_steps, threads)
+        except Exception as e:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

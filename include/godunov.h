@@ -116,7 +116,8 @@ typedef struct { int32_t kind; int32_t reserved; double data[4]; } gd_bc;
 typedef struct {
   int32_t scheme, precision, flux, limiter;
   double limiter_eps;      /* 1e-12 (computeslope2d.jl:116) */
-  double venkat_K;         /* ns: eps^2 = (K h)^3 */
+  double venkat_K;         /* ns: eps^2 = (K h)^3 s_v^2, h = 2A/perimeter */
+  double venkat_scale[4];  /* ns: s_v for rho, u, v, p */
   int32_t viscous;         /* ref: calcDiffTerm Laplacian on/off; ns: Sutherland stresses on/off */
   int32_t energy_law;      /* ref: GD_ENERGY_* */
   double mu_const;         /* ref: 1e-2 (calcDiffTerm.jl:110) */

@@ -59,7 +59,8 @@ class OrcParams(C.Structure):
 
 class OrcNsParams(C.Structure):
     _fields_ = [("gamma", C.c_double), ("R", C.c_double), ("Pr", C.c_double), ("mu_ref", C.c_double),
-                ("T_ref", C.c_double), ("S", C.c_double), ("venkat_K", C.c_double), ("flux", C.c_int32),
+                ("T_ref", C.c_double), ("S", C.c_double), ("venkat_K", C.c_double),
+                ("venkat_scale", C.c_double * 4), ("flux", C.c_int32),
                 ("viscous", C.c_int32), ("mixed", C.c_int32), ("pad", C.c_int32), ("bc", OrcBC * (MAX_BC + 1))]
 
 
@@ -103,7 +104,11 @@ def lib():
         L.orc_update_variables.argtypes = [ip, C.c_int32, C.c_int64, C.c_double, dp, dp, dp, C.POINTER(OrcFields)]
         L.orc_ref_tau.restype = C.c_double
         L.orc_ref_tau.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcFields), C.c_double]
+        L.orc_local_dt.restype = C.c_double
+        L.orc_local_dt.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcFields), C.c_double]
         L.orc_ns_step.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcNsParams), C.c_double, dp, dp, dp, dp, dp]
+        L.orc_set_threads.argtypes = [C.c_int32]
+        L.orc_set_threads.restype = None
         L.orc_ns_dt.restype = C.c_double
         L.orc_ns_dt.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcNsParams), dp, C.c_double]
         _lib = L
@@ -287,3 +292,40 @@ def update_variables(fields, gamma, UNew, UOld, n_threads=2):
 def ref_tau(mv: MeshView, fields, cfl):
     f = fields_struct(fields)
     return lib().orc_ref_tau(C.byref(mv.c), C.byref(f), cfl)
+
+
+def local_dt(mv: MeshView, fields, cfl):
+    f = fields_struct(fields)
+    return lib().orc_local_dt(C.byref(mv.c), C.byref(f), cfl)
+
+
+def ns_params(gamma=1.4, R=287.058, Pr=0.72, mu_ref=1.8476e-5, T_ref=300.0, S=116.0, venkat_K=5.0,
+              venkat_scale=(1.0, 1.0, 1.0, 1.0), flux=FLUX_AUSM_PLUS, viscous=True, mixed=False, bcs=None):
+    p = OrcNsParams()
+    p.gamma, p.R, p.Pr, p.mu_ref, p.T_ref, p.S, p.venkat_K = gamma, R, Pr, mu_ref, T_ref, S, venkat_K
+    for q in range(4):
+        p.venkat_scale[q] = venkat_scale[q]
+    p.flux, p.viscous, p.mixed = flux, int(viscous), int(mixed)
+    t = bc_table(bcs)
+    for q in range(MAX_BC + 1):
+        p.bc[q] = t[q]
+    return p
+
+
+def ns_step(mv: MeshView, prm, dt, U, want_grad=False):
+    """One ns-scheme step; returns (U_new, grad8 or None, phi4 or None, wall force (fx, fy))."""
+    n = mv.mesh.nCells
+    UN = np.zeros((n, 4), order="F")
+    g8 = np.zeros((n, 8), order="F") if want_grad else None
+    p4 = np.zeros((n, 4), order="F") if want_grad else None
+    f2 = np.zeros(2)
+    _rc(lib().orc_ns_step(C.byref(mv.c), C.byref(prm), dt, _p(_chk_f64(U)), _p(UN), _p(g8), _p(p4), _p(f2)), "ns")
+    return UN, g8, p4, f2
+
+
+def ns_dt(mv: MeshView, prm, U, cfl):
+    return lib().orc_ns_dt(C.byref(mv.c), C.byref(prm), _p(_chk_f64(U)), cfl)
+
+
+def set_threads(n: int):
+    lib().orc_set_threads(int(n))

@@ -597,4 +597,21 @@ double orc_ref_tau(const orc_mesh* m, const orc_fields* f, double cfl) {
   return cfl * maxEdge / jmax(vmax, 1.0e-6);
 }
 
+// CFL-limited step: cfl * min_i h_i / VMAX_i with h = 2 A / perimeter (the
+// inscribed-circle diameter of a triangle); the reference's own tau uses the
+// longest edge instead, which is not a stable bound on triangles.
+double orc_local_dt(const orc_mesh* m, const orc_fields* f, double cfl) {
+  const int64_t n = m->nCells;
+  double best = INFINITY;
+  for (int64_t i = 1; i <= n; ++i) {
+    const int nv = I(m->mesh_connectivity, n, i, 3);
+    double perim = 0.0;
+    for (int k = 1; k <= nv; ++k) perim += D(m->cell_edges_length, n, i, k);
+    const double h = 2.0 * m->cell_areas[i - 1] / perim;
+    const double r = h / f->VMAXCells[i - 1];
+    if (r < best) best = r;
+  }
+  return cfl * best;
+}
+
 }  // extern "C"

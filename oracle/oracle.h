@@ -94,12 +94,14 @@ int orc_green_gauss_gradient(const int32_t* cellsThreads, int32_t nThreads, cons
 int orc_update_variables(const int32_t* cellsThreads, int32_t nThreads, int64_t nCells, double gamma,
                          const double* UNew, double* UOld, double* Delta, const orc_fields* f);
 double orc_ref_tau(const orc_mesh* m, const orc_fields* f, double cfl);
+double orc_local_dt(const orc_mesh* m, const orc_fields* f, double cfl);
 
 /* ---- north-star (ns) mode ---- */
 typedef struct {
   double gamma, R, Pr;
   double mu_ref, T_ref, S;  /* Sutherland: mu = mu_ref (T/T_ref)^1.5 (T_ref+S)/(T+S) */
   double venkat_K;
+  double venkat_scale[4];   /* per-variable scales s_v in eps^2 = (K h)^3 s_v^2 */
   int32_t flux;             /* ORC_FLUX_AUSM_PLUS or ORC_FLUX_ROE */
   int32_t viscous;          /* 0 inviscid, 1 Sutherland stresses + heat flux */
   int32_t mixed;            /* 1: slopes/limiter/reconstruction/fluxes in float */
@@ -111,6 +113,7 @@ int orc_ns_step(const orc_mesh* m, const orc_ns_params* prm, double dt, const do
                 double* grad8 /* optional nCells x 8 */, double* phi4 /* optional nCells x 4 */,
                 double* wall_force2 /* optional: pressure+viscous force on code-wall faces */);
 double orc_ns_dt(const orc_mesh* m, const orc_ns_params* prm, const double* U, double cfl);
+void orc_set_threads(int32_t n); /* OpenMP threads of the ns oracle (default 1) */
 
 #ifdef __cplusplus
 }

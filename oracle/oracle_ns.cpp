@@ -1,6 +1,404 @@
-// placeholder, replaced below
+// CPU oracle of the ns scheme: TEST INFRASTRUCTURE ONLY.
+//
+// The reference has no cell-gradient MUSCL, no Venkatakrishnan limiter, no
+// Sutherland stress fluxes and no isothermal wall (its viscous term is the
+// mu*Laplacian of calcDiffTerm.jl); the configurations that use them define
+// this scheme, and this file states it plainly, cell by cell, in the mesh
+// layout of mesh2d_Int32 (1-based, column-major):
+//
+//  pass G  least-squares gradients of q = (rho, u, v, p) from face neighbours
+//          (ghost values at boundary faces), weights w_k = 1/|d_k|^2;
+//          Venkatakrishnan limiter (1993) per variable,
+//          eps^2 = (K h)^3 s_v^2 with h = 2A/perimeter and s_v a scale per variable
+//  pass F  reconstruction q_f = q + phi (grad q . r_f), first order where the
+//          reconstructed rho or p is not positive; AUSM+ or Roe with the
+//          neighbour side as "L" and inward normals (the reference's
+//          convention, computeslope2d.jl:129); viscous flux with the
+//          edge-corrected average face gradient, Sutherland mu(T),
+//          kappa = mu cp / Pr; isothermal no-slip walls get a one-sided
+//          normal gradient; explicit Euler update in FP64.
+//
+// 'mixed' evaluates slopes, limiter, reconstruction and face fluxes in float,
+// and keeps geometry, the flux sum, the update and p/T recovery in double.
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
 #include "oracle.h"
-extern "C" {
-int orc_ns_step(const orc_mesh*, const orc_ns_params*, double, const double*, double*, double*, double*, double*) { return -1; }
-double orc_ns_dt(const orc_mesh*, const orc_ns_params*, const double*, double) { return 0.0; }
+
+namespace {
+
+int g_threads = 1;
+
+inline int32_t I(const int32_t* a, int64_t n, int64_t i, int k) { return a[(int64_t)(k - 1) * n + (i - 1)]; }
+inline double D(const double* a, int64_t n, int64_t i, int k) { return a[(int64_t)(k - 1) * n + (i - 1)]; }
+
+struct Geo {
+  int nf;
+  int32_t nb[4];       // 0-based neighbour or -1
+  int32_t code[4];
+  double nx[4], ny[4], len[4], rx[4], ry[4], dx[4], dy[4], wx[4], wy[4];
+  double area, rarea, hmin;
+};
+
+void face_mid(const orc_mesh* m, int64_t i1, int k, double* r) {
+  const int64_t n = m->nCells;
+  const int32_t a = I(m->cells2nodes, n, i1, 2 * k - 1), b = I(m->cells2nodes, n, i1, 2 * k);
+  const int nv = I(m->mesh_connectivity, n, i1, 3);
+  double xa = 0, ya = 0, xb = 0, yb = 0;
+  for (int v = 0; v < nv; ++v) {
+    const int32_t id = I(m->mesh_connectivity, n, i1, 4 + v);
+    if (id == a) { xa = D(m->cell_nodes_X, n, i1, v + 1); ya = D(m->cell_nodes_Y, n, i1, v + 1); }
+    if (id == b) { xb = D(m->cell_nodes_X, n, i1, v + 1); yb = D(m->cell_nodes_Y, n, i1, v + 1); }
+  }
+  r[0] = 0.5 * (xa + xb) - D(m->cell_mid_points, n, i1, 1);
+  r[1] = 0.5 * (ya + yb) - D(m->cell_mid_points, n, i1, 2);
 }
+
+std::vector<Geo> geometry(const orc_mesh* m) {
+  const int64_t n = m->nCells;
+  std::vector<Geo> g(n);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int64_t i1 = 1; i1 <= n; ++i1) {
+    Geo& G = g[i1 - 1];
+    G.nf = I(m->mesh_connectivity, n, i1, 3);
+    G.area = m->cell_areas[i1 - 1];
+    G.rarea = 1.0 / G.area;
+    double perim = 0.0, M11 = 0, M12 = 0, M22 = 0, wk[4];
+    for (int k = 1; k <= G.nf; ++k) {
+      const int q = k - 1;
+      const int32_t e = I(m->cell_stiffness, n, i1, k);
+      G.nx[q] = D(m->cell_edges_Nx, n, i1, k);
+      G.ny[q] = D(m->cell_edges_Ny, n, i1, k);
+      G.len[q] = D(m->cell_edges_length, n, i1, k);
+      perim += G.len[q];
+      double r[2];
+      face_mid(m, i1, k, r);
+      G.rx[q] = r[0]; G.ry[q] = r[1];
+      if (e >= 1) {
+        G.nb[q] = e - 1; G.code[q] = 0;
+        int kj = 0;
+        for (int t = 1; t <= I(m->mesh_connectivity, n, e, 3); ++t)
+          if (I(m->cell_stiffness, n, e, t) == i1) kj = t;
+        double rj[2];
+        face_mid(m, e, kj, rj);
+        G.dx[q] = r[0] - rj[0]; G.dy[q] = r[1] - rj[1];
+      } else {
+        G.nb[q] = -1; G.code[q] = e;
+        const double rn = r[0] * G.nx[q] + r[1] * G.ny[q];
+        G.dx[q] = 2.0 * rn * G.nx[q]; G.dy[q] = 2.0 * rn * G.ny[q];
+      }
+      const double w = 1.0 / (G.dx[q] * G.dx[q] + G.dy[q] * G.dy[q]);
+      wk[q] = w;
+      M11 += w * G.dx[q] * G.dx[q];
+      M12 += w * G.dx[q] * G.dy[q];
+      M22 += w * G.dy[q] * G.dy[q];
+    }
+    G.hmin = 2.0 * G.area / perim;
+    const double det = M11 * M22 - M12 * M12;
+    for (int q = 0; q < G.nf; ++q) {
+      const double ix = (M22 * G.dx[q] - M12 * G.dy[q]) / det;
+      const double iy = (-M12 * G.dx[q] + M11 * G.dy[q]) / det;
+      G.wx[q] = wk[q] * ix; G.wy[q] = wk[q] * iy;
+    }
+  }
+  return g;
+}
+
+// geometry of the last mesh seen (keyed by its arrays), so that repeated steps
+// on one mesh do not recompute it
+const std::vector<Geo>& cached_geometry(const orc_mesh* m) {
+  static std::vector<Geo> g;
+  static const double* key_area = nullptr;
+  static const int32_t* key_stiff = nullptr;
+  static int64_t key_n = -1;
+  if (key_area != m->cell_areas || key_stiff != m->cell_stiffness || key_n != m->nCells) {
+    g = geometry(m);
+    key_area = m->cell_areas; key_stiff = m->cell_stiffness; key_n = m->nCells;
+  }
+  return g;
+}
+
+inline void prims(const double* U, int64_t n, int64_t i, double g, double* q) {
+  const double r = U[i], mx = U[n + i], my = U[2 * n + i], E = U[3 * n + i];
+  q[0] = r;
+  q[1] = mx / r;
+  q[2] = my / r;
+  q[3] = (g - 1.0) * (E - 0.5 * (mx * mx + my * my) / r);
+}
+
+template <class R>
+inline R sutherland(R T, const orc_ns_params* p) {
+  const R Tr = (R)p->T_ref, S = (R)p->S;
+  return (R)p->mu_ref * std::pow(T / Tr, (R)1.5) * (Tr + S) / (T + S);
+}
+
+// ghost state of a boundary face for the ns scheme
+template <class R>
+inline void ghost(const orc_ns_params* p, int code, const R* q, R nx, R ny, R* g) {
+  const int idx = -code;
+  const orc_bc* b = (idx >= 0 && idx <= ORC_MAX_BC) ? &p->bc[idx] : nullptr;
+  const int kind = b ? b->kind : ORC_BC_EXTRAPOLATE;
+  if (kind == ORC_BC_FIXED) {
+    for (int v = 0; v < 4; ++v) g[v] = (R)b->data[v];
+  } else if (kind == ORC_BC_SLIP_WALL || kind == ORC_BC_ISOTHERMAL_WALL) {
+    const R un = q[1] * nx + q[2] * ny;
+    g[0] = q[0]; g[1] = q[1] - (R)2 * un * nx; g[2] = q[2] - (R)2 * un * ny; g[3] = q[3];
+  } else {
+    for (int v = 0; v < 4; ++v) g[v] = q[v];
+  }
+}
+
+// AUSM+ per unit length, neighbour side "L", normal into the own cell:
+// returns -(F . n) (same splitting as the reference scheme)
+template <class R>
+inline void ausm(const R* L, const R* Rr, R nx, R ny, R g, R* out) {
+  auto mp = [](R M) { const R d = M * M - 1; return std::fabs(M) >= 1 ? (R)0.5 * (M + std::fabs(M)) : (R)0.25 * (M + 1) * (M + 1) + (R)0.125 * d * d; };
+  auto mm = [](R M) { const R d = M * M - 1; return std::fabs(M) >= 1 ? (R)0.5 * (M - std::fabs(M)) : -(R)0.25 * (M - 1) * (M - 1) - (R)0.125 * d * d; };
+  auto pp = [](R M) { const R d = M * M - 1; return std::fabs(M) >= 1 ? (M > 0 ? (R)1 : (R)0) : (R)0.25 * (M + 1) * (M + 1) * (2 - M) + (R)0.1875 * M * d * d; };
+  auto pm = [](R M) { const R d = M * M - 1; return std::fabs(M) >= 1 ? (M < 0 ? (R)1 : (R)0) : (R)0.25 * (M - 1) * (M - 1) * (2 + M) - (R)0.1875 * M * d * d; };
+  const R rL = L[0], uL = L[1], vL = L[2], pL = L[3];
+  const R rR = Rr[0], uR = Rr[1], vR = Rr[2], pR = Rr[3];
+  const R htL = (R)0.5 * (uL * uL + vL * vL) + g / (g - 1) * pL / rL;
+  const R htR = (R)0.5 * (uR * uR + vR * vR) + g / (g - 1) * pR / rR;
+  const R c = 2 * (g - 1) / (g + 1);
+  const R unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
+  const R asL = std::sqrt(c * htL), asR = std::sqrt(c * htR);
+  const R aL = asL * std::fmin((R)1, asL / std::fabs(unL));
+  const R aR = asR * std::fmin((R)1, asR / std::fabs(unR));
+  const R a12 = std::fmin(aL, aR);
+  const R ML = unL / a12, MR = unR / a12;
+  const R m = mp(ML) + mm(MR);
+  const R p12 = pp(ML) * pL + pm(MR) * pR;
+  const R mpl = (R)0.5 * (m + std::fabs(m)), mmi = (R)0.5 * (m - std::fabs(m));
+  out[0] = -(a12 * (mpl * rL + mmi * rR));
+  out[1] = -(a12 * (mpl * rL * uL + mmi * rR * uR) + p12 * nx);
+  out[2] = -(a12 * (mpl * rL * vL + mmi * rR * vR) + p12 * ny);
+  out[3] = -(a12 * (mpl * rL * htL + mmi * rR * htR));
+}
+
+// Roe per unit length with Harten's fix (width 0.2 of the wave speed scale, as
+// RoeFlux2dFast.jl:85-96 but relative to the Roe sound speed so that it is
+// independent of the units)
+template <class R>
+inline void roe(const R* L, const R* Rr, R nx, R ny, R g, R* out) {
+  const R rL = L[0], uL = L[1], vL = L[2], pL = L[3];
+  const R rR = Rr[0], uR = Rr[1], vR = Rr[2], pR = Rr[3];
+  const R mx = -ny, my = nx;
+  const R unL = uL * nx + vL * ny, umL = uL * mx + vL * my;
+  const R unR = uR * nx + vR * ny, umR = uR * mx + vR * my;
+  const R HL = g / (g - 1) * pL / rL + (R)0.5 * (uL * uL + vL * vL);
+  const R HR = g / (g - 1) * pR / rR + (R)0.5 * (uR * uR + vR * vR);
+  const R RT = std::sqrt(rR / rL);
+  const R rho = RT * rL;
+  const R u = (uL + RT * uR) / (1 + RT), v = (vL + RT * vR) / (1 + RT), H = (HL + RT * HR) / (1 + RT);
+  const R a = std::sqrt((g - 1) * (H - (R)0.5 * (u * u + v * v)));
+  const R un = u * nx + v * ny, um = u * mx + v * my;
+  const R dp = pR - pL, dun = unR - unL, dum = umR - umL, drho = rR - rL;
+  const R w1 = (dp - rho * a * dun) / (2 * a * a), w2 = rho * dum, w3 = drho - dp / (a * a),
+          w4 = (dp + rho * a * dun) / (2 * a * a);
+  R s1 = std::fabs(un - a), s4 = std::fabs(un + a);
+  const R s2 = std::fabs(un);
+  const R dw = (R)0.2 * a;
+  if (s1 < dw) s1 = (R)0.5 * (s1 * s1 / dw + dw);
+  if (s4 < dw) s4 = (R)0.5 * (s4 * s4 / dw + dw);
+  const R d1 = s1 * w1 + s2 * w3 + s4 * w4;
+  const R d2 = s1 * w1 * (u - a * nx) + s2 * w2 * mx + s2 * w3 * u + s4 * w4 * (u + a * nx);
+  const R d3 = s1 * w1 * (v - a * ny) + s2 * w2 * my + s2 * w3 * v + s4 * w4 * (v + a * ny);
+  const R d4 = s1 * w1 * (H - un * a) + s2 * w2 * um + s2 * w3 * (R)0.5 * (u * u + v * v) + s4 * w4 * (H + un * a);
+  out[0] = -(R)0.5 * (rL * unL + rR * unR - d1);
+  out[1] = -(R)0.5 * (rL * unL * uL + pL * nx + rR * unR * uR + pR * nx - d2);
+  out[2] = -(R)0.5 * (rL * unL * vL + pL * ny + rR * unR * vR + pR * ny - d3);
+  out[3] = -(R)0.5 * (rL * unL * HL + rR * unR * HR - d4);
+}
+
+template <class R>
+int ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* U, double* Un, double* grad8,
+            double* phi4, double* force2) {
+  const int64_t n = m->nCells;
+  const std::vector<Geo>& G = cached_geometry(m);
+  const double g = p->gamma;
+  const R gR = (R)g, RR = (R)p->R;
+  std::vector<R> q(4 * n), gr(8 * n), ph(4 * n);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    double qq[4];
+    prims(U, n, i, g, qq);
+    for (int v = 0; v < 4; ++v) q[v * n + i] = (R)qq[v];
+  }
+  // ---- pass G
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const Geo& c = G[i];
+    R qi[4], gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0}, qmx[4], qmn[4];
+    for (int v = 0; v < 4; ++v) { qi[v] = q[v * n + i]; qmx[v] = qi[v]; qmn[v] = qi[v]; }
+    for (int k = 0; k < c.nf; ++k) {
+      R qk[4];
+      if (c.nb[k] >= 0) for (int v = 0; v < 4; ++v) qk[v] = q[v * n + c.nb[k]];
+      else ghost<R>(p, c.code[k], qi, (R)c.nx[k], (R)c.ny[k], qk);
+      for (int v = 0; v < 4; ++v) {
+        const R d = qk[v] - qi[v];
+        gx[v] += (R)c.wx[k] * d;
+        gy[v] += (R)c.wy[k] * d;
+        qmx[v] = std::fmax(qmx[v], qk[v]);
+        qmn[v] = std::fmin(qmn[v], qk[v]);
+      }
+    }
+    const R kh = (R)(p->venkat_K * c.hmin);
+    const R kh3 = kh * kh * kh;
+    for (int v = 0; v < 4; ++v) {
+      const R e2 = kh3 * (R)(p->venkat_scale[v] * p->venkat_scale[v]);
+      R phi = 1;
+      for (int k = 0; k < c.nf; ++k) {
+        const R d2 = gx[v] * (R)c.rx[k] + gy[v] * (R)c.ry[k];
+        R d1;
+        if (d2 > 0) d1 = qmx[v] - qi[v];
+        else if (d2 < 0) d1 = qmn[v] - qi[v];
+        else continue;
+        const R psi = ((d1 * d1 + e2) * d2 + 2 * d2 * d2 * d1) / (d1 * d1 + 2 * d2 * d2 + d1 * d2 + e2) / d2;
+        phi = std::fmin(phi, psi);
+      }
+      ph[v * n + i] = std::fmax(phi, (R)0);
+      gr[(2 * v) * n + i] = gx[v];
+      gr[(2 * v + 1) * n + i] = gy[v];
+    }
+  }
+  if (grad8) for (int64_t t = 0; t < 8 * n; ++t) grad8[t] = (double)gr[t];
+  if (phi4) for (int64_t t = 0; t < 4 * n; ++t) phi4[t] = (double)ph[t];
+  // ---- pass F
+  const R cp = gR * RR / (gR - 1);
+  std::vector<double> wall(force2 ? 2 * n : 0, 0.0);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const Geo& c = G[i];
+    R qi[4];
+    for (int v = 0; v < 4; ++v) qi[v] = q[v * n + i];
+    double F[4] = {0, 0, 0, 0};
+    for (int k = 0; k < c.nf; ++k) {
+      const R nx = (R)c.nx[k], ny = (R)c.ny[k];
+      R qa[4];  // own side at the face
+      for (int v = 0; v < 4; ++v)
+        qa[v] = qi[v] + ph[v * n + i] * (gr[(2 * v) * n + i] * (R)c.rx[k] + gr[(2 * v + 1) * n + i] * (R)c.ry[k]);
+      if (!(qa[0] > 0) || !(qa[3] > 0)) for (int v = 0; v < 4; ++v) qa[v] = qi[v];
+      R qb[4];  // neighbour side at the face
+      const int32_t j = c.nb[k];
+      if (j >= 0) {
+        const R rxj = (R)(c.rx[k] - c.dx[k]), ryj = (R)(c.ry[k] - c.dy[k]);
+        for (int v = 0; v < 4; ++v)
+          qb[v] = q[v * n + j] + ph[v * n + j] * (gr[(2 * v) * n + j] * rxj + gr[(2 * v + 1) * n + j] * ryj);
+        if (!(qb[0] > 0) || !(qb[3] > 0)) for (int v = 0; v < 4; ++v) qb[v] = q[v * n + j];
+      } else {
+        ghost<R>(p, c.code[k], qa, nx, ny, qb);
+      }
+      R e[4];
+      if (p->flux == ORC_FLUX_ROE) roe<R>(qb, qa, nx, ny, gR, e);
+      else ausm<R>(qb, qa, nx, ny, gR, e);
+      R gv[4] = {0, 0, 0, 0};
+      if (p->viscous) {
+        const int idx = j >= 0 ? 0 : -c.code[k];
+        const int kind = (j < 0 && idx <= ORC_MAX_BC) ? p->bc[idx].kind : -1;
+        if (j >= 0) {
+          // edge-corrected average of the two cell gradients
+          R qj[4];
+          for (int v = 0; v < 4; ++v) qj[v] = q[v * n + j];
+          const R Ti = qi[3] / (qi[0] * RR), Tj = qj[3] / (qj[0] * RR);
+          R g_i[3][2], g_j[3][2];  // u, v, T
+          for (int s = 0; s < 2; ++s) {
+            g_i[0][s] = gr[(2 + s) * n + i]; g_i[1][s] = gr[(4 + s) * n + i];
+            g_j[0][s] = gr[(2 + s) * n + j]; g_j[1][s] = gr[(4 + s) * n + j];
+            g_i[2][s] = (gr[(6 + s) * n + i] - Ti * RR * gr[s * n + i]) / (qi[0] * RR);
+            g_j[2][s] = (gr[(6 + s) * n + j] - Tj * RR * gr[s * n + j]) / (qj[0] * RR);
+          }
+          const R ddx = (R)c.dx[k], ddy = (R)c.dy[k];
+          const R dl = std::sqrt(ddx * ddx + ddy * ddy);
+          const R tx = ddx / dl, ty = ddy / dl;
+          const R dphi[3] = {qj[1] - qi[1], qj[2] - qi[2], Tj - Ti};
+          R gf[3][2];
+          for (int s = 0; s < 3; ++s) {
+            const R ax = (R)0.5 * (g_i[s][0] + g_j[s][0]), ay = (R)0.5 * (g_i[s][1] + g_j[s][1]);
+            const R corr = dphi[s] / dl - (ax * tx + ay * ty);
+            gf[s][0] = ax + corr * tx; gf[s][1] = ay + corr * ty;
+          }
+          const R Tf = (R)0.5 * (Ti + Tj);
+          const R uf = (R)0.5 * (qi[1] + qj[1]), vf = (R)0.5 * (qi[2] + qj[2]);
+          const R mu = sutherland<R>(Tf, p);
+          const R kap = mu * cp / (R)p->Pr;
+          const R div = gf[0][0] + gf[1][1];
+          const R txx = mu * (2 * gf[0][0] - (R)(2.0 / 3.0) * div);
+          const R tyy = mu * (2 * gf[1][1] - (R)(2.0 / 3.0) * div);
+          const R txy = mu * (gf[0][1] + gf[1][0]);
+          gv[1] = txx * nx + txy * ny;
+          gv[2] = txy * nx + tyy * ny;
+          gv[3] = uf * gv[1] + vf * gv[2] + kap * (gf[2][0] * nx + gf[2][1] * ny);
+        } else if (kind == ORC_BC_ISOTHERMAL_WALL) {
+          // one-sided normal gradient to the wall (u = 0, T = T_w)
+          const R Tw = (R)p->bc[idx].data[0];
+          const R Ti = qi[3] / (qi[0] * RR);
+          const R dperp = -((R)c.rx[k] * nx + (R)c.ry[k] * ny);
+          // outward normal is -n: d(phi)/dn_out = (phi_w - phi_i)/dperp
+          const R dudn = (0 - qi[1]) / dperp, dvdn = (0 - qi[2]) / dperp, dTdn = (Tw - Ti) / dperp;
+          const R ox = -nx, oy = -ny;
+          const R gu[2] = {dudn * ox, dudn * oy}, gvv[2] = {dvdn * ox, dvdn * oy}, gT[2] = {dTdn * ox, dTdn * oy};
+          const R mu = sutherland<R>(Tw, p);
+          const R kap = mu * cp / (R)p->Pr;
+          const R div = gu[0] + gvv[1];
+          const R txx = mu * (2 * gu[0] - (R)(2.0 / 3.0) * div);
+          const R tyy = mu * (2 * gvv[1] - (R)(2.0 / 3.0) * div);
+          const R txy = mu * (gu[1] + gvv[0]);
+          gv[1] = txx * nx + txy * ny;
+          gv[2] = txy * nx + tyy * ny;
+          gv[3] = kap * (gT[0] * nx + gT[1] * ny);
+        }
+      }
+      double fk[4];
+      for (int v = 0; v < 4; ++v) fk[v] = ((double)e[v] + (double)gv[v]) * c.len[k];
+      for (int v = 0; v < 4; ++v) F[v] += fk[v];
+      if (j < 0 && force2) {
+        const int idx = -c.code[k];
+        if (idx <= ORC_MAX_BC && (p->bc[idx].kind == ORC_BC_ISOTHERMAL_WALL || p->bc[idx].kind == ORC_BC_SLIP_WALL)) {
+          wall[2 * i] += fk[1]; wall[2 * i + 1] += fk[2];
+        }
+      }
+    }
+    for (int v = 0; v < 4; ++v) Un[v * n + i] = U[v * n + i] - F[v] * dt * c.rarea;
+  }
+  if (force2) {
+    double fx = 0.0, fy = 0.0;
+    for (int64_t i = 0; i < n; ++i) { fx += wall[2 * i]; fy += wall[2 * i + 1]; }
+    force2[0] = fx; force2[1] = fy;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_set_threads(int32_t n) { g_threads = n > 0 ? n : 1; }
+
+int orc_ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* U, double* Un, double* grad8,
+                double* phi4, double* force2) {
+  if (p->mixed) return ns_step<float>(m, p, dt, U, Un, grad8, phi4, force2);
+  return ns_step<double>(m, p, dt, U, Un, grad8, phi4, force2);
+}
+
+// CFL dt of the ns scheme: cfl * min_i h_i / (|u|+a + 2 gamma mu/(Pr rho h_i))
+double orc_ns_dt(const orc_mesh* m, const orc_ns_params* p, const double* U, double cfl) {
+  const int64_t n = m->nCells;
+  const std::vector<Geo>& G = cached_geometry(m);
+  double best = INFINITY;
+  for (int64_t i = 0; i < n; ++i) {
+    double q[4];
+    prims(U, n, i, p->gamma, q);
+    const double a = std::sqrt(p->gamma * q[3] / q[0]);
+    double lam = std::sqrt(q[1] * q[1] + q[2] * q[2]) + a;
+    if (p->viscous) {
+      const double T = q[3] / (q[0] * p->R);
+      lam += 2.0 * p->gamma * sutherland<double>(T, p) / (p->Pr * q[0] * G[i].hmin);
+    }
+    best = std::fmin(best, G[i].hmin / lam);
+  }
+  return cfl * best;
+}
+
+}  // extern "C"

@@ -1,0 +1,166 @@
+"""GPU parity of the ns scheme (cell gradients + Venkatakrishnan + AUSM+/Roe +
+Sutherland viscous fluxes + isothermal wall) against the CPU oracle.
+
+FP64: the device kernels follow the oracle's operation order without FMA
+contraction, so the bar is relative L-inf <= 1e-12 per step (only pow() in the
+Sutherland law may differ by an ulp).  FP32-mixed: against the float oracle
+<= 2e-5 per step and against the FP64 oracle <= 5e-4 per step (single-precision
+slopes and fluxes, FP64 accumulation and update).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import pkg
+from helpers import SEED, rel_linf, random_fields
+
+pytestmark = pytest.mark.gpu
+
+
+def _box(quads="mixed", nx=24, ny=16):
+    M = pkg().mesh
+    return M.make_box_mesh(nx, ny, 2.0, 1.0, quads=quads)
+
+
+def _box_state(mesh, g=1.4):
+    f = random_fields(mesh, seed=SEED, rho=(0.8, 1.4), vel=(-0.6, 0.9), p=(0.8, 1.6))
+    U = np.zeros((mesh.nCells, 4), order="F")
+    r, u, v, p = f.densityCells, f.UxCells, f.UyCells, f.pressureCells
+    U[:, 0] = r; U[:, 1] = r * u; U[:, 2] = r * v; U[:, 3] = p / (g - 1) + 0.5 * r * (u * u + v * v)
+    return U
+
+
+BCS = {-1: (O.BC_FIXED, (1.1, 0.8, 0.1, 1.2)), -2: (O.BC_EXTRAPOLATE, ()), -3: (O.BC_ISOTHERMAL_WALL, (1.05,)),
+       -4: (O.BC_SLIP_WALL, ())}
+GAS = dict(R=1.0, mu_ref=2e-3, T_ref=1.0, S=0.4, Pr=0.72, venkat_K=3.0, venkat_scale=(1.0, 1.0, 1.0, 1.0))
+
+
+def _dev_params(flux=0, viscous=True, mixed=False):
+    S = pkg().stage
+    L = pkg()._lib
+    return S.make_params(scheme=L.GD_SCHEME_NS, precision=L.GD_FP32_MIXED if mixed else L.GD_FP64, flux=flux,
+                         viscous=viscous, venkat_K=GAS["venkat_K"], venkat_scale=GAS["venkat_scale"], RGAS=GAS["R"],
+                         Gamma=1.4, mu_ref=GAS["mu_ref"], T_ref=GAS["T_ref"], S_suth=GAS["S"], Pr=GAS["Pr"], bcs=BCS)
+
+
+def _orc_params(flux=0, viscous=True, mixed=False):
+    return O.ns_params(gamma=1.4, flux=flux, viscous=viscous, mixed=mixed, bcs=BCS, **GAS)
+
+
+def _device_steps(mesh, U, n, dt, params, order=None, perm=None, wall_code=0):
+    S = pkg().stage
+    L = pkg()._lib
+    s = S.GodunovSolver(mesh, params, order=L.GD_ORDER_AS_GIVEN if order is None else order, perm=perm)
+    s.state.set_array(L.GD_ARR_U_OLD, U)
+    st = s.run(n, dt=dt, wall_code=wall_code)
+    out = s.conservative()
+    s.close()
+    return out, st
+
+
+@pytest.mark.parametrize("quads", ["none", "mixed"])
+@pytest.mark.parametrize("flux", [O.FLUX_AUSM_PLUS, O.FLUX_ROE])
+@pytest.mark.parametrize("viscous", [False, True])
+def test_ns_fp64_one_step(quads, flux, viscous):
+    mesh = _box(quads)
+    U = _box_state(mesh)
+    mv = O.MeshView(mesh)
+    UNo, _, _, fo = O.ns_step(mv, _orc_params(flux, viscous), 2e-3, U)
+    UN, st = _device_steps(mesh, U, 1, 2e-3, _dev_params(flux, viscous), wall_code=-3)
+    err = rel_linf(UN, UNo)
+    print("ns fp64 one step", quads, flux, viscous, err)
+    assert err <= 1e-12
+
+
+def test_ns_fp64_ten_steps_and_wall_force():
+    mesh = _box("mixed")
+    U = _box_state(mesh)
+    mv = O.MeshView(mesh)
+    prm = _orc_params()
+    Uo = U.copy(order="F")
+    for _ in range(10):
+        Uo, _, _, fo = O.ns_step(mv, prm, 1e-3, Uo)
+    UN, st = _device_steps(mesh, U, 10, 1e-3, _dev_params(), wall_code=-3)
+    assert rel_linf(UN, Uo) <= 1e-11
+    # wall force of the last step: the isothermal wall (-3) only; the oracle also adds slip walls (-4)
+    prm_iso = O.ns_params(gamma=1.4, bcs={**BCS, -4: (O.BC_EXTRAPOLATE, ())}, **GAS)
+    U9 = U.copy(order="F")
+    for _ in range(9):
+        U9, _, _, _ = O.ns_step(mv, prm, 1e-3, U9)
+    _, _, _, f_iso = O.ns_step(mv, prm_iso, 1e-3, U9)
+    # forces from the same state, summed in a different order
+    assert np.allclose([st.wall_force[0], st.wall_force[1]], f_iso, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("flux", [O.FLUX_AUSM_PLUS, O.FLUX_ROE])
+def test_ns_mixed_vs_float_and_fp64_oracle(flux):
+    mesh = _box("mixed")
+    U = _box_state(mesh)
+    mv = O.MeshView(mesh)
+    UNf, _, _, _ = O.ns_step(mv, _orc_params(flux, True, mixed=True), 2e-3, U)
+    UNd, _, _, _ = O.ns_step(mv, _orc_params(flux, True, mixed=False), 2e-3, U)
+    UN, _ = _device_steps(mesh, U, 1, 2e-3, _dev_params(flux, True, mixed=True))
+    # compare the increments, the part that carries the single-precision error
+    e_f = rel_linf(UN - U, UNf - U)
+    e_d = rel_linf(UN - U, UNd - U)
+    print("mixed vs float oracle", e_f, "vs fp64 oracle", e_d)
+    assert e_f <= 2e-5
+    assert e_d <= 5e-4
+
+
+def test_ns_reordering_invisible_fp64():
+    L = pkg()._lib
+    M = pkg().mesh
+    mesh = _box("mixed", 30, 20)
+    U = _box_state(mesh)
+    a, _ = _device_steps(mesh, U, 8, 1e-3, _dev_params())
+    b, _ = _device_steps(mesh, U, 8, 1e-3, _dev_params(), order=L.GD_ORDER_RCM)
+    c, _ = _device_steps(mesh, U, 8, 1e-3, _dev_params(), order=L.GD_ORDER_PERMUTATION,
+                         perm=M.random_permutation(mesh.nCells, SEED))
+    assert np.array_equal(a, b)
+    assert np.array_equal(a, c)
+
+
+def test_ns_free_stream_preserved_periodic():
+    M = pkg().mesh
+    mesh = M.make_tgv_periodic_mesh(24)
+    n = mesh.nCells
+    U = np.zeros((n, 4), order="F")
+    U[:, 0] = 1.2; U[:, 1] = 1.2 * 0.3; U[:, 2] = -1.2 * 0.2; U[:, 3] = 2.0 / 0.4 + 0.5 * 1.2 * 0.13
+    UN, _ = _device_steps(mesh, U, 5, 1e-3, _dev_params())
+    assert rel_linf(UN, U) <= 1e-14
+    UN, _ = _device_steps(mesh, U, 5, 1e-3, _dev_params(mixed=True))
+    assert rel_linf(UN, U) <= 1e-7
+
+
+def test_ns_small_cylinder_fp64_vs_oracle_50_steps():
+    """The supersonic cylinder set-up on a coarse O-grid: 50 FP64 steps with
+    the CFL-0.4 viscous dt, device vs oracle (same dt each step)."""
+    M = pkg().mesh
+    S = pkg().stage
+    L = pkg()._lib
+    case = S.CylinderCase()
+    mesh = M.make_cylinder_ogrid_mesh(72, 41, 0.5, 10.0, 2e-3)
+    mv = O.MeshView(mesh)
+    n = mesh.nCells
+    r, u, v, p = case.free_stream(n)
+    U = np.zeros((n, 4), order="F")
+    U[:, 0] = r; U[:, 1] = r * u; U[:, 2] = r * v; U[:, 3] = p / 0.4 + 0.5 * r * (u * u + v * v)
+    bcs = {-1: (O.BC_FIXED, (case.rho_inf, case.u_inf, 0.0, case.p_inf)), -2: (O.BC_EXTRAPOLATE, ()),
+           -3: (O.BC_ISOTHERMAL_WALL, (case.T_wall,))}
+    prm = O.ns_params(gamma=1.4, R=case.R, Pr=case.Pr, mu_ref=case.mu_inf, T_ref=case.T_inf, S=case.S,
+                      venkat_K=5.0, venkat_scale=(case.rho_inf, case.u_inf, case.u_inf, case.p_inf), bcs=bcs)
+    s = S.GodunovSolver(mesh, case.params())
+    s.state.set_array(L.GD_ARR_U_OLD, U)
+    Uo = U.copy(order="F")
+    t = 0.0
+    for k in range(50):
+        dt = O.ns_dt(mv, prm, Uo, 0.4)
+        st = s.run(1, cfl=0.4, dt_policy=L.GD_DT_LOCAL, wall_code=-3)
+        assert abs(st.last_dt - dt) <= 1e-12 * dt, (k, st.last_dt, dt)
+        Uo, _, _, f = O.ns_step(mv, prm, dt, Uo)
+        t += dt
+    err = rel_linf(s.conservative(), Uo)
+    print("cylinder 50 steps fp64", err, "drag", st.wall_force[0], f[0])
+    assert err <= 1e-11
+    assert abs(st.wall_force[0] - f[0]) <= 1e-9 * max(abs(f[0]), case.p_inf)

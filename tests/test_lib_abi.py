@@ -38,7 +38,7 @@ def test_version_and_errors_without_device():
 
 def test_struct_sizes_match_header_layout():
     L = pkg()._lib
-    # 4 ints, 2 doubles, 2 ints, 8 doubles, 17 x (2 ints + 4 doubles)
-    assert ctypes.sizeof(L.gd_params) == 4 * 4 + 2 * 8 + 2 * 4 + 8 * 8 + 17 * (8 + 32)
+    # 4 ints, 6 doubles, 2 ints, 8 doubles, 17 x (2 ints + 4 doubles)
+    assert ctypes.sizeof(L.gd_params) == 4 * 4 + 6 * 8 + 2 * 4 + 8 * 8 + 17 * (8 + 32)
     assert ctypes.sizeof(L.gd_run_params) == 4 + 4 + 4 * 8 + 4 + 4
     assert ctypes.sizeof(L.gd_mesh_desc) == 8 + 8 + 4 + 4 + 17 * 8
