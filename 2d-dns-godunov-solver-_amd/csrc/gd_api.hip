@@ -476,12 +476,9 @@ int gd_state_create(gd_mesh* m, const gd_params* p, gd_state** out) {
   rc |= dev_alloc(al, &S->phiN, 3 * nn);
   rc |= dev_alloc(al, &S->g6, 6 * nc);
   rc |= dev_alloc(al, &S->g6N, 6 * nn);
-  double* gbuf = nullptr;
-  rc |= dev_alloc(al, &gbuf, 8 * nc);
-  S->grad = gbuf;
-  double* pbuf = nullptr;
-  rc |= dev_alloc(al, &pbuf, 4 * nc);
-  S->phi = pbuf;
+  double* rbuf = nullptr;
+  rc |= dev_alloc(al, &rbuf, 16 * nc);
+  S->rec = rbuf;
   double* qt0 = nullptr;
   double* qt1 = nullptr;
   rc |= dev_alloc(al, &qt0, 4 * nc);
@@ -777,10 +774,10 @@ static double bytes_ref_gg_nodes(const MeshView& v) {
 }
 // ns: t = bytes of the compute type; gathered neighbour data at unique size
 static double bytes_ns_grad(const MeshView& v, double t) {
-  return (double)v.nc * (4 * t /*q*/ + 12 /*nb*/ + 48 /*lsq*/ + 48 /*rf*/ + 8 /*hmin*/ + 12 * t /*grad+phi*/);
+  return (double)v.nc * (4 * t /*q*/ + 12 /*nb*/ + 48 /*lsq*/ + 48 /*rf*/ + 8 /*hmin*/ + 16 * t /*record*/);
 }
 static double bytes_ns_flux(const MeshView& v, double t) {
-  return (double)v.nc * (16 * t /*q, grad, phi*/ + 12 /*nb*/ + 72 /*nx,ny,len*/ + 48 /*rf*/ + 48 /*dc*/ +
+  return (double)v.nc * (16 * t /*record*/ + 12 /*nb*/ + 72 /*nx,ny,len*/ + 48 /*rf*/ + 48 /*dc*/ +
                          8 /*rarea*/ + 8 /*hmin*/ + 32 /*U old*/ + 32 /*U new*/ + 4 * t /*q new*/);
 }
 
@@ -855,8 +852,7 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       a.Un = S->U[1 - S->cur];
       a.Q = S->Qt[tq];
       a.Qw = S->Qt[1 - tq];
-      a.grad = S->grad;
-      a.phi = S->phi;
+      a.rec = S->rec;
       a.acc = S->acc;
       a.step = s;
       a.dt_policy = rp->dt_policy;

@@ -15,6 +15,17 @@
 
 namespace gd {
 
+// MI355X dispatches workgroup b to XCD (b mod 8).  Remapping the launch order
+// so that each XCD receives one contiguous range of logical blocks keeps the
+// neighbours of a cell range (which, in RCM order, sit in nearby blocks) in the
+// same 4 MB L2 instead of spreading the halo over eight L2s.
+constexpr unsigned kNumXcd = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned x = b % kNumXcd, k = b / kNumXcd;
+  const unsigned q = nb / kNumXcd, r = nb % kNumXcd;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // Everything a kernel needs about the mesh, passed by value as a kernel argument.
 struct MeshView {
   int64_t nc, nn;
@@ -94,9 +105,8 @@ struct State {
   double* phiN = nullptr;               // [3][nn]
   double* g6 = nullptr;                 // [6][nc]
   double* g6N = nullptr;                // [6][nn]
-  void* grad = nullptr;                 // ns: [8][nc] (double or float)
-  void* phi = nullptr;                  // ns: [4][nc]
-  void* Qt[2] = {nullptr, nullptr};     // ns: primitives in the compute type [4][nc]
+  void* rec = nullptr;                  // ns: per-cell record [nc][16] (q, grad q, limiter), compute type
+  void* Qt[2] = {nullptr, nullptr};     // ns: primitives [nc][4], compute type
   double* wall_f = nullptr;             // [n_wall][2]
   double* drag = nullptr;               // [4]: last step force x,y; running sums
   Acc* acc = nullptr;                   // [3]
@@ -152,10 +162,9 @@ struct NsStepArgs {
   gd_params prm;
   const double* Uo;
   double* Un;
-  const void* Q;        // current primitives, compute type
+  const void* Q;        // current primitives [nc][4], compute type
   void* Qw;             // next primitives
-  void* grad;           // [8][nc]
-  void* phi;            // [4][nc]
+  void* rec;            // [nc][16]: q, (d/dx, d/dy) of rho,u,v,p, limiter of rho,u,v,p
   Acc* acc;
   int32_t step;
   int32_t dt_policy;

@@ -11,8 +11,7 @@ static ns::NsKArgs<float> cast(const NsStepArgs& a) {
   k.Un = a.Un;
   k.Q = static_cast<const float*>(a.Q);
   k.Qw = static_cast<float*>(a.Qw);
-  k.grad = static_cast<float*>(a.grad);
-  k.phi = static_cast<float*>(a.phi);
+  k.rec = static_cast<float*>(a.rec);
   k.acc = a.acc;
   k.step = a.step;
   k.dt_policy = a.dt_policy;

@@ -11,8 +11,7 @@ static ns::NsKArgs<double> cast(const NsStepArgs& a) {
   k.Un = a.Un;
   k.Q = static_cast<const double*>(a.Q);
   k.Qw = static_cast<double*>(a.Qw);
-  k.grad = static_cast<double*>(a.grad);
-  k.phi = static_cast<double*>(a.phi);
+  k.rec = static_cast<double*>(a.rec);
   k.acc = a.acc;
   k.step = a.step;
   k.dt_policy = a.dt_policy;

@@ -8,6 +8,11 @@
 //   k_ns_grad  least-squares gradient of (rho,u,v,p) + Venkatakrishnan limiter
 //   k_ns_flux  reconstruction, AUSM+/Roe, viscous stress/heat flux, explicit
 //              Euler update, primitive recovery, dt and wall-force partials
+// Everything read across a face is stored as one record per cell so that a
+// neighbour gather is one contiguous load: primitives [nc][4] (16 B in FP32)
+// and the face-reconstruction record [nc][16] = (q, d/dx d/dy of each of
+// rho,u,v,p, limiter of each) (64 B in FP32).  Per-cell geometry is
+// face-slot-major SoA, read unit-stride.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -24,10 +29,9 @@ struct NsKArgs {
   gd_params prm;
   const double* Uo;
   double* Un;
-  const T* Q;       // current primitives (rho,u,v,p) [4][nc]
-  T* Qw;            // next primitives
-  T* grad;          // [8][nc]: d/dx, d/dy of rho, u, v, p
-  T* phi;           // [4][nc]
+  const T* Q;       // current primitives (rho,u,v,p) [nc][4]
+  T* Qw;            // next primitives [nc][4]
+  T* rec;           // [nc][16]: q[4], (d/dx, d/dy) of rho,u,v,p [8], limiter [4]
   Acc* acc;
   int32_t step;
   int32_t dt_policy;
@@ -36,10 +40,26 @@ struct NsKArgs {
   int32_t wall_code;
 };
 
+// 4 consecutive values: one 16 B load (float) or two (double)
+__device__ __forceinline__ void ld4(const float* p, float* o) {
+  const float4 v = *reinterpret_cast<const float4*>(p);
+  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ __forceinline__ void ld4(const double* p, double* o) {
+  const double2 a = *reinterpret_cast<const double2*>(p), b = *reinterpret_cast<const double2*>(p + 2);
+  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ __forceinline__ void st4(float* p, float a, float b, float c, float d) {
+  *reinterpret_cast<float4*>(p) = make_float4(a, b, c, d);
+}
+__device__ __forceinline__ void st4(double* p, double a, double b, double c, double d) {
+  *reinterpret_cast<double2*>(p) = make_double2(a, b);
+  *reinterpret_cast<double2*>(p + 2) = make_double2(c, d);
+}
 template <class T>
-__device__ __forceinline__ T tmin(T a, T b) { return b < a ? b : a; }
-template <class T>
-__device__ __forceinline__ T tmax(T a, T b) { return b > a ? b : a; }
+__device__ __forceinline__ void ld16(const T* p, T* o) {
+  ld4(p, o); ld4(p + 4, o + 4); ld4(p + 8, o + 8); ld4(p + 12, o + 12);
+}
 
 template <class T>
 __device__ __forceinline__ T sutherland(T Tt, const gd_params& p) {
@@ -146,8 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_prims(int64_t nc, double g, const
   if (i >= nc) return;
   double q[4];
   prims64(U, nc, i, g, q);
-#pragma unroll
-  for (int v = 0; v < 4; ++v) Q[v * nc + i] = (T)q[v];
+  st4(Q + 4 * i, (T)q[0], (T)q[1], (T)q[2], (T)q[3]);
 }
 
 // ---- pass G: gradients + limiter
@@ -155,12 +174,13 @@ template <class T, bool QUADS>
 __global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
   const MeshView& m = a.m;
   const int64_t nc = m.nc;
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= nc) return;
   const int nf = QUADS ? m.nv[i] : 3;
   T qi[4], gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0}, qmx[4], qmn[4];
+  ld4(a.Q + 4 * i, qi);
 #pragma unroll
-  for (int v = 0; v < 4; ++v) { qi[v] = a.Q[v * nc + i]; qmx[v] = qi[v]; qmn[v] = qi[v]; }
+  for (int v = 0; v < 4; ++v) { qmx[v] = qi[v]; qmn[v] = qi[v]; }
   T rx[4], ry[4];
 #pragma unroll
   for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
@@ -169,8 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
       const int32_t j = m.nb[s];
       T qk[4];
       if (j >= 0) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) qk[v] = a.Q[v * nc + j];
+        ld4(a.Q + 4 * (int64_t)j, qk);
       } else {
         ghost<T>(a.prm, j + 1, qi, (T)m.fnx[s], (T)m.fny[s], qk);
       }
@@ -189,6 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
   }
   const T kh = (T)(a.prm.venkat_K * m.hmin[i]);
   const T kh3 = kh * kh * kh;
+  T ph[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const T e2 = kh3 * (T)(a.prm.venkat_scale[v] * a.prm.venkat_scale[v]);
@@ -204,10 +224,13 @@ __global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
         }
       }
     }
-    a.phi[v * nc + i] = fmax(phi, (T)0);
-    a.grad[(2 * v) * nc + i] = gx[v];
-    a.grad[(2 * v + 1) * nc + i] = gy[v];
+    ph[v] = fmax(phi, (T)0);
   }
+  T* r = a.rec + 16 * i;
+  st4(r, qi[0], qi[1], qi[2], qi[3]);
+  st4(r + 4, gx[0], gy[0], gx[1], gy[1]);
+  st4(r + 8, gx[2], gy[2], gx[3], gy[3]);
+  st4(r + 12, ph[0], ph[1], ph[2], ph[3]);
 }
 
 __device__ __forceinline__ double wave_max64(double v) {
@@ -229,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
   const MeshView& m = a.m;
   const gd_params& P = a.prm;
   const int64_t nc = m.nc;
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const Acc r = a.acc[a.step % 3];
   double dt = a.dt;
   if (a.dt_policy == GD_DT_LOCAL) dt = a.cfl * __longlong_as_double((long long)r.hdt_bits);
@@ -243,11 +266,11 @@ __global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
     const T g = (T)P.Gamma, Rg = (T)P.RGAS;
     const T cp = g * Rg / (g - 1);
     const int nf = QUADS ? m.nv[i] : 3;
-    T qi[4], phi_i[4], gi[8];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) { qi[v] = a.Q[v * nc + i]; phi_i[v] = a.phi[v * nc + i]; }
-#pragma unroll
-    for (int c = 0; c < 8; ++c) gi[c] = a.grad[c * nc + i];
+    T ri[16];
+    ld16(a.rec + 16 * i, ri);
+    const T* qi = ri;
+    const T* gi = ri + 4;
+    const T* phi_i = ri + 12;
     double F[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
@@ -265,24 +288,24 @@ __global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) qa[v] = qi[v];
         }
-        T qb[4], qj[4], gj[8];
+        T qb[4], rj[16];
+        const T* qj = rj;
+        const T* gj = rj + 4;
         double dxd = 0.0, dyd = 0.0;
         if (j >= 0) {
           dxd = m.dc[(2 * k) * nc + i];
           dyd = m.dc[(2 * k + 1) * nc + i];
           const T rxj = (T)(rxd - dxd), ryj = (T)(ryd - dyd);
+          ld16(a.rec + 16 * (int64_t)j, rj);
 #pragma unroll
-          for (int c = 0; c < 8; ++c) gj[c] = a.grad[c * nc + j];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            qj[v] = a.Q[v * nc + j];
-            qb[v] = qj[v] + a.phi[v * nc + j] * (gj[2 * v] * rxj + gj[2 * v + 1] * ryj);
-          }
+          for (int v = 0; v < 4; ++v) qb[v] = qj[v] + rj[12 + v] * (gj[2 * v] * rxj + gj[2 * v + 1] * ryj);
           if (!(qb[0] > 0) || !(qb[3] > 0)) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) qb[v] = qj[v];
           }
         } else {
+#pragma unroll
+          for (int c = 0; c < 16; ++c) rj[c] = 0;
           ghost<T>(P, j + 1, qa, nx, ny, qb);
         }
         T e[4];
@@ -365,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
     const double G = P.Gamma;
     const double rho = Un[0], ux = Un[1] / rho, uy = Un[2] / rho;
     const double p = (G - 1.0) * (Un[3] - 0.5 * (Un[1] * Un[1] + Un[2] * Un[2]) / rho);
-    a.Qw[i] = (T)rho; a.Qw[nc + i] = (T)ux; a.Qw[2 * nc + i] = (T)uy; a.Qw[3 * nc + i] = (T)p;
+    st4(a.Qw + 4 * i, (T)rho, (T)ux, (T)uy, (T)p);
     const double as = sqrt(G * p / rho);
     const double vm = sqrt(ux * ux + uy * uy) + as;
     double lam = vm;

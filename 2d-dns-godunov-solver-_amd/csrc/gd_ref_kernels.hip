@@ -180,7 +180,7 @@ template <bool QUADS, bool VISC, bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_ref_step(RefStepArgs a) {
   const MeshView& m = a.m;
   const int64_t nc = m.nc;
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   double dt = a.dt, t = a.flow_time;
   if (FUSED) {
     const Acc r = a.acc[a.step % 3];
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_step(RefStepArgs a) {
 // (calcGradients.jl:3-18) for rho*u, rho*v, E.
 __global__ __launch_bounds__(kBlock) void k_ref_nodes(MeshView m, const double* Q, const double* U, double* Qn,
                                                       double* Un, double* phiN) {
-  const int64_t J = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t J = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (J >= m.nn) return;
   const int64_t nc = m.nc, nn = m.nn;
   double det = 0.0, r = 0.0, u = 0.0, v = 0.0, p = 0.0;
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_nodes(MeshView m, const double* 
 // rho*u, rho*v, E the pair ("gradX", "gradY") = (d/dy, d/dx).
 template <bool QUADS>
 __global__ __launch_bounds__(kBlock) void k_ref_gg1(MeshView m, const double* phiN, double* g6) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= m.nc) return;
   const int nf = QUADS ? m.nv[i] : 3;
   const double A = m.area[i];
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_gg1(MeshView m, const double* ph
 // Cluster means of the six gradient fields (calcPhiInCellCluster applied
 // inside the second gradient level, calcDiffTerm.jl:88-95).
 __global__ __launch_bounds__(kBlock) void k_ref_gg_nodes(MeshView m, const double* g6, double* g6N) {
-  const int64_t J = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t J = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (J >= m.nn) return;
   const int64_t nc = m.nc, nn = m.nn;
   double s[6] = {0, 0, 0, 0, 0, 0};

@@ -137,10 +137,11 @@ def main():
     ap.add_argument("--n-r", type=int, default=1445)
     ap.add_argument("--dr0", type=float, default=2e-4)
     ap.add_argument("--order", choices=["random", "given"], default="random")
-    ap.add_argument("--device-order", choices=["given", "rcm"], default="given")
+    ap.add_argument("--device-order", choices=["given", "rcm"], default="rcm",
+                    help="cell order of the HBM layout; variants ending in -given keep the caller's order")
     ap.add_argument("--flux", choices=["ausm", "roe"], default="ausm")
     ap.add_argument("--cfl", type=float, default=0.4)
-    ap.add_argument("--variants", default="fp64,mixed,mixed-rcm")
+    ap.add_argument("--variants", default="fp64,mixed,fp64-given,mixed-given")
     ap.add_argument("--cpu-sample-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--no-cpu", action="store_true")
@@ -164,7 +165,7 @@ def main():
     res = {}
     for var in a.variants.split(","):
         prec = L.GD_FP64 if var.startswith("fp64") else L.GD_FP32_MIXED
-        order = L.GD_ORDER_RCM if var.endswith("rcm") else dev_order
+        order = L.GD_ORDER_AS_GIVEN if var.endswith("-given") else dev_order
         res[var] = run_variant(mesh, case, prec, flux, order, U0, a.warmup, a.steps, a.cfl)
         r = res[var]
         log(f"{var}: {r['ms_per_step']:.4f} ms/step, {mesh.nCells / (r['ms_per_step'] * 1e-3):.4e} cell-updates/s, "
