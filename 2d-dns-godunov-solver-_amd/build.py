@@ -24,7 +24,8 @@ UNITS = {
     "gd_api.hip": [],
     "gd_ref_kernels.hip": ["-ffp-contract=off"],
     "gd_ns_fp64.hip": ["-ffp-contract=off"],
-    "gd_ns_fp32.hip": [],
+    # FP32-mixed: hardware reciprocal / square root for float division and sqrt
+    "gd_ns_fp32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 

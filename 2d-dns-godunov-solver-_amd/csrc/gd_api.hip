@@ -396,6 +396,23 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
   rc |= upload(al, &v.rf, h_rf, B);
   rc |= upload(al, &v.dc, h_dc, B);
   rc |= upload(al, &v.wall_id, h_wid, B);
+  {
+    std::vector<float> f_nx(h_nx.size()), f_ny(h_ny.size()), f_lsq(h_lsq.size()), f_rf(h_rf.size()),
+        f_rj(h_rf.size()), f_dc(h_dc.size());
+    for (size_t q = 0; q < h_nx.size(); ++q) { f_nx[q] = (float)h_nx[q]; f_ny[q] = (float)h_ny[q]; }
+    for (size_t q = 0; q < h_lsq.size(); ++q) {
+      f_lsq[q] = (float)h_lsq[q];
+      f_rf[q] = (float)h_rf[q];
+      f_rj[q] = (float)(h_rf[q] - h_dc[q]);
+      f_dc[q] = (float)h_dc[q];
+    }
+    rc |= upload(al, &v.fnx32, f_nx, B);
+    rc |= upload(al, &v.fny32, f_ny, B);
+    rc |= upload(al, &v.lsq32, f_lsq, B);
+    rc |= upload(al, &v.rf32, f_rf, B);
+    rc |= upload(al, &v.rj32, f_rj, B);
+    rc |= upload(al, &v.dc32, f_dc, B);
+  }
   rc |= upload(al, &v.cl, h_cl, B);
   rc |= upload(al, &v.w, h_w, B);
   rc |= upload(al, &v.xn, h_xn, B);
@@ -772,13 +789,15 @@ static double bytes_ref_gg1(const MeshView& v) {
 static double bytes_ref_gg_nodes(const MeshView& v) {
   return 3.0 * v.nc * 4 + (double)v.nc * 48 + (double)v.nn * 48;
 }
-// ns: t = bytes of the compute type; gathered neighbour data at unique size
+// ns: t = bytes of the compute type (geometry read at t except len, rarea,
+// hmin); gathered neighbour data counted once at unique size
 static double bytes_ns_grad(const MeshView& v, double t) {
-  return (double)v.nc * (4 * t /*q*/ + 12 /*nb*/ + 48 /*lsq*/ + 48 /*rf*/ + 8 /*hmin*/ + 16 * t /*record*/);
+  return (double)v.nc * (4 * t /*q*/ + 12 /*nb*/ + 6 * t /*lsq*/ + 6 * t /*rf*/ + 8 /*hmin*/ + 16 * t /*record*/);
 }
 static double bytes_ns_flux(const MeshView& v, double t) {
-  return (double)v.nc * (16 * t /*record*/ + 12 /*nb*/ + 72 /*nx,ny,len*/ + 48 /*rf*/ + 48 /*dc*/ +
-                         8 /*rarea*/ + 8 /*hmin*/ + 32 /*U old*/ + 32 /*U new*/ + 4 * t /*q new*/);
+  const double face = t == 4.0 ? 32 /*nx,ny,rf,rj,dc in float*/ : 48 /*nx,ny,rf,dc in double*/;
+  return (double)v.nc * (16 * t /*record*/ + 12 /*nb*/ + 3 * (face + 8 /*len*/) + 8 /*rarea*/ + 8 /*hmin*/ +
+                         32 /*U old*/ + 32 /*U new*/ + 4 * t /*q new*/);
 }
 
 int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats) {

@@ -56,6 +56,14 @@ struct MeshView {
   const double* rf;    // [nf][2][nc] face midpoint - own centroid
   const double* dc;    // [nf][2][nc] neighbour centroid - own centroid (periodic safe)
   const int32_t* wall_id; // [nf][nc] index into the wall list or -1
+  // float copies for the FP32-mixed kernels of values that those kernels round
+  // to float at first use (bit-identical results, half the bytes)
+  const float* fnx32;  // [nf][nc]
+  const float* fny32;
+  const float* lsq32;  // [nf][2][nc]
+  const float* rf32;   // [nf][2][nc]
+  const float* rj32;   // [nf][2][nc] face midpoint - neighbour centroid (= rf - dc, rounded once)
+  const float* dc32;   // [nf][2][nc]
   // per node
   const int32_t* cl;   // [ncl][nn] cluster cell ids, -1 padded
   const double* w;     // [ncl][nn] interpolation weights

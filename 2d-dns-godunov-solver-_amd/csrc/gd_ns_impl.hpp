@@ -61,10 +61,13 @@ __device__ __forceinline__ void ld16(const T* p, T* o) {
   ld4(p, o); ld4(p + 4, o + 4); ld4(p + 8, o + 8); ld4(p + 12, o + 12);
 }
 
+// Sutherland's law mu_ref (T/T_ref)^(3/2) (T_ref + S)/(T + S), with
+// x^(3/2) = x sqrt(x)
 template <class T>
 __device__ __forceinline__ T sutherland(T Tt, const gd_params& p) {
   const T Tr = (T)p.T_ref, S = (T)p.S_suth;
-  return (T)p.mu_ref * pow(Tt / Tr, (T)1.5) * (Tr + S) / (Tt + S);
+  const T t = Tt / Tr;
+  return (T)p.mu_ref * (t * sqrt(t)) * (Tr + S) / (Tt + S);
 }
 
 // ghost primitive state of a boundary face (first order)
@@ -169,44 +172,80 @@ __global__ __launch_bounds__(kBlock) void k_ns_prims(int64_t nc, double g, const
   st4(Q + 4 * i, (T)q[0], (T)q[1], (T)q[2], (T)q[3]);
 }
 
+// face geometry at the compute type: FP64 arrays for the FP64 step, the float
+// copies for the FP32-mixed step
+template <class T>
+struct FaceGeo {
+  __device__ __forceinline__ static T nx(const MeshView& m, int64_t s) {
+    if constexpr (sizeof(T) == 4) return m.fnx32[s]; else return m.fnx[s];
+  }
+  __device__ __forceinline__ static T ny(const MeshView& m, int64_t s) {
+    if constexpr (sizeof(T) == 4) return m.fny32[s]; else return m.fny[s];
+  }
+  // face midpoint - own centroid, component c of slot k
+  __device__ __forceinline__ static T rf(const MeshView& m, int k, int c, int64_t i) {
+    if constexpr (sizeof(T) == 4) return m.rf32[(2 * k + c) * m.nc + i]; else return m.rf[(2 * k + c) * m.nc + i];
+  }
+  __device__ __forceinline__ static T lsq(const MeshView& m, int k, int c, int64_t i) {
+    if constexpr (sizeof(T) == 4) return m.lsq32[(2 * k + c) * m.nc + i]; else return m.lsq[(2 * k + c) * m.nc + i];
+  }
+  // neighbour centroid - own centroid
+  __device__ __forceinline__ static T dc(const MeshView& m, int k, int c, int64_t i) {
+    if constexpr (sizeof(T) == 4) return m.dc32[(2 * k + c) * m.nc + i]; else return m.dc[(2 * k + c) * m.nc + i];
+  }
+  // face midpoint - neighbour centroid (rf - dc, formed in FP64 and rounded once)
+  __device__ __forceinline__ static T rj(const MeshView& m, int k, int c, int64_t i) {
+    if constexpr (sizeof(T) == 4) return m.rj32[(2 * k + c) * m.nc + i];
+    else return m.rf[(2 * k + c) * m.nc + i] - m.dc[(2 * k + c) * m.nc + i];
+  }
+};
+
 // ---- pass G: gradients + limiter
 template <class T, bool QUADS>
 __global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
+  constexpr int NF = QUADS ? 4 : 3;
+  using G = FaceGeo<T>;
   const MeshView& m = a.m;
   const int64_t nc = m.nc;
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= nc) return;
   const int nf = QUADS ? m.nv[i] : 3;
-  T qi[4], gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0}, qmx[4], qmn[4];
+  // every load first: neighbour ids, then all neighbour states (boundary and
+  // absent faces read the own cell, overwritten below), then geometry
+  int32_t jn[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) jn[k] = m.nb[k * nc + i];
+  T qi[4], qk[NF][4];
   ld4(a.Q + 4 * i, qi);
 #pragma unroll
-  for (int v = 0; v < 4; ++v) { qmx[v] = qi[v]; qmn[v] = qi[v]; }
-  T rx[4], ry[4];
+  for (int k = 0; k < NF; ++k) ld4(a.Q + 4 * (int64_t)(jn[k] >= 0 ? jn[k] : i), qk[k]);
+  T wx[NF], wy[NF], rx[NF], ry[NF];
 #pragma unroll
-  for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
+  for (int k = 0; k < NF; ++k) {
+    wx[k] = G::lsq(m, k, 0, i); wy[k] = G::lsq(m, k, 1, i);
+    rx[k] = G::rf(m, k, 0, i); ry[k] = G::rf(m, k, 1, i);
+  }
+  const T kh = (T)(a.prm.venkat_K * m.hmin[i]);
+  T gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0}, qmx[4], qmn[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) { qmx[v] = qi[v]; qmn[v] = qi[v]; }
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
     if (k < nf) {
-      const int64_t s = k * nc + i;
-      const int32_t j = m.nb[s];
-      T qk[4];
-      if (j >= 0) {
-        ld4(a.Q + 4 * (int64_t)j, qk);
-      } else {
-        ghost<T>(a.prm, j + 1, qi, (T)m.fnx[s], (T)m.fny[s], qk);
+      if (jn[k] < 0) {
+        const int64_t s = k * nc + i;
+        ghost<T>(a.prm, jn[k] + 1, qi, G::nx(m, s), G::ny(m, s), qk[k]);
       }
-      const T wx = (T)m.lsq[(2 * k) * nc + i], wy = (T)m.lsq[(2 * k + 1) * nc + i];
-      rx[k] = (T)m.rf[(2 * k) * nc + i];
-      ry[k] = (T)m.rf[(2 * k + 1) * nc + i];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const T d = qk[v] - qi[v];
-        gx[v] += wx * d;
-        gy[v] += wy * d;
-        qmx[v] = fmax(qmx[v], qk[v]);
-        qmn[v] = fmin(qmn[v], qk[v]);
+        const T d = qk[k][v] - qi[v];
+        gx[v] += wx[k] * d;
+        gy[v] += wy[k] * d;
+        qmx[v] = fmax(qmx[v], qk[k][v]);
+        qmn[v] = fmin(qmn[v], qk[k][v]);
       }
     }
   }
-  const T kh = (T)(a.prm.venkat_K * m.hmin[i]);
   const T kh3 = kh * kh * kh;
   T ph[4];
 #pragma unroll
@@ -214,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
     const T e2 = kh3 * (T)(a.prm.venkat_scale[v] * a.prm.venkat_scale[v]);
     T phi = 1;
 #pragma unroll
-    for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
+    for (int k = 0; k < NF; ++k) {
       if (k < nf) {
         const T d2 = gx[v] * rx[k] + gy[v] * ry[k];
         if (d2 != 0) {
@@ -246,9 +285,29 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
   return v;
 }
 
+// viscous flux per unit length through a face with inward normal n, from the
+// face gradients of u, v, T (gf[var][x|y]) and face values u, v, T:
+// (0, tau.n, (tau.u).n + kappa grad T.n)
+template <class T>
+__device__ __forceinline__ void viscous_face(const gd_params& P, T cp, T uf, T vf, T Tf, const T (*gf)[2], T nx,
+                                             T ny, T* gv) {
+  const T mu = sutherland<T>(Tf, P);
+  const T kap = mu * cp / (T)P.Pr;
+  const T div = gf[0][0] + gf[1][1];
+  const T txx = mu * (2 * gf[0][0] - (T)(2.0 / 3.0) * div);
+  const T tyy = mu * (2 * gf[1][1] - (T)(2.0 / 3.0) * div);
+  const T txy = mu * (gf[0][1] + gf[1][0]);
+  gv[0] = 0;
+  gv[1] = txx * nx + txy * ny;
+  gv[2] = txy * nx + tyy * ny;
+  gv[3] = uf * gv[1] + vf * gv[2] + kap * (gf[2][0] * nx + gf[2][1] * ny);
+}
+
 // ---- pass F: fluxes, update, primitives, dt inputs
 template <class T, bool QUADS, bool VISC>
 __global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
+  constexpr int NF = QUADS ? 4 : 3;
+  using G = FaceGeo<T>;
   const MeshView& m = a.m;
   const gd_params& P = a.prm;
   const int64_t nc = m.nc;
@@ -266,21 +325,39 @@ __global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
     const T g = (T)P.Gamma, Rg = (T)P.RGAS;
     const T cp = g * Rg / (g - 1);
     const int nf = QUADS ? m.nv[i] : 3;
+    // ---- loads: ids, own record, neighbour records (boundary faces read the
+    // own record, unused), geometry, conservative state
+    int32_t jn[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) jn[k] = m.nb[k * nc + i];
     T ri[16];
     ld16(a.rec + 16 * i, ri);
+    T rn[NF][16];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) ld16(a.rec + 16 * (int64_t)(jn[k] >= 0 ? jn[k] : i), rn[k]);
+    double Uo[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Uo[v] = a.Uo[v * nc + i];
     const T* qi = ri;
     const T* gi = ri + 4;
     const T* phi_i = ri + 12;
+    // own temperature and its gradient (T = p / (rho R))
+    const T rRi = 1 / (qi[0] * Rg);
+    const T Ti = qi[3] * rRi;
+    T gTi[2];
+    if (VISC) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) gTi[c] = (gi[6 + c] - Ti * Rg * gi[c]) * rRi;
+    }
     double F[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
+    for (int k = 0; k < NF; ++k) {
       if (k < nf) {
         const int64_t s = k * nc + i;
-        const int32_t j = m.nb[s];
-        const T nx = (T)m.fnx[s], ny = (T)m.fny[s];
+        const int32_t j = jn[k];
+        const T nx = G::nx(m, s), ny = G::ny(m, s);
         const double len = m.flen[s];
-        const double rxd = m.rf[(2 * k) * nc + i], ryd = m.rf[(2 * k + 1) * nc + i];
-        const T rx = (T)rxd, ry = (T)ryd;
+        const T rx = G::rf(m, k, 0, i), ry = G::rf(m, k, 1, i);
         T qa[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) qa[v] = qi[v] + phi_i[v] * (gi[2 * v] * rx + gi[2 * v + 1] * ry);
@@ -288,24 +365,18 @@ __global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
 #pragma unroll
           for (int v = 0; v < 4; ++v) qa[v] = qi[v];
         }
-        T qb[4], rj[16];
-        const T* qj = rj;
-        const T* gj = rj + 4;
-        double dxd = 0.0, dyd = 0.0;
+        const T* qj = rn[k];
+        const T* gj = rn[k] + 4;
+        T qb[4];
         if (j >= 0) {
-          dxd = m.dc[(2 * k) * nc + i];
-          dyd = m.dc[(2 * k + 1) * nc + i];
-          const T rxj = (T)(rxd - dxd), ryj = (T)(ryd - dyd);
-          ld16(a.rec + 16 * (int64_t)j, rj);
+          const T rxj = G::rj(m, k, 0, i), ryj = G::rj(m, k, 1, i);
 #pragma unroll
-          for (int v = 0; v < 4; ++v) qb[v] = qj[v] + rj[12 + v] * (gj[2 * v] * rxj + gj[2 * v + 1] * ryj);
+          for (int v = 0; v < 4; ++v) qb[v] = qj[v] + rn[k][12 + v] * (gj[2 * v] * rxj + gj[2 * v + 1] * ryj);
           if (!(qb[0] > 0) || !(qb[3] > 0)) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) qb[v] = qj[v];
           }
         } else {
-#pragma unroll
-          for (int c = 0; c < 16; ++c) rj[c] = 0;
           ghost<T>(P, j + 1, qa, nx, ny, qb);
         }
         T e[4];
@@ -314,56 +385,40 @@ __global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
         T gv[4] = {0, 0, 0, 0};
         if (VISC) {
           if (j >= 0) {
-            const T Ti = qi[3] / (qi[0] * Rg), Tj = qj[3] / (qj[0] * Rg);
+            // edge-corrected average of the two cell gradients of u, v, T
+            const T rRj = 1 / (qj[0] * Rg);
+            const T Tj = qj[3] * rRj;
             T g_i[3][2], g_j[3][2];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-              g_i[0][c] = gi[2 + c]; g_i[1][c] = gi[4 + c];
+              g_i[0][c] = gi[2 + c]; g_i[1][c] = gi[4 + c]; g_i[2][c] = gTi[c];
               g_j[0][c] = gj[2 + c]; g_j[1][c] = gj[4 + c];
-              g_i[2][c] = (gi[6 + c] - Ti * Rg * gi[c]) / (qi[0] * Rg);
-              g_j[2][c] = (gj[6 + c] - Tj * Rg * gj[c]) / (qj[0] * Rg);
+              g_j[2][c] = (gj[6 + c] - Tj * Rg * gj[c]) * rRj;
             }
-            const T ddx = (T)dxd, ddy = (T)dyd;
-            const T dl = sqrt(ddx * ddx + ddy * ddy);
-            const T tx = ddx / dl, ty = ddy / dl;
+            const T ddx = G::dc(m, k, 0, i), ddy = G::dc(m, k, 1, i);
+            const T rdl = 1 / sqrt(ddx * ddx + ddy * ddy);
+            const T tx = ddx * rdl, ty = ddy * rdl;
             const T dphi[3] = {qj[1] - qi[1], qj[2] - qi[2], Tj - Ti};
             T gf[3][2];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
               const T ax = (T)0.5 * (g_i[c][0] + g_j[c][0]), ay = (T)0.5 * (g_i[c][1] + g_j[c][1]);
-              const T corr = dphi[c] / dl - (ax * tx + ay * ty);
+              const T corr = dphi[c] * rdl - (ax * tx + ay * ty);
               gf[c][0] = ax + corr * tx; gf[c][1] = ay + corr * ty;
             }
-            const T Tf = (T)0.5 * (Ti + Tj);
-            const T uf = (T)0.5 * (qi[1] + qj[1]), vf = (T)0.5 * (qi[2] + qj[2]);
-            const T mu = sutherland<T>(Tf, P);
-            const T kap = mu * cp / (T)P.Pr;
-            const T div = gf[0][0] + gf[1][1];
-            const T txx = mu * (2 * gf[0][0] - (T)(2.0 / 3.0) * div);
-            const T tyy = mu * (2 * gf[1][1] - (T)(2.0 / 3.0) * div);
-            const T txy = mu * (gf[0][1] + gf[1][0]);
-            gv[1] = txx * nx + txy * ny;
-            gv[2] = txy * nx + tyy * ny;
-            gv[3] = uf * gv[1] + vf * gv[2] + kap * (gf[2][0] * nx + gf[2][1] * ny);
+            viscous_face<T>(P, cp, (T)0.5 * (qi[1] + qj[1]), (T)0.5 * (qi[2] + qj[2]), (T)0.5 * (Ti + Tj), gf, nx,
+                            ny, gv);
           } else {
             const int idx = -(j + 1);
             if (idx <= GD_MAX_BC && P.bc[idx].kind == GD_BC_ISOTHERMAL_WALL) {
+              // one-sided normal gradient to the wall (u = 0, T = T_w); the
+              // outward normal of the fluid is -n
               const T Tw = (T)P.bc[idx].data[0];
-              const T Ti = qi[3] / (qi[0] * Rg);
-              const T dperp = -(rx * nx + ry * ny);
-              const T dudn = (0 - qi[1]) / dperp, dvdn = (0 - qi[2]) / dperp, dTdn = (Tw - Ti) / dperp;
+              const T rdp = -1 / (rx * nx + ry * ny);
+              const T dudn = (0 - qi[1]) * rdp, dvdn = (0 - qi[2]) * rdp, dTdn = (Tw - Ti) * rdp;
               const T ox = -nx, oy = -ny;
-              const T gu0 = dudn * ox, gu1 = dudn * oy, gv0 = dvdn * ox, gv1 = dvdn * oy;
-              const T gT0 = dTdn * ox, gT1 = dTdn * oy;
-              const T mu = sutherland<T>(Tw, P);
-              const T kap = mu * cp / (T)P.Pr;
-              const T div = gu0 + gv1;
-              const T txx = mu * (2 * gu0 - (T)(2.0 / 3.0) * div);
-              const T tyy = mu * (2 * gv1 - (T)(2.0 / 3.0) * div);
-              const T txy = mu * (gu1 + gv0);
-              gv[1] = txx * nx + txy * ny;
-              gv[2] = txy * nx + tyy * ny;
-              gv[3] = kap * (gT0 * nx + gT1 * ny);
+              const T gf[3][2] = {{dudn * ox, dudn * oy}, {dvdn * ox, dvdn * oy}, {dTdn * ox, dTdn * oy}};
+              viscous_face<T>(P, cp, (T)0, (T)0, Tw, gf, nx, ny, gv);
             }
           }
         }
@@ -382,20 +437,21 @@ __global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
     double Un[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      Un[v] = a.Uo[v * nc + i] - F[v] * dt * ra;
+      Un[v] = Uo[v] - F[v] * dt * ra;
       a.Un[v * nc + i] = Un[v];
     }
-    const double G = P.Gamma;
+    const double Gm = P.Gamma;
     const double rho = Un[0], ux = Un[1] / rho, uy = Un[2] / rho;
-    const double p = (G - 1.0) * (Un[3] - 0.5 * (Un[1] * Un[1] + Un[2] * Un[2]) / rho);
+    const double p = (Gm - 1.0) * (Un[3] - 0.5 * (Un[1] * Un[1] + Un[2] * Un[2]) / rho);
     st4(a.Qw + 4 * i, (T)rho, (T)ux, (T)uy, (T)p);
-    const double as = sqrt(G * p / rho);
+    const double as = sqrt(Gm * p / rho);
     const double vm = sqrt(ux * ux + uy * uy) + as;
     double lam = vm;
     if (VISC) {
       const double Tc = p / (rho * P.RGAS);
-      const double mu = P.mu_ref * pow(Tc / P.T_ref, 1.5) * (P.T_ref + P.S_suth) / (Tc + P.S_suth);
-      lam = vm + 2.0 * G * mu / (P.Pr * rho * m.hmin[i]);
+      const double t = Tc / P.T_ref;
+      const double mu = P.mu_ref * (t * sqrt(t)) * (P.T_ref + P.S_suth) / (Tc + P.S_suth);
+      lam = vm + 2.0 * Gm * mu / (P.Pr * rho * m.hmin[i]);
     }
     vmx = vm;
     hdt = m.hmin[i] / lam;

@@ -374,7 +374,8 @@ __global__ __launch_bounds__(kBlock) void k_prime_acc(MeshView m, gd_params p, c
     double lam = vm;
     if (viscous_ns) {
       const double T = pr / (rho * p.RGAS);
-      const double mu = p.mu_ref * pow(T / p.T_ref, 1.5) * (p.T_ref + p.S_suth) / (T + p.S_suth);
+      const double t = T / p.T_ref;
+      const double mu = p.mu_ref * (t * sqrt(t)) * (p.T_ref + p.S_suth) / (T + p.S_suth);
       lam = vm + 2.0 * p.Gamma * mu / (p.Pr * rho * m.hmin[i]);
     }
     hdt = m.hmin[i] / lam;

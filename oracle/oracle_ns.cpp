@@ -127,10 +127,12 @@ inline void prims(const double* U, int64_t n, int64_t i, double g, double* q) {
   q[3] = (g - 1.0) * (E - 0.5 * (mx * mx + my * my) / r);
 }
 
+// Sutherland: mu_ref (T/T_ref)^(3/2) (T_ref + S)/(T + S), x^(3/2) = x sqrt(x)
 template <class R>
 inline R sutherland(R T, const orc_ns_params* p) {
   const R Tr = (R)p->T_ref, S = (R)p->S;
-  return (R)p->mu_ref * std::pow(T / Tr, (R)1.5) * (Tr + S) / (T + S);
+  const R t = T / Tr;
+  return (R)p->mu_ref * (t * std::sqrt(t)) * (Tr + S) / (T + S);
 }
 
 // ghost state of a boundary face for the ns scheme
@@ -210,6 +212,22 @@ inline void roe(const R* L, const R* Rr, R nx, R ny, R g, R* out) {
   out[1] = -(R)0.5 * (rL * unL * uL + pL * nx + rR * unR * uR + pR * nx - d2);
   out[2] = -(R)0.5 * (rL * unL * vL + pL * ny + rR * unR * vR + pR * ny - d3);
   out[3] = -(R)0.5 * (rL * unL * HL + rR * unR * HR - d4);
+}
+
+// viscous flux per unit length through a face with inward normal n:
+// (0, tau.n, (tau.u).n + kappa grad T.n), kappa = mu cp / Pr
+template <class R>
+inline void visc_face(const orc_ns_params* p, R cp, R uf, R vf, R Tf, const R (*gf)[2], R nx, R ny, R* gv) {
+  const R mu = sutherland<R>(Tf, p);
+  const R kap = mu * cp / (R)p->Pr;
+  const R div = gf[0][0] + gf[1][1];
+  const R txx = mu * (2 * gf[0][0] - (R)(2.0 / 3.0) * div);
+  const R tyy = mu * (2 * gf[1][1] - (R)(2.0 / 3.0) * div);
+  const R txy = mu * (gf[0][1] + gf[1][0]);
+  gv[0] = 0;
+  gv[1] = txx * nx + txy * ny;
+  gv[2] = txy * nx + tyy * ny;
+  gv[3] = uf * gv[1] + vf * gv[2] + kap * (gf[2][0] * nx + gf[2][1] * ny);
 }
 
 template <class R>
@@ -298,56 +316,39 @@ int ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* 
         const int idx = j >= 0 ? 0 : -c.code[k];
         const int kind = (j < 0 && idx <= ORC_MAX_BC) ? p->bc[idx].kind : -1;
         if (j >= 0) {
-          // edge-corrected average of the two cell gradients
+          // edge-corrected average of the two cell gradients of u, v, T
           R qj[4];
           for (int v = 0; v < 4; ++v) qj[v] = q[v * n + j];
-          const R Ti = qi[3] / (qi[0] * RR), Tj = qj[3] / (qj[0] * RR);
-          R g_i[3][2], g_j[3][2];  // u, v, T
+          const R rRi = 1 / (qi[0] * RR), rRj = 1 / (qj[0] * RR);
+          const R Ti = qi[3] * rRi, Tj = qj[3] * rRj;
+          R g_i[3][2], g_j[3][2];
           for (int s = 0; s < 2; ++s) {
             g_i[0][s] = gr[(2 + s) * n + i]; g_i[1][s] = gr[(4 + s) * n + i];
             g_j[0][s] = gr[(2 + s) * n + j]; g_j[1][s] = gr[(4 + s) * n + j];
-            g_i[2][s] = (gr[(6 + s) * n + i] - Ti * RR * gr[s * n + i]) / (qi[0] * RR);
-            g_j[2][s] = (gr[(6 + s) * n + j] - Tj * RR * gr[s * n + j]) / (qj[0] * RR);
+            g_i[2][s] = (gr[(6 + s) * n + i] - Ti * RR * gr[s * n + i]) * rRi;
+            g_j[2][s] = (gr[(6 + s) * n + j] - Tj * RR * gr[s * n + j]) * rRj;
           }
           const R ddx = (R)c.dx[k], ddy = (R)c.dy[k];
-          const R dl = std::sqrt(ddx * ddx + ddy * ddy);
-          const R tx = ddx / dl, ty = ddy / dl;
+          const R rdl = 1 / std::sqrt(ddx * ddx + ddy * ddy);
+          const R tx = ddx * rdl, ty = ddy * rdl;
           const R dphi[3] = {qj[1] - qi[1], qj[2] - qi[2], Tj - Ti};
           R gf[3][2];
           for (int s = 0; s < 3; ++s) {
             const R ax = (R)0.5 * (g_i[s][0] + g_j[s][0]), ay = (R)0.5 * (g_i[s][1] + g_j[s][1]);
-            const R corr = dphi[s] / dl - (ax * tx + ay * ty);
+            const R corr = dphi[s] * rdl - (ax * tx + ay * ty);
             gf[s][0] = ax + corr * tx; gf[s][1] = ay + corr * ty;
           }
-          const R Tf = (R)0.5 * (Ti + Tj);
-          const R uf = (R)0.5 * (qi[1] + qj[1]), vf = (R)0.5 * (qi[2] + qj[2]);
-          const R mu = sutherland<R>(Tf, p);
-          const R kap = mu * cp / (R)p->Pr;
-          const R div = gf[0][0] + gf[1][1];
-          const R txx = mu * (2 * gf[0][0] - (R)(2.0 / 3.0) * div);
-          const R tyy = mu * (2 * gf[1][1] - (R)(2.0 / 3.0) * div);
-          const R txy = mu * (gf[0][1] + gf[1][0]);
-          gv[1] = txx * nx + txy * ny;
-          gv[2] = txy * nx + tyy * ny;
-          gv[3] = uf * gv[1] + vf * gv[2] + kap * (gf[2][0] * nx + gf[2][1] * ny);
+          visc_face<R>(p, cp, (R)0.5 * (qi[1] + qj[1]), (R)0.5 * (qi[2] + qj[2]), (R)0.5 * (Ti + Tj), gf, nx, ny, gv);
         } else if (kind == ORC_BC_ISOTHERMAL_WALL) {
-          // one-sided normal gradient to the wall (u = 0, T = T_w)
+          // one-sided normal gradient to the wall (u = 0, T = T_w); the fluid's
+          // outward normal is -n
           const R Tw = (R)p->bc[idx].data[0];
-          const R Ti = qi[3] / (qi[0] * RR);
-          const R dperp = -((R)c.rx[k] * nx + (R)c.ry[k] * ny);
-          // outward normal is -n: d(phi)/dn_out = (phi_w - phi_i)/dperp
-          const R dudn = (0 - qi[1]) / dperp, dvdn = (0 - qi[2]) / dperp, dTdn = (Tw - Ti) / dperp;
+          const R Ti = qi[3] * (1 / (qi[0] * RR));
+          const R rdp = -1 / ((R)c.rx[k] * nx + (R)c.ry[k] * ny);
+          const R dudn = (0 - qi[1]) * rdp, dvdn = (0 - qi[2]) * rdp, dTdn = (Tw - Ti) * rdp;
           const R ox = -nx, oy = -ny;
-          const R gu[2] = {dudn * ox, dudn * oy}, gvv[2] = {dvdn * ox, dvdn * oy}, gT[2] = {dTdn * ox, dTdn * oy};
-          const R mu = sutherland<R>(Tw, p);
-          const R kap = mu * cp / (R)p->Pr;
-          const R div = gu[0] + gvv[1];
-          const R txx = mu * (2 * gu[0] - (R)(2.0 / 3.0) * div);
-          const R tyy = mu * (2 * gvv[1] - (R)(2.0 / 3.0) * div);
-          const R txy = mu * (gu[1] + gvv[0]);
-          gv[1] = txx * nx + txy * ny;
-          gv[2] = txy * nx + tyy * ny;
-          gv[3] = kap * (gT[0] * nx + gT[1] * ny);
+          const R gf[3][2] = {{dudn * ox, dudn * oy}, {dvdn * ox, dvdn * oy}, {dTdn * ox, dTdn * oy}};
+          visc_face<R>(p, cp, (R)0, (R)0, Tw, gf, nx, ny, gv);
         }
       }
       double fk[4];

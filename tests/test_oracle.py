@@ -1,0 +1,421 @@
+"""Pins the CPU oracle with facts fixed by the mathematics, not by its own code.
+
+Each check is chosen so that a plausible restatement mistake (a dropped
+term, a swapped L/R, a wrong sign or index, the wrong limiter branch) fails:
+hand-computed values, flux consistency, upwinding and symmetry identities,
+Rankine-Hugoniot jumps, free-stream preservation, conservation on periodic
+meshes, exactness on linear fields and second-order convergence.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import pkg
+from helpers import SEED, random_fields, tgv_fields
+
+G = 1.4
+
+
+def M():
+    return pkg().mesh
+
+
+def exact_flux(q, nx, ny, g=G):
+    r, u, v, p = q
+    un = u * nx + v * ny
+    ht = g / (g - 1) * p / r + 0.5 * (u * u + v * v)
+    return np.array([r * un, r * un * u + p * nx, r * un * v + p * ny, r * un * ht])
+
+
+def cons(q, g=G):
+    r, u, v, p = q
+    return np.array([r, r * u, r * v, p / (g - 1) + 0.5 * r * (u * u + v * v)])
+
+
+# ------------------------------------------------------------------ limiters
+
+def test_harmonic_minmod_limiter_values():
+    L = O.LIM_REF_HARMONIC
+    assert O.limiter(L, 1.0, 3.0, 0.0) == pytest.approx(1.5)          # 2xy/(x+y)
+    assert O.limiter(L, -2.0, -6.0, 0.0) == pytest.approx(-3.0)
+    assert O.limiter(L, 1.0, -1.0, 1e-12) == 0.0                      # opposite signs
+    assert O.limiter(L, 0.0, 5.0, 1e-12) == 0.0
+    assert O.limiter(L, 0.7, 0.7, 0.0) == pytest.approx(0.7)          # lim(x,x) = x
+    x, c = 0.3, 0.05
+    assert O.limiter(L, x, x, c) == pytest.approx(2 * x * x / (2 * x + c))
+    rng = np.random.default_rng(SEED)
+    for a, b in rng.normal(size=(50, 2)):
+        assert O.limiter(L, a, b, 1e-12) == pytest.approx(O.limiter(L, b, a, 1e-12), rel=1e-15)
+        assert O.limiter(L, -a, -b, 1e-12) == pytest.approx(-O.limiter(L, a, b, 1e-12), rel=1e-15)
+        # a harmonic mean lies between 0 and 2 min(|a|,|b|)
+        assert abs(O.limiter(L, a, b, 0.0)) <= 2 * min(abs(a), abs(b)) + 1e-15
+
+
+def test_other_reference_limiters():
+    assert O.limiter(O.LIM_MINMOD, 1.0, 3.0, 0.0) == 1.0
+    assert O.limiter(O.LIM_MINMOD, -1.0, 3.0, 0.0) == 0.0
+    assert O.limiter(O.LIM_MINMOD, -4.0, -2.5, 0.0) == -2.5
+    assert O.limiter(O.LIM_VAN_LEER_A, 1.0, 3.0, 0.0) == pytest.approx(1.5)  # (ab+|ab|)/(a+b)
+    assert O.limiter(O.LIM_VAN_LEER_A, 1.0, -3.0, 0.0) == 0.0
+    assert O.limiter(O.LIM_VAN_ALBADA_A, 2.0, 2.0, 0.0) == pytest.approx(2.0)
+    assert O.limiter(O.LIM_VAN_ALBADA_A, 1.0, 0.0, 0.0) == pytest.approx(0.0)  # a b^2 + b a^2 = 0
+
+
+# ------------------------------------------------------------------ AUSM+
+
+STATES = [
+    (1.2, 0.3, -0.1, 1.5),     # subsonic
+    (0.9, 2.5, 0.4, 0.7),      # supersonic along x
+    (1.1766, 1232.6445, 0.0, 101325.0),  # the reference's inflow, M = 3.55
+    (1.0, -0.2, 0.05, 1.0 / 1.4),
+]
+
+
+@pytest.mark.parametrize("q", STATES)
+@pytest.mark.parametrize("n", [(1.0, 0.0), (0.6, 0.8), (-0.28, 0.96)])
+def test_ausm_plus_consistency(q, n):
+    """F(U, U) is the exact Euler flux: this needs M+ + M- = M and P+ + P- = 1."""
+    F = O.ausm_plus_flux(np.array(q), np.array(q), n[0], n[1], 2.5, G)
+    assert np.allclose(F, -2.5 * exact_flux(q, *n), rtol=1e-13, atol=1e-13 * abs(q[3]))
+
+
+def test_ausm_plus_supersonic_upwinding():
+    """Both sides supersonic along n: the flux is the upstream ("L") flux."""
+    qL = np.array([1.0, 3.0, 0.2, 0.8])
+    qR = np.array([1.3, 2.6, -0.1, 1.1])
+    F = O.ausm_plus_flux(qL, qR, 1.0, 0.0, 1.0, G)
+    assert np.allclose(F, -exact_flux(qL, 1.0, 0.0), rtol=1e-14)
+    F = O.ausm_plus_flux(qR * np.array([1, -1, 1, 1]), qL * np.array([1, -1, 1, 1]), 1.0, 0.0, 1.0, G)
+    assert np.allclose(F, -exact_flux(qL * np.array([1, -1, 1, 1]), 1.0, 0.0), rtol=1e-14)
+
+
+def test_ausm_plus_reflective_wall_zero_mass_and_energy_flux():
+    """The slip-wall ghost (bcInviscidWall.jl) mirrors the normal velocity:
+    AUSM+ then carries no mass or energy through the face, only pressure."""
+    nx, ny = 0.6, -0.8
+    q = np.array([1.1, 0.4, 0.3, 0.9])
+    un = q[1] * nx + q[2] * ny
+    g = q.copy()
+    g[1] = q[1] * (ny * ny - nx * nx) - 2 * nx * ny * q[2]
+    g[2] = q[2] * (nx * nx - ny * ny) - 2 * nx * ny * q[1]
+    assert g[1] * nx + g[2] * ny == pytest.approx(-un)
+    F = O.ausm_plus_flux(g, q, nx, ny, 1.0, G)
+    assert abs(F[0]) < 1e-15 and abs(F[3]) < 1e-14
+    # the momentum flux is normal to the wall
+    assert abs(F[1] * ny - F[2] * nx) < 1e-14
+
+
+def test_ausm_plus_antisymmetry_and_continuity():
+    rng = np.random.default_rng(SEED)
+    for _ in range(20):
+        a = np.array([rng.uniform(0.5, 2), rng.normal(), rng.normal(), rng.uniform(0.5, 2)])
+        b = np.array([rng.uniform(0.5, 2), rng.normal(), rng.normal(), rng.uniform(0.5, 2)])
+        t = rng.uniform(0, 2 * math.pi)
+        n = (math.cos(t), math.sin(t))
+        assert np.allclose(O.ausm_plus_flux(a, b, *n), -O.ausm_plus_flux(b, a, -n[0], -n[1]), rtol=1e-13,
+                           atol=1e-14)
+    # C1 across |M| = 1: the splitting polynomials meet the supersonic branches
+    for eps in (1e-7,):
+        base = np.array([1.0, 0.0, 0.0, 1.0 / G])
+        lo = base.copy(); lo[1] = 1.0 - eps
+        hi = base.copy(); hi[1] = 1.0 + eps
+        d = O.ausm_plus_flux(lo, lo, 1.0, 0.0) - O.ausm_plus_flux(hi, hi, 1.0, 0.0)
+        assert np.max(np.abs(d)) < 1e-5
+
+
+@pytest.mark.parametrize("q", STATES)
+def test_ausm_plus_m2020_consistency(q):
+    n = (0.6, 0.8)
+    F = O.ausm_plus_m2020_flux(np.array(q), np.array(q), n[0], n[1], 1.0, G)
+    assert np.allclose(F, -exact_flux(q, *n), rtol=1e-12, atol=1e-12 * abs(q[3]))
+
+
+# ------------------------------------------------------------------ Roe
+
+@pytest.mark.parametrize("q", STATES)
+def test_roe_consistency(q):
+    n = (-0.28, 0.96)
+    F = O.roe_flux(np.array(q), np.array(q), n[0], n[1], 1.7, G)
+    assert np.allclose(F, -1.7 * exact_flux(q, *n), rtol=1e-13, atol=1e-13 * abs(q[3]))
+
+
+def oblique_shock_pair(m1=2.9, beta_deg=29.0, g=G):
+    """Yee's reflected-shock inflow (testFluxes2d.jl:10-24 uses these states
+    rounded): upstream rho=1, p=1/gamma*... with a=100, u=290, and the exact
+    downstream state of a stationary shock at angle beta."""
+    rho1, p1 = 1.0, 7142.857142857143          # a1 = sqrt(g p1/rho1) = 100
+    a1 = math.sqrt(g * p1 / rho1)
+    u1 = m1 * a1
+    beta = math.radians(beta_deg)
+    n = (math.sin(beta), -math.cos(beta))      # shock normal, from upstream to downstream
+    t = (math.cos(beta), math.sin(beta))
+    un1 = u1 * n[0]
+    ut1 = u1 * t[0]
+    mn = un1 / a1
+    r = (g + 1) * mn * mn / ((g - 1) * mn * mn + 2)
+    p2 = p1 * (1 + 2 * g / (g + 1) * (mn * mn - 1))
+    rho2 = rho1 * r
+    un2 = un1 / r
+    u2 = un2 * n[0] + ut1 * t[0]
+    v2 = un2 * n[1] + ut1 * t[1]
+    return np.array([rho1, u1, 0.0, p1]), np.array([rho2, u2, v2, p2]), n
+
+
+def test_oblique_shock_pair_is_rankine_hugoniot():
+    qL, qR, n = oblique_shock_pair()
+    assert np.allclose(exact_flux(qL, *n), exact_flux(qR, *n), rtol=1e-13)
+    # the reference's rounded downstream state is close to the exact one
+    assert qR[0] == pytest.approx(1.7, abs=2e-3) and qR[3] == pytest.approx(15282, rel=1e-3)
+
+
+def test_roe_stationary_shock_with_harten_fix():
+    """For a stationary RH jump only the wave with lambda = 0 carries the jump,
+    so sum |lambda_k| alpha_k r_k = |lambda_s| (U_R - U_L).  Harten's fix with
+    width 0.2 (RoeFlux2dFast.jl:85-96) replaces |lambda_s| = 0 by 0.1:
+    F = -(F(U_L) - 0.05 (U_R - U_L)) * side."""
+    qL, qR, n = oblique_shock_pair()
+    F = O.roe_flux(qL, qR, n[0], n[1], 1.0, G)
+    expect = -(exact_flux(qL, *n) - 0.5 * 0.1 * (cons(qR) - cons(qL)))
+    assert np.allclose(F, expect, rtol=1e-11, atol=1e-9)
+
+
+# ------------------------------------------------------------------ thermo
+
+def test_sutherland_reference_values():
+    # 1.4792e-6 sqrt(T) / (1 + 116/T), hand evaluated
+    assert O.sutherland_viscosity(300.0) == pytest.approx(1.8476319e-5, rel=1e-7)
+    assert O.sutherland_viscosity(273.15) == pytest.approx(1.7159778e-5, rel=1e-7)
+
+
+def test_air_conductivity_structure_and_branches():
+    """k = (15/4)(R/M) mu (4/15 cp M/R + 1/3) = mu cp + (5/4)(R/M) mu with the
+    power-law mu; cp is frozen below 200 K; T = 200 exactly takes the last branch."""
+    R_M = 8.314462 / 28.9647
+
+    def mu(T):
+        return 1.716e-5 * (T / 273.11) ** (2.0 / 3.0)
+    ratios = [O.air_conductivity(T) / mu(T) - 1.25 * R_M for T in (50.0, 120.0, 199.0)]
+    assert np.ptp(ratios) < 1e-9 * abs(ratios[0])
+    cp300 = O.air_conductivity(300.0) / mu(300.0) - 1.25 * R_M
+    assert 995.0 < cp300 < 1015.0                     # air, J/(kg K)
+    cp1500 = O.air_conductivity(1500.0) / mu(1500.0) - 1.25 * R_M
+    assert 1180.0 < cp1500 < 1230.0
+    k_lo = O.air_conductivity(199.999999) / mu(199.999999)
+    k_at = O.air_conductivity(200.0) / mu(200.0)
+    assert abs(k_at - k_lo) > 1.0                      # the source's T == 200 gap
+
+
+# ------------------------------------------------------------------ partition
+
+def test_distribute_cells_in_threads():
+    t = O.threads_table(4, 10)
+    assert t.tolist() == [[1, 4], [5, 6], [7, 8], [9, 10]]
+    assert O.threads_table(1, 10).tolist() == [[0, 0]]
+    assert M().distibuteCellsInThreadsSA(4, 10).tolist() == t.tolist()
+    t = O.threads_table(3, 20000)
+    assert t[0, 0] == 1 and t[-1, 1] == 20000 and all(t[k + 1, 0] == t[k, 1] + 1 for k in range(2))
+
+
+# ------------------------------------------------------------------ the step
+
+def _setup(mesh, f, g=G):
+    mv = O.MeshView(mesh)
+    U = O.phs2dcns(f, g, mesh.nCells)
+    O.cells2nodes(mv, f, U)
+    return mv, U
+
+
+def test_free_stream_preserved_periodic_and_boundaries():
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+    for mesh, bcs in ((M().make_tgv_periodic_mesh(20), None),
+                      (M().make_box_mesh(9, 7, 1.0, 1.0, quads="mixed"),
+                       {-1: (O.BC_FIXED, (1.2, 0.3, -0.2, 2.0)), -2: (O.BC_EXTRAPOLATE, ()),
+                        -3: (O.BC_FIXED, (1.2, 0.3, -0.2, 2.0)), -4: (O.BC_EXTRAPOLATE, ())})):
+        f = M().fields2d.zeros(mesh.nCells, mesh.nNodes)
+        f.densityCells[:] = 1.2; f.UxCells[:] = 0.3; f.UyCells[:] = -0.2; f.pressureCells[:] = 2.0
+        mv, U = _setup(mesh, f)
+        for flux in (O.FLUX_AUSM_PLUS, O.FLUX_ROE, O.FLUX_AUSM_PLUS_M2020):
+            prm = O.params(flux=flux, bcs=bcs)
+            _, UN = O.calc_one_stage_muscl_cpu_fast(mv, f, th, prm, U, None, 1.0, 1e-3, 0.0)
+            assert np.max(np.abs(UN - U)) < 1e-14
+
+
+def test_conservation_periodic_inviscid_and_viscous():
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+    mesh = M().make_tgv_periodic_mesh(32)
+    f = tgv_fields(mesh, noise=1e-2)
+    mv, U = _setup(mesh, f)
+    A = mesh.cell_areas
+    prm = O.params(energy_law=O.ENERGY_CONST, mu_const=0.05, k_const=1.4)
+    Ud, _ = O.calc_diff_term(mv, f, th, prm, U)
+    for ud in (None, Ud):
+        _, UN = O.calc_one_stage_muscl_cpu_fast(mv, f, th, prm, U, ud, 1.0, 2e-3, 0.0)
+        tot0 = (A[:, None] * U).sum(0)
+        tot1 = (A[:, None] * UN).sum(0)
+        assert np.all(np.abs(tot1 - tot0) <= 1e-13 * np.abs(U).max() * A.sum())
+    # the Laplacian term itself integrates to zero for rho*u, rho*v (and E with constant k)
+    assert np.all(np.abs((A[:, None] * Ud[:, 1:]).sum(0)) < 1e-10)
+
+
+@pytest.mark.parametrize("quads", ["none", "all", "mixed"])
+def test_slip_walls_conserve_mass_and_energy(quads):
+    """A closed box of slip walls: the wall faces carry no mass or energy and
+    interior faces cancel pairwise, since each side's face state uses the
+    other side's opposite vertex.  At a triangle/quad interface the two sides
+    use different stencils (one node vs the mean of two, computeslope2d.jl:48-80),
+    so the reference scheme is not conservative there; that is pinned too."""
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+    mesh = M().make_box_mesh(12, 10, 1.0, 1.0, quads=quads)
+    f = random_fields(mesh)
+    mv, U = _setup(mesh, f)
+    prm = O.params(bcs={c: (O.BC_SLIP_WALL, ()) for c in (-1, -2, -3, -4)})
+    _, UN = O.calc_one_stage_muscl_cpu_fast(mv, f, th, prm, U, None, 1.0, 1e-3, 0.0)
+    A = mesh.cell_areas
+    for q in (0, 3):
+        d = abs((A * (UN[:, q] - U[:, q])).sum()) / (A * U[:, q]).sum()
+        if quads == "mixed":
+            assert d > 1e-9
+        else:
+            assert d < 1e-14
+
+
+def test_green_gauss_exact_for_linear_fields_with_axis_swap():
+    """Interior cells of the SW-NE split: node cluster means are exact for
+    linear fields, so GG is exact; the source's normal swap makes "gradX" the
+    y-derivative (calcGradients.jl:116-129)."""
+    mesh = M().make_box_mesh(12, 12, 1.0, 1.0)
+    mv = O.MeshView(mesh)
+    x, y = mesh.cell_mid_points[:, 0], mesh.cell_mid_points[:, 1]
+    gx, gy = O.green_gauss_gradient(mv, 2.0 * x + 3.0 * y - 1.0)
+    nodes = mesh.cells2nodes[:, :6] - 1
+    cnt = (mesh.cell_clusters > 0).sum(1)
+    interior = (cnt[nodes] == 6).all(1)
+    assert interior.sum() > 50
+    assert np.allclose(gx[interior], 3.0, atol=1e-12)
+    assert np.allclose(gy[interior], 2.0, atol=1e-12)
+
+
+def test_diff_term_laplacian_converges():
+    """calcDiffTerm's Laplacian of rho*u = sin x cos y (rho = 1) tends to
+    -2 sin x cos y on the periodic square; the two node-averaged Green-Gauss
+    levels are first-order accurate in the max norm on this mesh (the error
+    ratio per halving of h falls from 3.0 towards 2: 0.219, 0.072, 0.026, 0.011)."""
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+    errs = []
+    for n in (16, 32, 64):
+        mesh = M().make_tgv_periodic_mesh(n)
+        f = tgv_fields(mesh, noise=0.0)
+        f.densityCells[:] = 1.0
+        mv, U = _setup(mesh, f)
+        _, lap = O.calc_diff_term(mv, f, th, O.params(energy_law=O.ENERGY_CONST, mu_const=1.0, k_const=1.0), U)
+        x, y = mesh.cell_mid_points[:, 0], mesh.cell_mid_points[:, 1]
+        errs.append(np.max(np.abs(lap[:, 0] + 2 * np.sin(x) * np.cos(y))))
+    assert errs[0] / errs[1] > 2.5 and errs[1] / errs[2] > 2.5
+    assert errs[0] == pytest.approx(0.21899, rel=1e-4)
+
+
+def test_node_interpolation_constant_and_convex():
+    mesh = M().make_box_mesh(8, 6, 1.0, 1.0, quads="mixed")
+    f = random_fields(mesh)
+    mv = O.MeshView(mesh)
+    O.cells2nodes(mv, f)
+    cl = mesh.cell_clusters
+    for name_c, name_n in (("densityCells", "densityNodes"), ("pressureCells", "pressureNodes")):
+        c = getattr(f, name_c); nd = getattr(f, name_n)
+        for J in range(mesh.nNodes):
+            ids = cl[J][cl[J] > 0] - 1
+            assert c[ids].min() - 1e-15 <= nd[J] <= c[ids].max() + 1e-15
+    f2 = M().fields2d.zeros(mesh.nCells, mesh.nNodes)
+    f2.densityCells[:] = 1.3
+    O.cells2nodes(mv, f2)
+    assert np.allclose(f2.densityNodes, 1.3, rtol=0, atol=1e-15)
+
+
+def test_update_variables_round_trip():
+    mesh = M().make_box_mesh(5, 4)
+    f = random_fields(mesh)
+    U = O.phs2dcns(f, G, mesh.nCells)
+    Uold = np.zeros_like(U)
+    g = f.copy()
+    D = O.update_variables(g, G, U, Uold)
+    assert np.array_equal(Uold, U) and np.array_equal(D, U)
+    for a, b in (("densityCells",) * 2, ("UxCells",) * 2, ("UyCells",) * 2, ("pressureCells",) * 2):
+        assert np.allclose(getattr(g, a), getattr(f, b), rtol=1e-14)
+    assert np.allclose(g.aSoundCells, np.sqrt(G * f.pressureCells / f.densityCells), rtol=1e-15)
+
+
+# ------------------------------------------------------------------ ns scheme
+
+NS_GAS = dict(R=1.0, mu_ref=2e-3, T_ref=1.0, S=0.4, Pr=0.72, venkat_K=3.0)
+
+
+def test_ns_lsq_exact_for_linear_fields():
+    mesh = M().make_box_mesh(10, 9, 1.0, 1.0, quads="mixed")
+    mv = O.MeshView(mesh)
+    x, y = mesh.cell_mid_points[:, 0], mesh.cell_mid_points[:, 1]
+    rho = 1 + 0.1 * x + 0.2 * y; u = 0.3 - 0.1 * x + 0.05 * y; v = 0.1 * x; p = 1 + 0.2 * x - 0.1 * y
+    U = np.zeros((mesh.nCells, 4), order="F")
+    U[:, 0] = rho; U[:, 1] = rho * u; U[:, 2] = rho * v; U[:, 3] = p / 0.4 + 0.5 * rho * (u * u + v * v)
+    prm = O.ns_params(bcs={c: (O.BC_EXTRAPOLATE, ()) for c in (-1, -2, -3, -4)}, **NS_GAS)
+    _, g8, p4, _ = O.ns_step(mv, prm, 1e-4, U, want_grad=True)
+    nv = mesh.mesh_connectivity[:, 2]
+    interior = np.array([(mesh.cell_stiffness[i, :nv[i]] > 0).all() for i in range(mesh.nCells)])
+    want = [0.1, 0.2, -0.1, 0.05, 0.1, 0.0, 0.2, -0.1]
+    for c in range(8):
+        assert np.allclose(g8[interior, c], want[c], atol=1e-12)
+    assert np.all((p4 >= 0) & (p4 <= 1))
+
+
+def test_ns_venkatakrishnan_bounds_and_limits():
+    mesh = M().make_tgv_periodic_mesh(16)
+    mv = O.MeshView(mesh)
+    f = random_fields(mesh)
+    U = O.phs2dcns(f, G, mesh.nCells)
+    for K in (0.3, 3.0):
+        _, _, p4, _ = O.ns_step(mv, O.ns_params(venkat_K=K, **{k: v for k, v in NS_GAS.items() if k != "venkat_K"}),
+                                1e-4, U, want_grad=True)
+        assert np.all((p4 >= 0) & (p4 <= 1))
+    # smooth field: phi -> 1 as K grows (eps dominates)
+    f = tgv_fields(mesh, noise=0.0)
+    U = O.phs2dcns(f, G, mesh.nCells)
+    _, _, p_big, _ = O.ns_step(mv, O.ns_params(venkat_K=1e4, **{k: v for k, v in NS_GAS.items() if k != "venkat_K"}),
+                               1e-4, U, want_grad=True)
+    assert p_big.min() > 0.999
+
+
+def test_ns_free_stream_and_conservation_periodic():
+    mesh = M().make_tgv_periodic_mesh(20)
+    mv = O.MeshView(mesh)
+    n = mesh.nCells
+    U = np.zeros((n, 4), order="F")
+    U[:, 0] = 1.2; U[:, 1] = 0.36; U[:, 2] = -0.24; U[:, 3] = 5.078
+    for flux in (O.FLUX_AUSM_PLUS, O.FLUX_ROE):
+        UN, _, _, _ = O.ns_step(mv, O.ns_params(flux=flux, **NS_GAS), 1e-3, U)
+        assert np.max(np.abs(UN - U)) < 1e-14
+    f = tgv_fields(mesh, noise=1e-2)
+    U = O.phs2dcns(f, G, n)
+    A = mesh.cell_areas
+    for flux in (O.FLUX_AUSM_PLUS, O.FLUX_ROE):
+        UN, _, _, _ = O.ns_step(mv, O.ns_params(flux=flux, **NS_GAS), 1e-3, U)
+        d = (A[:, None] * (UN - U)).sum(0)
+        assert np.all(np.abs(d) <= 1e-12 * (A[:, None] * np.abs(U)).sum(0))
+
+
+def test_ns_isothermal_wall_at_rest():
+    """Gas at rest at the wall temperature: no heat or momentum flux through
+    the wall except the pressure, so the wall force is -p * wall length (the
+    fluid pushes the wall, whose outward normal from the fluid is -y)."""
+    mesh = M().make_box_mesh(8, 6, 2.0, 1.0)
+    mv = O.MeshView(mesh)
+    n = mesh.nCells
+    p0, r0 = 1.5, 1.2
+    U = np.zeros((n, 4), order="F"); U[:, 0] = r0; U[:, 3] = p0 / 0.4
+    Tw = p0 / (r0 * NS_GAS["R"])
+    bcs = {-1: (O.BC_EXTRAPOLATE, ()), -2: (O.BC_EXTRAPOLATE, ()), -3: (O.BC_ISOTHERMAL_WALL, (Tw,)),
+           -4: (O.BC_EXTRAPOLATE, ())}
+    UN, _, _, F = O.ns_step(mv, O.ns_params(bcs=bcs, **NS_GAS), 1e-3, U)
+    assert np.max(np.abs(UN - U)) < 1e-14
+    assert F[0] == pytest.approx(0.0, abs=1e-13)
+    assert F[1] == pytest.approx(-p0 * 2.0, rel=1e-13)
