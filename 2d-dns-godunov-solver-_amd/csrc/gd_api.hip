@@ -505,6 +505,7 @@ int gd_state_create(gd_mesh* m, const gd_params* p, gd_state** out) {
   rc |= dev_alloc(al, &S->wall_f, 2 * (size_t)std::max<int32_t>(m->v.n_wall, 1));
   rc |= dev_alloc(al, &S->drag, 4);
   rc |= dev_alloc(al, &S->acc, 3);
+  rc |= dev_alloc(al, &S->part, 3 * ((nc + 255) / 256) + 3);
   rc |= dev_alloc(al, &S->red, 8 + 4 * ((nc + 255) / 256) + 8);
   if (rc) {
     std::string msg = g_err;
@@ -814,8 +815,9 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
                                    S->stream));
   else
     GD_HIP(launch_fields_derived(v.nc, P.Gamma, S->Q[S->qc], S->aS, S->vmax, S->stream));
-  GD_HIP(launch_prime_acc(v, P, S->Q[S->qc], S->vmax, S->acc, rp->flow_time, rp->dt_policy, ns && P.viscous,
-                          S->stream));
+  GD_HIP(launch_prime_acc(v, P, S->Q[S->qc], S->vmax, S->acc, S->part, rp->flow_time, rp->dt_policy,
+                          ns && P.viscous, S->stream));
+  const int64_t nparts = (v.nc + 255) / 256;
   if (wall) {
     GD_HIP(launch_fill(S->wall_f, 2 * (int64_t)v.n_wall, 0.0, S->stream));
     GD_HIP(launch_fill(S->drag, 4, 0.0, S->stream));
@@ -839,6 +841,7 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       a.Un = S->U[1 - S->cur];
       a.Qw = S->Q[1 - S->qc];
       a.acc = S->acc;
+      a.part = S->part;
       a.step = s;
       a.dt_policy = rp->dt_policy;
       a.dt = rp->dt;
@@ -848,6 +851,7 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       a.wall_f = wall ? S->wall_f : nullptr;
       a.wall_code = rp->wall_code;
       GD_HIP(T.run("ref_step", bytes_ref_step(v, visc), [&] { return launch_ref_step(a, S->stream); }));
+      GD_HIP(launch_acc_fold(S->part, nparts, &S->acc[(s + 1) % 3], S->stream));
       if (wall) GD_HIP(launch_wall_sum(S->wall_f, v.n_wall, S->drag, S->stream));
       S->cur ^= 1;
       S->qc ^= 1;
@@ -873,6 +877,7 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       a.Qw = S->Qt[1 - tq];
       a.rec = S->rec;
       a.acc = S->acc;
+      a.part = S->part;
       a.step = s;
       a.dt_policy = rp->dt_policy;
       a.dt = rp->dt;
@@ -881,6 +886,7 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       a.wall_code = rp->wall_code;
       GD_HIP(T.run(mixed ? "ns_grad_fp32" : "ns_grad_fp64", bytes_ns_grad(v, t), [&] { return grad(a, S->stream); }));
       GD_HIP(T.run(mixed ? "ns_flux_fp32" : "ns_flux_fp64", bytes_ns_flux(v, t), [&] { return flux(a, S->stream); }));
+      GD_HIP(launch_acc_fold(S->part, nparts, &S->acc[(s + 1) % 3], S->stream));
       if (wall) GD_HIP(launch_wall_sum(S->wall_f, v.n_wall, S->drag, S->stream));
       S->cur ^= 1;
       tq ^= 1;

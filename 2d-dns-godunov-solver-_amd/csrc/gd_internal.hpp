@@ -118,6 +118,7 @@ struct State {
   double* wall_f = nullptr;             // [n_wall][2]
   double* drag = nullptr;               // [4]: last step force x,y; running sums
   Acc* acc = nullptr;                   // [3]
+  double* part = nullptr;               // [3 x cell blocks] partial reductions
   double* red = nullptr;                // reduction scratch
   std::vector<void*> allocs;
 };
@@ -137,6 +138,7 @@ struct RefStepArgs {
   double* aS;
   double* vmax;
   Acc* acc;             // fused run: accumulators (3 slots) or nullptr
+  double* part;         // fused run: per-block partials of the dt inputs [3 x blocks]
   int32_t step;         // rotates the accumulator slots
   int32_t dt_policy;
   double dt, cfl, betta, flow_time;
@@ -156,7 +158,8 @@ hipError_t launch_ref_diff_term(const MeshView& m, const gd_params& prm, const d
 hipError_t launch_update_variables(const MeshView& m, double gamma, const double* Unew, double* Uold, double* Q,
                                    double* aS, double* vmax, double* Delta, hipStream_t s);
 hipError_t launch_prime_acc(const MeshView& m, const gd_params& prm, const double* Q, const double* vmax, Acc* acc,
-                            double flow_time, int32_t dt_policy, int32_t viscous_ns, hipStream_t s);
+                            double* part, double flow_time, int32_t dt_policy, int32_t viscous_ns, hipStream_t s);
+hipError_t launch_acc_fold(const double* part, int64_t nparts, Acc* slot, hipStream_t s);
 hipError_t launch_residuals(const MeshView& m, const double* Unew, const double* Uold, double* red, hipStream_t s);
 hipError_t launch_wall_sum(const double* wall_f, int32_t n, double* out4, hipStream_t s);
 hipError_t launch_fields_derived(int64_t nc, double gamma, const double* Q, double* aS, double* vmax, hipStream_t s);
@@ -174,6 +177,7 @@ struct NsStepArgs {
   void* Qw;             // next primitives
   void* rec;            // [nc][16]: q, (d/dx, d/dy) of rho,u,v,p, limiter of rho,u,v,p
   Acc* acc;
+  double* part;         // per-block partials of the dt inputs [3 x blocks]
   int32_t step;
   int32_t dt_policy;
   double dt, cfl;

@@ -13,6 +13,7 @@ static ns::NsKArgs<float> cast(const NsStepArgs& a) {
   k.Qw = static_cast<float*>(a.Qw);
   k.rec = static_cast<float*>(a.rec);
   k.acc = a.acc;
+  k.part = a.part;
   k.step = a.step;
   k.dt_policy = a.dt_policy;
   k.dt = a.dt;

@@ -13,6 +13,7 @@ static ns::NsKArgs<double> cast(const NsStepArgs& a) {
   k.Qw = static_cast<double*>(a.Qw);
   k.rec = static_cast<double*>(a.rec);
   k.acc = a.acc;
+  k.part = a.part;
   k.step = a.step;
   k.dt_policy = a.dt_policy;
   k.dt = a.dt;

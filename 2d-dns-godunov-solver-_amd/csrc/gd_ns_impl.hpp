@@ -33,6 +33,7 @@ struct NsKArgs {
   T* Qw;            // next primitives [nc][4]
   T* rec;           // [nc][16]: q[4], (d/dx, d/dy) of rho,u,v,p [8], limiter [4]
   Acc* acc;
+  double* part;     // per-block partials of the next dt [3 x blocks]
   int32_t step;
   int32_t dt_policy;
   double dt, cfl;
@@ -469,17 +470,13 @@ __global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int q = 1; q < kBlock / 64; ++q) { vmx = fmax(vmx, s_max[q]); hdt = fmin(hdt, s_min[q]); bad += s_bad[q]; }
-    Acc* w = &a.acc[(a.step + 1) % 3];
-    atomicMax(&w->vmax_bits, (unsigned long long)__double_as_longlong(vmx));
-    atomicMin(&w->hdt_bits, (unsigned long long)__double_as_longlong(hdt));
-    if (bad) atomicAdd(&w->nonphysical, bad);
+    a.part[3 * (int64_t)blockIdx.x] = vmx;
+    a.part[3 * (int64_t)blockIdx.x + 1] = hdt;
+    a.part[3 * (int64_t)blockIdx.x + 2] = (double)bad;
     if (blockIdx.x == 0) {
+      Acc* w = &a.acc[(a.step + 1) % 3];
       w->time = r.time + dt;
       w->dt = dt;
-      Acc* z = &a.acc[(a.step + 2) % 3];
-      z->vmax_bits = 0ull;
-      z->hdt_bits = 0x7ff0000000000000ull;
-      z->nonphysical = 0u;
     }
   }
 }

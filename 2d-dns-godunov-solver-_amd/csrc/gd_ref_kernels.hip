@@ -31,10 +31,11 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
   return v;
 }
 
-// Block-reduce the step's dt inputs and non-physical count into the next
-// accumulator slot (max/min of positive doubles as integer atomics: the
-// result does not depend on the order of arrival).
-__device__ __forceinline__ void reduce_acc(double vmax, double hdt, unsigned bad, Acc* w) {
+// Block-reduce the step's dt inputs and non-physical count and store the
+// block's partial (max |u|+a, min local dt, count) in part[3 b .. 3 b + 2];
+// k_acc_fold combines the partials in one block.  (Same-address device-scope
+// atomics from every block serialise across the XCDs.)
+__device__ __forceinline__ void reduce_acc(double vmax, double hdt, unsigned bad, double* part) {
   __shared__ double s_max[kBlock / 64], s_min[kBlock / 64];
   __shared__ unsigned s_bad[kBlock / 64];
   vmax = wave_max(vmax);
@@ -47,9 +48,9 @@ __device__ __forceinline__ void reduce_acc(double vmax, double hdt, unsigned bad
     for (int q = 1; q < kBlock / 64; ++q) {
       vmax = fmax(vmax, s_max[q]); hdt = fmin(hdt, s_min[q]); bad += s_bad[q];
     }
-    atomicMax(&w->vmax_bits, (unsigned long long)__double_as_longlong(vmax));
-    atomicMin(&w->hdt_bits, (unsigned long long)__double_as_longlong(hdt));
-    if (bad) atomicAdd(&w->nonphysical, bad);
+    part[3 * (int64_t)blockIdx.x] = vmax;
+    part[3 * (int64_t)blockIdx.x + 1] = hdt;
+    part[3 * (int64_t)blockIdx.x + 2] = (double)bad;
   }
 }
 
@@ -64,10 +65,6 @@ __device__ __forceinline__ void rotate_acc(Acc* acc, int step, double t, double 
     Acc* w = &acc[(step + 1) % 3];
     w->time = t + dt;
     w->dt = dt;
-    Acc* z = &acc[(step + 2) % 3];
-    z->vmax_bits = 0ull;
-    z->hdt_bits = 0x7ff0000000000000ull;  // +inf
-    z->nonphysical = 0u;
   }
 }
 
@@ -230,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_step(RefStepArgs a) {
     }
   }
   if (FUSED) {
-    reduce_acc(vmx, hdt, bad, &a.acc[(a.step + 1) % 3]);
+    reduce_acc(vmx, hdt, bad, a.part);
     rotate_acc(a.acc, a.step, t, dt);
   }
 }
@@ -363,7 +360,7 @@ __global__ void k_acc_reset(Acc* acc, double t) {
 // Fills accumulator slot 0 from the current state (the reference takes
 // max(VMAXCells) at the top of each step, testViscousFluxesTGV2d.jl:420).
 __global__ __launch_bounds__(kBlock) void k_prime_acc(MeshView m, gd_params p, const double* Q, const double* vmax,
-                                                      Acc* acc, int32_t viscous_ns) {
+                                                      double* part, int32_t viscous_ns) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double vm = 0.0, hdt = __longlong_as_double(0x7ff0000000000000ll);
   unsigned bad = 0;
@@ -382,7 +379,34 @@ __global__ __launch_bounds__(kBlock) void k_prime_acc(MeshView m, gd_params p, c
     bad = (!(rho > 0.0) || !(pr > 0.0) || vm != vm) ? 1u : 0u;
     if (vm != vm) { vm = 0.0; hdt = __longlong_as_double(0x7ff0000000000000ll); }
   }
-  reduce_acc(vm, hdt, bad, &acc[0]);
+  reduce_acc(vm, hdt, bad, part);
+}
+
+// Folds the per-block partials of reduce_acc into one accumulator slot, in a
+// fixed order (max and min are exact; the count is an integer sum).
+__global__ __launch_bounds__(1024) void k_acc_fold(const double* part, int64_t nparts, Acc* w) {
+  __shared__ double s_max[1024], s_min[1024], s_bad[1024];
+  double vmax = 0.0, hdt = __longlong_as_double(0x7ff0000000000000ll), bad = 0.0;
+  for (int64_t b = threadIdx.x; b < nparts; b += 1024) {
+    vmax = fmax(vmax, part[3 * b]);
+    hdt = fmin(hdt, part[3 * b + 1]);
+    bad += part[3 * b + 2];
+  }
+  s_max[threadIdx.x] = vmax; s_min[threadIdx.x] = hdt; s_bad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + o]);
+      s_min[threadIdx.x] = fmin(s_min[threadIdx.x], s_min[threadIdx.x + o]);
+      s_bad[threadIdx.x] += s_bad[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    w->vmax_bits = (unsigned long long)__double_as_longlong(s_max[0]);
+    w->hdt_bits = (unsigned long long)__double_as_longlong(s_min[0]);
+    w->nonphysical = (unsigned)s_bad[0];
+  }
 }
 
 // Deterministic sums of (Unew - Uold)^2 per variable: per-block partials, then
@@ -544,10 +568,16 @@ hipError_t launch_update_variables(const MeshView& m, double gamma, const double
 }
 
 hipError_t launch_prime_acc(const MeshView& m, const gd_params& prm, const double* Q, const double* vmax, Acc* acc,
-                            double flow_time, int32_t dt_policy, int32_t viscous_ns, hipStream_t s) {
+                            double* part, double flow_time, int32_t dt_policy, int32_t viscous_ns, hipStream_t s) {
   (void)dt_policy;
   hipLaunchKernelGGL(k_acc_reset, dim3(1), dim3(64), 0, s, acc, flow_time);
-  hipLaunchKernelGGL(k_prime_acc, dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m, prm, Q, vmax, acc, viscous_ns);
+  hipLaunchKernelGGL(k_prime_acc, dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m, prm, Q, vmax, part, viscous_ns);
+  hipLaunchKernelGGL(k_acc_fold, dim3(1), dim3(1024), 0, s, part, (int64_t)nblocks(m.nc), acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_acc_fold(const double* part, int64_t nparts, Acc* slot, hipStream_t s) {
+  hipLaunchKernelGGL(k_acc_fold, dim3(1), dim3(1024), 0, s, part, nparts, slot);
   return hipGetLastError();
 }
 
