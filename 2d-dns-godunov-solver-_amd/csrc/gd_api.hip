@@ -250,7 +250,9 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
       h_up2(nf * nc, 0), h_ea(nf * nc, 0), h_eb(nf * nc, 0), h_wid(nf * nc, -1);
   std::vector<double> h_nx(nf * nc, 0.0), h_ny(nf * nc, 0.0), h_len(nf * nc, 0.0), h_area(nc), h_rarea(nc), h_hx(nc),
       h_hmin(nc), h_xc(nc), h_yc(nc);
-  std::vector<double> h_lsq(nf * 2 * nc, 0.0), h_rf(nf * 2 * nc, 0.0), h_dc(nf * 2 * nc, 0.0);
+  std::vector<double> h_lsq(nf * 2 * nc, 0.0), h_rf(nf * 2 * nc, 0.0), h_dc(nf * 2 * nc, 0.0),
+      h_rn(nf * 2 * nc, 0.0);
+  std::vector<int32_t> h_kn(nf * nc, -1);
   double max_hx = 0.0;
   int64_t bandwidth = 0;
   int32_t n_wall = 0;
@@ -313,6 +315,9 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
         rface(oj, kj, rj);
         dcx[k - 1] = r[0] - rj[0];
         dcy[k - 1] = r[1] - rj[1];
+        h_kn[s] = kj - 1;
+        h_rn[((int64_t)(k - 1) * 2 + 0) * nc + c] = rj[0];
+        h_rn[((int64_t)(k - 1) * 2 + 1) * nc + c] = rj[1];
       } else {
         h_nb[s] = e - 1;  // code - 1 < 0
         h_wid[s] = n_wall++;
@@ -370,6 +375,51 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
     h_yn[jn] = d->yNodes[j];
   }
 
+  // --- face list of the ns flux pass: each interior face once, owned by the
+  // cell with the lower caller id; faces numbered in owner order
+  std::vector<int32_t> h_fc0, h_fc1, h_fwall, h_cface(nf * nc, -1);
+  std::vector<double> h_fnx, h_fny, h_flen, h_fr0[2], h_fr1[2], h_fd[2];
+  for (int64_t c = 0; c < nc; ++c) {
+    for (int k = 0; k < h_nv[c]; ++k) {
+      const int64_t s = (int64_t)k * nc + c;
+      const int32_t j = h_nb[s];
+      if (j >= 0 && cp[j] < cp[c]) continue;  // the neighbour owns it
+      const int32_t f = (int32_t)h_fc0.size();
+      h_fc0.push_back((int32_t)c);
+      h_fc1.push_back(j);
+      h_fwall.push_back(j >= 0 ? -1 : h_wid[s]);
+      h_cface[s] = 2 * f;
+      if (j >= 0) h_cface[(int64_t)h_kn[s] * nc + j] = 2 * f + 1;
+      h_fnx.push_back(h_nx[s]);
+      h_fny.push_back(h_ny[s]);
+      h_flen.push_back(h_len[s]);
+      for (int q = 0; q < 2; ++q) {
+        h_fr0[q].push_back(h_rf[((int64_t)k * 2 + q) * nc + c]);
+        h_fr1[q].push_back(h_rn[((int64_t)k * 2 + q) * nc + c]);
+        h_fd[q].push_back(h_dc[((int64_t)k * 2 + q) * nc + c]);
+      }
+    }
+  }
+  const int64_t nfaces = (int64_t)h_fc0.size();
+  for (int64_t c = 0; c < nc; ++c)
+    for (int k = 0; k < h_nv[c]; ++k)
+      if (h_cface[(int64_t)k * nc + c] < 0) {
+        delete M;
+        return fail(GD_ERR_INVALID, "a cell lists the same neighbour on two faces");
+      }
+  auto pair =[&](const std::vector<double>* a) {
+    std::vector<double> o(2 * nfaces);
+    std::copy(a[0].begin(), a[0].end(), o.begin());
+    std::copy(a[1].begin(), a[1].end(), o.begin() + nfaces);
+    return o;
+  };
+  auto to_float = [](const std::vector<double>& a) {
+    std::vector<float> o(a.size());
+    for (size_t q = 0; q < a.size(); ++q) o[q] = (float)a[q];
+    return o;
+  };
+  const std::vector<double> h_fr0p = pair(h_fr0), h_fr1p = pair(h_fr1), h_fdp = pair(h_fd);
+
   MeshView& v = M->v;
   v.nc = nc; v.nn = nn; v.nf = nf; v.ncl = ncl; v.has_quads = has_quads; v.n_wall = n_wall; v.max_hx = max_hx;
   int64_t* B = &M->device_bytes;
@@ -394,25 +444,32 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
   rc |= upload(al, &v.yc, h_yc, B);
   rc |= upload(al, &v.lsq, h_lsq, B);
   rc |= upload(al, &v.rf, h_rf, B);
-  rc |= upload(al, &v.dc, h_dc, B);
   rc |= upload(al, &v.wall_id, h_wid, B);
   {
-    std::vector<float> f_nx(h_nx.size()), f_ny(h_ny.size()), f_lsq(h_lsq.size()), f_rf(h_rf.size()),
-        f_rj(h_rf.size()), f_dc(h_dc.size());
+    std::vector<float> f_nx(h_nx.size()), f_ny(h_ny.size()), f_lsq(h_lsq.size()), f_rf(h_rf.size());
     for (size_t q = 0; q < h_nx.size(); ++q) { f_nx[q] = (float)h_nx[q]; f_ny[q] = (float)h_ny[q]; }
-    for (size_t q = 0; q < h_lsq.size(); ++q) {
-      f_lsq[q] = (float)h_lsq[q];
-      f_rf[q] = (float)h_rf[q];
-      f_rj[q] = (float)(h_rf[q] - h_dc[q]);
-      f_dc[q] = (float)h_dc[q];
-    }
+    for (size_t q = 0; q < h_lsq.size(); ++q) { f_lsq[q] = (float)h_lsq[q]; f_rf[q] = (float)h_rf[q]; }
     rc |= upload(al, &v.fnx32, f_nx, B);
     rc |= upload(al, &v.fny32, f_ny, B);
     rc |= upload(al, &v.lsq32, f_lsq, B);
     rc |= upload(al, &v.rf32, f_rf, B);
-    rc |= upload(al, &v.rj32, f_rj, B);
-    rc |= upload(al, &v.dc32, f_dc, B);
   }
+  v.n_faces = nfaces;
+  rc |= upload(al, &v.fc0, h_fc0, B);
+  rc |= upload(al, &v.fc1, h_fc1, B);
+  rc |= upload(al, &v.fwall, h_fwall, B);
+  rc |= upload(al, &v.f_nx, h_fnx, B);
+  rc |= upload(al, &v.f_ny, h_fny, B);
+  rc |= upload(al, &v.f_len, h_flen, B);
+  rc |= upload(al, &v.f_r0, h_fr0p, B);
+  rc |= upload(al, &v.f_r1, h_fr1p, B);
+  rc |= upload(al, &v.f_d, h_fdp, B);
+  rc |= upload(al, &v.f_nx32, to_float(h_fnx), B);
+  rc |= upload(al, &v.f_ny32, to_float(h_fny), B);
+  rc |= upload(al, &v.f_r032, to_float(h_fr0p), B);
+  rc |= upload(al, &v.f_r132, to_float(h_fr1p), B);
+  rc |= upload(al, &v.f_d32, to_float(h_fdp), B);
+  rc |= upload(al, &v.cface, h_cface, B);
   rc |= upload(al, &v.cl, h_cl, B);
   rc |= upload(al, &v.w, h_w, B);
   rc |= upload(al, &v.xn, h_xn, B);
@@ -496,6 +553,9 @@ int gd_state_create(gd_mesh* m, const gd_params* p, gd_state** out) {
   double* rbuf = nullptr;
   rc |= dev_alloc(al, &rbuf, 16 * nc);
   S->rec = rbuf;
+  double* fbuf = nullptr;
+  rc |= dev_alloc(al, &fbuf, 4 * (size_t)std::max<int64_t>(m->v.n_faces, 1));
+  S->fflux = fbuf;
   double* qt0 = nullptr;
   double* qt1 = nullptr;
   rc |= dev_alloc(al, &qt0, 4 * nc);
@@ -795,10 +855,15 @@ static double bytes_ref_gg_nodes(const MeshView& v) {
 static double bytes_ns_grad(const MeshView& v, double t) {
   return (double)v.nc * (4 * t /*q*/ + 12 /*nb*/ + 6 * t /*lsq*/ + 6 * t /*rf*/ + 8 /*hmin*/ + 16 * t /*record*/);
 }
-static double bytes_ns_flux(const MeshView& v, double t) {
-  const double face = t == 4.0 ? 32 /*nx,ny,rf,rj,dc in float*/ : 48 /*nx,ny,rf,dc in double*/;
-  return (double)v.nc * (16 * t /*record*/ + 12 /*nb*/ + 3 * (face + 8 /*len*/) + 8 /*rarea*/ + 8 /*hmin*/ +
-                         32 /*U old*/ + 32 /*U new*/ + 4 * t /*q new*/);
+static double bytes_ns_face(const MeshView& v, double t) {
+  const double geo = t == 4.0 ? 32 /*nx,ny,r0,r1,d in float*/ : 64 /*in double*/;
+  return (double)v.n_faces * (8 /*cells*/ + geo + 8 /*len*/ + 4 * t /*flux write*/) +
+         (double)v.nc * 16 * t /*records, gathered at unique size*/;
+}
+static double bytes_ns_update(const MeshView& v, double t) {
+  return (double)v.nc * (4.0 * v.nf /*face ids*/ + 32 /*U old*/ + 32 /*U new*/ + 8 /*1/A*/ + 8 /*h*/ +
+                         4 * t /*q new*/) +
+         (double)v.n_faces * 4 * t /*face fluxes, unique*/;
 }
 
 int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats) {
@@ -864,7 +929,8 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
     const double t = mixed ? 4.0 : 8.0;
     auto prims = mixed ? launch_ns_prims_f32 : launch_ns_prims_f64;
     auto grad = mixed ? launch_ns_grad_f32 : launch_ns_grad_f64;
-    auto flux = mixed ? launch_ns_flux_f32 : launch_ns_flux_f64;
+    auto face = mixed ? launch_ns_face_f32 : launch_ns_face_f64;
+    auto update = mixed ? launch_ns_update_f32 : launch_ns_update_f64;
     int tq = 0;
     GD_HIP(prims(v.nc, P.Gamma, S->U[S->cur], S->Qt[tq], S->stream));
     for (int32_t s = 0; s < n_steps; ++s) {
@@ -876,6 +942,7 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       a.Q = S->Qt[tq];
       a.Qw = S->Qt[1 - tq];
       a.rec = S->rec;
+      a.fflux = S->fflux;
       a.acc = S->acc;
       a.part = S->part;
       a.step = s;
@@ -885,7 +952,9 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       a.wall_f = wall ? S->wall_f : nullptr;
       a.wall_code = rp->wall_code;
       GD_HIP(T.run(mixed ? "ns_grad_fp32" : "ns_grad_fp64", bytes_ns_grad(v, t), [&] { return grad(a, S->stream); }));
-      GD_HIP(T.run(mixed ? "ns_flux_fp32" : "ns_flux_fp64", bytes_ns_flux(v, t), [&] { return flux(a, S->stream); }));
+      GD_HIP(T.run(mixed ? "ns_face_fp32" : "ns_face_fp64", bytes_ns_face(v, t), [&] { return face(a, S->stream); }));
+      GD_HIP(T.run(mixed ? "ns_update_fp32" : "ns_update_fp64", bytes_ns_update(v, t),
+                   [&] { return update(a, S->stream); }));
       GD_HIP(launch_acc_fold(S->part, nparts, &S->acc[(s + 1) % 3], S->stream));
       if (wall) GD_HIP(launch_wall_sum(S->wall_f, v.n_wall, S->drag, S->stream));
       S->cur ^= 1;

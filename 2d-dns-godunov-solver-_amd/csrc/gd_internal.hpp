@@ -54,7 +54,6 @@ struct MeshView {
   // ns geometry
   const double* lsq;   // [nf][2][nc] least-squares weights: grad = sum_k w_k (q_nb - q_i)
   const double* rf;    // [nf][2][nc] face midpoint - own centroid
-  const double* dc;    // [nf][2][nc] neighbour centroid - own centroid (periodic safe)
   const int32_t* wall_id; // [nf][nc] index into the wall list or -1
   // float copies for the FP32-mixed kernels of values that those kernels round
   // to float at first use (bit-identical results, half the bytes)
@@ -62,8 +61,24 @@ struct MeshView {
   const float* fny32;
   const float* lsq32;  // [nf][2][nc]
   const float* rf32;   // [nf][2][nc]
-  const float* rj32;   // [nf][2][nc] face midpoint - neighbour centroid (= rf - dc, rounded once)
-  const float* dc32;   // [nf][2][nc]
+  // face list of the ns flux pass: each face once, owned by the cell with the
+  // lower caller id; the owner's inward normal; [2][n_faces] pairs are x, y
+  int64_t n_faces;
+  const int32_t* fc0;   // owner cell
+  const int32_t* fc1;   // neighbour cell, or (code - 1) < 0 at a boundary
+  const int32_t* fwall; // wall-list index of a boundary face, or -1
+  const double* f_nx;
+  const double* f_ny;
+  const double* f_len;
+  const double* f_r0;   // [2][n_faces] face midpoint - owner centroid
+  const double* f_r1;   // [2][n_faces] face midpoint - neighbour centroid (its own vertices)
+  const double* f_d;    // [2][n_faces] neighbour centroid - owner centroid (ghost offset at a boundary)
+  const float* f_nx32;
+  const float* f_ny32;
+  const float* f_r032;
+  const float* f_r132;
+  const float* f_d32;
+  const int32_t* cface; // [nf][nc]: 2 f + (1 if the cell is the face's neighbour), -1 for no face
   // per node
   const int32_t* cl;   // [ncl][nn] cluster cell ids, -1 padded
   const double* w;     // [ncl][nn] interpolation weights
@@ -114,6 +129,7 @@ struct State {
   double* g6 = nullptr;                 // [6][nc]
   double* g6N = nullptr;                // [6][nn]
   void* rec = nullptr;                  // ns: per-cell record [nc][16] (q, grad q, limiter), compute type
+  void* fflux = nullptr;                // ns: [4][n_faces] face fluxes, compute type
   void* Qt[2] = {nullptr, nullptr};     // ns: primitives [nc][4], compute type
   double* wall_f = nullptr;             // [n_wall][2]
   double* drag = nullptr;               // [4]: last step force x,y; running sums
@@ -176,6 +192,7 @@ struct NsStepArgs {
   const void* Q;        // current primitives [nc][4], compute type
   void* Qw;             // next primitives
   void* rec;            // [nc][16]: q, (d/dx, d/dy) of rho,u,v,p, limiter of rho,u,v,p
+  void* fflux;          // [4][n_faces] flux through each face, leaving its owner
   Acc* acc;
   double* part;         // per-block partials of the dt inputs [3 x blocks]
   int32_t step;
@@ -186,9 +203,13 @@ struct NsStepArgs {
 };
 hipError_t launch_ns_prims_f64(int64_t nc, double gamma, const double* U, void* Q, hipStream_t s);
 hipError_t launch_ns_grad_f64(const NsStepArgs& a, hipStream_t s);
-hipError_t launch_ns_flux_f64(const NsStepArgs& a, hipStream_t s);
+hipError_t launch_ns_flux_f64(const NsStepArgs& a, hipStream_t s);  // face fluxes + cell update
 hipError_t launch_ns_prims_f32(int64_t nc, double gamma, const double* U, void* Q, hipStream_t s);
 hipError_t launch_ns_grad_f32(const NsStepArgs& a, hipStream_t s);
 hipError_t launch_ns_flux_f32(const NsStepArgs& a, hipStream_t s);
+hipError_t launch_ns_face_f64(const NsStepArgs& a, hipStream_t s);
+hipError_t launch_ns_update_f64(const NsStepArgs& a, hipStream_t s);
+hipError_t launch_ns_face_f32(const NsStepArgs& a, hipStream_t s);
+hipError_t launch_ns_update_f32(const NsStepArgs& a, hipStream_t s);
 
 }  // namespace gd

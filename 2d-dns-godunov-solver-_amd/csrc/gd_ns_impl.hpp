@@ -5,9 +5,11 @@
 //
 // Two launches per step, both cell-centric gathers without atomics on the
 // state (deterministic, numbering independent):
-//   k_ns_grad  least-squares gradient of (rho,u,v,p) + Venkatakrishnan limiter
-//   k_ns_flux  reconstruction, AUSM+/Roe, viscous stress/heat flux, explicit
-//              Euler update, primitive recovery, dt and wall-force partials
+//   k_ns_grad    least-squares gradient of (rho,u,v,p) + Venkatakrishnan limiter
+//   k_ns_face    one flux per face (owner = lower caller id): reconstruction of
+//                both sides, AUSM+/Roe, viscous stress/heat flux, wall force
+//   k_ns_update  per cell: sum of +-face fluxes in slot order (FP64), explicit
+//                Euler update, primitive recovery, next-dt partials
 // Everything read across a face is stored as one record per cell so that a
 // neighbour gather is one contiguous load: primitives [nc][4] (16 B in FP32)
 // and the face-reconstruction record [nc][16] = (q, d/dx d/dy of each of
@@ -32,6 +34,7 @@ struct NsKArgs {
   const T* Q;       // current primitives (rho,u,v,p) [nc][4]
   T* Qw;            // next primitives [nc][4]
   T* rec;           // [nc][16]: q[4], (d/dx, d/dy) of rho,u,v,p [8], limiter [4]
+  T* fflux;         // [4][n_faces]: flux leaving the face's owner
   Acc* acc;
   double* part;     // per-block partials of the next dt [3 x blocks]
   int32_t step;
@@ -190,14 +193,21 @@ struct FaceGeo {
   __device__ __forceinline__ static T lsq(const MeshView& m, int k, int c, int64_t i) {
     if constexpr (sizeof(T) == 4) return m.lsq32[(2 * k + c) * m.nc + i]; else return m.lsq[(2 * k + c) * m.nc + i];
   }
-  // neighbour centroid - own centroid
-  __device__ __forceinline__ static T dc(const MeshView& m, int k, int c, int64_t i) {
-    if constexpr (sizeof(T) == 4) return m.dc32[(2 * k + c) * m.nc + i]; else return m.dc[(2 * k + c) * m.nc + i];
+  // face-list geometry (owner's inward normal, offsets)
+  __device__ __forceinline__ static T fnx(const MeshView& m, int64_t f) {
+    if constexpr (sizeof(T) == 4) return m.f_nx32[f]; else return m.f_nx[f];
   }
-  // face midpoint - neighbour centroid (rf - dc, formed in FP64 and rounded once)
-  __device__ __forceinline__ static T rj(const MeshView& m, int k, int c, int64_t i) {
-    if constexpr (sizeof(T) == 4) return m.rj32[(2 * k + c) * m.nc + i];
-    else return m.rf[(2 * k + c) * m.nc + i] - m.dc[(2 * k + c) * m.nc + i];
+  __device__ __forceinline__ static T fny(const MeshView& m, int64_t f) {
+    if constexpr (sizeof(T) == 4) return m.f_ny32[f]; else return m.f_ny[f];
+  }
+  __device__ __forceinline__ static T fr0(const MeshView& m, int c, int64_t f) {
+    if constexpr (sizeof(T) == 4) return m.f_r032[c * m.n_faces + f]; else return m.f_r0[c * m.n_faces + f];
+  }
+  __device__ __forceinline__ static T fr1(const MeshView& m, int c, int64_t f) {
+    if constexpr (sizeof(T) == 4) return m.f_r132[c * m.n_faces + f]; else return m.f_r1[c * m.n_faces + f];
+  }
+  __device__ __forceinline__ static T fd(const MeshView& m, int c, int64_t f) {
+    if constexpr (sizeof(T) == 4) return m.f_d32[c * m.n_faces + f]; else return m.f_d[c * m.n_faces + f];
   }
 };
 
@@ -304,14 +314,113 @@ __device__ __forceinline__ void viscous_face(const gd_params& P, T cp, T uf, T v
   gv[3] = uf * gv[1] + vf * gv[2] + kap * (gf[2][0] * nx + gf[2][1] * ny);
 }
 
-// ---- pass F: fluxes, update, primitives, dt inputs
-template <class T, bool QUADS, bool VISC>
-__global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
-  constexpr int NF = QUADS ? 4 : 3;
+// ---- pass F1: one flux per face
+template <class T, bool VISC>
+__global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
   using G = FaceGeo<T>;
   const MeshView& m = a.m;
   const gd_params& P = a.prm;
-  const int64_t nc = m.nc;
+  const int64_t nF = m.n_faces;
+  const int64_t f = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (f >= nF) return;
+  const int32_t c0 = m.fc0[f], c1 = m.fc1[f];
+  // loads first: both records (a boundary face reads the owner's twice), geometry
+  T r0[16], r1[16];
+  ld16(a.rec + 16 * (int64_t)c0, r0);
+  ld16(a.rec + 16 * (int64_t)(c1 >= 0 ? c1 : c0), r1);
+  const T nx = G::fnx(m, f), ny = G::fny(m, f);
+  const double len = m.f_len[f];
+  const T rx = G::fr0(m, 0, f), ry = G::fr0(m, 1, f);
+  const T g = (T)P.Gamma, Rg = (T)P.RGAS;
+  const T* qi = r0;
+  const T* gi = r0 + 4;
+  T qa[4];  // owner side at the face
+#pragma unroll
+  for (int v = 0; v < 4; ++v) qa[v] = qi[v] + r0[12 + v] * (gi[2 * v] * rx + gi[2 * v + 1] * ry);
+  if (!(qa[0] > 0) || !(qa[3] > 0)) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) qa[v] = qi[v];
+  }
+  const T* qj = r1;
+  const T* gj = r1 + 4;
+  T qb[4];  // neighbour side, from its own offset
+  if (c1 >= 0) {
+    const T rxj = G::fr1(m, 0, f), ryj = G::fr1(m, 1, f);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) qb[v] = qj[v] + r1[12 + v] * (gj[2 * v] * rxj + gj[2 * v + 1] * ryj);
+    if (!(qb[0] > 0) || !(qb[3] > 0)) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) qb[v] = qj[v];
+    }
+  } else {
+    ghost<T>(P, c1 + 1, qa, nx, ny, qb);
+  }
+  T e[4];
+  if (P.flux == GD_FLUX_ROE) roe<T>(qb, qa, nx, ny, g, e);
+  else ausm<T>(qb, qa, nx, ny, g, e);
+  T gv[4] = {0, 0, 0, 0};
+  if (VISC) {
+    const T cp = g * Rg / (g - 1);
+    const T rRi = 1 / (qi[0] * Rg);
+    const T Ti = qi[3] * rRi;
+    if (c1 >= 0) {
+      // edge-corrected average of the two cell gradients of u, v, T
+      const T rRj = 1 / (qj[0] * Rg);
+      const T Tj = qj[3] * rRj;
+      T g_i[3][2], g_j[3][2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        g_i[0][c] = gi[2 + c]; g_i[1][c] = gi[4 + c];
+        g_i[2][c] = (gi[6 + c] - Ti * Rg * gi[c]) * rRi;
+        g_j[0][c] = gj[2 + c]; g_j[1][c] = gj[4 + c];
+        g_j[2][c] = (gj[6 + c] - Tj * Rg * gj[c]) * rRj;
+      }
+      const T ddx = G::fd(m, 0, f), ddy = G::fd(m, 1, f);
+      const T rdl = 1 / sqrt(ddx * ddx + ddy * ddy);
+      const T tx = ddx * rdl, ty = ddy * rdl;
+      const T dphi[3] = {qj[1] - qi[1], qj[2] - qi[2], Tj - Ti};
+      T gf[3][2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T ax = (T)0.5 * (g_i[c][0] + g_j[c][0]), ay = (T)0.5 * (g_i[c][1] + g_j[c][1]);
+        const T corr = dphi[c] * rdl - (ax * tx + ay * ty);
+        gf[c][0] = ax + corr * tx; gf[c][1] = ay + corr * ty;
+      }
+      viscous_face<T>(P, cp, (T)0.5 * (qi[1] + qj[1]), (T)0.5 * (qi[2] + qj[2]), (T)0.5 * (Ti + Tj), gf, nx, ny,
+                      gv);
+    } else {
+      const int idx = -(c1 + 1);
+      if (idx <= GD_MAX_BC && P.bc[idx].kind == GD_BC_ISOTHERMAL_WALL) {
+        // one-sided normal gradient to the wall (u = 0, T = T_w); the fluid's
+        // outward normal is -n
+        const T Tw = (T)P.bc[idx].data[0];
+        const T rdp = -1 / (rx * nx + ry * ny);
+        const T dudn = (0 - qi[1]) * rdp, dvdn = (0 - qi[2]) * rdp, dTdn = (Tw - Ti) * rdp;
+        const T ox = -nx, oy = -ny;
+        const T gf[3][2] = {{dudn * ox, dudn * oy}, {dvdn * ox, dvdn * oy}, {dTdn * ox, dTdn * oy}};
+        viscous_face<T>(P, cp, (T)0, (T)0, Tw, gf, nx, ny, gv);
+      }
+    }
+  }
+  double fk[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    fk[v] = ((double)e[v] + (double)gv[v]) * len;
+    a.fflux[v * nF + f] = (T)fk[v];
+  }
+  if (c1 < 0 && a.wall_f && a.wall_code == c1 + 1) {
+    const int32_t wid = m.fwall[f];
+    if (wid >= 0) { a.wall_f[2 * wid] = fk[1]; a.wall_f[2 * wid + 1] = fk[2]; }
+  }
+}
+
+// ---- pass F2: per-cell flux sum, update, primitives, dt inputs
+template <class T, bool QUADS, bool VISC>
+__global__ __launch_bounds__(kBlock) void k_ns_update(NsKArgs<T> a) {
+  constexpr int NF = QUADS ? 4 : 3;
+  const MeshView& m = a.m;
+  const gd_params& P = a.prm;
+  const int64_t nc = m.nc, nF = m.n_faces;
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const Acc r = a.acc[a.step % 3];
   double dt = a.dt;
@@ -323,114 +432,23 @@ __global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
   double vmx = 0.0, hdt = __longlong_as_double(0x7ff0000000000000ll);
   unsigned bad = 0;
   if (i < nc) {
-    const T g = (T)P.Gamma, Rg = (T)P.RGAS;
-    const T cp = g * Rg / (g - 1);
     const int nf = QUADS ? m.nv[i] : 3;
-    // ---- loads: ids, own record, neighbour records (boundary faces read the
-    // own record, unused), geometry, conservative state
-    int32_t jn[NF];
+    int32_t fc[NF];
 #pragma unroll
-    for (int k = 0; k < NF; ++k) jn[k] = m.nb[k * nc + i];
-    T ri[16];
-    ld16(a.rec + 16 * i, ri);
-    T rn[NF][16];
-#pragma unroll
-    for (int k = 0; k < NF; ++k) ld16(a.rec + 16 * (int64_t)(jn[k] >= 0 ? jn[k] : i), rn[k]);
+    for (int k = 0; k < NF; ++k) fc[k] = m.cface[k * nc + i];
     double Uo[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) Uo[v] = a.Uo[v * nc + i];
-    const T* qi = ri;
-    const T* gi = ri + 4;
-    const T* phi_i = ri + 12;
-    // own temperature and its gradient (T = p / (rho R))
-    const T rRi = 1 / (qi[0] * Rg);
-    const T Ti = qi[3] * rRi;
-    T gTi[2];
-    if (VISC) {
-#pragma unroll
-      for (int c = 0; c < 2; ++c) gTi[c] = (gi[6 + c] - Ti * Rg * gi[c]) * rRi;
-    }
     double F[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < NF; ++k) {
       if (k < nf) {
-        const int64_t s = k * nc + i;
-        const int32_t j = jn[k];
-        const T nx = G::nx(m, s), ny = G::ny(m, s);
-        const double len = m.flen[s];
-        const T rx = G::rf(m, k, 0, i), ry = G::rf(m, k, 1, i);
-        T qa[4];
+        const int64_t fid = fc[k] >> 1;
+        const bool neg = (fc[k] & 1) != 0;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) qa[v] = qi[v] + phi_i[v] * (gi[2 * v] * rx + gi[2 * v + 1] * ry);
-        if (!(qa[0] > 0) || !(qa[3] > 0)) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) qa[v] = qi[v];
-        }
-        const T* qj = rn[k];
-        const T* gj = rn[k] + 4;
-        T qb[4];
-        if (j >= 0) {
-          const T rxj = G::rj(m, k, 0, i), ryj = G::rj(m, k, 1, i);
-#pragma unroll
-          for (int v = 0; v < 4; ++v) qb[v] = qj[v] + rn[k][12 + v] * (gj[2 * v] * rxj + gj[2 * v + 1] * ryj);
-          if (!(qb[0] > 0) || !(qb[3] > 0)) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) qb[v] = qj[v];
-          }
-        } else {
-          ghost<T>(P, j + 1, qa, nx, ny, qb);
-        }
-        T e[4];
-        if (P.flux == GD_FLUX_ROE) roe<T>(qb, qa, nx, ny, g, e);
-        else ausm<T>(qb, qa, nx, ny, g, e);
-        T gv[4] = {0, 0, 0, 0};
-        if (VISC) {
-          if (j >= 0) {
-            // edge-corrected average of the two cell gradients of u, v, T
-            const T rRj = 1 / (qj[0] * Rg);
-            const T Tj = qj[3] * rRj;
-            T g_i[3][2], g_j[3][2];
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-              g_i[0][c] = gi[2 + c]; g_i[1][c] = gi[4 + c]; g_i[2][c] = gTi[c];
-              g_j[0][c] = gj[2 + c]; g_j[1][c] = gj[4 + c];
-              g_j[2][c] = (gj[6 + c] - Tj * Rg * gj[c]) * rRj;
-            }
-            const T ddx = G::dc(m, k, 0, i), ddy = G::dc(m, k, 1, i);
-            const T rdl = 1 / sqrt(ddx * ddx + ddy * ddy);
-            const T tx = ddx * rdl, ty = ddy * rdl;
-            const T dphi[3] = {qj[1] - qi[1], qj[2] - qi[2], Tj - Ti};
-            T gf[3][2];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              const T ax = (T)0.5 * (g_i[c][0] + g_j[c][0]), ay = (T)0.5 * (g_i[c][1] + g_j[c][1]);
-              const T corr = dphi[c] * rdl - (ax * tx + ay * ty);
-              gf[c][0] = ax + corr * tx; gf[c][1] = ay + corr * ty;
-            }
-            viscous_face<T>(P, cp, (T)0.5 * (qi[1] + qj[1]), (T)0.5 * (qi[2] + qj[2]), (T)0.5 * (Ti + Tj), gf, nx,
-                            ny, gv);
-          } else {
-            const int idx = -(j + 1);
-            if (idx <= GD_MAX_BC && P.bc[idx].kind == GD_BC_ISOTHERMAL_WALL) {
-              // one-sided normal gradient to the wall (u = 0, T = T_w); the
-              // outward normal of the fluid is -n
-              const T Tw = (T)P.bc[idx].data[0];
-              const T rdp = -1 / (rx * nx + ry * ny);
-              const T dudn = (0 - qi[1]) * rdp, dvdn = (0 - qi[2]) * rdp, dTdn = (Tw - Ti) * rdp;
-              const T ox = -nx, oy = -ny;
-              const T gf[3][2] = {{dudn * ox, dudn * oy}, {dvdn * ox, dvdn * oy}, {dTdn * ox, dTdn * oy}};
-              viscous_face<T>(P, cp, (T)0, (T)0, Tw, gf, nx, ny, gv);
-            }
-          }
-        }
-        double fk[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) fk[v] = ((double)e[v] + (double)gv[v]) * len;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) F[v] += fk[v];
-        if (j < 0 && a.wall_f && a.wall_code == j + 1) {
-          const int32_t wid = m.wall_id[s];
-          if (wid >= 0) { a.wall_f[2 * wid] = fk[1]; a.wall_f[2 * wid + 1] = fk[2]; }
+        for (int v = 0; v < 4; ++v) {
+          const double x = (double)a.fflux[v * nF + fid];
+          F[v] += neg ? -x : x;
         }
       }
     }
@@ -459,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_flux(NsKArgs<T> a) {
     bad = (!(rho > 0.0) || !(p > 0.0) || vm != vm) ? 1u : 0u;
     if (vm != vm || lam != lam) { vmx = 0.0; hdt = __longlong_as_double(0x7ff0000000000000ll); }
   }
-  // block reduction of the next step's dt inputs
+  // block reduction of the next step's dt inputs into per-block partials
   __shared__ double s_max[kBlock / 64], s_min[kBlock / 64];
   __shared__ unsigned s_bad[kBlock / 64];
   vmx = wave_max64(vmx);
@@ -491,15 +509,23 @@ hipError_t launch_grad(const NsKArgs<T>& a, hipStream_t s) {
 }
 
 template <class T>
-hipError_t launch_flux(const NsKArgs<T>& a, hipStream_t s) {
+hipError_t launch_face(const NsKArgs<T>& a, hipStream_t s) {
+  const dim3 g(nblocks(a.m.n_faces)), b(kBlock);
+  if (a.prm.viscous) hipLaunchKernelGGL((k_ns_face<T, true>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_ns_face<T, false>), g, b, 0, s, a);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_update(const NsKArgs<T>& a, hipStream_t s) {
   const dim3 g(nblocks(a.m.nc)), b(kBlock);
   const bool visc = a.prm.viscous != 0;
   if (a.m.has_quads) {
-    if (visc) hipLaunchKernelGGL((k_ns_flux<T, true, true>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_ns_flux<T, true, false>), g, b, 0, s, a);
+    if (visc) hipLaunchKernelGGL((k_ns_update<T, true, true>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_ns_update<T, true, false>), g, b, 0, s, a);
   } else {
-    if (visc) hipLaunchKernelGGL((k_ns_flux<T, false, true>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_ns_flux<T, false, false>), g, b, 0, s, a);
+    if (visc) hipLaunchKernelGGL((k_ns_update<T, false, true>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_ns_update<T, false, false>), g, b, 0, s, a);
   }
   return hipGetLastError();
 }

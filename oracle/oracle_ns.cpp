@@ -10,13 +10,17 @@
 //          (ghost values at boundary faces), weights w_k = 1/|d_k|^2;
 //          Venkatakrishnan limiter (1993) per variable,
 //          eps^2 = (K h)^3 s_v^2 with h = 2A/perimeter and s_v a scale per variable
-//  pass F  reconstruction q_f = q + phi (grad q . r_f), first order where the
-//          reconstructed rho or p is not positive; AUSM+ or Roe with the
-//          neighbour side as "L" and inward normals (the reference's
-//          convention, computeslope2d.jl:129); viscous flux with the
-//          edge-corrected average face gradient, Sutherland mu(T),
-//          kappa = mu cp / Pr; isothermal no-slip walls get a one-sided
-//          normal gradient; explicit Euler update in FP64.
+//  pass F  one flux per face, evaluated for the cell with the lower id (the
+//          other cell takes its negative): reconstruction of each side to the
+//          face midpoint, q_f = q + phi (grad q . r_f) with that side's own
+//          offset r_f, first order where the reconstructed rho or p is not
+//          positive; AUSM+ or Roe with the neighbour side as "L" and the
+//          owner's inward normal (the reference's convention,
+//          computeslope2d.jl:129); viscous flux with the edge-corrected
+//          average face gradient, Sutherland mu(T), kappa = mu cp / Pr;
+//          isothermal no-slip walls get a one-sided normal gradient; each cell
+//          sums its faces in slot order in FP64 and takes an explicit Euler
+//          step.
 //
 // 'mixed' evaluates slopes, limiter, reconstruction and face fluxes in float,
 // and keeps geometry, the flux sum, the update and p/T recovery in double.
@@ -36,8 +40,10 @@ inline double D(const double* a, int64_t n, int64_t i, int k) { return a[(int64_
 struct Geo {
   int nf;
   int32_t nb[4];       // 0-based neighbour or -1
+  int32_t kn[4];       // the neighbour's slot of the shared face
   int32_t code[4];
   double nx[4], ny[4], len[4], rx[4], ry[4], dx[4], dy[4], wx[4], wy[4];
+  double rnx[4], rny[4];  // face midpoint - neighbour centroid, from the neighbour's own vertices
   double area, rarea, hmin;
 };
 
@@ -82,9 +88,11 @@ std::vector<Geo> geometry(const orc_mesh* m) {
           if (I(m->cell_stiffness, n, e, t) == i1) kj = t;
         double rj[2];
         face_mid(m, e, kj, rj);
+        G.kn[q] = kj - 1;
+        G.rnx[q] = rj[0]; G.rny[q] = rj[1];
         G.dx[q] = r[0] - rj[0]; G.dy[q] = r[1] - rj[1];
       } else {
-        G.nb[q] = -1; G.code[q] = e;
+        G.nb[q] = -1; G.code[q] = e; G.kn[q] = -1; G.rnx[q] = 0.0; G.rny[q] = 0.0;
         const double rn = r[0] * G.nx[q] + r[1] * G.ny[q];
         G.dx[q] = 2.0 * rn * G.nx[q]; G.dy[q] = 2.0 * rn * G.ny[q];
       }
@@ -283,25 +291,26 @@ int ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* 
   }
   if (grad8) for (int64_t t = 0; t < 8 * n; ++t) grad8[t] = (double)gr[t];
   if (phi4) for (int64_t t = 0; t < 4 * n; ++t) phi4[t] = (double)ph[t];
-  // ---- pass F
+  // ---- pass F: face fluxes (owner = lower cell id), then per-cell sums
   const R cp = gR * RR / (gR - 1);
+  std::vector<double> ff((size_t)16 * n, 0.0);  // [slot][var][cell]: flux leaving the cell through the face
   std::vector<double> wall(force2 ? 2 * n : 0, 0.0);
 #pragma omp parallel for num_threads(g_threads) schedule(static)
   for (int64_t i = 0; i < n; ++i) {
     const Geo& c = G[i];
     R qi[4];
     for (int v = 0; v < 4; ++v) qi[v] = q[v * n + i];
-    double F[4] = {0, 0, 0, 0};
     for (int k = 0; k < c.nf; ++k) {
+      const int32_t j = c.nb[k];
+      if (j >= 0 && j < i) continue;  // evaluated by the neighbour
       const R nx = (R)c.nx[k], ny = (R)c.ny[k];
-      R qa[4];  // own side at the face
+      R qa[4];  // owner side at the face
       for (int v = 0; v < 4; ++v)
         qa[v] = qi[v] + ph[v * n + i] * (gr[(2 * v) * n + i] * (R)c.rx[k] + gr[(2 * v + 1) * n + i] * (R)c.ry[k]);
       if (!(qa[0] > 0) || !(qa[3] > 0)) for (int v = 0; v < 4; ++v) qa[v] = qi[v];
-      R qb[4];  // neighbour side at the face
-      const int32_t j = c.nb[k];
+      R qb[4];  // neighbour side at the face, from the neighbour's own offset
       if (j >= 0) {
-        const R rxj = (R)(c.rx[k] - c.dx[k]), ryj = (R)(c.ry[k] - c.dy[k]);
+        const R rxj = (R)c.rnx[k], ryj = (R)c.rny[k];
         for (int v = 0; v < 4; ++v)
           qb[v] = q[v * n + j] + ph[v * n + j] * (gr[(2 * v) * n + j] * rxj + gr[(2 * v + 1) * n + j] * ryj);
         if (!(qb[0] > 0) || !(qb[3] > 0)) for (int v = 0; v < 4; ++v) qb[v] = q[v * n + j];
@@ -353,14 +362,23 @@ int ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* 
       }
       double fk[4];
       for (int v = 0; v < 4; ++v) fk[v] = ((double)e[v] + (double)gv[v]) * c.len[k];
-      for (int v = 0; v < 4; ++v) F[v] += fk[v];
-      if (j < 0 && force2) {
+      for (int v = 0; v < 4; ++v) ff[((size_t)k * 4 + v) * n + i] = fk[v];
+      if (j >= 0) {
+        for (int v = 0; v < 4; ++v) ff[((size_t)c.kn[k] * 4 + v) * n + j] = -fk[v];
+      } else if (force2) {
         const int idx = -c.code[k];
         if (idx <= ORC_MAX_BC && (p->bc[idx].kind == ORC_BC_ISOTHERMAL_WALL || p->bc[idx].kind == ORC_BC_SLIP_WALL)) {
           wall[2 * i] += fk[1]; wall[2 * i + 1] += fk[2];
         }
       }
     }
+  }
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const Geo& c = G[i];
+    double F[4] = {0, 0, 0, 0};
+    for (int k = 0; k < c.nf; ++k)
+      for (int v = 0; v < 4; ++v) F[v] += ff[((size_t)k * 4 + v) * n + i];
     for (int v = 0; v < 4; ++v) Un[v * n + i] = U[v * n + i] - F[v] * dt * c.rarea;
   }
   if (force2) {
