@@ -378,7 +378,7 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
   // --- face list of the ns flux pass: each interior face once, owned by the
   // cell with the lower caller id; faces numbered in owner order
   std::vector<int32_t> h_fc0, h_fc1, h_fwall, h_cface(nf * nc, -1);
-  std::vector<double> h_fnx, h_fny, h_flen, h_fr0[2], h_fr1[2], h_fd[2];
+  std::vector<double> h_fnx, h_fny, h_flen, h_fr0[2], h_fr1[2];
   for (int64_t c = 0; c < nc; ++c) {
     for (int k = 0; k < h_nv[c]; ++k) {
       const int64_t s = (int64_t)k * nc + c;
@@ -396,7 +396,6 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
       for (int q = 0; q < 2; ++q) {
         h_fr0[q].push_back(h_rf[((int64_t)k * 2 + q) * nc + c]);
         h_fr1[q].push_back(h_rn[((int64_t)k * 2 + q) * nc + c]);
-        h_fd[q].push_back(h_dc[((int64_t)k * 2 + q) * nc + c]);
       }
     }
   }
@@ -418,7 +417,7 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
     for (size_t q = 0; q < a.size(); ++q) o[q] = (float)a[q];
     return o;
   };
-  const std::vector<double> h_fr0p = pair(h_fr0), h_fr1p = pair(h_fr1), h_fdp = pair(h_fd);
+  const std::vector<double> h_fr0p = pair(h_fr0), h_fr1p = pair(h_fr1);
 
   MeshView& v = M->v;
   v.nc = nc; v.nn = nn; v.nf = nf; v.ncl = ncl; v.has_quads = has_quads; v.n_wall = n_wall; v.max_hx = max_hx;
@@ -463,12 +462,12 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
   rc |= upload(al, &v.f_len, h_flen, B);
   rc |= upload(al, &v.f_r0, h_fr0p, B);
   rc |= upload(al, &v.f_r1, h_fr1p, B);
-  rc |= upload(al, &v.f_d, h_fdp, B);
   rc |= upload(al, &v.f_nx32, to_float(h_fnx), B);
   rc |= upload(al, &v.f_ny32, to_float(h_fny), B);
   rc |= upload(al, &v.f_r032, to_float(h_fr0p), B);
   rc |= upload(al, &v.f_r132, to_float(h_fr1p), B);
-  rc |= upload(al, &v.f_d32, to_float(h_fdp), B);
+  rc |= upload(al, &v.f_len32, to_float(h_flen), B);
+  rc |= upload(al, &v.hmin32, to_float(h_hmin), B);
   rc |= upload(al, &v.cface, h_cface, B);
   rc |= upload(al, &v.cl, h_cl, B);
   rc |= upload(al, &v.w, h_w, B);
@@ -853,15 +852,14 @@ static double bytes_ref_gg_nodes(const MeshView& v) {
 // ns: t = bytes of the compute type (geometry read at t except len, rarea,
 // hmin); gathered neighbour data counted once at unique size
 static double bytes_ns_grad(const MeshView& v, double t) {
-  return (double)v.nc * (4 * t /*q*/ + 12 /*nb*/ + 6 * t /*lsq*/ + 6 * t /*rf*/ + 8 /*hmin*/ + 16 * t /*record*/);
+  return (double)v.nc * (4 * t /*q*/ + 12 /*nb*/ + 6 * t /*lsq*/ + 6 * t /*rf*/ + t /*hmin*/ + 16 * t /*record*/);
 }
 static double bytes_ns_face(const MeshView& v, double t) {
-  const double geo = t == 4.0 ? 32 /*nx,ny,r0,r1,d in float*/ : 64 /*in double*/;
-  return (double)v.n_faces * (8 /*cells*/ + geo + 8 /*len*/ + 4 * t /*flux write*/) +
+  return (double)v.n_faces * (8 /*cells*/ + 7 * t /*nx,ny,len,r0,r1*/ + 4 * t /*flux write*/) +
          (double)v.nc * 16 * t /*records, gathered at unique size*/;
 }
 static double bytes_ns_update(const MeshView& v, double t) {
-  return (double)v.nc * (4.0 * v.nf /*face ids*/ + 32 /*U old*/ + 32 /*U new*/ + 8 /*1/A*/ + 8 /*h*/ +
+  return (double)v.nc * (4.0 * v.nf /*face ids*/ + 32 /*U old*/ + 32 /*U new*/ + 8 /*1/A*/ + t /*h*/ +
                          4 * t /*q new*/) +
          (double)v.n_faces * 4 * t /*face fluxes, unique*/;
 }
@@ -951,14 +949,19 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       a.cfl = rp->cfl;
       a.wall_f = wall ? S->wall_f : nullptr;
       a.wall_code = rp->wall_code;
+      a.fold_prev = s > 0;
+      a.nparts = nparts;
+      a.drag = S->drag;
       GD_HIP(T.run(mixed ? "ns_grad_fp32" : "ns_grad_fp64", bytes_ns_grad(v, t), [&] { return grad(a, S->stream); }));
       GD_HIP(T.run(mixed ? "ns_face_fp32" : "ns_face_fp64", bytes_ns_face(v, t), [&] { return face(a, S->stream); }));
       GD_HIP(T.run(mixed ? "ns_update_fp32" : "ns_update_fp64", bytes_ns_update(v, t),
                    [&] { return update(a, S->stream); }));
-      GD_HIP(launch_acc_fold(S->part, nparts, &S->acc[(s + 1) % 3], S->stream));
-      if (wall) GD_HIP(launch_wall_sum(S->wall_f, v.n_wall, S->drag, S->stream));
       S->cur ^= 1;
       tq ^= 1;
+    }
+    if (n_steps > 0) {  // the last step's reductions (earlier ones ride on the next gradient launch)
+      GD_HIP(launch_acc_fold(S->part, nparts, &S->acc[n_steps % 3], S->stream));
+      if (wall) GD_HIP(launch_wall_sum(S->wall_f, v.n_wall, S->drag, S->stream));
     }
     GD_HIP(launch_update_variables(v, P.Gamma, S->U[S->cur], S->U[S->cur], S->Q[S->qc], S->aS, S->vmax, nullptr,
                                    S->stream));

@@ -49,6 +49,7 @@ struct MeshView {
   const double* rarea; // 1/A
   const double* hx;    // longest edge (HX)
   const double* hmin;  // 2A / perimeter
+  const float* hmin32;
   const double* xc;    // centroid (cell_mid_points)
   const double* yc;
   // ns geometry
@@ -72,12 +73,11 @@ struct MeshView {
   const double* f_len;
   const double* f_r0;   // [2][n_faces] face midpoint - owner centroid
   const double* f_r1;   // [2][n_faces] face midpoint - neighbour centroid (its own vertices)
-  const double* f_d;    // [2][n_faces] neighbour centroid - owner centroid (ghost offset at a boundary)
   const float* f_nx32;
   const float* f_ny32;
   const float* f_r032;
   const float* f_r132;
-  const float* f_d32;
+  const float* f_len32;
   const int32_t* cface; // [nf][nc]: 2 f + (1 if the cell is the face's neighbour), -1 for no face
   // per node
   const int32_t* cl;   // [ncl][nn] cluster cell ids, -1 padded
@@ -200,13 +200,17 @@ struct NsStepArgs {
   double dt, cfl;
   double* wall_f;
   int32_t wall_code;
+  // fold_prev != 0: two extra blocks of the gradient launch fold the previous
+  // update's partials into acc[step % 3] and sum wall_f (the previous face
+  // pass) into drag
+  int32_t fold_prev;
+  int64_t nparts;
+  double* drag;
 };
 hipError_t launch_ns_prims_f64(int64_t nc, double gamma, const double* U, void* Q, hipStream_t s);
 hipError_t launch_ns_grad_f64(const NsStepArgs& a, hipStream_t s);
-hipError_t launch_ns_flux_f64(const NsStepArgs& a, hipStream_t s);  // face fluxes + cell update
 hipError_t launch_ns_prims_f32(int64_t nc, double gamma, const double* U, void* Q, hipStream_t s);
 hipError_t launch_ns_grad_f32(const NsStepArgs& a, hipStream_t s);
-hipError_t launch_ns_flux_f32(const NsStepArgs& a, hipStream_t s);
 hipError_t launch_ns_face_f64(const NsStepArgs& a, hipStream_t s);
 hipError_t launch_ns_update_f64(const NsStepArgs& a, hipStream_t s);
 hipError_t launch_ns_face_f32(const NsStepArgs& a, hipStream_t s);

@@ -18,6 +18,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "gd_fold.hpp"
 #include "gd_internal.hpp"
 
 namespace gd {
@@ -42,7 +43,35 @@ struct NsKArgs {
   double dt, cfl;
   double* wall_f;
   int32_t wall_code;
+  int32_t fold_prev;  // see NsStepArgs
+  int64_t nparts;
+  double* drag;
 };
+
+template <class T>
+NsKArgs<T> kargs(const NsStepArgs& a) {
+  NsKArgs<T> k;
+  k.m = a.m;
+  k.prm = a.prm;
+  k.Uo = a.Uo;
+  k.Un = a.Un;
+  k.Q = static_cast<const T*>(a.Q);
+  k.Qw = static_cast<T*>(a.Qw);
+  k.rec = static_cast<T*>(a.rec);
+  k.fflux = static_cast<T*>(a.fflux);
+  k.acc = a.acc;
+  k.part = a.part;
+  k.step = a.step;
+  k.dt_policy = a.dt_policy;
+  k.dt = a.dt;
+  k.cfl = a.cfl;
+  k.wall_f = a.wall_f;
+  k.wall_code = a.wall_code;
+  k.fold_prev = a.fold_prev;
+  k.nparts = a.nparts;
+  k.drag = a.drag;
+  return k;
+}
 
 // 4 consecutive values: one 16 B load (float) or two (double)
 __device__ __forceinline__ void ld4(const float* p, float* o) {
@@ -177,37 +206,38 @@ __global__ __launch_bounds__(kBlock) void k_ns_prims(int64_t nc, double g, const
 }
 
 // face geometry at the compute type: FP64 arrays for the FP64 step, the float
-// copies for the FP32-mixed step
+// copies for the FP32-mixed step.  Geometry is read once per step, so it is
+// loaded non-temporally and leaves the caches to the gathered cell records.
+template <class U>
+__device__ __forceinline__ U ntl(const U* p) { return __builtin_nontemporal_load(p); }
+
 template <class T>
 struct FaceGeo {
-  __device__ __forceinline__ static T nx(const MeshView& m, int64_t s) {
-    if constexpr (sizeof(T) == 4) return m.fnx32[s]; else return m.fnx[s];
-  }
-  __device__ __forceinline__ static T ny(const MeshView& m, int64_t s) {
-    if constexpr (sizeof(T) == 4) return m.fny32[s]; else return m.fny[s];
-  }
+  static constexpr bool F = sizeof(T) == 4;
+  __device__ __forceinline__ static T nx(const MeshView& m, int64_t s) { return F ? ntl(&m.fnx32[s]) : ntl(&m.fnx[s]); }
+  __device__ __forceinline__ static T ny(const MeshView& m, int64_t s) { return F ? ntl(&m.fny32[s]) : ntl(&m.fny[s]); }
   // face midpoint - own centroid, component c of slot k
   __device__ __forceinline__ static T rf(const MeshView& m, int k, int c, int64_t i) {
-    if constexpr (sizeof(T) == 4) return m.rf32[(2 * k + c) * m.nc + i]; else return m.rf[(2 * k + c) * m.nc + i];
+    const int64_t o = (2 * k + c) * m.nc + i;
+    return F ? ntl(&m.rf32[o]) : ntl(&m.rf[o]);
   }
   __device__ __forceinline__ static T lsq(const MeshView& m, int k, int c, int64_t i) {
-    if constexpr (sizeof(T) == 4) return m.lsq32[(2 * k + c) * m.nc + i]; else return m.lsq[(2 * k + c) * m.nc + i];
+    const int64_t o = (2 * k + c) * m.nc + i;
+    return F ? ntl(&m.lsq32[o]) : ntl(&m.lsq[o]);
   }
-  // face-list geometry (owner's inward normal, offsets)
-  __device__ __forceinline__ static T fnx(const MeshView& m, int64_t f) {
-    if constexpr (sizeof(T) == 4) return m.f_nx32[f]; else return m.f_nx[f];
+  __device__ __forceinline__ static T hmin(const MeshView& m, int64_t i) { return F ? ntl(&m.hmin32[i]) : ntl(&m.hmin[i]); }
+  // face-list geometry: the owner's inward normal, length, offsets
+  __device__ __forceinline__ static T fnx(const MeshView& m, int64_t f) { return F ? ntl(&m.f_nx32[f]) : ntl(&m.f_nx[f]); }
+  __device__ __forceinline__ static T fny(const MeshView& m, int64_t f) { return F ? ntl(&m.f_ny32[f]) : ntl(&m.f_ny[f]); }
+  __device__ __forceinline__ static T flen(const MeshView& m, int64_t f) {
+    return F ? ntl(&m.f_len32[f]) : ntl(&m.f_len[f]);
   }
-  __device__ __forceinline__ static T fny(const MeshView& m, int64_t f) {
-    if constexpr (sizeof(T) == 4) return m.f_ny32[f]; else return m.f_ny[f];
-  }
+  // face midpoint - owner centroid (fr0) and - neighbour centroid (fr1), component c
   __device__ __forceinline__ static T fr0(const MeshView& m, int c, int64_t f) {
-    if constexpr (sizeof(T) == 4) return m.f_r032[c * m.n_faces + f]; else return m.f_r0[c * m.n_faces + f];
+    return F ? ntl(&m.f_r032[c * m.n_faces + f]) : ntl(&m.f_r0[c * m.n_faces + f]);
   }
   __device__ __forceinline__ static T fr1(const MeshView& m, int c, int64_t f) {
-    if constexpr (sizeof(T) == 4) return m.f_r132[c * m.n_faces + f]; else return m.f_r1[c * m.n_faces + f];
-  }
-  __device__ __forceinline__ static T fd(const MeshView& m, int c, int64_t f) {
-    if constexpr (sizeof(T) == 4) return m.f_d32[c * m.n_faces + f]; else return m.f_d[c * m.n_faces + f];
+    return F ? ntl(&m.f_r132[c * m.n_faces + f]) : ntl(&m.f_r1[c * m.n_faces + f]);
   }
 };
 
@@ -218,7 +248,14 @@ __global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
   using G = FaceGeo<T>;
   const MeshView& m = a.m;
   const int64_t nc = m.nc;
-  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  // blocks 0 and 1 of a folding launch close the previous step's reductions
+  const unsigned nx = a.fold_prev ? 2u : 0u;
+  if (blockIdx.x < nx) {
+    if (blockIdx.x == 0) fold_acc_block(a.part, a.nparts, &a.acc[a.step % 3]);
+    else if (a.wall_f) wall_sum_block(a.wall_f, m.n_wall, a.drag);
+    return;
+  }
+  const int64_t i = (int64_t)xcd_block(blockIdx.x - nx, gridDim.x - nx) * kBlock + threadIdx.x;
   if (i >= nc) return;
   const int nf = QUADS ? m.nv[i] : 3;
   // every load first: neighbour ids, then all neighbour states (boundary and
@@ -236,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
     wx[k] = G::lsq(m, k, 0, i); wy[k] = G::lsq(m, k, 1, i);
     rx[k] = G::rf(m, k, 0, i); ry[k] = G::rf(m, k, 1, i);
   }
-  const T kh = (T)(a.prm.venkat_K * m.hmin[i]);
+  const T kh = (T)a.prm.venkat_K * G::hmin(m, i);
   T gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0}, qmx[4], qmn[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) { qmx[v] = qi[v]; qmn[v] = qi[v]; }
@@ -329,7 +366,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
   ld16(a.rec + 16 * (int64_t)c0, r0);
   ld16(a.rec + 16 * (int64_t)(c1 >= 0 ? c1 : c0), r1);
   const T nx = G::fnx(m, f), ny = G::fny(m, f);
-  const double len = m.f_len[f];
+  const T len = G::flen(m, f);
   const T rx = G::fr0(m, 0, f), ry = G::fr0(m, 1, f);
   const T g = (T)P.Gamma, Rg = (T)P.RGAS;
   const T* qi = r0;
@@ -375,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
         g_j[0][c] = gj[2 + c]; g_j[1][c] = gj[4 + c];
         g_j[2][c] = (gj[6 + c] - Tj * Rg * gj[c]) * rRj;
       }
-      const T ddx = G::fd(m, 0, f), ddy = G::fd(m, 1, f);
+      const T ddx = rx - G::fr1(m, 0, f), ddy = ry - G::fr1(m, 1, f);  // centroid j - centroid i
       const T rdl = 1 / sqrt(ddx * ddx + ddy * ddy);
       const T tx = ddx * rdl, ty = ddy * rdl;
       const T dphi[3] = {qj[1] - qi[1], qj[2] - qi[2], Tj - Ti};
@@ -405,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
   double fk[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    fk[v] = ((double)e[v] + (double)gv[v]) * len;
+    fk[v] = (double)((e[v] + gv[v]) * len);
     a.fflux[v * nF + f] = (T)fk[v];
   }
   if (c1 < 0 && a.wall_f && a.wall_code == c1 + 1) {
@@ -465,15 +502,16 @@ __global__ __launch_bounds__(kBlock) void k_ns_update(NsKArgs<T> a) {
     st4(a.Qw + 4 * i, (T)rho, (T)ux, (T)uy, (T)p);
     const double as = sqrt(Gm * p / rho);
     const double vm = sqrt(ux * ux + uy * uy) + as;
+    const double h = (double)FaceGeo<T>::hmin(m, i);
     double lam = vm;
     if (VISC) {
       const double Tc = p / (rho * P.RGAS);
       const double t = Tc / P.T_ref;
       const double mu = P.mu_ref * (t * sqrt(t)) * (P.T_ref + P.S_suth) / (Tc + P.S_suth);
-      lam = vm + 2.0 * Gm * mu / (P.Pr * rho * m.hmin[i]);
+      lam = vm + 2.0 * Gm * mu / (P.Pr * rho * h);
     }
     vmx = vm;
-    hdt = m.hmin[i] / lam;
+    hdt = h / lam;
     bad = (!(rho > 0.0) || !(p > 0.0) || vm != vm) ? 1u : 0u;
     if (vm != vm || lam != lam) { vmx = 0.0; hdt = __longlong_as_double(0x7ff0000000000000ll); }
   }
@@ -503,8 +541,9 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock
 
 template <class T>
 hipError_t launch_grad(const NsKArgs<T>& a, hipStream_t s) {
-  if (a.m.has_quads) hipLaunchKernelGGL((k_ns_grad<T, true>), dim3(nblocks(a.m.nc)), dim3(kBlock), 0, s, a);
-  else hipLaunchKernelGGL((k_ns_grad<T, false>), dim3(nblocks(a.m.nc)), dim3(kBlock), 0, s, a);
+  const dim3 g(nblocks(a.m.nc) + (a.fold_prev ? 2 : 0));
+  if (a.m.has_quads) hipLaunchKernelGGL((k_ns_grad<T, true>), g, dim3(kBlock), 0, s, a);
+  else hipLaunchKernelGGL((k_ns_grad<T, false>), g, dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 

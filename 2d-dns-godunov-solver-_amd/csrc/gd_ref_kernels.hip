@@ -10,6 +10,7 @@
 //                 and unweighted cluster means of rho*u, rho*v, E
 //   k_ref_gg1     Green-Gauss gradients of rho*u, rho*v, E   (viscous only)
 //   k_ref_gg_nodes  cluster means of those gradients         (viscous only)
+#include "gd_fold.hpp"
 #include "gd_internal.hpp"
 #include "gd_ref_math.hpp"
 
@@ -385,28 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_prime_acc(MeshView m, gd_params p, c
 // Folds the per-block partials of reduce_acc into one accumulator slot, in a
 // fixed order (max and min are exact; the count is an integer sum).
 __global__ __launch_bounds__(1024) void k_acc_fold(const double* part, int64_t nparts, Acc* w) {
-  __shared__ double s_max[1024], s_min[1024], s_bad[1024];
-  double vmax = 0.0, hdt = __longlong_as_double(0x7ff0000000000000ll), bad = 0.0;
-  for (int64_t b = threadIdx.x; b < nparts; b += 1024) {
-    vmax = fmax(vmax, part[3 * b]);
-    hdt = fmin(hdt, part[3 * b + 1]);
-    bad += part[3 * b + 2];
-  }
-  s_max[threadIdx.x] = vmax; s_min[threadIdx.x] = hdt; s_bad[threadIdx.x] = bad;
-  __syncthreads();
-  for (int o = 512; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
-      s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + o]);
-      s_min[threadIdx.x] = fmin(s_min[threadIdx.x], s_min[threadIdx.x + o]);
-      s_bad[threadIdx.x] += s_bad[threadIdx.x + o];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    w->vmax_bits = (unsigned long long)__double_as_longlong(s_max[0]);
-    w->hdt_bits = (unsigned long long)__double_as_longlong(s_min[0]);
-    w->nonphysical = (unsigned)s_bad[0];
-  }
+  fold_acc_block(part, nparts, w);
 }
 
 // Deterministic sums of (Unew - Uold)^2 per variable: per-block partials, then
@@ -450,16 +430,7 @@ __global__ __launch_bounds__(kBlock) void k_fold(const double* part, int64_t npa
 // wall force of one step: fold the per-face contributions in a fixed order;
 // out = {fx, fy, running sum fx, running sum fy}
 __global__ __launch_bounds__(kBlock) void k_wall_sum(const double* wf, int32_t n, double* out) {
-  __shared__ double sx[kBlock], sy[kBlock];
-  double ax = 0.0, ay = 0.0;
-  for (int b = threadIdx.x; b < n; b += kBlock) { ax += wf[2 * b]; ay += wf[2 * b + 1]; }
-  sx[threadIdx.x] = ax; sy[threadIdx.x] = ay;
-  __syncthreads();
-  for (int o = kBlock / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < o) { sx[threadIdx.x] += sx[threadIdx.x + o]; sy[threadIdx.x] += sy[threadIdx.x + o]; }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) { out[0] = sx[0]; out[1] = sy[0]; out[2] += sx[0]; out[3] += sy[0]; }
+  wall_sum_block(wf, n, out);
 }
 
 // aSoundCells and VMAXCells from the primitives (initfields_TGV2d.jl:47-48,
