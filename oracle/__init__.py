@@ -108,6 +108,8 @@ def lib():
         L.orc_local_dt.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcFields), C.c_double]
         L.orc_ns_step.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcNsParams), C.c_double, dp, dp, dp, dp, dp]
         L.orc_set_threads.argtypes = [C.c_int32]
+        L.orc_ns_step_tl.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcNsParams), C.c_double, dp, dp, dp, dp, dp,
+                                     dp, dp]
         L.orc_set_threads.restype = None
         L.orc_ns_dt.restype = C.c_double
         L.orc_ns_dt.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcNsParams), dp, C.c_double]
@@ -321,6 +323,21 @@ def ns_step(mv: MeshView, prm, dt, U, want_grad=False):
     f2 = np.zeros(2)
     _rc(lib().orc_ns_step(C.byref(mv.c), C.byref(prm), dt, _p(_chk_f64(U)), _p(UN), _p(g8), _p(p4), _p(f2)), "ns")
     return UN, g8, p4, f2
+
+
+def ns_step_tl(mv: MeshView, prm, dt, U, dU, dbc=None):
+    """Tangent-linear FP64 ns step (dual numbers): returns (U_new, dU_new, force2, dforce2).
+    dbc: tangent of the fixed boundary states, shape (MAX_BC + 1, 4), or None."""
+    n = mv.mesh.nCells
+    UN = np.zeros((n, 4), order="F")
+    dUN = np.zeros((n, 4), order="F")
+    f2, df2 = np.zeros(2), np.zeros(2)
+    db = None if dbc is None else np.ascontiguousarray(dbc, dtype=np.float64).reshape(-1)
+    if db is not None and db.size != 4 * (MAX_BC + 1):
+        raise ValueError("dbc must have (MAX_BC + 1) x 4 entries")
+    _rc(lib().orc_ns_step_tl(C.byref(mv.c), C.byref(prm), dt, _p(_chk_f64(U)), _p(_chk_f64(dU)), _p(db), _p(UN),
+                             _p(dUN), _p(f2), _p(df2)), "ns_tl")
+    return UN, dUN, f2, df2
 
 
 def ns_dt(mv: MeshView, prm, U, cfl):

@@ -113,6 +113,9 @@ int orc_ns_step(const orc_mesh* m, const orc_ns_params* prm, double dt, const do
                 double* grad8 /* optional nCells x 8 */, double* phi4 /* optional nCells x 4 */,
                 double* wall_force2 /* optional: pressure+viscous force on code-wall faces */);
 double orc_ns_dt(const orc_mesh* m, const orc_ns_params* prm, const double* U, double cfl);
+/* tangent-linear FP64 ns step (forward-mode dual numbers); dbc: [ORC_MAX_BC+1][4] or NULL */
+int orc_ns_step_tl(const orc_mesh* m, const orc_ns_params* prm, double dt, const double* UOld, const double* dUOld,
+                   const double* dbc, double* UNew, double* dUNew, double* force2, double* dforce2);
 void orc_set_threads(int32_t n); /* OpenMP threads of the ns oracle (default 1) */
 
 #ifdef __cplusplus

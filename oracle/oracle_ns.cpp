@@ -34,6 +34,46 @@ namespace {
 
 int g_threads = 1;
 
+using std::fabs;
+using std::fmax;
+using std::fmin;
+using std::sqrt;
+
+// Forward-mode dual number (value, one directional derivative): the ns step
+// instantiated with it gives the exact tangent-linear step, the reference
+// against which a discrete adjoint is checked (<adjoint, dU0> = dJ[dU0]).
+struct Dual {
+  double v, d;
+  Dual() : v(0.0), d(0.0) {}
+  Dual(double x) : v(x), d(0.0) {}  // NOLINT: constants enter the scheme as duals
+  Dual(double x, double dx) : v(x), d(dx) {}
+};
+inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+inline Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
+inline Dual& operator-=(Dual& a, Dual b) { a = a - b; return a; }
+inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
+inline bool operator>(Dual a, Dual b) { return a.v > b.v; }
+inline bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
+inline bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+inline bool operator!=(Dual a, Dual b) { return a.v != b.v; }
+inline Dual sqrt(Dual a) {
+  const double r = std::sqrt(a.v);
+  return {r, a.d / (2.0 * r)};
+}
+inline Dual fabs(Dual a) { return a.v < 0.0 ? -a : a; }
+inline Dual fmin(Dual a, Dual b) { return b.v < a.v ? b : a; }
+inline Dual fmax(Dual a, Dual b) { return b.v > a.v ? b : a; }
+inline double val(double x) { return x; }
+inline double val(float x) { return x; }
+inline double val(Dual x) { return x.v; }
+
 inline int32_t I(const int32_t* a, int64_t n, int64_t i, int k) { return a[(int64_t)(k - 1) * n + (i - 1)]; }
 inline double D(const double* a, int64_t n, int64_t i, int k) { return a[(int64_t)(k - 1) * n + (i - 1)]; }
 
@@ -127,8 +167,9 @@ const std::vector<Geo>& cached_geometry(const orc_mesh* m) {
   return g;
 }
 
-inline void prims(const double* U, int64_t n, int64_t i, double g, double* q) {
-  const double r = U[i], mx = U[n + i], my = U[2 * n + i], E = U[3 * n + i];
+template <class D>
+inline void prims(const D* U, int64_t n, int64_t i, double g, D* q) {
+  const D r = U[i], mx = U[n + i], my = U[2 * n + i], E = U[3 * n + i];
   q[0] = r;
   q[1] = mx / r;
   q[2] = my / r;
@@ -140,17 +181,18 @@ template <class R>
 inline R sutherland(R T, const orc_ns_params* p) {
   const R Tr = (R)p->T_ref, S = (R)p->S;
   const R t = T / Tr;
-  return (R)p->mu_ref * (t * std::sqrt(t)) * (Tr + S) / (T + S);
+  return (R)p->mu_ref * (t * sqrt(t)) * (Tr + S) / (T + S);
 }
 
 // ghost state of a boundary face for the ns scheme
+// (bcf[idx] = the fixed state of code -idx at the compute type)
 template <class R>
-inline void ghost(const orc_ns_params* p, int code, const R* q, R nx, R ny, R* g) {
+inline void ghost(const orc_ns_params* p, const R (*bcf)[4], int code, const R* q, R nx, R ny, R* g) {
   const int idx = -code;
   const orc_bc* b = (idx >= 0 && idx <= ORC_MAX_BC) ? &p->bc[idx] : nullptr;
   const int kind = b ? b->kind : ORC_BC_EXTRAPOLATE;
   if (kind == ORC_BC_FIXED) {
-    for (int v = 0; v < 4; ++v) g[v] = (R)b->data[v];
+    for (int v = 0; v < 4; ++v) g[v] = bcf[idx][v];
   } else if (kind == ORC_BC_SLIP_WALL || kind == ORC_BC_ISOTHERMAL_WALL) {
     const R un = q[1] * nx + q[2] * ny;
     g[0] = q[0]; g[1] = q[1] - (R)2 * un * nx; g[2] = q[2] - (R)2 * un * ny; g[3] = q[3];
@@ -163,24 +205,24 @@ inline void ghost(const orc_ns_params* p, int code, const R* q, R nx, R ny, R* g
 // returns -(F . n) (same splitting as the reference scheme)
 template <class R>
 inline void ausm(const R* L, const R* Rr, R nx, R ny, R g, R* out) {
-  auto mp = [](R M) { const R d = M * M - 1; return std::fabs(M) >= 1 ? (R)0.5 * (M + std::fabs(M)) : (R)0.25 * (M + 1) * (M + 1) + (R)0.125 * d * d; };
-  auto mm = [](R M) { const R d = M * M - 1; return std::fabs(M) >= 1 ? (R)0.5 * (M - std::fabs(M)) : -(R)0.25 * (M - 1) * (M - 1) - (R)0.125 * d * d; };
-  auto pp = [](R M) { const R d = M * M - 1; return std::fabs(M) >= 1 ? (M > 0 ? (R)1 : (R)0) : (R)0.25 * (M + 1) * (M + 1) * (2 - M) + (R)0.1875 * M * d * d; };
-  auto pm = [](R M) { const R d = M * M - 1; return std::fabs(M) >= 1 ? (M < 0 ? (R)1 : (R)0) : (R)0.25 * (M - 1) * (M - 1) * (2 + M) - (R)0.1875 * M * d * d; };
+  auto mp = [](R M) { const R d = M * M - 1; return fabs(M) >= 1 ? (R)0.5 * (M + fabs(M)) : (R)0.25 * (M + 1) * (M + 1) + (R)0.125 * d * d; };
+  auto mm = [](R M) { const R d = M * M - 1; return fabs(M) >= 1 ? (R)0.5 * (M - fabs(M)) : -(R)0.25 * (M - 1) * (M - 1) - (R)0.125 * d * d; };
+  auto pp = [](R M) { const R d = M * M - 1; return fabs(M) >= 1 ? (M > 0 ? (R)1 : (R)0) : (R)0.25 * (M + 1) * (M + 1) * (2 - M) + (R)0.1875 * M * d * d; };
+  auto pm = [](R M) { const R d = M * M - 1; return fabs(M) >= 1 ? (M < 0 ? (R)1 : (R)0) : (R)0.25 * (M - 1) * (M - 1) * (2 + M) - (R)0.1875 * M * d * d; };
   const R rL = L[0], uL = L[1], vL = L[2], pL = L[3];
   const R rR = Rr[0], uR = Rr[1], vR = Rr[2], pR = Rr[3];
   const R htL = (R)0.5 * (uL * uL + vL * vL) + g / (g - 1) * pL / rL;
   const R htR = (R)0.5 * (uR * uR + vR * vR) + g / (g - 1) * pR / rR;
   const R c = 2 * (g - 1) / (g + 1);
   const R unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
-  const R asL = std::sqrt(c * htL), asR = std::sqrt(c * htR);
-  const R aL = asL * std::fmin((R)1, asL / std::fabs(unL));
-  const R aR = asR * std::fmin((R)1, asR / std::fabs(unR));
-  const R a12 = std::fmin(aL, aR);
+  const R asL = sqrt(c * htL), asR = sqrt(c * htR);
+  const R aL = asL * fmin((R)1, asL / fabs(unL));
+  const R aR = asR * fmin((R)1, asR / fabs(unR));
+  const R a12 = fmin(aL, aR);
   const R ML = unL / a12, MR = unR / a12;
   const R m = mp(ML) + mm(MR);
   const R p12 = pp(ML) * pL + pm(MR) * pR;
-  const R mpl = (R)0.5 * (m + std::fabs(m)), mmi = (R)0.5 * (m - std::fabs(m));
+  const R mpl = (R)0.5 * (m + fabs(m)), mmi = (R)0.5 * (m - fabs(m));
   out[0] = -(a12 * (mpl * rL + mmi * rR));
   out[1] = -(a12 * (mpl * rL * uL + mmi * rR * uR) + p12 * nx);
   out[2] = -(a12 * (mpl * rL * vL + mmi * rR * vR) + p12 * ny);
@@ -199,16 +241,16 @@ inline void roe(const R* L, const R* Rr, R nx, R ny, R g, R* out) {
   const R unR = uR * nx + vR * ny, umR = uR * mx + vR * my;
   const R HL = g / (g - 1) * pL / rL + (R)0.5 * (uL * uL + vL * vL);
   const R HR = g / (g - 1) * pR / rR + (R)0.5 * (uR * uR + vR * vR);
-  const R RT = std::sqrt(rR / rL);
+  const R RT = sqrt(rR / rL);
   const R rho = RT * rL;
   const R u = (uL + RT * uR) / (1 + RT), v = (vL + RT * vR) / (1 + RT), H = (HL + RT * HR) / (1 + RT);
-  const R a = std::sqrt((g - 1) * (H - (R)0.5 * (u * u + v * v)));
+  const R a = sqrt((g - 1) * (H - (R)0.5 * (u * u + v * v)));
   const R un = u * nx + v * ny, um = u * mx + v * my;
   const R dp = pR - pL, dun = unR - unL, dum = umR - umL, drho = rR - rL;
   const R w1 = (dp - rho * a * dun) / (2 * a * a), w2 = rho * dum, w3 = drho - dp / (a * a),
           w4 = (dp + rho * a * dun) / (2 * a * a);
-  R s1 = std::fabs(un - a), s4 = std::fabs(un + a);
-  const R s2 = std::fabs(un);
+  R s1 = fabs(un - a), s4 = fabs(un + a);
+  const R s2 = fabs(un);
   const R dw = (R)0.2 * a;
   if (s1 < dw) s1 = (R)0.5 * (s1 * s1 / dw + dw);
   if (s4 < dw) s4 = (R)0.5 * (s4 * s4 / dw + dw);
@@ -238,9 +280,11 @@ inline void visc_face(const orc_ns_params* p, R cp, R uf, R vf, R Tf, const R (*
   gv[3] = uf * gv[1] + vf * gv[2] + kap * (gf[2][0] * nx + gf[2][1] * ny);
 }
 
-template <class R>
-int ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* U, double* Un, double* grad8,
-            double* phi4, double* force2) {
+// R: slope/limiter/flux arithmetic; D: state, flux sum and update (double, or
+// Dual for the tangent-linear step).  bcf: fixed states per code.
+template <class R, class D>
+int ns_step(const orc_mesh* m, const orc_ns_params* p, const R (*bcf)[4], double dt, const D* U, D* Un,
+            double* grad8, double* phi4, D* force2) {
   const int64_t n = m->nCells;
   const std::vector<Geo>& G = cached_geometry(m);
   const double g = p->gamma;
@@ -248,7 +292,7 @@ int ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* 
   std::vector<R> q(4 * n), gr(8 * n), ph(4 * n);
 #pragma omp parallel for num_threads(g_threads) schedule(static)
   for (int64_t i = 0; i < n; ++i) {
-    double qq[4];
+    D qq[4];
     prims(U, n, i, g, qq);
     for (int v = 0; v < 4; ++v) q[v * n + i] = (R)qq[v];
   }
@@ -261,13 +305,13 @@ int ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* 
     for (int k = 0; k < c.nf; ++k) {
       R qk[4];
       if (c.nb[k] >= 0) for (int v = 0; v < 4; ++v) qk[v] = q[v * n + c.nb[k]];
-      else ghost<R>(p, c.code[k], qi, (R)c.nx[k], (R)c.ny[k], qk);
+      else ghost<R>(p, bcf, c.code[k], qi, (R)c.nx[k], (R)c.ny[k], qk);
       for (int v = 0; v < 4; ++v) {
         const R d = qk[v] - qi[v];
         gx[v] += (R)c.wx[k] * d;
         gy[v] += (R)c.wy[k] * d;
-        qmx[v] = std::fmax(qmx[v], qk[v]);
-        qmn[v] = std::fmin(qmn[v], qk[v]);
+        qmx[v] = fmax(qmx[v], qk[v]);
+        qmn[v] = fmin(qmn[v], qk[v]);
       }
     }
     const R kh = (R)(p->venkat_K * c.hmin);
@@ -282,19 +326,19 @@ int ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* 
         else if (d2 < 0) d1 = qmn[v] - qi[v];
         else continue;
         const R psi = ((d1 * d1 + e2) * d2 + 2 * d2 * d2 * d1) / (d1 * d1 + 2 * d2 * d2 + d1 * d2 + e2) / d2;
-        phi = std::fmin(phi, psi);
+        phi = fmin(phi, psi);
       }
-      ph[v * n + i] = std::fmax(phi, (R)0);
+      ph[v * n + i] = fmax(phi, (R)0);
       gr[(2 * v) * n + i] = gx[v];
       gr[(2 * v + 1) * n + i] = gy[v];
     }
   }
-  if (grad8) for (int64_t t = 0; t < 8 * n; ++t) grad8[t] = (double)gr[t];
-  if (phi4) for (int64_t t = 0; t < 4 * n; ++t) phi4[t] = (double)ph[t];
+  if (grad8) for (int64_t t = 0; t < 8 * n; ++t) grad8[t] = val(gr[t]);
+  if (phi4) for (int64_t t = 0; t < 4 * n; ++t) phi4[t] = val(ph[t]);
   // ---- pass F: face fluxes (owner = lower cell id), then per-cell sums
   const R cp = gR * RR / (gR - 1);
-  std::vector<double> ff((size_t)16 * n, 0.0);  // [slot][var][cell]: flux leaving the cell through the face
-  std::vector<double> wall(force2 ? 2 * n : 0, 0.0);
+  std::vector<D> ff((size_t)16 * n, D(0.0));  // [slot][var][cell]: flux leaving the cell through the face
+  std::vector<D> wall(force2 ? 2 * n : 0, D(0.0));
 #pragma omp parallel for num_threads(g_threads) schedule(static)
   for (int64_t i = 0; i < n; ++i) {
     const Geo& c = G[i];
@@ -315,7 +359,7 @@ int ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* 
           qb[v] = q[v * n + j] + ph[v * n + j] * (gr[(2 * v) * n + j] * rxj + gr[(2 * v + 1) * n + j] * ryj);
         if (!(qb[0] > 0) || !(qb[3] > 0)) for (int v = 0; v < 4; ++v) qb[v] = q[v * n + j];
       } else {
-        ghost<R>(p, c.code[k], qa, nx, ny, qb);
+        ghost<R>(p, bcf, c.code[k], qa, nx, ny, qb);
       }
       R e[4];
       if (p->flux == ORC_FLUX_ROE) roe<R>(qb, qa, nx, ny, gR, e);
@@ -338,7 +382,7 @@ int ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* 
             g_j[2][s] = (gr[(6 + s) * n + j] - Tj * RR * gr[s * n + j]) * rRj;
           }
           const R ddx = (R)c.dx[k], ddy = (R)c.dy[k];
-          const R rdl = 1 / std::sqrt(ddx * ddx + ddy * ddy);
+          const R rdl = 1 / sqrt(ddx * ddx + ddy * ddy);
           const R tx = ddx * rdl, ty = ddy * rdl;
           const R dphi[3] = {qj[1] - qi[1], qj[2] - qi[2], Tj - Ti};
           R gf[3][2];
@@ -360,8 +404,8 @@ int ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* 
           visc_face<R>(p, cp, (R)0, (R)0, Tw, gf, nx, ny, gv);
         }
       }
-      double fk[4];
-      for (int v = 0; v < 4; ++v) fk[v] = ((double)e[v] + (double)gv[v]) * c.len[k];
+      D fk[4];
+      for (int v = 0; v < 4; ++v) fk[v] = ((D)e[v] + (D)gv[v]) * c.len[k];
       for (int v = 0; v < 4; ++v) ff[((size_t)k * 4 + v) * n + i] = fk[v];
       if (j >= 0) {
         for (int v = 0; v < 4; ++v) ff[((size_t)c.kn[k] * 4 + v) * n + j] = -fk[v];
@@ -376,13 +420,13 @@ int ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* 
 #pragma omp parallel for num_threads(g_threads) schedule(static)
   for (int64_t i = 0; i < n; ++i) {
     const Geo& c = G[i];
-    double F[4] = {0, 0, 0, 0};
+    D F[4] = {0.0, 0.0, 0.0, 0.0};
     for (int k = 0; k < c.nf; ++k)
       for (int v = 0; v < 4; ++v) F[v] += ff[((size_t)k * 4 + v) * n + i];
     for (int v = 0; v < 4; ++v) Un[v * n + i] = U[v * n + i] - F[v] * dt * c.rarea;
   }
   if (force2) {
-    double fx = 0.0, fy = 0.0;
+    D fx = 0.0, fy = 0.0;
     for (int64_t i = 0; i < n; ++i) { fx += wall[2 * i]; fy += wall[2 * i + 1]; }
     force2[0] = fx; force2[1] = fy;
   }
@@ -397,8 +441,36 @@ void orc_set_threads(int32_t n) { g_threads = n > 0 ? n : 1; }
 
 int orc_ns_step(const orc_mesh* m, const orc_ns_params* p, double dt, const double* U, double* Un, double* grad8,
                 double* phi4, double* force2) {
-  if (p->mixed) return ns_step<float>(m, p, dt, U, Un, grad8, phi4, force2);
-  return ns_step<double>(m, p, dt, U, Un, grad8, phi4, force2);
+  if (p->mixed) {
+    float bcf[ORC_MAX_BC + 1][4];
+    for (int c = 0; c <= ORC_MAX_BC; ++c)
+      for (int v = 0; v < 4; ++v) bcf[c][v] = (float)p->bc[c].data[v];
+    return ns_step<float, double>(m, p, bcf, dt, U, Un, grad8, phi4, force2);
+  }
+  double bcf[ORC_MAX_BC + 1][4];
+  for (int c = 0; c <= ORC_MAX_BC; ++c)
+    for (int v = 0; v < 4; ++v) bcf[c][v] = p->bc[c].data[v];
+  return ns_step<double, double>(m, p, bcf, dt, U, Un, grad8, phi4, force2);
+}
+
+// Tangent-linear FP64 step: (U, dU) -> (Un, dUn) and the wall force with its
+// derivative; dbc (may be NULL) is the tangent of the fixed boundary states,
+// [ORC_MAX_BC + 1][4].
+int orc_ns_step_tl(const orc_mesh* m, const orc_ns_params* p, double dt, const double* U, const double* dU,
+                   const double* dbc, double* Un, double* dUn, double* force2, double* dforce2) {
+  if (p->mixed) return -1;
+  const int64_t n = m->nCells;
+  std::vector<Dual> u(4 * n), un(4 * n);
+  for (int64_t t = 0; t < 4 * n; ++t) u[t] = Dual(U[t], dU ? dU[t] : 0.0);
+  Dual bcf[ORC_MAX_BC + 1][4];
+  for (int c = 0; c <= ORC_MAX_BC; ++c)
+    for (int v = 0; v < 4; ++v) bcf[c][v] = Dual(p->bc[c].data[v], dbc ? dbc[4 * c + v] : 0.0);
+  Dual f2[2];
+  const int rc = ns_step<Dual, Dual>(m, p, bcf, dt, u.data(), un.data(), nullptr, nullptr, f2);
+  for (int64_t t = 0; t < 4 * n; ++t) { Un[t] = un[t].v; dUn[t] = un[t].d; }
+  if (force2) { force2[0] = f2[0].v; force2[1] = f2[1].v; }
+  if (dforce2) { dforce2[0] = f2[0].d; dforce2[1] = f2[1].d; }
+  return rc;
 }
 
 // CFL dt of the ns scheme: cfl * min_i h_i / (|u|+a + 2 gamma mu/(Pr rho h_i))
@@ -408,14 +480,14 @@ double orc_ns_dt(const orc_mesh* m, const orc_ns_params* p, const double* U, dou
   double best = INFINITY;
   for (int64_t i = 0; i < n; ++i) {
     double q[4];
-    prims(U, n, i, p->gamma, q);
-    const double a = std::sqrt(p->gamma * q[3] / q[0]);
-    double lam = std::sqrt(q[1] * q[1] + q[2] * q[2]) + a;
+    prims<double>(U, n, i, p->gamma, q);
+    const double a = sqrt(p->gamma * q[3] / q[0]);
+    double lam = sqrt(q[1] * q[1] + q[2] * q[2]) + a;
     if (p->viscous) {
       const double T = q[3] / (q[0] * p->R);
       lam += 2.0 * p->gamma * sutherland<double>(T, p) / (p->Pr * q[0] * G[i].hmin);
     }
-    best = std::fmin(best, G[i].hmin / lam);
+    best = fmin(best, G[i].hmin / lam);
   }
   return cfl * best;
 }
