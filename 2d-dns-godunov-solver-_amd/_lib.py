@@ -36,6 +36,7 @@ EXPORTS = (
     "gd_state_destroy", "gd_state_set_params", "gd_state_set_fields", "gd_state_get_fields", "gd_state_set_array",
     "gd_state_get_array", "gd_calc_one_stage", "gd_calc_diff_term", "gd_update_variables", "gd_cells2nodes",
     "gd_run", "gd_sync", "gd_state_device_u", "gd_device_count", "gd_last_error", "gd_version",
+    "gd_adjoint_run",
 )
 
 
@@ -81,6 +82,18 @@ class gd_run_stats(C.Structure):  # noqa: N801
                 ("kernel_bytes", C.c_double * GD_MAX_KERNELS)]
 
 
+class gd_adjoint_params(C.Structure):  # noqa: N801
+    _fields_ = [("n_steps", C.c_int32), ("checkpoint_every", C.c_int32), ("dt", C.c_double),
+                ("flow_time", C.c_double), ("wall_code", C.c_int32), ("reserved", C.c_int32),
+                ("force_weight", C.c_double * 2)]
+
+
+class gd_adjoint_stats(C.Structure):  # noqa: N801
+    _fields_ = [("J", C.c_double), ("force_sum", C.c_double * 2), ("flow_time", C.c_double),
+                ("recomputed_steps", C.c_int32), ("n_checkpoints", C.c_int32), ("storage_bytes", C.c_int64),
+                ("ms_forward", C.c_double), ("ms_reverse", C.c_double)]
+
+
 class GodunovError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"libgodunov error {code}: {msg}")
@@ -117,6 +130,7 @@ def load():
         f.argtypes = [vp]
     L.gd_run.argtypes = [vp, C.c_int32, C.POINTER(gd_run_params), C.POINTER(gd_run_stats)]
     L.gd_state_device_u.argtypes = [vp, C.POINTER(dp)]
+    L.gd_adjoint_run.argtypes = [vp, C.POINTER(gd_adjoint_params), dp, dp, dp, C.POINTER(gd_adjoint_stats)]
     L.gd_device_count.argtypes = [C.POINTER(C.c_int32)]
     L.gd_last_error.argtypes = [C.c_char_p, C.c_size_t]
     L.gd_version.restype = C.c_char_p
@@ -268,6 +282,20 @@ class DeviceState:
             return st
         check(rc)
         return st
+
+    def adjoint(self, ap: gd_adjoint_params, dJ_dUN=None):
+        """gd_adjoint_run: returns (dJ/dU0 (n_cells, 4), dJ/dbc (GD_MAX_BC + 1, 4), stats)."""
+        n = self.mesh.n_cells
+        out = np.zeros((n, 4), order="F")
+        dbc = np.zeros((GD_MAX_BC + 1, 4))
+        st = gd_adjoint_stats()
+        src = None
+        if dJ_dUN is not None:
+            src = np.asfortranarray(dJ_dUN, dtype=np.float64)
+            if src.shape != (n, 4):
+                raise ValueError(f"dJ_dUN must be ({n}, 4)")
+        check(load().gd_adjoint_run(self.handle, C.byref(ap), ptr(src), ptr(out), ptr(dbc), C.byref(st)))
+        return out, dbc, st
 
     def sync(self):
         check(load().gd_sync(self.handle))

@@ -26,6 +26,9 @@ UNITS = {
     "gd_ns_fp64.hip": ["-ffp-contract=off"],
     # FP32-mixed: hardware reciprocal / square root for float division and sqrt
     "gd_ns_fp32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    # discrete adjoint of the ns step
+    "gd_adj_fp64.hip": [],
+    "gd_adj_fp32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 

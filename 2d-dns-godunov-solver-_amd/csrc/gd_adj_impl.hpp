@@ -1,0 +1,637 @@
+// Discrete adjoint of one ns step (AUSM+), templated on the face arithmetic
+// type like the forward kernels (double: FP64, float: FP32-mixed).
+//
+// Forward step n, dt fixed:  Q = prims(U^n);  rec = (q, grad q, phi)(Q);
+//   f_face = Phi(rec_owner, rec_neighbour);  U^{n+1} = U^n - dt/A sum(+-f_face);
+//   J += w . (wall force of step n) = sum over wall faces of w . (f_1, f_2).
+// Reverse step n, given lamU^{n+1} and lamF = -dt/A lamU^{n+1} per cell:
+//   k_adj_cell   four lanes per cell: for each of its faces, lam_f =
+//                lamF(owner) - lamF(neighbour) (+ w on wall faces) and the
+//                transposed Jacobian of the face flux with respect to this
+//                cell's record (the face is evaluated once from each side, in
+//                the owner's orientation); then the transposed Jacobian of the
+//                gradient + limiter with respect to the stencil's primitives
+//                -> own part, and one part per neighbour slot (nrev); ghost
+//                states chained back to the cell or, for fixed states, to the
+//                boundary data
+//   k_adj_state  per cell: lamQ = own part + the neighbours' parts for this
+//                cell; lamU^n = lamU^{n+1} + (dQ/dU)^T lamQ; next lamF
+// Every derivative follows the forward code's branches as a forward-mode dual
+// number would (fabs'(0) = +1; fmin/fmax take the first argument on ties), so
+// the transposed step is exactly the transpose of the tangent-linear step.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gd_ns_impl.hpp"
+
+namespace gd {
+namespace adj {
+
+using ns::FaceGeo;
+using ns::kBlock;
+using ns::ld16;
+using ns::ld4;
+using ns::st4;
+
+template <class T>
+struct AdjKArgs {
+  MeshView m;
+  gd_params prm;
+  const T* rec;        // [nc][16] forward record of the step
+  T* lamF;             // [nc][4]  -dt/A lamU^{n+1}
+  T* selfq;            // [nc][4]
+  T* nrev;             // [nc][nf][4]
+  double* bcsum;       // [n_wall][4] fixed-state adjoint summed over steps
+  const double* lamU_in;
+  double* lamU_out;
+  double dt;
+  int32_t wall_code;
+  double fw[2];
+};
+
+template <class T>
+AdjKArgs<T> kargs(const AdjStepArgs& a) {
+  AdjKArgs<T> k;
+  k.m = a.m;
+  k.prm = a.prm;
+  k.rec = static_cast<const T*>(a.rec);
+  k.lamF = static_cast<T*>(a.lamF);
+  k.selfq = static_cast<T*>(a.selfq);
+  k.nrev = static_cast<T*>(a.nrev);
+  k.bcsum = a.bcsum;
+  k.lamU_in = a.lamU_in;
+  k.lamU_out = a.lamU_out;
+  k.dt = a.dt;
+  k.wall_code = a.wall_code;
+  k.fw[0] = a.fw[0];
+  k.fw[1] = a.fw[1];
+  return k;
+}
+
+// x[v] for a run-time v without a dynamically indexed (scratch) array
+template <class T>
+__device__ __forceinline__ T pick4(const T* x, int v) {
+  return v == 0 ? x[0] : v == 1 ? x[1] : v == 2 ? x[2] : x[3];
+}
+
+// derivative sign of fabs as a dual number takes it
+template <class T>
+__device__ __forceinline__ T dsgn(T x) { return x < 0 ? (T)-1 : (T)1; }
+
+// Transposed Jacobian of ns::ausm(L, R, n): lo = adjoint of the 4 outputs;
+// adds to lL, lR.
+template <class T>
+__device__ __forceinline__ void ausm_rev(const T* L, const T* R, T nx, T ny, T g, const T* lo, T* lL, T* lR) {
+  const T rL = L[0], uL = L[1], vL = L[2], pL = L[3];
+  const T rR = R[0], uR = R[1], vR = R[2], pR = R[3];
+  const T kg = g / (g - 1);
+  const T htL = (T)0.5 * (uL * uL + vL * vL) + kg * pL / rL;
+  const T htR = (T)0.5 * (uR * uR + vR * vR) + kg * pR / rR;
+  const T c = 2 * (g - 1) / (g + 1);
+  const T unL = uL * nx + vL * ny, unR = uR * nx + vR * ny;
+  const T asL = sqrt(c * htL), asR = sqrt(c * htR);
+  const T xL = asL / fabs(unL), xR = asR / fabs(unR);
+  const T aL = asL * fmin((T)1, xL);
+  const T aR = asR * fmin((T)1, xR);
+  const bool pickR = aR < aL;  // fmin(aL, aR) as a dual number picks
+  const T a12 = pickR ? aR : aL;
+  const T ML = unL / a12, MR = unR / a12;
+  T mp, pp, dmp, dpp, mm, pm, dmm, dpm;
+  {
+    const T M = ML, d = M * M - 1;
+    if (fabs(M) >= 1) {
+      mp = (T)0.5 * (M + fabs(M)); pp = M > 0 ? (T)1 : (T)0;
+      dmp = M > 0 ? (T)1 : (T)0; dpp = 0;
+    } else {
+      mp = (T)0.25 * (M + 1) * (M + 1) + (T)0.125 * d * d;
+      pp = (T)0.25 * (M + 1) * (M + 1) * (2 - M) + (T)0.1875 * M * d * d;
+      dmp = (T)0.5 * (M + 1) + (T)0.5 * M * d;
+      dpp = (T)0.5 * (M + 1) * (2 - M) - (T)0.25 * (M + 1) * (M + 1) + (T)0.1875 * d * d + (T)0.75 * M * M * d;
+    }
+  }
+  {
+    const T M = MR, d = M * M - 1;
+    if (fabs(M) >= 1) {
+      mm = (T)0.5 * (M - fabs(M)); pm = M < 0 ? (T)1 : (T)0;
+      dmm = M < 0 ? (T)1 : (T)0; dpm = 0;
+    } else {
+      mm = -(T)0.25 * (M - 1) * (M - 1) - (T)0.125 * d * d;
+      pm = (T)0.25 * (M - 1) * (M - 1) * (2 + M) - (T)0.1875 * M * d * d;
+      dmm = -(T)0.5 * (M - 1) - (T)0.5 * M * d;
+      dpm = (T)0.5 * (M - 1) * (2 + M) + (T)0.25 * (M - 1) * (M - 1) - (T)0.1875 * d * d - (T)0.75 * M * M * d;
+    }
+  }
+  const T m = mp + mm;
+  const bool mpos = !(m < 0);  // fabs'(0) = +1
+  const T mpl = mpos ? m : (T)0, mmi = mpos ? (T)0 : m;
+  const T A = mpl * rL, B = mmi * rR;
+  // out = -(a12 (A psiL + B psiR) + p12 (0, nx, ny, 0)), psi = (1, u, v, ht)
+  T la12 = -(lo[0] * (A + B) + lo[1] * (A * uL + B * uR) + lo[2] * (A * vL + B * vR) + lo[3] * (A * htL + B * htR));
+  const T lA = -a12 * (lo[0] + lo[1] * uL + lo[2] * vL + lo[3] * htL);
+  const T lB = -a12 * (lo[0] + lo[1] * uR + lo[2] * vR + lo[3] * htR);
+  T luL = -a12 * A * lo[1], lvL = -a12 * A * lo[2], lhtL = -a12 * A * lo[3];
+  T luR = -a12 * B * lo[1], lvR = -a12 * B * lo[2], lhtR = -a12 * B * lo[3];
+  const T lp12 = -(lo[1] * nx + lo[2] * ny);
+  T lrL = lA * mpl, lrR = lB * mmi;
+  const T lm = mpos ? lA * rL : lB * rR;
+  T lpL = lp12 * pp, lpR = lp12 * pm;
+  const T lML = lm * dmp + lp12 * pL * dpp;
+  const T lMR = lm * dmm + lp12 * pR * dpm;
+  T lunL = lML / a12, lunR = lMR / a12;
+  la12 -= (lML * ML + lMR * MR) / a12;
+  const T laL = pickR ? (T)0 : la12, laR = pickR ? la12 : (T)0;
+  T lasL, lasR;
+  if (xL < 1) {  // aL = asL^2 / |unL|
+    lasL = laL * 2 * xL;
+    lunL += -laL * xL * xL * dsgn(unL);
+  } else {
+    lasL = laL;
+  }
+  if (xR < 1) {
+    lasR = laR * 2 * xR;
+    lunR += -laR * xR * xR * dsgn(unR);
+  } else {
+    lasR = laR;
+  }
+  lhtL += lasL * c / (2 * asL);
+  lhtR += lasR * c / (2 * asR);
+  luL += lhtL * uL + lunL * nx;
+  lvL += lhtL * vL + lunL * ny;
+  lpL += lhtL * kg / rL;
+  lrL -= lhtL * kg * pL / (rL * rL);
+  luR += lhtR * uR + lunR * nx;
+  lvR += lhtR * vR + lunR * ny;
+  lpR += lhtR * kg / rR;
+  lrR -= lhtR * kg * pR / (rR * rR);
+  lL[0] += lrL; lL[1] += luL; lL[2] += lvL; lL[3] += lpL;
+  lR[0] += lrR; lR[1] += luR; lR[2] += lvR; lR[3] += lpR;
+}
+
+// Transposed Jacobian of ns::viscous_face with respect to (uf, vf, Tf, gf);
+// lg = adjoint of gv[1..3] (gv[0] = 0).
+template <class T>
+__device__ __forceinline__ void viscous_face_rev(const gd_params& P, T cp, T uf, T vf, T Tf, const T (*gf)[2], T nx,
+                                                 T ny, const T* lg, T* luf, T* lvf, T* lTf, T (*lgf)[2]) {
+  const T mu = ns::sutherland<T>(Tf, P);
+  const T kap = mu * cp / (T)P.Pr;
+  const T div = gf[0][0] + gf[1][1];
+  const T sxx = 2 * gf[0][0] - (T)(2.0 / 3.0) * div, syy = 2 * gf[1][1] - (T)(2.0 / 3.0) * div;
+  const T sxy = gf[0][1] + gf[1][0];
+  const T gv1 = mu * sxx * nx + mu * sxy * ny, gv2 = mu * sxy * nx + mu * syy * ny;
+  const T l1 = lg[1] + uf * lg[3], l2 = lg[2] + vf * lg[3];
+  *luf = gv1 * lg[3];
+  *lvf = gv2 * lg[3];
+  const T ltxx = l1 * nx, ltxy = l1 * ny + l2 * nx, ltyy = l2 * ny;
+  const T qn = gf[2][0] * nx + gf[2][1] * ny;
+  const T lkap = lg[3] * qn;
+  lgf[2][0] = lg[3] * kap * nx;
+  lgf[2][1] = lg[3] * kap * ny;
+  const T lmu = ltxx * sxx + ltyy * syy + ltxy * sxy + lkap * cp / (T)P.Pr;
+  const T ltr = (T)(2.0 / 3.0) * (ltxx + ltyy);
+  lgf[0][0] = mu * (2 * ltxx - ltr);
+  lgf[1][1] = mu * (2 * ltyy - ltr);
+  lgf[0][1] = mu * ltxy;
+  lgf[1][0] = mu * ltxy;
+  const T S = (T)P.S_suth;
+  *lTf = lmu * mu * ((T)1.5 / Tf - 1 / (Tf + S));
+}
+
+// adjoint of T = p / (rho R) and of the T gradient (gp - T R grho) / (rho R)
+// of one cell record, added into its record adjoint lr
+template <class T>
+__device__ __forceinline__ void temp_rev(const T* r, T Rg, T lT, const T* lgT, T* lr) {
+  const T rho = r[0], p = r[3];
+  const T rR = 1 / (rho * Rg);
+  const T Tc = p * rR;
+  T lrR = 0;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const T grho = r[4 + c], gp = r[10 + c];
+    lr[10 + c] += lgT[c] * rR;
+    lr[4 + c] -= lgT[c] * Tc * Rg * rR;
+    lT -= lgT[c] * Rg * grho * rR;
+    lrR += lgT[c] * (gp - Tc * Rg * grho);
+  }
+  lr[3] += lT * rR;
+  lrR += lT * p;
+  lr[0] -= lrR * rR / rho;
+}
+
+// Transposed face flux for one side of face f.  r0 / r1: owner's and
+// neighbour's records (a boundary face passes the owner's twice); l0 / l1:
+// their lamF (l1 = 0 at a boundary).  Adds the adjoint of the record of the
+// given side (0: owner, 1: neighbour) to lr; at a boundary face returns the
+// fixed-state adjoint in bq (zero unless the code is GD_BC_FIXED).
+template <class T, bool VISC>
+__device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, int64_t f, int32_t c1, int side, const T* r0,
+                                         const T* r1, const T* l0, const T* l1, T* lr, double* bq) {
+  using G = FaceGeo<T>;
+  const MeshView& m = a.m;
+  const gd_params& P = a.prm;
+  const T nx = G::fnx(m, f), ny = G::fny(m, f);
+  const T len = G::flen(m, f);
+  const T rx = G::fr0(m, 0, f), ry = G::fr0(m, 1, f);
+  const T g = (T)P.Gamma, Rg = (T)P.RGAS;
+  T le[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) le[v] = l0[v] - l1[v];
+  if (c1 < 0 && a.wall_code == c1 + 1 && m.fwall[f] >= 0) { le[1] += (T)a.fw[0]; le[2] += (T)a.fw[1]; }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) le[v] *= len;
+  // forward recompute of both face states
+  const T* gi = r0 + 4;
+  T qa[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) qa[v] = r0[v] + r0[12 + v] * (gi[2 * v] * rx + gi[2 * v + 1] * ry);
+  const bool fa = !(qa[0] > 0) || !(qa[3] > 0);
+  if (fa) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) qa[v] = r0[v];
+  }
+  const T* gj = r1 + 4;
+  T qb[4];
+  bool fb = false;
+  T rxj = 0, ryj = 0;
+  if (c1 >= 0) {
+    rxj = G::fr1(m, 0, f); ryj = G::fr1(m, 1, f);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) qb[v] = r1[v] + r1[12 + v] * (gj[2 * v] * rxj + gj[2 * v + 1] * ryj);
+    fb = !(qb[0] > 0) || !(qb[3] > 0);
+    if (fb) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) qb[v] = r1[v];
+    }
+  } else {
+    ns::ghost<T>(P, c1 + 1, qa, nx, ny, qb);
+  }
+  T lqa[4] = {0, 0, 0, 0}, lqb[4] = {0, 0, 0, 0};
+  ausm_rev<T>(qb, qa, nx, ny, g, le, lqb, lqa);
+  if (c1 < 0) {
+    const int idx = -(c1 + 1);
+    const int kind = idx <= GD_MAX_BC ? P.bc[idx].kind : GD_BC_EXTRAPOLATE;
+    if (kind == GD_BC_FIXED) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) bq[v] = (double)lqb[v];
+    } else if (kind == GD_BC_SLIP_WALL || kind == GD_BC_ISOTHERMAL_WALL) {
+      const T sn = lqb[1] * nx + lqb[2] * ny;
+      lqa[0] += lqb[0];
+      lqa[1] += lqb[1] - 2 * nx * sn;
+      lqa[2] += lqb[2] - 2 * ny * sn;
+      lqa[3] += lqb[3];
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) lqa[v] += lqb[v];
+    }
+  }
+  // this side's state and reconstruction (element-wise selects keep one code
+  // path for both sides and no dynamically indexed private arrays)
+  const bool sd = side != 0;
+  T rs[16], lqs[4];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) rs[t] = sd ? r1[t] : r0[t];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) lqs[v] = sd ? lqb[v] : lqa[v];
+  const T* gs = rs + 4;
+  const bool fs = sd ? fb : fa;
+  const T sx = sd ? rxj : rx, sy = sd ? ryj : ry;
+  if (VISC) {
+    T lgv[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) lgv[v] = le[v];
+    const T cp = g * Rg / (g - 1);
+    const T rRi = 1 / (r0[0] * Rg);
+    const T Ti = r0[3] * rRi;
+    if (c1 >= 0) {
+      const T rRj = 1 / (r1[0] * Rg);
+      const T Tj = r1[3] * rRj;
+      T g_i[3][2], g_j[3][2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        g_i[0][c] = gi[2 + c]; g_i[1][c] = gi[4 + c];
+        g_i[2][c] = (gi[6 + c] - Ti * Rg * gi[c]) * rRi;
+        g_j[0][c] = gj[2 + c]; g_j[1][c] = gj[4 + c];
+        g_j[2][c] = (gj[6 + c] - Tj * Rg * gj[c]) * rRj;
+      }
+      const T ddx = rx - rxj, ddy = ry - ryj;
+      const T rdl = 1 / sqrt(ddx * ddx + ddy * ddy);
+      const T tx = ddx * rdl, ty = ddy * rdl;
+      const T dphi[3] = {r1[1] - r0[1], r1[2] - r0[2], Tj - Ti};
+      T gf[3][2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T ax = (T)0.5 * (g_i[c][0] + g_j[c][0]), ay = (T)0.5 * (g_i[c][1] + g_j[c][1]);
+        const T corr = dphi[c] * rdl - (ax * tx + ay * ty);
+        gf[c][0] = ax + corr * tx; gf[c][1] = ay + corr * ty;
+      }
+      T luf, lvf, lTf, lgf[3][2];
+      viscous_face_rev<T>(P, cp, (T)0.5 * (r0[1] + r1[1]), (T)0.5 * (r0[2] + r1[2]), (T)0.5 * (Ti + Tj), gf, nx, ny,
+                          lgv, &luf, &lvf, &lTf, lgf);
+      // both sides take half the averaged-gradient adjoint; the difference
+      // term has opposite signs
+      const T sg = side ? (T)1 : (T)-1;
+      T lg[3][2], ldphi[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T tl = tx * lgf[c][0] + ty * lgf[c][1];
+        lg[c][0] = (T)0.5 * (lgf[c][0] - tx * tl);
+        lg[c][1] = (T)0.5 * (lgf[c][1] - ty * tl);
+        ldphi[c] = rdl * tl;
+      }
+      lr[1] += (T)0.5 * luf + sg * ldphi[0];
+      lr[2] += (T)0.5 * lvf + sg * ldphi[1];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        lr[6 + c] += lg[0][c];
+        lr[8 + c] += lg[1][c];
+      }
+      const T lgT[2] = {lg[2][0], lg[2][1]};
+      temp_rev<T>(rs, Rg, (T)0.5 * lTf + sg * ldphi[2], lgT, lr);
+    } else {
+      const int idx = -(c1 + 1);
+      if (idx <= GD_MAX_BC && P.bc[idx].kind == GD_BC_ISOTHERMAL_WALL) {
+        const T Tw = (T)P.bc[idx].data[0];
+        const T rdp = -1 / (rx * nx + ry * ny);
+        const T dudn = (0 - r0[1]) * rdp, dvdn = (0 - r0[2]) * rdp, dTdn = (Tw - Ti) * rdp;
+        const T ox = -nx, oy = -ny;
+        const T gf[3][2] = {{dudn * ox, dudn * oy}, {dvdn * ox, dvdn * oy}, {dTdn * ox, dTdn * oy}};
+        T luf, lvf, lTf, lgf[3][2];
+        viscous_face_rev<T>(P, cp, (T)0, (T)0, Tw, gf, nx, ny, lgv, &luf, &lvf, &lTf, lgf);
+        const T ldn[3] = {lgf[0][0] * ox + lgf[0][1] * oy, lgf[1][0] * ox + lgf[1][1] * oy,
+                          lgf[2][0] * ox + lgf[2][1] * oy};
+        lr[1] -= ldn[0] * rdp;
+        lr[2] -= ldn[1] * rdp;
+        const T lT = -ldn[2] * rdp;  // T_i = p rR
+        lr[3] += lT * rRi;
+        lr[0] -= lT * Ti / r0[0];
+      }
+    }
+  }
+  // reconstruction q_f = q + phi (grad q . r), first order where it fell back
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    lr[v] += lqs[v];
+    if (!fs) {
+      lr[4 + 2 * v] += lqs[v] * rs[12 + v] * sx;
+      lr[5 + 2 * v] += lqs[v] * rs[12 + v] * sy;
+      lr[12 + v] += lqs[v] * (gs[2 * v] * sx + gs[2 * v + 1] * sy);
+    }
+  }
+}
+
+// ---- cell pass: four lanes per cell.  Phase 1, lane k = face slot k: the
+// transposed flux of that face for this cell's side (each face is transposed
+// once from each side, in the owner's orientation, so nothing per face is
+// stored).  Phase 2, lane v = variable v: the transposed gradient + limiter
+// of that variable over the stencil.  Phase 3, lane k: the neighbour part of
+// slot k (nrev) or, at a boundary, the ghost chain back to the cell or to the
+// fixed state.  Phase 4, lane 0: the cell's own part (selfq).
+template <class T, bool QUADS, bool VISC>
+__global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
+  constexpr int NF = QUADS ? 4 : 3;
+  constexpr int CPB = kBlock / 4;  // cells per block
+  __shared__ T sLR[CPB][4][16];     // record adjoint from each face slot
+  __shared__ T sLQ[CPB][NF + 1][4]; // primitive adjoint of each stencil point
+  __shared__ T sGC[CPB][4][4];      // ghost chains back to the cell
+  using G = FaceGeo<T>;
+  const MeshView& m = a.m;
+  const int64_t nc = m.nc;
+  const int lc = threadIdx.x >> 2, r = threadIdx.x & 3;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * CPB + lc;
+  const bool live = i < nc;
+  const int nf = live ? (QUADS ? m.nv[i] : 3) : 0;
+  const bool has_slot = live && r < nf;
+  // ---- phase 1
+  int32_t jn = 0;
+  double bq[4] = {0.0, 0.0, 0.0, 0.0};
+  {
+    T lr[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) lr[t] = 0;
+    if (has_slot) {
+      jn = m.nb[r * nc + i];
+      const int32_t fc = m.cface[r * nc + i];
+      T ri[16], rj[16], li[4], lj[4] = {0, 0, 0, 0};
+      ld16(a.rec + 16 * i, ri);
+      ld4(a.lamF + 4 * i, li);
+      if (jn >= 0) {
+        ld16(a.rec + 16 * (int64_t)jn, rj);
+        ld4(a.lamF + 4 * (int64_t)jn, lj);
+      } else {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) rj[t] = ri[t];
+      }
+      const int64_t f = fc >> 1;
+      const int32_t c1 = jn >= 0 ? 0 : jn;  // only its sign and code are used
+      // owner's record first, by element-wise selects (one code path)
+      const bool sd = (fc & 1) != 0;
+      T r0[16], r1[16], l0[4], l1[4];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) { r0[t] = sd ? rj[t] : ri[t]; r1[t] = sd ? ri[t] : rj[t]; }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { l0[v] = sd ? lj[v] : li[v]; l1[v] = sd ? li[v] : lj[v]; }
+      face_rev<T, VISC>(a, f, c1, sd ? 1 : 0, r0, r1, l0, l1, lr, bq);
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) sLR[lc][r][t] = lr[t];
+  }
+  __syncthreads();
+  // ---- phase 2: variable v = r
+  if (live) {
+    const int v = r;
+    T lq[NF + 1];
+#pragma unroll
+    for (int p = 0; p <= NF; ++p) lq[p] = 0;
+    T lqv = 0, lgx = 0, lgy = 0, lph = 0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      lqv += sLR[lc][s][v];
+      lgx += sLR[lc][s][4 + 2 * v];
+      lgy += sLR[lc][s][5 + 2 * v];
+      lph += sLR[lc][s][12 + v];
+    }
+    lq[0] = lqv;
+    int32_t jk[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) jk[k] = m.nb[k * nc + i];
+    T qi[4], g2[2];
+    ld4(a.rec + 16 * i, qi);
+    g2[0] = a.rec[16 * i + 4 + 2 * v];
+    g2[1] = a.rec[16 * i + 5 + 2 * v];
+    T qk[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) qk[k] = a.rec[16 * (int64_t)(jk[k] >= 0 ? jk[k] : i) + v];
+    T wx[NF], wy[NF], rx[NF], ry[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      wx[k] = G::lsq(m, k, 0, i); wy[k] = G::lsq(m, k, 1, i);
+      rx[k] = G::rf(m, k, 0, i); ry[k] = G::rf(m, k, 1, i);
+    }
+    const T kh = (T)a.prm.venkat_K * G::hmin(m, i);
+#pragma unroll
+    for (int k = 0; k < NF; ++k)
+      if (k < nf && jk[k] < 0) {
+        const int64_t s = k * nc + i;
+        T gq[4];
+        ns::ghost<T>(a.prm, jk[k] + 1, qi, G::nx(m, s), G::ny(m, s), gq);
+        qk[k] = pick4(gq, v);
+      }
+    const T gx = g2[0], gy = g2[1], q0 = pick4(qi, v);
+    // stencil extremes with their points (strictly larger / smaller wins)
+    T qmx = q0, qmn = q0;
+    int amx = 0, amn = 0;
+#pragma unroll
+    for (int k = 0; k < NF; ++k)
+      if (k < nf) {
+        if (qk[k] > qmx) { qmx = qk[k]; amx = 1 + k; }
+        if (qk[k] < qmn) { qmn = qk[k]; amn = 1 + k; }
+      }
+    const T e2 = kh * kh * kh * (T)(a.prm.venkat_scale[v] * a.prm.venkat_scale[v]);
+    T phi = 1;
+    int ks = -1;
+#pragma unroll
+    for (int k = 0; k < NF; ++k)
+      if (k < nf) {
+        const T d2 = gx * rx[k] + gy * ry[k];
+        if (d2 != 0) {
+          const T d1 = d2 > 0 ? qmx - q0 : qmn - q0;
+          const T psi = ((d1 * d1 + e2) * d2 + 2 * d2 * d2 * d1) / (d1 * d1 + 2 * d2 * d2 + d1 * d2 + e2) / d2;
+          if (psi < phi) { phi = psi; ks = k; }
+        }
+      }
+    if (ks >= 0 && !(phi < 0)) {  // phi = psi of slot ks (not clipped at 0 or 1)
+      T rxs = 0, rys = 0;
+#pragma unroll
+      for (int k = 0; k < NF; ++k)
+        if (k == ks) { rxs = rx[k]; rys = ry[k]; }
+      const T d2 = gx * rxs + gy * rys;
+      const int ae = d2 > 0 ? amx : amn;
+      const T d1 = (d2 > 0 ? qmx : qmn) - q0;
+      const T D = d1 * d1 + 2 * d2 * d2 + d1 * d2 + e2;
+      const T dd1 = ((2 * d1 + 2 * d2) - phi * (2 * d1 + d2)) / D;
+      const T dd2 = (2 * d1 - phi * (4 * d2 + d1)) / D;
+      const T ld1 = lph * dd1, ld2 = lph * dd2;
+      lgx += ld2 * rxs;
+      lgy += ld2 * rys;
+#pragma unroll
+      for (int p = 0; p <= NF; ++p)
+        if (p == ae) lq[p] += ld1;
+      lq[0] -= ld1;
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k)
+      if (k < nf) {
+        const T t = wx[k] * lgx + wy[k] * lgy;
+        lq[1 + k] += t;
+        lq[0] -= t;
+      }
+#pragma unroll
+    for (int p = 0; p <= NF; ++p) sLQ[lc][p][v] = lq[p];
+  }
+  __syncthreads();
+  // ---- phase 3: slot k = r
+  {
+    T gc[4] = {0, 0, 0, 0};
+    if (has_slot) {
+      const int k = r;
+      T l[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) l[w] = sLQ[lc][1 + k][w];
+      if (jn >= 0) {
+        st4(a.nrev + 4 * (i * m.nf + k), l[0], l[1], l[2], l[3]);
+      } else {
+        const int idx = -(jn + 1);
+        const int kind = idx <= GD_MAX_BC ? a.prm.bc[idx].kind : GD_BC_EXTRAPOLATE;
+        const int64_t s = k * nc + i;
+        if (kind == GD_BC_FIXED) {
+          const int32_t wid = m.wall_id[s];
+          if (wid >= 0) {
+#pragma unroll
+            for (int w = 0; w < 4; ++w) a.bcsum[4 * (int64_t)wid + w] += bq[w] + (double)l[w];
+          }
+        } else if (kind == GD_BC_SLIP_WALL || kind == GD_BC_ISOTHERMAL_WALL) {
+          const T nx = G::nx(m, s), ny = G::ny(m, s);
+          const T sn = l[1] * nx + l[2] * ny;
+          gc[0] = l[0]; gc[1] = l[1] - 2 * nx * sn; gc[2] = l[2] - 2 * ny * sn; gc[3] = l[3];
+        } else {
+#pragma unroll
+          for (int w = 0; w < 4; ++w) gc[w] = l[w];
+        }
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) sGC[lc][r][w] = gc[w];
+  }
+  __syncthreads();
+  // ---- phase 4
+  if (live && r == 0) {
+    T o[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) o[w] = sLQ[lc][0][w] + sGC[lc][0][w] + sGC[lc][1][w] + sGC[lc][2][w] + sGC[lc][3][w];
+    st4(a.selfq + 4 * i, o[0], o[1], o[2], o[3]);
+  }
+}
+
+// ---- state pass: gather the primitive adjoint, transpose the primitive
+// recovery, add to lamU; seed the next step's lamF
+template <class T, bool GATHER>
+__global__ __launch_bounds__(kBlock) void k_adj_state(AdjKArgs<T> a) {
+  const MeshView& m = a.m;
+  const int64_t nc = m.nc;
+  const int64_t j = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (j >= nc) return;
+  double lU[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) lU[v] = a.lamU_in[v * nc + j];
+  if (GATHER) {
+    const int nf = m.has_quads ? m.nv[j] : 3;
+    T s4[4];
+    ld4(a.selfq + 4 * j, s4);
+    double lQ[4] = {(double)s4[0], (double)s4[1], (double)s4[2], (double)s4[3]};
+    for (int k = 0; k < nf; ++k) {
+      const int32_t jn = m.nb[k * nc + j];
+      if (jn >= 0) {
+        const int32_t kk = m.kn[k * nc + j];
+        T b[4];
+        ld4(a.nrev + 4 * ((int64_t)jn * m.nf + kk), b);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) lQ[v] += (double)b[v];
+      }
+    }
+    T q[4];
+    ld4(a.rec + 16 * j, q);
+    const double rho = q[0], u = q[1], v = q[2], gm1 = a.prm.Gamma - 1.0;
+    const double ri = 1.0 / rho;
+    lU[0] += lQ[0] - (u * lQ[1] + v * lQ[2]) * ri + gm1 * 0.5 * (u * u + v * v) * lQ[3];
+    lU[1] += lQ[1] * ri - gm1 * u * lQ[3];
+    lU[2] += lQ[2] * ri - gm1 * v * lQ[3];
+    lU[3] += gm1 * lQ[3];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) a.lamU_out[w * nc + j] = lU[w];
+  }
+  const double s = -a.dt * m.rarea[j];
+  st4(a.lamF + 4 * j, (T)(s * lU[0]), (T)(s * lU[1]), (T)(s * lU[2]), (T)(s * lU[3]));
+}
+
+template <class T>
+hipError_t launch_cell(const AdjKArgs<T>& a, hipStream_t s) {
+  const dim3 g((unsigned)((a.m.nc + kBlock / 4 - 1) / (kBlock / 4))), b(kBlock);
+  const bool visc = a.prm.viscous != 0;
+  if (a.m.has_quads) {
+    if (visc) hipLaunchKernelGGL((k_adj_cell<T, true, true>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_adj_cell<T, true, false>), g, b, 0, s, a);
+  } else {
+    if (visc) hipLaunchKernelGGL((k_adj_cell<T, false, true>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_adj_cell<T, false, false>), g, b, 0, s, a);
+  }
+  return hipGetLastError();
+}
+template <class T>
+hipError_t launch_state(const AdjKArgs<T>& a, bool gather, hipStream_t s) {
+  const dim3 g(ns::nblocks(a.m.nc)), b(kBlock);
+  if (gather) hipLaunchKernelGGL((k_adj_state<T, true>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_adj_state<T, false>), g, b, 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace adj
+}  // namespace gd

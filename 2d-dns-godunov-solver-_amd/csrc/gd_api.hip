@@ -111,7 +111,6 @@ std::vector<int64_t> rcm_order(int64_t n, const std::vector<int64_t>& xadj, cons
 struct gd_mesh : gd::Mesh {};
 struct gd_state : gd::State {
   std::vector<hipEvent_t> ev_pool;
-  std::vector<double> host_tmp;
 };
 
 // The definitions below take C linkage from their declarations in godunov.h.
@@ -241,6 +240,18 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
     for (int64_t j = 0; j < nn; ++j)
       if (M->niperm[j] < 0) { M->niperm[j] = (int64_t)np.size(); np.push_back(j); }
   }
+  M->cells_identity = M->nodes_identity = true;
+  for (int64_t c = 0; c < nc; ++c) M->cells_identity = M->cells_identity && cp[c] == c;
+  for (int64_t j = 0; j < nn; ++j) M->nodes_identity = M->nodes_identity && np[j] == j;
+  {
+    auto to32 = [](const std::vector<int64_t>& a) { return std::vector<int32_t>(a.begin(), a.end()); };
+    int rcp = 0;
+    rcp |= upload(M->allocs, &M->d_cperm, to32(M->cperm), &M->device_bytes);
+    rcp |= upload(M->allocs, &M->d_ciperm, to32(M->ciperm), &M->device_bytes);
+    rcp |= upload(M->allocs, &M->d_nperm, to32(M->nperm), &M->device_bytes);
+    rcp |= upload(M->allocs, &M->d_niperm, to32(M->niperm), &M->device_bytes);
+    if (rcp) { std::string msg = g_err; gd_mesh_destroy(M); return fail(rcp, msg); }
+  }
   const auto& ci = M->ciperm;
   const auto& ni = M->niperm;
   auto new_node = [&](int64_t id1) -> int32_t { return (id1 >= 1 && id1 <= nn) ? (int32_t)ni[id1 - 1] : 0; };
@@ -321,6 +332,7 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
       } else {
         h_nb[s] = e - 1;  // code - 1 < 0
         h_wid[s] = n_wall++;
+        M->wall_codes.push_back(e);
         // ghost centroid: the own centroid mirrored across the face
         const double rn = r[0] * h_nx[s] + r[1] * h_ny[s];
         dcx[k - 1] = 2.0 * rn * h_nx[s];
@@ -444,6 +456,7 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
   rc |= upload(al, &v.lsq, h_lsq, B);
   rc |= upload(al, &v.rf, h_rf, B);
   rc |= upload(al, &v.wall_id, h_wid, B);
+  rc |= upload(al, &v.kn, h_kn, B);
   {
     std::vector<float> f_nx(h_nx.size()), f_ny(h_ny.size()), f_lsq(h_lsq.size()), f_rf(h_rf.size());
     for (size_t q = 0; q < h_nx.size(); ++q) { f_nx[q] = (float)h_nx[q]; f_ny[q] = (float)h_ny[q]; }
@@ -566,6 +579,7 @@ int gd_state_create(gd_mesh* m, const gd_params* p, gd_state** out) {
   rc |= dev_alloc(al, &S->acc, 3);
   rc |= dev_alloc(al, &S->part, 3 * ((nc + 255) / 256) + 3);
   rc |= dev_alloc(al, &S->red, 8 + 4 * ((nc + 255) / 256) + 8);
+  rc |= dev_alloc(al, &S->xfer, 4 * (size_t)std::max(nc, nn));
   if (rc) {
     std::string msg = g_err;
     gd_state_destroy(S);
@@ -609,16 +623,18 @@ int gd_state_set_params(gd_state* S, const gd_params* p) {
   return GD_OK;
 }
 
-// host (caller order) <-> device (device order) column transfers
+// host (caller order) <-> device (device order) column transfers: one copy
+// of the caller's columns, then a gather on the device (or the reverse)
 static int put_cols(gd_state* S, double* dev, const double* host, int ncols, bool nodes) {
   const Mesh& M = *S->mesh;
   const int64_t n = nodes ? M.v.nn : M.v.nc;
-  const std::vector<int64_t>& perm = nodes ? M.nperm : M.cperm;
-  std::vector<double>& t = S->host_tmp;
-  t.resize((size_t)n * ncols);
-  for (int q = 0; q < ncols; ++q)
-    for (int64_t i = 0; i < n; ++i) t[(size_t)q * n + i] = host[(size_t)q * n + perm[i]];
-  GD_HIP(hipMemcpyAsync(dev, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, S->stream));
+  const size_t bytes = (size_t)n * ncols * sizeof(double);
+  if (nodes ? M.nodes_identity : M.cells_identity) {
+    GD_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, S->stream));
+  } else {
+    GD_HIP(hipMemcpyAsync(S->xfer, host, bytes, hipMemcpyHostToDevice, S->stream));
+    GD_HIP(launch_permute(dev, S->xfer, nodes ? M.d_nperm : M.d_cperm, n, ncols, S->stream));
+  }
   GD_HIP(hipStreamSynchronize(S->stream));
   return GD_OK;
 }
@@ -626,13 +642,14 @@ static int put_cols(gd_state* S, double* dev, const double* host, int ncols, boo
 static int get_cols(gd_state* S, const double* dev, double* host, int ncols, bool nodes) {
   const Mesh& M = *S->mesh;
   const int64_t n = nodes ? M.v.nn : M.v.nc;
-  const std::vector<int64_t>& perm = nodes ? M.nperm : M.cperm;
-  std::vector<double>& t = S->host_tmp;
-  t.resize((size_t)n * ncols);
-  GD_HIP(hipMemcpyAsync(t.data(), dev, t.size() * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  const size_t bytes = (size_t)n * ncols * sizeof(double);
+  if (nodes ? M.nodes_identity : M.cells_identity) {
+    GD_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, S->stream));
+  } else {
+    GD_HIP(launch_permute(S->xfer, dev, nodes ? M.d_niperm : M.d_ciperm, n, ncols, S->stream));
+    GD_HIP(hipMemcpyAsync(host, S->xfer, bytes, hipMemcpyDeviceToHost, S->stream));
+  }
   GD_HIP(hipStreamSynchronize(S->stream));
-  for (int q = 0; q < ncols; ++q)
-    for (int64_t i = 0; i < n; ++i) host[(size_t)q * n + perm[i]] = t[(size_t)q * n + i];
   return GD_OK;
 }
 
@@ -1001,3 +1018,219 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
   return GD_OK;
 }
 
+
+// ---- discrete adjoint of the ns loop ---------------------------------------
+// Forward sweep from the state's U keeps U every checkpoint_every steps and the
+// cell records of the current segment; the reverse sweep walks the segments
+// backwards, recomputing each (except the last) from its checkpoint into the
+// segment's record slots, then applies the transposed steps.
+namespace {
+struct DevScratch {
+  std::vector<void*> p;
+  ~DevScratch() {
+    for (void* q : p) (void)hipFree(q);
+  }
+};
+}  // namespace
+
+int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dUN, double* dJ_dU0, double* dJ_dbc,
+                   gd_adjoint_stats* stats) {
+  if (!S || !ap || !dJ_dU0) return fail(GD_ERR_INVALID, "null argument");
+  const gd_params& P = S->prm;
+  if (P.scheme != GD_SCHEME_NS) return fail(GD_ERR_UNSUPPORTED, "the adjoint covers the ns scheme");
+  if (P.flux != GD_FLUX_AUSM_PLUS) return fail(GD_ERR_UNSUPPORTED, "the adjoint covers the AUSM+ flux");
+  if (ap->n_steps < 1 || ap->checkpoint_every < 1) return fail(GD_ERR_INVALID, "n_steps and checkpoint_every >= 1");
+  if (!(ap->dt > 0.0)) return fail(GD_ERR_INVALID, "the adjoint needs a fixed dt > 0");
+  const MeshView& v = S->mesh->v;
+  const int64_t nc = v.nc;
+  const bool mixed = P.precision == GD_FP32_MIXED;
+  const size_t t = mixed ? 4 : 8;
+  const int32_t N = ap->n_steps;
+  const int32_t c = std::min(ap->checkpoint_every, N);
+  const int32_t nseg = (N + c - 1) / c;
+  const int64_t nparts = (nc + 255) / 256;
+  const bool wall = ap->wall_code != 0 && v.n_wall > 0;
+  const int64_t nw = std::max<int32_t>(v.n_wall, 1);
+
+  DevScratch D;
+  int64_t bytes = 0;
+  int rc = 0;
+  double* ck = nullptr;
+  rc |= dev_alloc(D.p, &ck, (size_t)nseg * 4 * nc, &bytes);
+  char* recs = nullptr;
+  rc |= dev_alloc(D.p, &recs, (size_t)c * 16 * nc * t, &bytes);
+  double *Uw0 = nullptr, *Uw1 = nullptr, *lU0 = nullptr, *lU1 = nullptr, *bcsum = nullptr;
+  rc |= dev_alloc(D.p, &Uw0, 4 * nc);
+  rc |= dev_alloc(D.p, &Uw1, 4 * nc);
+  rc |= dev_alloc(D.p, &lU0, 4 * nc);
+  rc |= dev_alloc(D.p, &lU1, 4 * nc);
+  rc |= dev_alloc(D.p, &bcsum, 4 * nw);
+  char *lamF = nullptr, *selfq = nullptr, *nrev = nullptr;
+  rc |= dev_alloc(D.p, &lamF, 4 * nc * t);
+  rc |= dev_alloc(D.p, &selfq, 4 * nc * t);
+  rc |= dev_alloc(D.p, &nrev, 4 * (size_t)v.nf * nc * t);
+  if (rc) return rc;
+  auto rec_slot = [&](int32_t s) { return (void*)(recs + (size_t)(s % c) * 16 * nc * t); };
+
+  auto prims = mixed ? launch_ns_prims_f32 : launch_ns_prims_f64;
+  auto grad = mixed ? launch_ns_grad_f32 : launch_ns_grad_f64;
+  auto face = mixed ? launch_ns_face_f32 : launch_ns_face_f64;
+  auto update = mixed ? launch_ns_update_f32 : launch_ns_update_f64;
+  auto acell = mixed ? launch_adj_cell_f32 : launch_adj_cell_f64;
+  auto astate = mixed ? launch_adj_state_f32 : launch_adj_state_f64;
+
+  // one forward step s from Uo into Un, the record into its segment slot
+  auto fstep = [&](int32_t s, const double* Uo, double* Un, int tq, bool first, bool with_wall) -> int {
+    NsStepArgs a{};
+    a.m = v;
+    a.prm = P;
+    a.Uo = Uo;
+    a.Un = Un;
+    a.Q = S->Qt[tq];
+    a.Qw = S->Qt[1 - tq];
+    a.rec = rec_slot(s);
+    a.fflux = S->fflux;
+    a.acc = S->acc;
+    a.part = S->part;
+    a.step = s;
+    a.dt_policy = GD_DT_FIXED;
+    a.dt = ap->dt;
+    a.cfl = 0.0;
+    a.wall_f = with_wall ? S->wall_f : nullptr;
+    a.wall_code = ap->wall_code;
+    a.fold_prev = first ? 0 : 1;
+    a.nparts = nparts;
+    a.drag = S->drag;
+    GD_HIP(grad(a, S->stream));
+    GD_HIP(face(a, S->stream));
+    GD_HIP(update(a, S->stream));
+    return GD_OK;
+  };
+
+  hipEvent_t ev[3];
+  for (auto& e : ev) GD_HIP(hipEventCreate(&e));
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int q = 0; q < 3; ++q) (void)hipEventDestroy(e[q]);
+    }
+  } evg{ev};
+
+  // ---- forward sweep
+  GD_HIP(hipEventRecord(ev[0], S->stream));
+  GD_HIP(launch_update_variables(v, P.Gamma, S->U[S->cur], S->U[S->cur], S->Q[S->qc], S->aS, S->vmax, nullptr,
+                                 S->stream));
+  GD_HIP(launch_prime_acc(v, P, S->Q[S->qc], S->vmax, S->acc, S->part, ap->flow_time, GD_DT_FIXED, P.viscous,
+                          S->stream));
+  GD_HIP(launch_fill(S->wall_f, 2 * nw, 0.0, S->stream));
+  GD_HIP(launch_fill(S->drag, 4, 0.0, S->stream));
+  int tq = 0;
+  GD_HIP(prims(nc, P.Gamma, S->U[S->cur], S->Qt[tq], S->stream));
+  for (int32_t s = 0; s < N; ++s) {
+    if (s % c == 0) GD_HIP(launch_copy(ck + (size_t)(s / c) * 4 * nc, S->U[S->cur], 4 * nc, S->stream));
+    if ((rc = fstep(s, S->U[S->cur], S->U[1 - S->cur], tq, s == 0, wall))) return rc;
+    S->cur ^= 1;
+    tq ^= 1;
+  }
+  GD_HIP(launch_acc_fold(S->part, nparts, &S->acc[N % 3], S->stream));
+  if (wall) GD_HIP(launch_wall_sum(S->wall_f, v.n_wall, S->drag, S->stream));
+  GD_HIP(hipEventRecord(ev[1], S->stream));
+  Acc acc[3];
+  double drag[4] = {0, 0, 0, 0};
+  GD_HIP(hipMemcpyAsync(acc, S->acc, sizeof(acc), hipMemcpyDeviceToHost, S->stream));
+  GD_HIP(hipMemcpyAsync(drag, S->drag, sizeof(drag), hipMemcpyDeviceToHost, S->stream));
+  GD_HIP(hipStreamSynchronize(S->stream));
+  if (acc[N % 3].nonphysical) {
+    char buf[128];
+    std::snprintf(buf, sizeof(buf), "%u cells with non-positive density/pressure or NaN", acc[N % 3].nonphysical);
+    return fail(GD_ERR_NONPHYSICAL, buf);
+  }
+  double J = ap->force_weight[0] * drag[2] + ap->force_weight[1] * drag[3];
+  if (dJ_dUN) {  // + <dJ/dU_N, U_N>
+    std::vector<double> UN((size_t)4 * nc);
+    if ((rc = get_cols(S, S->U[S->cur], UN.data(), 4, false))) return rc;
+    for (size_t q = 0; q < UN.size(); ++q) J += dJ_dUN[q] * UN[q];
+  }
+
+  // ---- reverse sweep
+  double* lU[2] = {lU0, lU1};
+  int li = 0;
+  if (dJ_dUN) {
+    if ((rc = put_cols(S, lU[0], dJ_dUN, 4, false))) return rc;
+  } else {
+    GD_HIP(launch_fill(lU[0], 4 * nc, 0.0, S->stream));
+  }
+  GD_HIP(launch_fill(bcsum, 4 * nw, 0.0, S->stream));
+  AdjStepArgs r{};
+  r.m = v;
+  r.prm = P;
+  r.lamF = lamF;
+  r.selfq = selfq;
+  r.nrev = nrev;
+  r.bcsum = bcsum;
+  r.dt = ap->dt;
+  r.wall_code = wall ? ap->wall_code : 0;
+  r.fw[0] = ap->force_weight[0];
+  r.fw[1] = ap->force_weight[1];
+  r.rec = rec_slot(N - 1);
+  r.lamU_in = lU[li];
+  GD_HIP(astate(r, false, S->stream));  // lamF of the last step from dJ/dU_N
+  int32_t recomputed = 0;
+  for (int32_t seg = nseg - 1; seg >= 0; --seg) {
+    const int32_t s0 = seg * c, s1 = std::min(N, s0 + c);
+    if (seg != nseg - 1) {
+      GD_HIP(launch_copy(Uw0, ck + (size_t)seg * 4 * nc, 4 * nc, S->stream));
+      double* Uw[2] = {Uw0, Uw1};
+      int wc = 0, rq = 0;
+      GD_HIP(prims(nc, P.Gamma, Uw[wc], S->Qt[rq], S->stream));
+      for (int32_t s = s0; s < s1; ++s) {
+        if ((rc = fstep(s, Uw[wc], Uw[1 - wc], rq, true, false))) return rc;
+        wc ^= 1;
+        rq ^= 1;
+        ++recomputed;
+      }
+    }
+    for (int32_t s = s1 - 1; s >= s0; --s) {
+      r.rec = rec_slot(s);
+      r.lamU_in = lU[li];
+      r.lamU_out = lU[1 - li];
+      GD_HIP(acell(r, S->stream));
+      GD_HIP(astate(r, true, S->stream));
+      li ^= 1;
+    }
+  }
+  GD_HIP(hipEventRecord(ev[2], S->stream));
+  if ((rc = get_cols(S, lU[li], dJ_dU0, 4, false))) return rc;
+  if (dJ_dbc) {
+    std::vector<double> bs((size_t)4 * nw);
+    GD_HIP(hipMemcpyAsync(bs.data(), bcsum, bs.size() * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    GD_HIP(hipStreamSynchronize(S->stream));
+    for (int q = 0; q < 4 * (GD_MAX_BC + 1); ++q) dJ_dbc[q] = 0.0;
+    const std::vector<int32_t>& wc = S->mesh->wall_codes;
+    for (int32_t w = 0; w < v.n_wall; ++w) {
+      const int idx = -wc[w];
+      if (idx < 0 || idx > GD_MAX_BC || P.bc[idx].kind != GD_BC_FIXED) continue;
+      for (int q = 0; q < 4; ++q) dJ_dbc[4 * idx + q] += bs[4 * (size_t)w + q];
+    }
+  }
+  // the state holds U_N, as after gd_run
+  GD_HIP(launch_update_variables(v, P.Gamma, S->U[S->cur], S->U[S->cur], S->Q[S->qc], S->aS, S->vmax, nullptr,
+                                 S->stream));
+  GD_HIP(launch_copy(S->U[1 - S->cur], S->U[S->cur], 4 * nc, S->stream));
+  GD_HIP(hipStreamSynchronize(S->stream));
+  if (stats) {
+    float f_ms = 0.f, r_ms = 0.f;
+    (void)hipEventElapsedTime(&f_ms, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&r_ms, ev[1], ev[2]);
+    stats->J = J;
+    stats->force_sum[0] = drag[2];
+    stats->force_sum[1] = drag[3];
+    stats->recomputed_steps = recomputed;
+    stats->n_checkpoints = nseg;
+    stats->storage_bytes = bytes;
+    stats->ms_forward = f_ms;
+    stats->ms_reverse = r_ms;
+    stats->flow_time = acc[N % 3].time;
+  }
+  return GD_OK;
+}

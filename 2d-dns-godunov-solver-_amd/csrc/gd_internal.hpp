@@ -56,6 +56,7 @@ struct MeshView {
   const double* lsq;   // [nf][2][nc] least-squares weights: grad = sum_k w_k (q_nb - q_i)
   const double* rf;    // [nf][2][nc] face midpoint - own centroid
   const int32_t* wall_id; // [nf][nc] index into the wall list or -1
+  const int32_t* kn;      // [nf][nc] the neighbour's slot of the shared face, -1 at a boundary
   // float copies for the FP32-mixed kernels of values that those kernels round
   // to float at first use (bit-identical results, half the bytes)
   const float* fnx32;  // [nf][nc]
@@ -97,6 +98,11 @@ struct Mesh {
   int64_t bandwidth = 0;
   int32_t max_cluster = 0;
   int device = 0;
+  std::vector<int32_t> wall_codes;  // boundary code of each wall-list entry
+  // the same permutations on the device, for column transfers; the identity
+  // is flagged and skipped
+  const int32_t *d_cperm = nullptr, *d_ciperm = nullptr, *d_nperm = nullptr, *d_niperm = nullptr;
+  bool cells_identity = false, nodes_identity = false;
 };
 
 // Run-time accumulator slot, rotated over three buffers (see gd_run).
@@ -136,6 +142,7 @@ struct State {
   Acc* acc = nullptr;                   // [3]
   double* part = nullptr;               // [3 x cell blocks] partial reductions
   double* red = nullptr;                // reduction scratch
+  double* xfer = nullptr;               // [4][max(nc, nn)] staging of host transfers
   std::vector<void*> allocs;
 };
 
@@ -181,6 +188,8 @@ hipError_t launch_wall_sum(const double* wall_f, int32_t n, double* out4, hipStr
 hipError_t launch_fields_derived(int64_t nc, double gamma, const double* Q, double* aS, double* vmax, hipStream_t s);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
+hipError_t launch_permute(double* dst, const double* src, const int32_t* idx, int64_t n, int32_t ncols,
+                          hipStream_t s);
 hipError_t launch_delta(int64_t nc, const double* Un, const double* Uo, double* D, hipStream_t s);
 
 // ---- ns scheme (gd_ns_fp64.hip, gd_ns_fp32.hip) ----
@@ -211,6 +220,26 @@ hipError_t launch_ns_prims_f64(int64_t nc, double gamma, const double* U, void* 
 hipError_t launch_ns_grad_f64(const NsStepArgs& a, hipStream_t s);
 hipError_t launch_ns_prims_f32(int64_t nc, double gamma, const double* U, void* Q, hipStream_t s);
 hipError_t launch_ns_grad_f32(const NsStepArgs& a, hipStream_t s);
+// ---- discrete adjoint of the ns step (gd_adj_fp64.hip / gd_adj_fp32.hip) ----
+struct AdjStepArgs {
+  MeshView m;
+  gd_params prm;
+  const void* rec;      // [nc][16] forward record of the step, compute type
+  void* lamF;           // [nc][4] -dt/A lamU^{n+1}, compute type
+  void* selfq;          // [nc][4] primitive adjoint, own part
+  void* nrev;           // [nc][nf][4] primitive adjoint sent to each neighbour
+  double* bcsum;        // [n_wall][4]
+  const double* lamU_in;
+  double* lamU_out;
+  double dt;
+  int32_t wall_code;
+  double fw[2];
+};
+hipError_t launch_adj_cell_f64(const AdjStepArgs& a, hipStream_t s);
+hipError_t launch_adj_state_f64(const AdjStepArgs& a, bool gather, hipStream_t s);
+hipError_t launch_adj_cell_f32(const AdjStepArgs& a, hipStream_t s);
+hipError_t launch_adj_state_f32(const AdjStepArgs& a, bool gather, hipStream_t s);
+
 hipError_t launch_ns_face_f64(const NsStepArgs& a, hipStream_t s);
 hipError_t launch_ns_update_f64(const NsStepArgs& a, hipStream_t s);
 hipError_t launch_ns_face_f32(const NsStepArgs& a, hipStream_t s);

@@ -455,6 +455,15 @@ __global__ __launch_bounds__(kBlock) void k_copy(double* d, const double* s, int
   if (i < n) d[i] = s[i];
 }
 
+// column-wise gather dst[q][o] = src[q][idx[o]] (host <-> device numbering)
+__global__ __launch_bounds__(kBlock) void k_permute(double* d, const double* s, const int32_t* idx, int64_t n,
+                                                    int32_t ncols) {
+  const int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (o >= n) return;
+  const int64_t j = idx[o];
+  for (int32_t q = 0; q < ncols; ++q) d[q * n + o] = s[q * n + j];
+}
+
 __global__ __launch_bounds__(kBlock) void k_delta(int64_t nc, const double* Un, const double* Uo, double* D) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i < 4 * nc) D[i] = Un[i] - Uo[i];
@@ -476,6 +485,12 @@ hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
 
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(nblocks(n)), dim3(kBlock), 0, s, dst, src, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_permute(double* dst, const double* src, const int32_t* idx, int64_t n, int32_t ncols,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_permute, dim3(nblocks(n)), dim3(kBlock), 0, s, dst, src, idx, n, ncols);
   return hipGetLastError();
 }
 

@@ -348,6 +348,23 @@ class GodunovSolver:
         self.flow_time = rp.flow_time
         return st
 
+    def adjoint(self, n_steps, dt, checkpoint_every=20, wall_code=0, force_weight=(1.0, 0.0), dJ_dUN=None):
+        """Discrete adjoint of ``n_steps`` fixed-dt ns steps from the current
+        state (which ends at U_N, as after ``run``) for
+        J = sum_n w . F_n + <dJ_dUN, U_N>, F_n the wall force of step n on
+        ``wall_code``.  Returns (dJ/dU0 (n_cells, 4), dJ/dbc (MAX_BC + 1, 4),
+        gd_adjoint_stats)."""
+        ap = L.gd_adjoint_params()
+        ap.n_steps = int(n_steps)
+        ap.checkpoint_every = int(checkpoint_every)
+        ap.dt = float(dt)
+        ap.flow_time = float(self.flow_time)
+        ap.wall_code = int(wall_code)
+        ap.force_weight[0], ap.force_weight[1] = float(force_weight[0]), float(force_weight[1])
+        g0, gbc, st = self.state.adjoint(ap, dJ_dUN)
+        self.flow_time = st.flow_time
+        return g0, gbc, st
+
     def conservative(self):
         return self.state.get_array(L.GD_ARR_U_OLD)
 
@@ -405,3 +422,21 @@ class CylinderCase:
 
     def free_stream(self, n):
         return (np.full(n, self.rho_inf), np.full(n, self.u_inf), np.zeros(n), np.full(n, self.p_inf))
+
+    def free_stream_conservative(self, n):
+        r, u, v, p = self.free_stream(n)
+        U = np.zeros((n, 4), order="F")
+        U[:, 0] = r; U[:, 1] = r * u; U[:, 2] = r * v
+        U[:, 3] = p / (self.gamma - 1.0) + 0.5 * r * (u * u + v * v)
+        return U
+
+    def mach_sensitivity(self, dJ_dU0, dJ_dbc, initial_free_stream=True):
+        """dJ/dM_inf at fixed T_inf, p_inf and viscosity (so Re_D moves with
+        M_inf): through the inflow state of code -1 and, when the run started
+        from the free stream, through the initial state."""
+        a_inf = (self.gamma * self.R * self.T_inf) ** 0.5
+        d = float(np.dot(dJ_dbc[1], (0.0, a_inf, 0.0, 0.0)))
+        if initial_free_stream:
+            r, u = self.rho_inf, self.u_inf
+            d += float(r * a_inf * dJ_dU0[:, 1].sum() + r * u * a_inf * dJ_dU0[:, 3].sum())
+        return d

@@ -65,16 +65,55 @@ def run_variant(mesh, case, precision, flux, order_dev, U0, warmup, steps, cfl):
         sol.run(warmup, cfl=cfl, dt_policy=L.GD_DT_LOCAL, wall_code=-3)
     sol.state.sync()
     t0 = time.perf_counter()
-    st = sol.run(steps, cfl=cfl, dt_policy=L.GD_DT_LOCAL, wall_code=-3, time_kernels=True)
+    st = sol.run(steps, cfl=cfl, dt_policy=L.GD_DT_LOCAL, wall_code=-3)
     sol.state.sync()
     el = time.perf_counter() - t0
-    kt = L.kernel_table(st)
+    # per-kernel HIP-event times from a short extra run (events between
+    # launches are kept out of the timed steps above)
+    kt = L.kernel_table(sol.run(max(10, steps // 10), cfl=cfl, dt_policy=L.GD_DT_LOCAL, wall_code=-3,
+                                time_kernels=True))
     U = sol.conservative()
     f = sol.fields()
     sol.close()
     return {"ms_per_step": 1e3 * el / steps, "seconds": el, "kernels": kt, "U": U, "rho": f[0], "p": f[3],
             "drag": float(st.wall_force[0]), "lift": float(st.wall_force[1]), "flow_time": float(st.flow_time),
             "nonphysical": int(st.nonphysical)}
+
+
+def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl):
+    """Discrete adjoint of the summed wall drag over ``steps`` FP32-mixed
+    steps (fixed dt, checkpoints every ``ckpt``) against the same forward
+    steps alone, from the state after 20 CFL steps."""
+    prm = case.params(precision=L.GD_FP32_MIXED)
+    sol = S.GodunovSolver(mesh, prm, order=order_dev)
+    sol.state.set_array(L.GD_ARR_U_OLD, U0)
+    st = sol.run(20, cfl=cfl, dt_policy=L.GD_DT_LOCAL, wall_code=-3)
+    dt = st.last_dt
+    U1 = sol.conservative()
+    sol.run(steps, dt=dt, wall_code=-3)  # warm-up of the fixed-dt path
+    sol.state.set_array(L.GD_ARR_U_OLD, U1)
+    sol.state.sync()
+    t0 = time.perf_counter()
+    st = sol.run(steps, dt=dt, wall_code=-3)
+    sol.state.sync()
+    fwd = time.perf_counter() - t0
+    J = st.wall_force_sum[0]
+    sol.state.set_array(L.GD_ARR_U_OLD, U1)
+    sol.adjoint(2, dt, checkpoint_every=1, wall_code=-3)  # warm-up
+    sol.state.set_array(L.GD_ARR_U_OLD, U1)
+    sol.state.sync()
+    t0 = time.perf_counter()
+    g0, gbc, ast = sol.adjoint(steps, dt, checkpoint_every=ckpt, wall_code=-3)
+    sol.state.sync()
+    adj = time.perf_counter() - t0
+    sol.close()
+    return {"steps": steps, "checkpoint_every": ckpt, "precision": "fp32-mixed", "dt": dt,
+            "forward_ms": round(fwd * 1e3, 3), "adjoint_ms": round(adj * 1e3, 3),
+            "cost_vs_forward": round(adj / fwd, 3),
+            "adjoint_forward_sweep_ms": round(ast.ms_forward, 3), "adjoint_reverse_ms": round(ast.ms_reverse, 3),
+            "recomputed_steps": ast.recomputed_steps, "storage_GB": round(ast.storage_bytes / 1e9, 3),
+            "J_sum_drag": ast.J, "J_rel_diff_vs_forward": abs(ast.J - J) / abs(J),
+            "dJ_dU0_linf": float(np.abs(g0).max()), "dJ_dinflow": [float(x) for x in gbc[1]]}
 
 
 def roofline(kt, steps):
@@ -146,6 +185,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="bytes; run only a device copy of this size")
+    ap.add_argument("--adjoint-steps", type=int, default=200, help="0: skip the adjoint measurement")
+    ap.add_argument("--adjoint-checkpoint", type=int, default=20)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU path is not built",
@@ -211,6 +252,12 @@ def main():
         d["roofline_frac"] = rf["frac"]
         out[var.replace("-", "_")] = d
     out["drag_fp64"] = r64["drag"]
+    if a.adjoint_steps > 0:
+        out["adjoint"] = adjoint_cost(mesh, case, dev_order, U0, a.adjoint_steps, a.adjoint_checkpoint, a.cfl)
+        ad = out["adjoint"]
+        log(f"adjoint: {ad['steps']} mixed steps, checkpoints every {ad['checkpoint_every']}: "
+            f"{ad['adjoint_ms']:.1f} ms vs forward {ad['forward_ms']:.1f} ms -> {ad['cost_vs_forward']:.2f}x "
+            f"(forward sweep {ad['adjoint_forward_sweep_ms']:.1f} ms, reverse {ad['adjoint_reverse_ms']:.1f} ms)")
     if not a.no_cpu:
         threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
         try:

@@ -190,6 +190,39 @@ typedef struct {
 } gd_run_stats;
 
 int gd_run(gd_state* st, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats);
+
+/* ---- discrete adjoint of the ns loop (AUSM+, FP64 or FP32-mixed) ----
+   Not in the reference, which has no adjoint; the objective is
+   J = sum_{n<N} w . F_n + <dJ_dUN, U_N>, F_n the wall force of step n on the
+   faces of code wall_code, at a fixed dt.  Runs N steps forward from the
+   state's U (which ends at U_N, as after gd_run), keeping U every
+   checkpoint_every steps and recomputing the steps between checkpoints on the
+   way back.  dJ_dU0 / dJ_dUN: (n_cells, 4) column-major, caller numbering.
+   dJ_dbc: [(GD_MAX_BC + 1)][4], derivative with respect to the fixed states
+   bc[-code].data[0..3] (rho, u, v, p) of GD_BC_FIXED codes, zero elsewhere. */
+typedef struct {
+  int32_t n_steps;
+  int32_t checkpoint_every;
+  double dt;
+  double flow_time;
+  int32_t wall_code;
+  int32_t reserved;
+  double force_weight[2];
+} gd_adjoint_params;
+
+typedef struct {
+  double J;
+  double force_sum[2];       /* sum over the N steps of the wall force */
+  double flow_time;
+  int32_t recomputed_steps;
+  int32_t n_checkpoints;
+  int64_t storage_bytes;     /* checkpoints + one segment of cell records */
+  double ms_forward;         /* forward sweep, HIP events */
+  double ms_reverse;         /* recomputation + transposed steps */
+} gd_adjoint_stats;
+
+int gd_adjoint_run(gd_state* st, const gd_adjoint_params* ap, const double* dJ_dUN, double* dJ_dU0, double* dJ_dbc,
+                   gd_adjoint_stats* stats);
 int gd_sync(gd_state* st);
 
 /* device pointers for zero-copy callers; valid until the next gd_run/step */
