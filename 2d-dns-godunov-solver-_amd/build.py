@@ -24,11 +24,13 @@ UNITS = {
     "gd_api.hip": [],
     "gd_ref_kernels.hip": ["-ffp-contract=off"],
     "gd_ns_fp64.hip": ["-ffp-contract=off"],
-    # FP32-mixed: hardware reciprocal / square root for float division and sqrt
-    "gd_ns_fp32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    # FP32-mixed: hardware reciprocal / square root for float division and
+    # sqrt; float denormals flushed, which drops the range-scaling steps from
+    # every float division (the values here are far from the denormal range)
+    "gd_ns_fp32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
     # discrete adjoint of the ns step
     "gd_adj_fp64.hip": [],
-    "gd_adj_fp32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "gd_adj_fp32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
 }
 
 

@@ -5,15 +5,15 @@
 //   f_face = Phi(rec_owner, rec_neighbour);  U^{n+1} = U^n - dt/A sum(+-f_face);
 //   J += w . (wall force of step n) = sum over wall faces of w . (f_1, f_2).
 // Reverse step n, given lamU^{n+1} and lamF = -dt/A lamU^{n+1} per cell:
-//   k_adj_cell   four lanes per cell: for each of its faces, lam_f =
-//                lamF(owner) - lamF(neighbour) (+ w on wall faces) and the
-//                transposed Jacobian of the face flux with respect to this
-//                cell's record (the face is evaluated once from each side, in
-//                the owner's orientation); then the transposed Jacobian of the
-//                gradient + limiter with respect to the stencil's primitives
-//                -> own part, and one part per neighbour slot (nrev); ghost
-//                states chained back to the cell or, for fixed states, to the
-//                boundary data
+//   k_adj_cell   per cell: for each of its faces, lam_f = lamF(owner) -
+//                lamF(neighbour) (+ w on wall faces) and the transposed
+//                Jacobian of the face flux with respect to this cell's record
+//                (the face is evaluated once from each side, in the owner's
+//                orientation, so nothing per face is stored); then the
+//                transposed Jacobian of the gradient + limiter with respect to
+//                the stencil's primitives -> own part, and one part per
+//                neighbour slot (nrev); ghost states chained back to the cell
+//                or, for fixed states, to the boundary data
 //   k_adj_state  per cell: lamQ = own part + the neighbours' parts for this
 //                cell; lamU^n = lamU^{n+1} + (dQ/dU)^T lamQ; next lamF
 // Every derivative follows the forward code's branches as a forward-mode dual
@@ -66,12 +66,6 @@ AdjKArgs<T> kargs(const AdjStepArgs& a) {
   k.fw[0] = a.fw[0];
   k.fw[1] = a.fw[1];
   return k;
-}
-
-// x[v] for a run-time v without a dynamically indexed (scratch) array
-template <class T>
-__device__ __forceinline__ T pick4(const T* x, int v) {
-  return v == 0 ? x[0] : v == 1 ? x[1] : v == 2 ? x[2] : x[3];
 }
 
 // derivative sign of fabs as a dual number takes it
@@ -217,25 +211,47 @@ __device__ __forceinline__ void temp_rev(const T* r, T Rg, T lT, const T* lgT, T
   lr[0] -= lrR * rR / rho;
 }
 
-// Transposed face flux for one side of face f.  r0 / r1: owner's and
-// neighbour's records (a boundary face passes the owner's twice); l0 / l1:
-// their lamF (l1 = 0 at a boundary).  Adds the adjoint of the record of the
-// given side (0: owner, 1: neighbour) to lr; at a boundary face returns the
-// fixed-state adjoint in bq (zero unless the code is GD_BC_FIXED).
-template <class T, bool VISC>
-__device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, int64_t f, int32_t c1, int side, const T* r0,
-                                         const T* r1, const T* l0, const T* l1, T* lr, double* bq) {
+// Geometry of one face as the transposed flux needs it (loaded ahead of use)
+template <class T>
+struct FaceIn {
+  T nx, ny, len, rx, ry, rxj, ryj;  // owner's inward normal, length, both offsets
+  bool on_wall;                      // its force enters J
+};
+
+template <class T>
+__device__ __forceinline__ FaceIn<T> load_face(const AdjKArgs<T>& a, int64_t f, int32_t c1) {
   using G = FaceGeo<T>;
   const MeshView& m = a.m;
+  FaceIn<T> g;
+  g.nx = G::fnx(m, f); g.ny = G::fny(m, f);
+  g.len = G::flen(m, f);
+  g.rx = G::fr0(m, 0, f); g.ry = G::fr0(m, 1, f);
+  g.rxj = 0; g.ryj = 0;
+  g.on_wall = false;
+  if (c1 >= 0) {
+    g.rxj = G::fr1(m, 0, f); g.ryj = G::fr1(m, 1, f);
+  } else {
+    g.on_wall = a.wall_code == c1 + 1 && m.fwall[f] >= 0;
+  }
+  return g;
+}
+
+// Transposed face flux for one side of a face.  r0 / r1: owner's and
+// neighbour's records (a boundary face passes the owner's twice); l0 / l1:
+// their lamF (l1 = 0 at a boundary); c1 < 0: the boundary code - 1.  Adds the
+// adjoint of the record of the given side (0: owner, 1: neighbour) to lr; at
+// a boundary face returns the fixed-state adjoint in bq (zero unless the code
+// is GD_BC_FIXED).
+template <class T, bool VISC>
+__device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& fg, int32_t c1, int side,
+                                         const T* r0, const T* r1, const T* l0, const T* l1, T* lr, double* bq) {
   const gd_params& P = a.prm;
-  const T nx = G::fnx(m, f), ny = G::fny(m, f);
-  const T len = G::flen(m, f);
-  const T rx = G::fr0(m, 0, f), ry = G::fr0(m, 1, f);
+  const T nx = fg.nx, ny = fg.ny, len = fg.len, rx = fg.rx, ry = fg.ry;
   const T g = (T)P.Gamma, Rg = (T)P.RGAS;
   T le[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) le[v] = l0[v] - l1[v];
-  if (c1 < 0 && a.wall_code == c1 + 1 && m.fwall[f] >= 0) { le[1] += (T)a.fw[0]; le[2] += (T)a.fw[1]; }
+  if (fg.on_wall) { le[1] += (T)a.fw[0]; le[2] += (T)a.fw[1]; }
 #pragma unroll
   for (int v = 0; v < 4; ++v) le[v] *= len;
   // forward recompute of both face states
@@ -251,9 +267,8 @@ __device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, int64_t f, int32_
   const T* gj = r1 + 4;
   T qb[4];
   bool fb = false;
-  T rxj = 0, ryj = 0;
+  const T rxj = fg.rxj, ryj = fg.ryj;
   if (c1 >= 0) {
-    rxj = G::fr1(m, 0, f); ryj = G::fr1(m, 1, f);
 #pragma unroll
     for (int v = 0; v < 4; ++v) qb[v] = r1[v] + r1[12 + v] * (gj[2 * v] * rxj + gj[2 * v + 1] * ryj);
     fb = !(qb[0] > 0) || !(qb[3] > 0);
@@ -266,12 +281,13 @@ __device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, int64_t f, int32_
   }
   T lqa[4] = {0, 0, 0, 0}, lqb[4] = {0, 0, 0, 0};
   ausm_rev<T>(qb, qa, nx, ny, g, le, lqb, lqa);
-  if (c1 < 0) {
-    const int idx = -(c1 + 1);
-    const int kind = idx <= GD_MAX_BC ? P.bc[idx].kind : GD_BC_EXTRAPOLATE;
-    if (kind == GD_BC_FIXED) {
+  const int bkind = c1 < 0 ? (-(c1 + 1) <= GD_MAX_BC ? P.bc[-(c1 + 1)].kind : GD_BC_EXTRAPOLATE) : -1;
+  // written unconditionally so that bq stays in registers
 #pragma unroll
-      for (int v = 0; v < 4; ++v) bq[v] = (double)lqb[v];
+  for (int v = 0; v < 4; ++v) bq[v] = bkind == GD_BC_FIXED ? (double)lqb[v] : 0.0;
+  if (c1 < 0) {
+    const int kind = bkind;
+    if (kind == GD_BC_FIXED) {
     } else if (kind == GD_BC_SLIP_WALL || kind == GD_BC_ISOTHERMAL_WALL) {
       const T sn = lqb[1] * nx + lqb[2] * ny;
       lqa[0] += lqb[0];
@@ -378,112 +394,92 @@ __device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, int64_t f, int32_
   }
 }
 
-// ---- cell pass: four lanes per cell.  Phase 1, lane k = face slot k: the
-// transposed flux of that face for this cell's side (each face is transposed
-// once from each side, in the owner's orientation, so nothing per face is
-// stored).  Phase 2, lane v = variable v: the transposed gradient + limiter
-// of that variable over the stencil.  Phase 3, lane k: the neighbour part of
-// slot k (nrev) or, at a boundary, the ghost chain back to the cell or to the
-// fixed state.  Phase 4, lane 0: the cell's own part (selfq).
+// ---- cell pass, one thread per cell: every load of the cell's faces is
+// issued before the first transposed flux, then the four variables' transposed
+// gradient + limiter run in the same thread.
 template <class T, bool QUADS, bool VISC>
 __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
   constexpr int NF = QUADS ? 4 : 3;
-  constexpr int CPB = kBlock / 4;  // cells per block
-  __shared__ T sLR[CPB][4][16];     // record adjoint from each face slot
-  __shared__ T sLQ[CPB][NF + 1][4]; // primitive adjoint of each stencil point
-  __shared__ T sGC[CPB][4][4];      // ghost chains back to the cell
   using G = FaceGeo<T>;
   const MeshView& m = a.m;
   const int64_t nc = m.nc;
-  const int lc = threadIdx.x >> 2, r = threadIdx.x & 3;
-  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * CPB + lc;
-  const bool live = i < nc;
-  const int nf = live ? (QUADS ? m.nv[i] : 3) : 0;
-  const bool has_slot = live && r < nf;
-  // ---- phase 1
-  int32_t jn = 0;
-  double bq[4] = {0.0, 0.0, 0.0, 0.0};
-  {
-    T lr[16];
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= nc) return;
+  const int nf = QUADS ? m.nv[i] : 3;
+  int32_t jn[NF], fc[NF];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) lr[t] = 0;
-    if (has_slot) {
-      jn = m.nb[r * nc + i];
-      const int32_t fc = m.cface[r * nc + i];
-      T ri[16], rj[16], li[4], lj[4] = {0, 0, 0, 0};
-      ld16(a.rec + 16 * i, ri);
-      ld4(a.lamF + 4 * i, li);
-      if (jn >= 0) {
-        ld16(a.rec + 16 * (int64_t)jn, rj);
-        ld4(a.lamF + 4 * (int64_t)jn, lj);
-      } else {
+  for (int k = 0; k < NF; ++k) { jn[k] = m.nb[k * nc + i]; fc[k] = m.cface[k * nc + i]; }
+  T ri[16], li[4], rj[NF][16], lj[NF][4];
+  ld16(a.rec + 16 * i, ri);
+  ld4(a.lamF + 4 * i, li);
+  FaceIn<T> fg[NF];
 #pragma unroll
-        for (int t = 0; t < 16; ++t) rj[t] = ri[t];
+  for (int k = 0; k < NF; ++k) {
+    const int64_t jj = jn[k] >= 0 ? jn[k] : i;
+    ld16(a.rec + 16 * jj, rj[k]);
+    ld4(a.lamF + 4 * jj, lj[k]);
+    if (k < nf) fg[k] = load_face(a, fc[k] >> 1, jn[k] >= 0 ? 0 : jn[k]);
+  }
+  T wx[NF], wy[NF], rx[NF], ry[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    wx[k] = G::lsq(m, k, 0, i); wy[k] = G::lsq(m, k, 1, i);
+    rx[k] = G::rf(m, k, 0, i); ry[k] = G::rf(m, k, 1, i);
+  }
+  const T kh = (T)a.prm.venkat_K * G::hmin(m, i);
+  T lr[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) lr[t] = 0;
+  double bqk[NF][4];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) bqk[k][v] = 0.0;
+    if (k < nf) {
+      const int32_t c1 = jn[k] >= 0 ? 0 : jn[k];
+      if (c1 < 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) lj[k][v] = 0;
       }
-      const int64_t f = fc >> 1;
-      const int32_t c1 = jn >= 0 ? 0 : jn;  // only its sign and code are used
-      // owner's record first, by element-wise selects (one code path)
-      const bool sd = (fc & 1) != 0;
+      const bool sd = (fc[k] & 1) != 0;
       T r0[16], r1[16], l0[4], l1[4];
 #pragma unroll
-      for (int t = 0; t < 16; ++t) { r0[t] = sd ? rj[t] : ri[t]; r1[t] = sd ? ri[t] : rj[t]; }
+      for (int t = 0; t < 16; ++t) { r0[t] = sd ? rj[k][t] : ri[t]; r1[t] = sd ? ri[t] : rj[k][t]; }
 #pragma unroll
-      for (int v = 0; v < 4; ++v) { l0[v] = sd ? lj[v] : li[v]; l1[v] = sd ? li[v] : lj[v]; }
-      face_rev<T, VISC>(a, f, c1, sd ? 1 : 0, r0, r1, l0, l1, lr, bq);
+      for (int v = 0; v < 4; ++v) { l0[v] = sd ? lj[k][v] : li[v]; l1[v] = sd ? li[v] : lj[k][v]; }
+      face_rev<T, VISC>(a, fg[k], c1, sd ? 1 : 0, r0, r1, l0, l1, lr, bqk[k]);
     }
-#pragma unroll
-    for (int t = 0; t < 16; ++t) sLR[lc][r][t] = lr[t];
   }
-  __syncthreads();
-  // ---- phase 2: variable v = r
-  if (live) {
-    const int v = r;
-    T lq[NF + 1];
+  const T* qi = ri;
+  T qk[NF][4];
 #pragma unroll
-    for (int p = 0; p <= NF; ++p) lq[p] = 0;
-    T lqv = 0, lgx = 0, lgy = 0, lph = 0;
+  for (int k = 0; k < NF; ++k) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      lqv += sLR[lc][s][v];
-      lgx += sLR[lc][s][4 + 2 * v];
-      lgy += sLR[lc][s][5 + 2 * v];
-      lph += sLR[lc][s][12 + v];
+    for (int v = 0; v < 4; ++v) qk[k][v] = rj[k][v];
+    if (k < nf && jn[k] < 0) {
+      const int64_t s = k * nc + i;
+      ns::ghost<T>(a.prm, jn[k] + 1, qi, G::nx(m, s), G::ny(m, s), qk[k]);
     }
-    lq[0] = lqv;
-    int32_t jk[NF];
+  }
+  T lq[NF + 1][4];
 #pragma unroll
-    for (int k = 0; k < NF; ++k) jk[k] = m.nb[k * nc + i];
-    T qi[4], g2[2];
-    ld4(a.rec + 16 * i, qi);
-    g2[0] = a.rec[16 * i + 4 + 2 * v];
-    g2[1] = a.rec[16 * i + 5 + 2 * v];
-    T qk[NF];
+  for (int v = 0; v < 4; ++v) {
+    lq[0][v] = lr[v];
 #pragma unroll
-    for (int k = 0; k < NF; ++k) qk[k] = a.rec[16 * (int64_t)(jk[k] >= 0 ? jk[k] : i) + v];
-    T wx[NF], wy[NF], rx[NF], ry[NF];
+    for (int k = 0; k < NF; ++k) lq[1 + k][v] = 0;
+  }
 #pragma unroll
-    for (int k = 0; k < NF; ++k) {
-      wx[k] = G::lsq(m, k, 0, i); wy[k] = G::lsq(m, k, 1, i);
-      rx[k] = G::rf(m, k, 0, i); ry[k] = G::rf(m, k, 1, i);
-    }
-    const T kh = (T)a.prm.venkat_K * G::hmin(m, i);
-#pragma unroll
-    for (int k = 0; k < NF; ++k)
-      if (k < nf && jk[k] < 0) {
-        const int64_t s = k * nc + i;
-        T gq[4];
-        ns::ghost<T>(a.prm, jk[k] + 1, qi, G::nx(m, s), G::ny(m, s), gq);
-        qk[k] = pick4(gq, v);
-      }
-    const T gx = g2[0], gy = g2[1], q0 = pick4(qi, v);
-    // stencil extremes with their points (strictly larger / smaller wins)
-    T qmx = q0, qmn = q0;
+  for (int v = 0; v < 4; ++v) {
+    const T gx = ri[4 + 2 * v], gy = ri[5 + 2 * v];
+    T lgx = lr[4 + 2 * v], lgy = lr[5 + 2 * v];
+    const T lph = lr[12 + v];
+    T qmx = qi[v], qmn = qi[v];
     int amx = 0, amn = 0;
 #pragma unroll
     for (int k = 0; k < NF; ++k)
       if (k < nf) {
-        if (qk[k] > qmx) { qmx = qk[k]; amx = 1 + k; }
-        if (qk[k] < qmn) { qmn = qk[k]; amn = 1 + k; }
+        if (qk[k][v] > qmx) { qmx = qk[k][v]; amx = 1 + k; }
+        if (qk[k][v] < qmn) { qmn = qk[k][v]; amn = 1 + k; }
       }
     const T e2 = kh * kh * kh * (T)(a.prm.venkat_scale[v] * a.prm.venkat_scale[v]);
     T phi = 1;
@@ -493,19 +489,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
       if (k < nf) {
         const T d2 = gx * rx[k] + gy * ry[k];
         if (d2 != 0) {
-          const T d1 = d2 > 0 ? qmx - q0 : qmn - q0;
+          const T d1 = d2 > 0 ? qmx - qi[v] : qmn - qi[v];
           const T psi = ((d1 * d1 + e2) * d2 + 2 * d2 * d2 * d1) / (d1 * d1 + 2 * d2 * d2 + d1 * d2 + e2) / d2;
           if (psi < phi) { phi = psi; ks = k; }
         }
       }
-    if (ks >= 0 && !(phi < 0)) {  // phi = psi of slot ks (not clipped at 0 or 1)
+    if (ks >= 0 && !(phi < 0)) {
       T rxs = 0, rys = 0;
 #pragma unroll
       for (int k = 0; k < NF; ++k)
         if (k == ks) { rxs = rx[k]; rys = ry[k]; }
       const T d2 = gx * rxs + gy * rys;
       const int ae = d2 > 0 ? amx : amn;
-      const T d1 = (d2 > 0 ? qmx : qmn) - q0;
+      const T d1 = (d2 > 0 ? qmx : qmn) - qi[v];
       const T D = d1 * d1 + 2 * d2 * d2 + d1 * d2 + e2;
       const T dd1 = ((2 * d1 + 2 * d2) - phi * (2 * d1 + d2)) / D;
       const T dd2 = (2 * d1 - phi * (4 * d2 + d1)) / D;
@@ -514,61 +510,47 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
       lgy += ld2 * rys;
 #pragma unroll
       for (int p = 0; p <= NF; ++p)
-        if (p == ae) lq[p] += ld1;
-      lq[0] -= ld1;
+        if (p == ae) lq[p][v] += ld1;
+      lq[0][v] -= ld1;
     }
 #pragma unroll
     for (int k = 0; k < NF; ++k)
       if (k < nf) {
         const T t = wx[k] * lgx + wy[k] * lgy;
-        lq[1 + k] += t;
-        lq[0] -= t;
+        lq[1 + k][v] += t;
+        lq[0][v] -= t;
       }
-#pragma unroll
-    for (int p = 0; p <= NF; ++p) sLQ[lc][p][v] = lq[p];
   }
-  __syncthreads();
-  // ---- phase 3: slot k = r
-  {
-    T gc[4] = {0, 0, 0, 0};
-    if (has_slot) {
-      const int k = r;
-      T l[4];
+  const int32_t nfs = m.nf;
 #pragma unroll
-      for (int w = 0; w < 4; ++w) l[w] = sLQ[lc][1 + k][w];
-      if (jn >= 0) {
-        st4(a.nrev + 4 * (i * m.nf + k), l[0], l[1], l[2], l[3]);
-      } else {
-        const int idx = -(jn + 1);
-        const int kind = idx <= GD_MAX_BC ? a.prm.bc[idx].kind : GD_BC_EXTRAPOLATE;
-        const int64_t s = k * nc + i;
-        if (kind == GD_BC_FIXED) {
-          const int32_t wid = m.wall_id[s];
-          if (wid >= 0) {
+  for (int k = 0; k < NF; ++k) {
+    if (k >= nf) continue;
+    if (jn[k] >= 0) {
+      st4(a.nrev + 4 * (i * nfs + k), lq[1 + k][0], lq[1 + k][1], lq[1 + k][2], lq[1 + k][3]);
+    } else {
+      const int idx = -(jn[k] + 1);
+      const int kind = idx <= GD_MAX_BC ? a.prm.bc[idx].kind : GD_BC_EXTRAPOLATE;
+      const int64_t s = k * nc + i;
+      if (kind == GD_BC_FIXED) {
+        const int32_t wid = m.wall_id[s];
+        if (wid >= 0) {
 #pragma unroll
-            for (int w = 0; w < 4; ++w) a.bcsum[4 * (int64_t)wid + w] += bq[w] + (double)l[w];
-          }
-        } else if (kind == GD_BC_SLIP_WALL || kind == GD_BC_ISOTHERMAL_WALL) {
-          const T nx = G::nx(m, s), ny = G::ny(m, s);
-          const T sn = l[1] * nx + l[2] * ny;
-          gc[0] = l[0]; gc[1] = l[1] - 2 * nx * sn; gc[2] = l[2] - 2 * ny * sn; gc[3] = l[3];
-        } else {
-#pragma unroll
-          for (int w = 0; w < 4; ++w) gc[w] = l[w];
+          for (int v = 0; v < 4; ++v) a.bcsum[4 * (int64_t)wid + v] += bqk[k][v] + (double)lq[1 + k][v];
         }
+      } else if (kind == GD_BC_SLIP_WALL || kind == GD_BC_ISOTHERMAL_WALL) {
+        const T nx = G::nx(m, s), ny = G::ny(m, s);
+        const T sn = lq[1 + k][1] * nx + lq[1 + k][2] * ny;
+        lq[0][0] += lq[1 + k][0];
+        lq[0][1] += lq[1 + k][1] - 2 * nx * sn;
+        lq[0][2] += lq[1 + k][2] - 2 * ny * sn;
+        lq[0][3] += lq[1 + k][3];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) lq[0][v] += lq[1 + k][v];
       }
     }
-#pragma unroll
-    for (int w = 0; w < 4; ++w) sGC[lc][r][w] = gc[w];
   }
-  __syncthreads();
-  // ---- phase 4
-  if (live && r == 0) {
-    T o[4];
-#pragma unroll
-    for (int w = 0; w < 4; ++w) o[w] = sLQ[lc][0][w] + sGC[lc][0][w] + sGC[lc][1][w] + sGC[lc][2][w] + sGC[lc][3][w];
-    st4(a.selfq + 4 * i, o[0], o[1], o[2], o[3]);
-  }
+  st4(a.selfq + 4 * i, lq[0][0], lq[0][1], lq[0][2], lq[0][3]);
 }
 
 // ---- state pass: gather the primitive adjoint, transpose the primitive
@@ -614,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_state(AdjKArgs<T> a) {
 
 template <class T>
 hipError_t launch_cell(const AdjKArgs<T>& a, hipStream_t s) {
-  const dim3 g((unsigned)((a.m.nc + kBlock / 4 - 1) / (kBlock / 4))), b(kBlock);
+  const dim3 g(ns::nblocks(a.m.nc)), b(kBlock);
   const bool visc = a.prm.viscous != 0;
   if (a.m.has_quads) {
     if (visc) hipLaunchKernelGGL((k_adj_cell<T, true, true>), g, b, 0, s, a);
@@ -625,6 +607,7 @@ hipError_t launch_cell(const AdjKArgs<T>& a, hipStream_t s) {
   }
   return hipGetLastError();
 }
+
 template <class T>
 hipError_t launch_state(const AdjKArgs<T>& a, bool gather, hipStream_t s) {
   const dim3 g(ns::nblocks(a.m.nc)), b(kBlock);

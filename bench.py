@@ -106,6 +106,13 @@ def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl):
     g0, gbc, ast = sol.adjoint(steps, dt, checkpoint_every=ckpt, wall_code=-3)
     sol.state.sync()
     adj = time.perf_counter() - t0
+    # the same with every step's record kept (one segment, nothing recomputed)
+    sol.state.set_array(L.GD_ARR_U_OLD, U1)
+    sol.state.sync()
+    t0 = time.perf_counter()
+    g1, _, ast1 = sol.adjoint(steps, dt, checkpoint_every=steps, wall_code=-3)
+    sol.state.sync()
+    adj1 = time.perf_counter() - t0
     sol.close()
     return {"steps": steps, "checkpoint_every": ckpt, "precision": "fp32-mixed", "dt": dt,
             "forward_ms": round(fwd * 1e3, 3), "adjoint_ms": round(adj * 1e3, 3),
@@ -113,7 +120,10 @@ def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl):
             "adjoint_forward_sweep_ms": round(ast.ms_forward, 3), "adjoint_reverse_ms": round(ast.ms_reverse, 3),
             "recomputed_steps": ast.recomputed_steps, "storage_GB": round(ast.storage_bytes / 1e9, 3),
             "J_sum_drag": ast.J, "J_rel_diff_vs_forward": abs(ast.J - J) / abs(J),
-            "dJ_dU0_linf": float(np.abs(g0).max()), "dJ_dinflow": [float(x) for x in gbc[1]]}
+            "dJ_dU0_linf": float(np.abs(g0).max()), "dJ_dinflow": [float(x) for x in gbc[1]],
+            "all_records": {"checkpoint_every": steps, "adjoint_ms": round(adj1 * 1e3, 3),
+                            "cost_vs_forward": round(adj1 / fwd, 3), "storage_GB": round(ast1.storage_bytes / 1e9, 3),
+                            "max_rel_diff_vs_checkpointed": float(np.abs(g1 - g0).max() / np.abs(g0).max())}}
 
 
 def roofline(kt, steps):
@@ -233,6 +243,7 @@ def main():
         "roofline": {**roof, "kernel": dom["name"], "bytes_per_launch": dom["bytes_per_launch"],
                      "avg_launch_us": round(dom["ms_avg"] * 1e3, 2)},
         "kernels_us": {k["name"]: round(k["ms_avg"] * 1e3, 2) for k in r64["kernels"]},
+        "kernel_bytes": {k["name"]: k["bytes_per_launch"] for r in res.values() for k in r["kernels"]},
     }
     for var, r in res.items():
         if var == head:
@@ -257,7 +268,8 @@ def main():
         ad = out["adjoint"]
         log(f"adjoint: {ad['steps']} mixed steps, checkpoints every {ad['checkpoint_every']}: "
             f"{ad['adjoint_ms']:.1f} ms vs forward {ad['forward_ms']:.1f} ms -> {ad['cost_vs_forward']:.2f}x "
-            f"(forward sweep {ad['adjoint_forward_sweep_ms']:.1f} ms, reverse {ad['adjoint_reverse_ms']:.1f} ms)")
+            f"(forward sweep {ad['adjoint_forward_sweep_ms']:.1f} ms, reverse {ad['adjoint_reverse_ms']:.1f} ms); "
+            f"all records kept: {ad['all_records']['adjoint_ms']:.1f} ms -> {ad['all_records']['cost_vs_forward']:.2f}x")
     if not a.no_cpu:
         threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
         try:
