@@ -255,69 +255,82 @@ __global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
     else if (a.wall_f) wall_sum_block(a.wall_f, m.n_wall, a.drag);
     return;
   }
-  const int64_t i = (int64_t)xcd_block(blockIdx.x - nx, gridDim.x - nx) * kBlock + threadIdx.x;
-  if (i >= nc) return;
-  const int nf = QUADS ? m.nv[i] : 3;
-  // every load first: neighbour ids, then all neighbour states (boundary and
-  // absent faces read the own cell, overwritten below), then geometry
-  int32_t jn[NF];
+  const int64_t base = (int64_t)xcd_block(blockIdx.x - nx, gridDim.x - nx) * kBlock;
+  const int64_t i = base + threadIdx.x;
+  // the block's records are staged in LDS and written out as one contiguous,
+  // fully coalesced range (a 64 B / 128 B record per lane would otherwise
+  // leave every store instruction scattered over 64 partial lines)
+  __shared__ T sRec[kBlock * 16];
+  if (i < nc) {
+    const int nf = QUADS ? m.nv[i] : 3;
+    // every load first: neighbour ids, then all neighbour states (boundary and
+    // absent faces read the own cell, overwritten below), then geometry
+    int32_t jn[NF];
 #pragma unroll
-  for (int k = 0; k < NF; ++k) jn[k] = m.nb[k * nc + i];
-  T qi[4], qk[NF][4];
-  ld4(a.Q + 4 * i, qi);
+    for (int k = 0; k < NF; ++k) jn[k] = m.nb[k * nc + i];
+    T qi[4], qk[NF][4];
+    ld4(a.Q + 4 * i, qi);
 #pragma unroll
-  for (int k = 0; k < NF; ++k) ld4(a.Q + 4 * (int64_t)(jn[k] >= 0 ? jn[k] : i), qk[k]);
-  T wx[NF], wy[NF], rx[NF], ry[NF];
+    for (int k = 0; k < NF; ++k) ld4(a.Q + 4 * (int64_t)(jn[k] >= 0 ? jn[k] : i), qk[k]);
+    T wx[NF], wy[NF], rx[NF], ry[NF];
 #pragma unroll
-  for (int k = 0; k < NF; ++k) {
-    wx[k] = G::lsq(m, k, 0, i); wy[k] = G::lsq(m, k, 1, i);
-    rx[k] = G::rf(m, k, 0, i); ry[k] = G::rf(m, k, 1, i);
-  }
-  const T kh = (T)a.prm.venkat_K * G::hmin(m, i);
-  T gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0}, qmx[4], qmn[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) { qmx[v] = qi[v]; qmn[v] = qi[v]; }
-#pragma unroll
-  for (int k = 0; k < NF; ++k) {
-    if (k < nf) {
-      if (jn[k] < 0) {
-        const int64_t s = k * nc + i;
-        ghost<T>(a.prm, jn[k] + 1, qi, G::nx(m, s), G::ny(m, s), qk[k]);
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const T d = qk[k][v] - qi[v];
-        gx[v] += wx[k] * d;
-        gy[v] += wy[k] * d;
-        qmx[v] = fmax(qmx[v], qk[k][v]);
-        qmn[v] = fmin(qmn[v], qk[k][v]);
-      }
+    for (int k = 0; k < NF; ++k) {
+      wx[k] = G::lsq(m, k, 0, i); wy[k] = G::lsq(m, k, 1, i);
+      rx[k] = G::rf(m, k, 0, i); ry[k] = G::rf(m, k, 1, i);
     }
-  }
-  const T kh3 = kh * kh * kh;
-  T ph[4];
+    const T kh = (T)a.prm.venkat_K * G::hmin(m, i);
+    T gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0}, qmx[4], qmn[4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const T e2 = kh3 * (T)(a.prm.venkat_scale[v] * a.prm.venkat_scale[v]);
-    T phi = 1;
+    for (int v = 0; v < 4; ++v) { qmx[v] = qi[v]; qmn[v] = qi[v]; }
 #pragma unroll
     for (int k = 0; k < NF; ++k) {
       if (k < nf) {
-        const T d2 = gx[v] * rx[k] + gy[v] * ry[k];
-        if (d2 != 0) {
-          const T d1 = d2 > 0 ? qmx[v] - qi[v] : qmn[v] - qi[v];
-          const T psi = ((d1 * d1 + e2) * d2 + 2 * d2 * d2 * d1) / (d1 * d1 + 2 * d2 * d2 + d1 * d2 + e2) / d2;
-          phi = fmin(phi, psi);
+        if (jn[k] < 0) {
+          const int64_t s = k * nc + i;
+          ghost<T>(a.prm, jn[k] + 1, qi, G::nx(m, s), G::ny(m, s), qk[k]);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const T d = qk[k][v] - qi[v];
+          gx[v] += wx[k] * d;
+          gy[v] += wy[k] * d;
+          qmx[v] = fmax(qmx[v], qk[k][v]);
+          qmn[v] = fmin(qmn[v], qk[k][v]);
         }
       }
     }
-    ph[v] = fmax(phi, (T)0);
+    const T kh3 = kh * kh * kh;
+    T ph[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const T e2 = kh3 * (T)(a.prm.venkat_scale[v] * a.prm.venkat_scale[v]);
+      T phi = 1;
+#pragma unroll
+      for (int k = 0; k < NF; ++k) {
+        if (k < nf) {
+          const T d2 = gx[v] * rx[k] + gy[v] * ry[k];
+          if (d2 != 0) {
+            const T d1 = d2 > 0 ? qmx[v] - qi[v] : qmn[v] - qi[v];
+            const T psi = ((d1 * d1 + e2) * d2 + 2 * d2 * d2 * d1) / (d1 * d1 + 2 * d2 * d2 + d1 * d2 + e2) / d2;
+            phi = fmin(phi, psi);
+          }
+        }
+      }
+      ph[v] = fmax(phi, (T)0);
+    }
+    T* r = sRec + 16 * threadIdx.x;
+    st4(r, qi[0], qi[1], qi[2], qi[3]);
+    st4(r + 4, gx[0], gy[0], gx[1], gy[1]);
+    st4(r + 8, gx[2], gy[2], gx[3], gy[3]);
+    st4(r + 12, ph[0], ph[1], ph[2], ph[3]);
   }
-  T* r = a.rec + 16 * i;
-  st4(r, qi[0], qi[1], qi[2], qi[3]);
-  st4(r + 4, gx[0], gy[0], gx[1], gy[1]);
-  st4(r + 8, gx[2], gy[2], gx[3], gy[3]);
-  st4(r + 12, ph[0], ph[1], ph[2], ph[3]);
+  __syncthreads();
+  const int64_t nvalid = nc - base < kBlock ? nc - base : kBlock;
+  T* out = a.rec + 16 * base;
+  for (int64_t c = threadIdx.x; c < 4 * nvalid; c += kBlock) {
+    const T* q = sRec + 4 * c;
+    st4(out + 4 * c, q[0], q[1], q[2], q[3]);
+  }
 }
 
 __device__ __forceinline__ double wave_max64(double v) {
