@@ -135,7 +135,7 @@ struct State {
   double* g6 = nullptr;                 // [6][nc]
   double* g6N = nullptr;                // [6][nn]
   void* rec = nullptr;                  // ns: per-cell record [nc][16] (q, grad q, limiter), compute type
-  void* fflux = nullptr;                // ns: [4][n_faces] face fluxes, compute type
+  void* fflux = nullptr;                // ns: [n_faces][4] face fluxes, compute type
   void* Qt[2] = {nullptr, nullptr};     // ns: primitives [nc][4], compute type
   double* wall_f = nullptr;             // [n_wall][2]
   double* drag = nullptr;               // [4]: last step force x,y; running sums
@@ -201,7 +201,7 @@ struct NsStepArgs {
   const void* Q;        // current primitives [nc][4], compute type
   void* Qw;             // next primitives
   void* rec;            // [nc][16]: q, (d/dx, d/dy) of rho,u,v,p, limiter of rho,u,v,p
-  void* fflux;          // [4][n_faces] flux through each face, leaving its owner
+  void* fflux;          // [n_faces][4] flux through each face, leaving its owner
   Acc* acc;
   double* part;         // per-block partials of the dt inputs [3 x blocks]
   int32_t step;

@@ -35,7 +35,7 @@ struct NsKArgs {
   const T* Q;       // current primitives (rho,u,v,p) [nc][4]
   T* Qw;            // next primitives [nc][4]
   T* rec;           // [nc][16]: q[4], (d/dx, d/dy) of rho,u,v,p [8], limiter [4]
-  T* fflux;         // [4][n_faces]: flux leaving the face's owner
+  T* fflux;         // [n_faces][4]: flux leaving the face's owner
   Acc* acc;
   double* part;     // per-block partials of the next dt [3 x blocks]
   int32_t step;
@@ -454,10 +454,8 @@ __global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
   }
   double fk[4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    fk[v] = (double)((e[v] + gv[v]) * len);
-    a.fflux[v * nF + f] = (T)fk[v];
-  }
+  for (int v = 0; v < 4; ++v) fk[v] = (double)((e[v] + gv[v]) * len);
+  st4(a.fflux + 4 * f, (T)fk[0], (T)fk[1], (T)fk[2], (T)fk[3]);
   if (c1 < 0 && a.wall_f && a.wall_code == c1 + 1) {
     const int32_t wid = m.fwall[f];
     if (wid >= 0) { a.wall_f[2 * wid] = fk[1]; a.wall_f[2 * wid + 1] = fk[2]; }
@@ -470,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_update(NsKArgs<T> a) {
   constexpr int NF = QUADS ? 4 : 3;
   const MeshView& m = a.m;
   const gd_params& P = a.prm;
-  const int64_t nc = m.nc, nF = m.n_faces;
+  const int64_t nc = m.nc;
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const Acc r = a.acc[a.step % 3];
   double dt = a.dt;
@@ -490,14 +488,17 @@ __global__ __launch_bounds__(kBlock) void k_ns_update(NsKArgs<T> a) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) Uo[v] = a.Uo[v * nc + i];
     double F[4] = {0.0, 0.0, 0.0, 0.0};
+    T fl[NF][4];
+#pragma unroll
+    for (int k = 0; k < NF; ++k)
+      if (k < nf) ld4(a.fflux + 4 * (int64_t)(fc[k] >> 1), fl[k]);
 #pragma unroll
     for (int k = 0; k < NF; ++k) {
       if (k < nf) {
-        const int64_t fid = fc[k] >> 1;
         const bool neg = (fc[k] & 1) != 0;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const double x = (double)a.fflux[v * nF + fid];
+          const double x = (double)fl[k][v];
           F[v] += neg ? -x : x;
         }
       }
