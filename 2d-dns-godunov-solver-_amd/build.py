@@ -28,8 +28,10 @@ UNITS = {
     # sqrt; float denormals flushed, which drops the range-scaling steps from
     # every float division (the values here are far from the denormal range)
     "gd_ns_fp32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
-    # discrete adjoint of the ns step
-    "gd_adj_fp64.hip": [],
+    # discrete adjoint of the ns step; FP64 without contraction, so that its
+    # recomputed forward values are the forward kernels' to the bit and every
+    # tie (min, max, limiter) is broken the same way
+    "gd_adj_fp64.hip": ["-ffp-contract=off"],
     "gd_adj_fp32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
 }
 

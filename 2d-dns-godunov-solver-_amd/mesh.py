@@ -350,7 +350,8 @@ def ogrid_radii(n_r: int, r0: float, r1: float, dr0: float) -> np.ndarray:
     lo, hi = 1.0 + 1e-12, 2.0
     for _ in range(200):
         q = 0.5 * (lo + hi)
-        s = dr0 * (q ** m - 1.0) / (q - 1.0)
+        lqm = m * np.log(q)  # q^m overflows for fine grids at the upper bracket
+        s = np.inf if lqm > 700.0 else dr0 * np.expm1(lqm) / (q - 1.0)
         if s > span:
             hi = q
         else:

@@ -97,6 +97,28 @@ def test_adjoint_fp64_dot_product_with_tangent_linear_oracle(quads, viscous):
     assert abs(float((gbc * dbc).sum()) - dJb) <= 1e-9 * abs(dJb)
 
 
+def test_adjoint_periodic_tgv_mesh_dot_product():
+    """The periodic 20,000-triangle Taylor-Green mesh (no boundary faces):
+    J = <w, U_N> after 8 viscous FP64 steps, against the dual-number oracle."""
+    M = pkg().mesh
+    mesh = M.make_tgv_periodic_mesh(100)
+    n = mesh.nCells
+    x, y = mesh.cell_mid_points[:, 0], mesh.cell_mid_points[:, 1]
+    r = 1.0 + 1e-3 * M.normal(SEED, n)
+    u, v = np.sin(x) * np.cos(y), -np.cos(x) * np.sin(y)
+    p = 1.0 / (1.4 * 0.01) + (np.cos(2 * x) + np.cos(2 * y)) / 4.0
+    U = np.zeros((n, 4), order="F")
+    U[:, 0] = r; U[:, 1] = r * u; U[:, 2] = r * v; U[:, 3] = p / 0.4 + 0.5 * r * (u * u + v * v)
+    w = np.asfortranarray(M.normal(SEED + 3, 4 * n).reshape(n, 4, order="F"))
+    g0, _, st, _ = _adjoint(mesh, U, 8, 3, w)
+    d = np.asfortranarray(M.normal(SEED + 4, 4 * n).reshape(n, 4, order="F") * np.abs(U).max(axis=0))
+    J, dJ = _tl_objective(mesh, U, 8, d, None, w)
+    adj = float((g0 * d).sum())
+    print("tgv <dJ/dU0, d>", adj, "tangent", dJ)
+    assert abs(adj - dJ) <= 1e-9 * abs(dJ)
+    assert abs(st.J - J) <= 1e-11 * abs(J)
+
+
 def test_adjoint_independent_of_checkpointing_and_numbering():
     L = pkg()._lib
     M = pkg().mesh
