@@ -85,7 +85,7 @@ class gd_run_stats(C.Structure):  # noqa: N801
 class gd_adjoint_params(C.Structure):  # noqa: N801
     _fields_ = [("n_steps", C.c_int32), ("checkpoint_every", C.c_int32), ("dt", C.c_double),
                 ("flow_time", C.c_double), ("wall_code", C.c_int32), ("reserved", C.c_int32),
-                ("force_weight", C.c_double * 2)]
+                ("force_weight", C.c_double * 2), ("record_budget", C.c_int64)]
 
 
 class gd_adjoint_stats(C.Structure):  # noqa: N801

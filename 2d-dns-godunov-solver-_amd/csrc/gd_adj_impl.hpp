@@ -14,8 +14,9 @@
 //                the stencil's primitives -> own part, and one part per
 //                neighbour slot (nrev); ghost states chained back to the cell
 //                or, for fixed states, to the boundary data
-//   k_adj_state  per cell: lamQ = own part + the neighbours' parts for this
-//                cell; lamU^n = lamU^{n+1} + (dQ/dU)^T lamQ; next lamF
+//   k_adj_state  per cell: lamU^n = lamU^{n+1} + own part + the neighbours'
+//                parts for this cell (the cell pass sends every part already
+//                through (dQ/dU)^T of the receiving cell); next lamF
 // Every derivative follows the forward code's branches as a forward-mode dual
 // number would (fabs'(0) = +1; fmin/fmax take the first argument on ties), so
 // the transposed step is exactly the transpose of the tangent-linear step.
@@ -39,8 +40,8 @@ struct AdjKArgs {
   gd_params prm;
   const T* rec;        // [nc][16] forward record of the step
   T* lamF;             // [nc][4]  -dt/A lamU^{n+1}
-  T* selfq;            // [nc][4]
-  T* nrev;             // [nc][nf][4]
+  T* selfq;            // [nc][4] own part, conservative variables
+  T* nrev;             // [nc][nf][4] part for each neighbour, its conservative variables
   double* bcsum;       // [n_wall][4] fixed-state adjoint summed over steps
   const double* lamU_in;
   double* lamU_out;
@@ -66,6 +67,17 @@ AdjKArgs<T> kargs(const AdjStepArgs& a) {
   k.fw[0] = a.fw[0];
   k.fw[1] = a.fw[1];
   return k;
+}
+
+// (dQ/dU)^T l for Q = (rho, u, v, p) of U = (rho, rho u, rho v, E) at the
+// primitive state q, in T (the cell pass applies it to every part it sends)
+template <class T>
+__device__ __forceinline__ void prims_rev(const T* q, T gm1, const T* l, T* o) {
+  const T ri = 1 / q[0], u = q[1], v = q[2];
+  o[0] = l[0] - (u * l[1] + v * l[2]) * ri + gm1 * (T)0.5 * (u * u + v * v) * l[3];
+  o[1] = l[1] * ri - gm1 * u * l[3];
+  o[2] = l[2] * ri - gm1 * v * l[3];
+  o[3] = gm1 * l[3];
 }
 
 // derivative sign of fabs as a dual number takes it
@@ -522,11 +534,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
       }
   }
   const int32_t nfs = m.nf;
+  const T gm1 = (T)(a.prm.Gamma - 1.0);
 #pragma unroll
   for (int k = 0; k < NF; ++k) {
     if (k >= nf) continue;
     if (jn[k] >= 0) {
-      st4(a.nrev + 4 * (i * nfs + k), lq[1 + k][0], lq[1 + k][1], lq[1 + k][2], lq[1 + k][3]);
+      T o[4];
+      prims_rev<T>(rj[k], gm1, lq[1 + k], o);  // in the neighbour's conservative variables
+      st4(a.nrev + 4 * (i * nfs + k), o[0], o[1], o[2], o[3]);
     } else {
       const int idx = -(jn[k] + 1);
       const int kind = idx <= GD_MAX_BC ? a.prm.bc[idx].kind : GD_BC_EXTRAPOLATE;
@@ -550,7 +565,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
       }
     }
   }
-  st4(a.selfq + 4 * i, lq[0][0], lq[0][1], lq[0][2], lq[0][3]);
+  T o[4];
+  prims_rev<T>(qi, gm1, lq[0], o);
+  st4(a.selfq + 4 * i, o[0], o[1], o[2], o[3]);
 }
 
 // ---- state pass: gather the primitive adjoint, transpose the primitive
@@ -568,7 +585,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_state(AdjKArgs<T> a) {
     const int nf = m.has_quads ? m.nv[j] : 3;
     T s4[4];
     ld4(a.selfq + 4 * j, s4);
-    double lQ[4] = {(double)s4[0], (double)s4[1], (double)s4[2], (double)s4[3]};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) lU[v] += (double)s4[v];
     for (int k = 0; k < nf; ++k) {
       const int32_t jn = m.nb[k * nc + j];
       if (jn >= 0) {
@@ -576,17 +594,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_state(AdjKArgs<T> a) {
         T b[4];
         ld4(a.nrev + 4 * ((int64_t)jn * m.nf + kk), b);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) lQ[v] += (double)b[v];
+        for (int v = 0; v < 4; ++v) lU[v] += (double)b[v];
       }
     }
-    T q[4];
-    ld4(a.rec + 16 * j, q);
-    const double rho = q[0], u = q[1], v = q[2], gm1 = a.prm.Gamma - 1.0;
-    const double ri = 1.0 / rho;
-    lU[0] += lQ[0] - (u * lQ[1] + v * lQ[2]) * ri + gm1 * 0.5 * (u * u + v * v) * lQ[3];
-    lU[1] += lQ[1] * ri - gm1 * u * lQ[3];
-    lU[2] += lQ[2] * ri - gm1 * v * lQ[3];
-    lU[3] += gm1 * lQ[3];
 #pragma unroll
     for (int w = 0; w < 4; ++w) a.lamU_out[w * nc + j] = lU[w];
   }

@@ -1052,13 +1052,17 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
   const bool wall = ap->wall_code != 0 && v.n_wall > 0;
   const int64_t nw = std::max<int32_t>(v.n_wall, 1);
 
+  if (ap->record_budget < 0) return fail(GD_ERR_INVALID, "negative record budget");
+  // records kept for the last `keep` segments; earlier segments are recomputed
+  const size_t seg_bytes = (size_t)c * 16 * nc * t;
+  const int32_t keep = (int32_t)std::max<int64_t>(1, std::min<int64_t>(nseg, ap->record_budget / (int64_t)seg_bytes));
   DevScratch D;
   int64_t bytes = 0;
   int rc = 0;
   double* ck = nullptr;
   rc |= dev_alloc(D.p, &ck, (size_t)nseg * 4 * nc, &bytes);
   char* recs = nullptr;
-  rc |= dev_alloc(D.p, &recs, (size_t)c * 16 * nc * t, &bytes);
+  rc |= dev_alloc(D.p, &recs, (size_t)keep * seg_bytes, &bytes);
   double *Uw0 = nullptr, *Uw1 = nullptr, *lU0 = nullptr, *lU1 = nullptr, *bcsum = nullptr;
   rc |= dev_alloc(D.p, &Uw0, 4 * nc);
   rc |= dev_alloc(D.p, &Uw1, 4 * nc);
@@ -1070,7 +1074,14 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
   rc |= dev_alloc(D.p, &selfq, 4 * nc * t);
   rc |= dev_alloc(D.p, &nrev, 4 * (size_t)v.nf * nc * t);
   if (rc) return rc;
-  auto rec_slot = [&](int32_t s) { return (void*)(recs + (size_t)(s % c) * 16 * nc * t); };
+  // record slot of step s: kept segments have their own region; the others
+  // (recomputed later) share the first kept region, which only later steps
+  // of the forward sweep overwrite and which the reverse sweep no longer needs
+  // when it reaches them
+  auto rec_slot = [&](int32_t s) {
+    const int32_t kidx = std::max<int32_t>(0, s / c - (nseg - keep));
+    return (void*)(recs + (size_t)kidx * seg_bytes + (size_t)(s % c) * 16 * nc * t);
+  };
 
   auto prims = mixed ? launch_ns_prims_f32 : launch_ns_prims_f64;
   auto grad = mixed ? launch_ns_grad_f32 : launch_ns_grad_f64;
@@ -1178,7 +1189,7 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
   int32_t recomputed = 0;
   for (int32_t seg = nseg - 1; seg >= 0; --seg) {
     const int32_t s0 = seg * c, s1 = std::min(N, s0 + c);
-    if (seg != nseg - 1) {
+    if (seg < nseg - keep) {
       GD_HIP(launch_copy(Uw0, ck + (size_t)seg * 4 * nc, 4 * nc, S->stream));
       double* Uw[2] = {Uw0, Uw1};
       int wc = 0, rq = 0;

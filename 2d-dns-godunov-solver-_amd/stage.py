@@ -348,12 +348,14 @@ class GodunovSolver:
         self.flow_time = rp.flow_time
         return st
 
-    def adjoint(self, n_steps, dt, checkpoint_every=20, wall_code=0, force_weight=(1.0, 0.0), dJ_dUN=None):
+    def adjoint(self, n_steps, dt, checkpoint_every=20, wall_code=0, force_weight=(1.0, 0.0), dJ_dUN=None,
+                record_budget=0):
         """Discrete adjoint of ``n_steps`` fixed-dt ns steps from the current
         state (which ends at U_N, as after ``run``) for
         J = sum_n w . F_n + <dJ_dUN, U_N>, F_n the wall force of step n on
-        ``wall_code``.  Returns (dJ/dU0 (n_cells, 4), dJ/dbc (MAX_BC + 1, 4),
-        gd_adjoint_stats)."""
+        ``wall_code``.  ``record_budget`` (bytes) keeps the cell records of as
+        many trailing checkpoint segments as fit instead of recomputing them.
+        Returns (dJ/dU0 (n_cells, 4), dJ/dbc (MAX_BC + 1, 4), gd_adjoint_stats)."""
         ap = L.gd_adjoint_params()
         ap.n_steps = int(n_steps)
         ap.checkpoint_every = int(checkpoint_every)
@@ -361,6 +363,7 @@ class GodunovSolver:
         ap.flow_time = float(self.flow_time)
         ap.wall_code = int(wall_code)
         ap.force_weight[0], ap.force_weight[1] = float(force_weight[0]), float(force_weight[1])
+        ap.record_budget = int(record_budget)
         g0, gbc, st = self.state.adjoint(ap, dJ_dUN)
         self.flow_time = st.flow_time
         return g0, gbc, st

@@ -197,7 +197,7 @@ int gd_run(gd_state* st, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats
    faces of code wall_code, at a fixed dt.  Runs N steps forward from the
    state's U (which ends at U_N, as after gd_run), keeping U every
    checkpoint_every steps and recomputing the steps between checkpoints on the
-   way back.  dJ_dU0 / dJ_dUN: (n_cells, 4) column-major, caller numbering.
+   way back (unless record_budget holds their records).  dJ_dU0 / dJ_dUN: (n_cells, 4) column-major, caller numbering.
    dJ_dbc: [(GD_MAX_BC + 1)][4], derivative with respect to the fixed states
    bc[-code].data[0..3] (rho, u, v, p) of GD_BC_FIXED codes, zero elsewhere. */
 typedef struct {
@@ -208,6 +208,9 @@ typedef struct {
   int32_t wall_code;
   int32_t reserved;
   double force_weight[2];
+  int64_t record_budget;     /* bytes of HBM for cell records: the records of as many trailing
+                                segments as fit are kept from the forward sweep instead of
+                                being recomputed (0: one segment) */
 } gd_adjoint_params;
 
 typedef struct {

@@ -61,13 +61,14 @@ def _tl_objective(mesh, U, n, dU, dbc, w, viscous=True):
     return J + float((w * u).sum()), dJ + float((w * du).sum())
 
 
-def _adjoint(mesh, U, n, c, w, mixed=False, viscous=True, order=None, perm=None):
+def _adjoint(mesh, U, n, c, w, mixed=False, viscous=True, order=None, perm=None, budget=0):
     S = pkg().stage
     L = pkg()._lib
     s = S.GodunovSolver(mesh, _dev_params(mixed, viscous), order=L.GD_ORDER_AS_GIVEN if order is None else order,
                         perm=perm)
     s.state.set_array(L.GD_ARR_U_OLD, U)
-    g0, gbc, st = s.adjoint(n, DT, checkpoint_every=c, wall_code=-3, force_weight=(1.0, 0.0), dJ_dUN=w)
+    g0, gbc, st = s.adjoint(n, DT, checkpoint_every=c, wall_code=-3, force_weight=(1.0, 0.0), dJ_dUN=w,
+                            record_budget=budget)
     UN = s.conservative()
     s.close()
     return g0, gbc, st, UN
@@ -127,6 +128,11 @@ def test_adjoint_independent_of_checkpointing_and_numbering():
     b = _adjoint(mesh, U, 5, 1, None)
     c = _adjoint(mesh, U, 5, 2, None, order=L.GD_ORDER_PERMUTATION, perm=M.random_permutation(mesh.nCells, SEED))
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # records of every segment kept (no recomputation), or of two of the five
+    d = _adjoint(mesh, U, 5, 1, None, budget=1 << 40)
+    e = _adjoint(mesh, U, 5, 1, None, budget=2 * 16 * 8 * mesh.nCells)
+    assert d[2].recomputed_steps == 0 and e[2].recomputed_steps == 3
+    assert np.array_equal(a[0], d[0]) and np.array_equal(a[0], e[0]) and np.array_equal(a[1], e[1])
     # another device numbering sums the same terms in another order
     assert np.allclose(a[0], c[0], rtol=1e-12, atol=1e-13 * np.abs(a[0]).max())
     assert np.array_equal(a[3], b[3])

@@ -42,5 +42,5 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(L.gd_params) == 4 * 4 + 6 * 8 + 2 * 4 + 8 * 8 + 17 * (8 + 32)
     assert ctypes.sizeof(L.gd_run_params) == 4 + 4 + 4 * 8 + 4 + 4
     assert ctypes.sizeof(L.gd_mesh_desc) == 8 + 8 + 4 + 4 + 17 * 8
-    assert ctypes.sizeof(L.gd_adjoint_params) == 4 + 4 + 8 + 8 + 4 + 4 + 2 * 8
+    assert ctypes.sizeof(L.gd_adjoint_params) == 4 + 4 + 8 + 8 + 4 + 4 + 2 * 8 + 8
     assert ctypes.sizeof(L.gd_adjoint_stats) == 8 + 2 * 8 + 8 + 4 + 4 + 8 + 8 + 8
