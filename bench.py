@@ -119,6 +119,9 @@ def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl):
     return {"steps": steps, "checkpoint_every": ckpt, "precision": "fp32-mixed", "dt": dt,
             "forward_ms": round(fwd * 1e3, 3), "adjoint_ms": round(adj * 1e3, 3),
             "cost_vs_forward": round(adj / fwd, 3),
+            # per step: the reverse sweep (recomputation included, HIP events)
+            # over the forward step
+            "reverse_step_vs_forward_step": round(ast.ms_reverse / (fwd * 1e3), 3),
             "adjoint_forward_sweep_ms": round(ast.ms_forward, 3), "adjoint_reverse_ms": round(ast.ms_reverse, 3),
             "recomputed_steps": ast.recomputed_steps, "storage_GB": round(ast.storage_bytes / 1e9, 3),
             "J_sum_drag": ast.J, "J_rel_diff_vs_forward": abs(ast.J - J) / abs(J),
@@ -126,6 +129,7 @@ def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl):
             "all_records": {"record_budget_GiB": 64, "recomputed_steps": ast1.recomputed_steps,
                             "adjoint_ms": round(adj1 * 1e3, 3),
                             "cost_vs_forward": round(adj1 / fwd, 3), "storage_GB": round(ast1.storage_bytes / 1e9, 3),
+                            "reverse_step_vs_forward_step": round(ast1.ms_reverse / (fwd * 1e3), 3),
                             "max_rel_diff_vs_checkpointed": float(np.abs(g1 - g0).max() / np.abs(g0).max())}}
 
 
@@ -183,7 +187,7 @@ def copy_probe(nbytes):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n-theta", type=int, default=720)
     ap.add_argument("--n-r", type=int, default=1445)
@@ -222,8 +226,10 @@ def main():
         order = L.GD_ORDER_AS_GIVEN if var.endswith("-given") else dev_order
         res[var] = run_variant(mesh, case, prec, flux, order, U0, a.warmup, a.steps, a.cfl)
         r = res[var]
+        bstep = sum(k["bytes_per_launch"] for k in r["kernels"])
         log(f"{var}: {r['ms_per_step']:.4f} ms/step, {mesh.nCells / (r['ms_per_step'] * 1e-3):.4e} cell-updates/s, "
-            f"drag {r['drag']:.6e}, nonphysical {r['nonphysical']}")
+            f"step {bstep / (r['ms_per_step'] * 1e-3) / 1e9:.0f} GB/s, drag {r['drag']:.6e}, "
+            f"nonphysical {r['nonphysical']}")
         for k in r["kernels"]:
             log(f"   {k['name']:16s} {k['ms_avg'] * 1e3:9.2f} us/launch  "
                 f"{k['bytes_per_launch'] / (k['ms_avg'] * 1e-3) / 1e9 if k['ms_avg'] > 0 else 0:8.1f} GB/s")
@@ -232,6 +238,11 @@ def main():
     value = mesh.nCells / (r64["ms_per_step"] * 1e-3)
     dom, roof = roofline(r64["kernels"], a.steps)
     roof["traffic"] = measured_traffic(dom["name"])
+
+    def step_frac(r):  # algorithmic bytes of one whole step over its time
+        b = sum(k["bytes_per_launch"] for k in r["kernels"])
+        return {"bytes_per_step": b, "achieved_GBs": round(b / (r["ms_per_step"] * 1e-3) / 1e9, 1),
+                "frac": round(b / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(r64["ms_per_step"], 5), "higher_is_better": True,
@@ -246,6 +257,7 @@ def main():
         "roofline": {**roof, "kernel": dom["name"], "bytes_per_launch": dom["bytes_per_launch"],
                      "avg_launch_us": round(dom["ms_avg"] * 1e3, 2)},
         "kernels_us": {k["name"]: round(k["ms_avg"] * 1e3, 2) for k in r64["kernels"]},
+        "step_hbm": step_frac(r64),
         "kernel_bytes": {k["name"]: k["bytes_per_launch"] for r in res.values() for k in r["kernels"]},
     }
     for var, r in res.items():
@@ -264,6 +276,7 @@ def main():
             d["steps_compared"] = a.warmup + a.steps
         _, rf = roofline(r["kernels"], a.steps)
         d["roofline_frac"] = rf["frac"]
+        d["step_hbm"] = step_frac(r)
         out[var.replace("-", "_")] = d
     out["drag_fp64"] = r64["drag"]
     if a.adjoint_steps > 0:
@@ -271,7 +284,8 @@ def main():
         ad = out["adjoint"]
         log(f"adjoint: {ad['steps']} mixed steps, checkpoints every {ad['checkpoint_every']}: "
             f"{ad['adjoint_ms']:.1f} ms vs forward {ad['forward_ms']:.1f} ms -> {ad['cost_vs_forward']:.2f}x "
-            f"(forward sweep {ad['adjoint_forward_sweep_ms']:.1f} ms, reverse {ad['adjoint_reverse_ms']:.1f} ms); "
+            f"(forward sweep {ad['adjoint_forward_sweep_ms']:.1f} ms, reverse {ad['adjoint_reverse_ms']:.1f} ms = "
+            f"{ad['reverse_step_vs_forward_step']:.2f}x per step); "
             f"all records kept: {ad['all_records']['adjoint_ms']:.1f} ms -> {ad['all_records']['cost_vs_forward']:.2f}x")
     if not a.no_cpu:
         threads = a.cpu_threads or min(os.cpu_count() or 1, 16)

@@ -167,3 +167,36 @@ def test_ns_small_cylinder_fp64_vs_oracle_50_steps():
     print("cylinder 50 steps fp64", err, "drag", st.wall_force[0], f[0])
     assert err <= 1e-11
     assert abs(st.wall_force[0] - f[0]) <= 1e-9 * max(abs(f[0]), case.p_inf)
+
+
+def test_ns_small_cylinder_fp64_1000_steps():
+    """1000 FP64 steps of the cylinder set-up (coarse O-grid, CFL-0.4 viscous
+    dt chosen on the device each step) against the oracle stepping with its
+    own dt: relative L-inf <= 1e-9 after 1000 steps, and the time reached."""
+    M = pkg().mesh
+    S = pkg().stage
+    L = pkg()._lib
+    case = S.CylinderCase()
+    mesh = M.make_cylinder_ogrid_mesh(48, 25, 0.5, 10.0, 5e-3)
+    mv = O.MeshView(mesh)
+    n = mesh.nCells
+    U = case.free_stream_conservative(n)
+    bcs = {-1: (O.BC_FIXED, (case.rho_inf, case.u_inf, 0.0, case.p_inf)), -2: (O.BC_EXTRAPOLATE, ()),
+           -3: (O.BC_ISOTHERMAL_WALL, (case.T_wall,))}
+    prm = O.ns_params(gamma=1.4, R=case.R, Pr=case.Pr, mu_ref=case.mu_inf, T_ref=case.T_inf, S=case.S,
+                      venkat_K=5.0, venkat_scale=(case.rho_inf, case.u_inf, case.u_inf, case.p_inf), bcs=bcs)
+    s = S.GodunovSolver(mesh, case.params(), order=L.GD_ORDER_RCM)
+    s.state.set_array(L.GD_ARR_U_OLD, U)
+    st = s.run(1000, cfl=0.4, dt_policy=L.GD_DT_LOCAL, wall_code=-3)
+    UN = s.conservative()
+    s.close()
+    O.set_threads(8)
+    Uo, t = U.copy(order="F"), 0.0
+    for _ in range(1000):
+        dt = O.ns_dt(mv, prm, Uo, 0.4)
+        Uo, _, _, _ = O.ns_step(mv, prm, dt, Uo)
+        t += dt
+    err = rel_linf(UN, Uo)
+    print("cylinder 1000 steps fp64", err, "time", st.flow_time, t)
+    assert err <= 1e-9
+    assert abs(st.flow_time - t) <= 1e-12 * t
