@@ -191,8 +191,8 @@ def test_adjoint_cylinder_against_central_differences():
     stream, checkpoints every 5: dJ/dM_inf (through the inflow and the
     initial state, at fixed viscosity) against central differences of the
     forward loop; random directions against the oracle's dual-number run; and
-    random directions of the developed state after those 20 steps, for the
-    next 20, against central differences."""
+    16 random directions of the developed state after those 20 steps, for the
+    next 20, against central differences (1e-4 relative)."""
     S = pkg().stage
     L = pkg()._lib
     M = pkg().mesh
@@ -236,8 +236,8 @@ def test_adjoint_cylinder_against_central_differences():
     # supported on the cells the flow has reached (the uniform far field has
     # equal face states, where AUSM+'s min(a_L, a_R) has a kink).  The limiter
     # is strongly curved where neighbour differences are small, so the step is
-    # 1e-9 (at 1e-7 the oracle's own central differences are off by 5 %, at
-    # 1e-9 within 2e-4 of its dual-number derivative).
+    # 1e-10: the oracle's own central differences are 5 % off at 1e-7, 2e-4 at
+    # 1e-9, 1e-5 at 1e-10 (and round-off takes over at 1e-11).  16 directions.
     s = S.GodunovSolver(mesh, base.params())
     s.state.set_array(L.GD_ARR_U_OLD, U0)
     s.run(N, dt=dt)
@@ -250,14 +250,15 @@ def test_adjoint_cylinder_against_central_differences():
     print("cells reached by the flow", int(moved.sum()), "of", n)
     scale = np.abs(U1).max(axis=0)
     errs = []
-    for k in range(4):
+    for k in range(16):
         d = M.normal(SEED + 200 + k, 4 * n).reshape(n, 4, order="F") * scale
         d = np.asfortranarray(d * moved[:, None])
-        eps = 1e-9
+        eps = 1e-10
         Jp = _cyl_forward(mesh, base, base, np.asfortranarray(U1 + eps * d), N, dt)
         Jm = _cyl_forward(mesh, base, base, np.asfortranarray(U1 - eps * d), N, dt)
         fd = (Jp - Jm) / (2 * eps)
         adj = float((g1 * d).sum())
         print("direction", k, "adjoint", adj, "central difference", fd)
         errs.append(abs(adj - fd) / max(abs(fd), 1e-8 * abs(J1)))
-    assert max(errs) <= 1e-3
+    print("max relative difference", max(errs))
+    assert max(errs) <= 1e-4
