@@ -380,7 +380,7 @@ class GodunovSolver:
 
 
 # ---------------------------------------------------------------------------
-# supersonic cylinder (north-star configurations): free stream and gas
+# supersonic cylinder configuration: free stream and gas
 # ---------------------------------------------------------------------------
 
 @dataclasses.dataclass

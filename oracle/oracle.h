@@ -96,7 +96,7 @@ int orc_update_variables(const int32_t* cellsThreads, int32_t nThreads, int64_t 
 double orc_ref_tau(const orc_mesh* m, const orc_fields* f, double cfl);
 double orc_local_dt(const orc_mesh* m, const orc_fields* f, double cfl);
 
-/* ---- north-star (ns) mode ---- */
+/* ---- Navier-Stokes (ns) scheme: cell gradients, Venkatakrishnan, Sutherland ---- */
 typedef struct {
   double gamma, R, Pr;
   double mu_ref, T_ref, S;  /* Sutherland: mu = mu_ref (T/T_ref)^1.5 (T_ref+S)/(T+S) */
