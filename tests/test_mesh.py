@@ -1,18 +1,15 @@
 """Mesh format conventions (mesh2d_Int32) of the generators, checked on the
 generated meshes and, where the container has it, on the mesh the reference
-ships (oblickShock2dl00F1n.h5, read with h5py from /opt/conda's python)."""
-import json
+ships (oblickShock2dl00F1n.h5, read with the package's HDF5 reader)."""
 import os
-import subprocess
 
 import numpy as np
 import pytest
 
 import oracle as O
-from conftest import ROOT, pkg
+from conftest import pkg
 
 REF_MESH = "/root/reference/oblickShock2dl00F1n.h5"
-CONDA_PY = "/opt/conda/bin/python3.9"
 
 
 def M():
@@ -132,35 +129,9 @@ def test_step_is_numbering_independent_in_the_oracle():
 
 
 def _load_reference_mesh(tmp_path):
-    if not (os.path.exists(REF_MESH) and os.path.exists(CONDA_PY)):
-        pytest.skip("reference mesh or h5py interpreter not available")
-    out = tmp_path / "ref_mesh.npz"
-    code = (
-        "import h5py, numpy as np, sys\n"
-        "f = h5py.File(sys.argv[1], 'r')\n"
-        "d = {k: np.array(f[k]) for k in f.keys()}\n"
-        "np.savez(sys.argv[2], **d)\n")
-    r = subprocess.run([CONDA_PY, "-c", code, REF_MESH, str(out)], capture_output=True, text=True)
-    if r.returncode != 0:
-        pytest.skip("h5py read failed: " + r.stderr[-200:])
-    d = np.load(out)
-    # HDF5 stores the Julia column-major arrays with reversed dims
-    T = lambda a: np.asfortranarray(np.asarray(a).T)  # noqa: E731
-    sc = lambda a: int(np.asarray(a).reshape(-1)[0])  # noqa: E731
-    mm = M()
-    return mm.mesh2d_Int32(
-        nCells=sc(d["nCells"]), nNodes=sc(d["nNodes"]), nNeibCells=sc(d["nNeibCells"]), nBSets=sc(d["nBSets"]),
-        xNodes=np.asarray(d["xNodes"], dtype=np.float64).reshape(-1),
-        yNodes=np.asarray(d["yNodes"], dtype=np.float64).reshape(-1),
-        mesh_connectivity=T(d["mesh_connectivity"]).astype(np.int32), bc_data=T(d["bc_data"]),
-        bc_indexes=np.asarray(d["bc_indexes"]).reshape(-1), cell_nodes_X=T(d["cell_nodes_X"]),
-        cell_nodes_Y=T(d["cell_nodes_Y"]), cell_mid_points=T(d["cell_mid_points"]),
-        cell_areas=np.asarray(d["cell_areas"], dtype=np.float64).reshape(-1),
-        HX=np.asarray(d["HX"], dtype=np.float64).reshape(-1), cell_edges_Nx=T(d["cell_edges_Nx"]),
-        cell_edges_Ny=T(d["cell_edges_Ny"]), cell_edges_length=T(d["cell_edges_length"]),
-        cell_stiffness=T(d["cell_stiffness"]).astype(np.int32), cell_clusters=T(d["cell_clusters"]).astype(np.int32),
-        node_stencils=T(d["node_stencils"]), node2cellsL2up=T(d["node2cellL2up"]).astype(np.int32),
-        node2cellsL2down=T(d["node2cellL2down"]).astype(np.int32), cells2nodes=T(d["cells2nodes"]).astype(np.int32))
+    if not os.path.exists(REF_MESH):
+        pytest.skip("reference mesh not available")
+    return pkg().read_mesh_hdf5(REF_MESH)
 
 
 def test_reference_shipped_mesh_conventions_and_oracle_step(tmp_path):
