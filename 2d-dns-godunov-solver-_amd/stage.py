@@ -374,6 +374,15 @@ class GodunovSolver:
     def fields(self):
         return self.state.get_fields()
 
+    def save_vtk(self, filename, binary=True):
+        """The node primitives (the node interpolation of the current cell
+        state) as a .vtu file, like saveResults4VTK (utilsIO.jl:43-53)."""
+        from .vtkio import save_results_vtk
+        self.state.cells2nodes()
+        f = self.state.get_fields()
+        return save_results_vtk(filename, self.mesh, {"density": f[6], "Ux": f[7], "Uy": f[8], "pressure": f[9]},
+                                binary=binary)
+
     def close(self):
         self.state.close()
         self.dmesh.close()
