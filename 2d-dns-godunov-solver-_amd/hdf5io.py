@@ -381,3 +381,28 @@ def write_mesh_hdf5(path: str, m: mesh2d_Int32) -> None:
     for k in _ARRAYS_F64:
         d[k] = np.asarray(getattr(m, _FILE_NAMES.get(k, k)), dtype=np.float64).T
     write_hdf5(path, d)
+
+
+def save_solution_hdf5(path: str, dt: float, flowTime: float, densityCells, UxCells, UyCells,  # noqa: N803
+                       pressureCells) -> None:
+    """A restart snapshot with the fields of ``solutionCellsT``
+    (primeObjects.jl:197-205): dt, flowTime, nCells and the cell primitives.
+    (The reference stores that struct with BSON.jl; HDF5 carries the same
+    fields in a form any HDF5 reader opens.)"""
+    rho = np.asarray(densityCells, dtype=np.float64).reshape(-1)
+    write_hdf5(path, {"dt": np.float64(dt), "flowTime": np.float64(flowTime), "nCells": np.int64(rho.size),
+                      "densityCells": rho, "UxCells": np.asarray(UxCells, dtype=np.float64).reshape(-1),
+                      "UyCells": np.asarray(UyCells, dtype=np.float64).reshape(-1),
+                      "pressureCells": np.asarray(pressureCells, dtype=np.float64).reshape(-1)})
+
+
+def load_solution_hdf5(path: str) -> dict:
+    """The snapshot of ``save_solution_hdf5`` (what ``loadPrevResults``,
+    loadPrevResults.jl:3-38, takes from it: the cell primitives and flowTime)."""
+    d = read_hdf5(path)
+    out = {"dt": float(d["dt"]), "flowTime": float(d["flowTime"]), "nCells": int(d["nCells"])}
+    for k in ("densityCells", "UxCells", "UyCells", "pressureCells"):
+        out[k] = d[k]
+        if out[k].size != out["nCells"]:
+            raise ValueError(f"{path}: {k} has {out[k].size} values for nCells = {out['nCells']}")
+    return out

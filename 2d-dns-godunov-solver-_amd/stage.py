@@ -374,6 +374,24 @@ class GodunovSolver:
     def fields(self):
         return self.state.get_fields()
 
+    def save_snapshot(self, path, dt=0.0):
+        """Cell primitives and flow time to an HDF5 restart file
+        (``solutionCellsT`` fields)."""
+        from .hdf5io import save_solution_hdf5
+        f = self.state.get_fields()
+        save_solution_hdf5(path, dt, self.flow_time, f[0], f[1], f[2], f[3])
+
+    def load_snapshot(self, path):
+        """Restart from ``save_snapshot`` (as loadPrevResults.jl:3-38): cell
+        primitives, derived node values and the flow time."""
+        from .hdf5io import load_solution_hdf5
+        d = load_solution_hdf5(path)
+        if d["nCells"] != self.mesh.nCells:
+            raise ValueError(f"snapshot has {d['nCells']} cells, the mesh {self.mesh.nCells}")
+        self.set_primitives(d["densityCells"], d["UxCells"], d["UyCells"], d["pressureCells"])
+        self.flow_time = d["flowTime"]
+        return d
+
     def save_vtk(self, filename, binary=True):
         """The node primitives (the node interpolation of the current cell
         state) as a .vtu file, like saveResults4VTK (utilsIO.jl:43-53)."""

@@ -109,3 +109,43 @@ def test_written_file_reads_in_h5py(tmp_path):
     ns = d["node_stencils"]
     assert ns[1] == "float64" and np.array_equal(np.array(ns[2]).reshape(8, m.nNodes).T, m.node_stencils)
     assert "node2cellL2up" in d and "node2cellL2down" in d
+
+
+def test_solution_snapshot_round_trip(tmp_path):
+    P = pkg()
+    rng = np.random.default_rng(5)
+    r, u, v, p = (rng.random(50) + 0.5 for _ in range(4))
+    fn = str(tmp_path / "snap.h5")
+    P.hdf5io.save_solution_hdf5(fn, 1e-3, 0.25, r, u, v, p)
+    d = P.hdf5io.load_solution_hdf5(fn)
+    assert d["dt"] == 1e-3 and d["flowTime"] == 0.25 and d["nCells"] == 50
+    for k, a in zip(("densityCells", "UxCells", "UyCells", "pressureCells"), (r, u, v, p)):
+        assert np.array_equal(d[k], a)
+
+
+@pytest.mark.gpu
+def test_solver_restart_continues(tmp_path):
+    """Ten steps, snapshot, ten more, against a solver restarted from the
+    snapshot for the last ten (ns FP64, fixed dt).  The snapshot holds
+    primitives, as the reference's does, so the restarted conservative state
+    agrees to rounding, not bit for bit."""
+    from helpers import random_fields
+    P = pkg()
+    L = P._lib
+    m = P.mesh.make_box_mesh(12, 8, quads="mixed")
+    f = random_fields(m, rho=(0.8, 1.4), vel=(-0.6, 0.9), p=(0.8, 1.6))
+    prm = P.make_params(scheme=L.GD_SCHEME_NS, viscous=True, RGAS=1.0, mu_ref=2e-3, T_ref=1.0, S_suth=0.4)
+    a = P.GodunovSolver(m, prm)
+    a.set_primitives(f.densityCells, f.UxCells, f.UyCells, f.pressureCells)
+    a.run(10, dt=1e-3)
+    fn = str(tmp_path / "snap.h5")
+    a.save_snapshot(fn, dt=1e-3)
+    a.run(10, dt=1e-3)
+    b = P.GodunovSolver(m, prm)
+    b.load_snapshot(fn)
+    assert abs(b.flow_time - (a.flow_time - 1e-2)) < 1e-15
+    b.run(10, dt=1e-3)
+    x, y = a.conservative(), b.conservative()
+    assert np.allclose(x, y, rtol=1e-12, atol=1e-12 * np.abs(x).max())
+    a.close()
+    b.close()
