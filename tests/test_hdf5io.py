@@ -134,7 +134,8 @@ def test_solver_restart_continues(tmp_path):
     L = P._lib
     m = P.mesh.make_box_mesh(12, 8, quads="mixed")
     f = random_fields(m, rho=(0.8, 1.4), vel=(-0.6, 0.9), p=(0.8, 1.6))
-    prm = P.make_params(scheme=L.GD_SCHEME_NS, viscous=True, RGAS=1.0, mu_ref=2e-3, T_ref=1.0, S_suth=0.4)
+    bcs = {code: (L.GD_BC_EXTRAPOLATE, ()) for code in (-1, -2, -3, -4)}
+    prm = P.make_params(scheme=L.GD_SCHEME_NS, viscous=True, RGAS=1.0, mu_ref=2e-3, T_ref=1.0, S_suth=0.4, bcs=bcs)
     a = P.GodunovSolver(m, prm)
     a.set_primitives(f.densityCells, f.UxCells, f.UyCells, f.pressureCells)
     a.run(10, dt=1e-3)
