@@ -165,10 +165,8 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
       !d->node2cellsL2down || !d->cells2nodes || !d->cell_mid_points || !d->cell_nodes_X || !d->cell_nodes_Y ||
       !d->xNodes || !d->yNodes)
     return fail(GD_ERR_INVALID, "missing mesh array");
-  int dev_count = 0;
-  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) return fail(GD_ERR_NO_DEVICE, "no HIP device");
-
-  // --- validate and collect topology in the caller's numbering (0-based)
+  // --- validate and collect topology in the caller's numbering (0-based); all
+  // of the host-side construction runs (and is checked) before any device call
   std::vector<int32_t> nv(nc);
   int32_t has_quads = 0;
   for (int64_t i = 0; i < nc; ++i) {
@@ -243,15 +241,6 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
   M->cells_identity = M->nodes_identity = true;
   for (int64_t c = 0; c < nc; ++c) M->cells_identity = M->cells_identity && cp[c] == c;
   for (int64_t j = 0; j < nn; ++j) M->nodes_identity = M->nodes_identity && np[j] == j;
-  {
-    auto to32 = [](const std::vector<int64_t>& a) { return std::vector<int32_t>(a.begin(), a.end()); };
-    int rcp = 0;
-    rcp |= upload(M->allocs, &M->d_cperm, to32(M->cperm), &M->device_bytes);
-    rcp |= upload(M->allocs, &M->d_ciperm, to32(M->ciperm), &M->device_bytes);
-    rcp |= upload(M->allocs, &M->d_nperm, to32(M->nperm), &M->device_bytes);
-    rcp |= upload(M->allocs, &M->d_niperm, to32(M->niperm), &M->device_bytes);
-    if (rcp) { std::string msg = g_err; gd_mesh_destroy(M); return fail(rcp, msg); }
-  }
   const auto& ci = M->ciperm;
   const auto& ni = M->niperm;
   auto new_node = [&](int64_t id1) -> int32_t { return (id1 >= 1 && id1 <= nn) ? (int32_t)ni[id1 - 1] : 0; };
@@ -431,6 +420,20 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
   };
   const std::vector<double> h_fr0p = pair(h_fr0), h_fr1p = pair(h_fr1);
 
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) {
+    delete M;
+    return fail(GD_ERR_NO_DEVICE, "no HIP device");
+  }
+  {
+    auto to32 = [](const std::vector<int64_t>& a) { return std::vector<int32_t>(a.begin(), a.end()); };
+    int rcp = 0;
+    rcp |= upload(M->allocs, &M->d_cperm, to32(M->cperm), &M->device_bytes);
+    rcp |= upload(M->allocs, &M->d_ciperm, to32(M->ciperm), &M->device_bytes);
+    rcp |= upload(M->allocs, &M->d_nperm, to32(M->nperm), &M->device_bytes);
+    rcp |= upload(M->allocs, &M->d_niperm, to32(M->niperm), &M->device_bytes);
+    if (rcp) { std::string msg = g_err; gd_mesh_destroy(M); return fail(rcp, msg); }
+  }
   MeshView& v = M->v;
   v.nc = nc; v.nn = nn; v.nf = nf; v.ncl = ncl; v.has_quads = has_quads; v.n_wall = n_wall; v.max_hx = max_hx;
   int64_t* B = &M->device_bytes;
