@@ -580,8 +580,8 @@ int gd_state_create(gd_mesh* m, const gd_params* p, gd_state** out) {
   rc |= dev_alloc(al, &S->wall_f, 2 * (size_t)std::max<int32_t>(m->v.n_wall, 1));
   rc |= dev_alloc(al, &S->drag, 4);
   rc |= dev_alloc(al, &S->acc, 3);
-  rc |= dev_alloc(al, &S->part, 3 * ((nc + 255) / 256) + 3);
-  rc |= dev_alloc(al, &S->red, 8 + 4 * ((nc + 255) / 256) + 8);
+  rc |= dev_alloc(al, &S->part, 3 * ((nc + kThreads - 1) / kThreads) + 3);
+  rc |= dev_alloc(al, &S->red, 8 + 4 * ((nc + kThreads - 1) / kThreads) + 8);
   rc |= dev_alloc(al, &S->xfer, 4 * (size_t)std::max(nc, nn));
   if (rc) {
     std::string msg = g_err;
@@ -900,7 +900,7 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
     GD_HIP(launch_fields_derived(v.nc, P.Gamma, S->Q[S->qc], S->aS, S->vmax, S->stream));
   GD_HIP(launch_prime_acc(v, P, S->Q[S->qc], S->vmax, S->acc, S->part, rp->flow_time, rp->dt_policy,
                           ns && P.viscous, S->stream));
-  const int64_t nparts = (v.nc + 255) / 256;
+  const int64_t nparts = (v.nc + kThreads - 1) / kThreads;
   if (wall) {
     GD_HIP(launch_fill(S->wall_f, 2 * (int64_t)v.n_wall, 0.0, S->stream));
     GD_HIP(launch_fill(S->drag, 4, 0.0, S->stream));
@@ -1051,7 +1051,7 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
   const int32_t N = ap->n_steps;
   const int32_t c = std::min(ap->checkpoint_every, N);
   const int32_t nseg = (N + c - 1) / c;
-  const int64_t nparts = (nc + 255) / 256;
+  const int64_t nparts = (nc + kThreads - 1) / kThreads;
   const bool wall = ap->wall_code != 0 && v.n_wall > 0;
   const int64_t nw = std::max<int32_t>(v.n_wall, 1);
 

@@ -20,6 +20,9 @@ namespace gd {
 // neighbours of a cell range (which, in RCM order, sit in nearby blocks) in the
 // same 4 MB L2 instead of spreading the halo over eight L2s.
 constexpr unsigned kNumXcd = 8;
+// threads per block of every cell / face / node kernel (the per-block partial
+// reductions are sized by it)
+constexpr int kThreads = 256;
 __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
   const unsigned x = b % kNumXcd, k = b / kNumXcd;
   const unsigned q = nb / kNumXcd, r = nb % kNumXcd;

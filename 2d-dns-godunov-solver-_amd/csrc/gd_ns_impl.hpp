@@ -24,7 +24,7 @@
 namespace gd {
 namespace ns {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = kThreads;
 
 template <class T>
 struct NsKArgs {

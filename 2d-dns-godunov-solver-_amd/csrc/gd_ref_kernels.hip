@@ -17,7 +17,7 @@
 namespace gd {
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = kThreads;
 
 __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
