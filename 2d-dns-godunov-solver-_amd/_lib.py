@@ -36,7 +36,7 @@ EXPORTS = (
     "gd_state_destroy", "gd_state_set_params", "gd_state_set_fields", "gd_state_get_fields", "gd_state_set_array",
     "gd_state_get_array", "gd_calc_one_stage", "gd_calc_diff_term", "gd_update_variables", "gd_cells2nodes",
     "gd_run", "gd_sync", "gd_state_device_u", "gd_device_count", "gd_last_error", "gd_version",
-    "gd_adjoint_run",
+    "gd_adjoint_run", "gd_state_device_adjoint",
 )
 
 
@@ -131,6 +131,7 @@ def load():
     L.gd_run.argtypes = [vp, C.c_int32, C.POINTER(gd_run_params), C.POINTER(gd_run_stats)]
     L.gd_state_device_u.argtypes = [vp, C.POINTER(dp)]
     L.gd_adjoint_run.argtypes = [vp, C.POINTER(gd_adjoint_params), dp, dp, dp, C.POINTER(gd_adjoint_stats)]
+    L.gd_state_device_adjoint.argtypes = [vp, C.POINTER(dp)]
     L.gd_device_count.argtypes = [C.POINTER(C.c_int32)]
     L.gd_last_error.argtypes = [C.c_char_p, C.c_size_t]
     L.gd_version.restype = C.c_char_p
@@ -283,10 +284,11 @@ class DeviceState:
         check(rc)
         return st
 
-    def adjoint(self, ap: gd_adjoint_params, dJ_dUN=None):
-        """gd_adjoint_run: returns (dJ/dU0 (n_cells, 4), dJ/dbc (GD_MAX_BC + 1, 4), stats)."""
+    def adjoint(self, ap: gd_adjoint_params, dJ_dUN=None, to_host=True):
+        """gd_adjoint_run: returns (dJ/dU0 (n_cells, 4) or None when kept on
+        the device, dJ/dbc (GD_MAX_BC + 1, 4), stats)."""
         n = self.mesh.n_cells
-        out = np.zeros((n, 4), order="F")
+        out = np.zeros((n, 4), order="F") if to_host else None
         dbc = np.zeros((GD_MAX_BC + 1, 4))
         st = gd_adjoint_stats()
         src = None

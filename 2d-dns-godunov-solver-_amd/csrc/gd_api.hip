@@ -727,6 +727,13 @@ int gd_state_device_u(gd_state* S, double** dev) {
   return GD_OK;
 }
 
+int gd_state_device_adjoint(gd_state* S, double** dev) {
+  if (!S || !dev) return fail(GD_ERR_INVALID, "null argument");
+  if (!S->grad0) return fail(GD_ERR_INVALID, "no adjoint has run on this state");
+  *dev = S->grad0;
+  return GD_OK;
+}
+
 int gd_sync(gd_state* S) {
   if (!S) return fail(GD_ERR_INVALID, "null state");
   GD_HIP(hipStreamSynchronize(S->stream));
@@ -1038,7 +1045,7 @@ struct DevScratch {
 
 int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dUN, double* dJ_dU0, double* dJ_dbc,
                    gd_adjoint_stats* stats) {
-  if (!S || !ap || !dJ_dU0) return fail(GD_ERR_INVALID, "null argument");
+  if (!S || !ap) return fail(GD_ERR_INVALID, "null argument");
   const gd_params& P = S->prm;
   if (P.scheme != GD_SCHEME_NS) return fail(GD_ERR_UNSUPPORTED, "the adjoint covers the ns scheme");
   if (P.flux != GD_FLUX_AUSM_PLUS) return fail(GD_ERR_UNSUPPORTED, "the adjoint covers the AUSM+ flux");
@@ -1214,7 +1221,9 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
     }
   }
   GD_HIP(hipEventRecord(ev[2], S->stream));
-  if ((rc = get_cols(S, lU[li], dJ_dU0, 4, false))) return rc;
+  if (!S->grad0 && (rc = dev_alloc(S->allocs, &S->grad0, 4 * nc))) return rc;
+  GD_HIP(launch_copy(S->grad0, lU[li], 4 * nc, S->stream));
+  if (dJ_dU0 && (rc = get_cols(S, lU[li], dJ_dU0, 4, false))) return rc;
   if (dJ_dbc) {
     std::vector<double> bs((size_t)4 * nw);
     GD_HIP(hipMemcpyAsync(bs.data(), bcsum, bs.size() * sizeof(double), hipMemcpyDeviceToHost, S->stream));

@@ -146,6 +146,7 @@ struct State {
   double* part = nullptr;               // [3 x cell blocks] partial reductions
   double* red = nullptr;                // reduction scratch
   double* xfer = nullptr;               // [4][max(nc, nn)] staging of host transfers
+  double* grad0 = nullptr;              // [4][nc] dJ/dU0 of the last adjoint run (allocated by the first)
   std::vector<void*> allocs;
 };
 

@@ -349,13 +349,15 @@ class GodunovSolver:
         return st
 
     def adjoint(self, n_steps, dt, checkpoint_every=20, wall_code=0, force_weight=(1.0, 0.0), dJ_dUN=None,
-                record_budget=0):
+                record_budget=0, to_host=True):
         """Discrete adjoint of ``n_steps`` fixed-dt ns steps from the current
         state (which ends at U_N, as after ``run``) for
         J = sum_n w . F_n + <dJ_dUN, U_N>, F_n the wall force of step n on
         ``wall_code``.  ``record_budget`` (bytes) keeps the cell records of as
         many trailing checkpoint segments as fit instead of recomputing them.
-        Returns (dJ/dU0 (n_cells, 4), dJ/dbc (MAX_BC + 1, 4), gd_adjoint_stats)."""
+        ``to_host=False`` leaves dJ/dU0 on the device (gd_state_device_adjoint).
+        Returns (dJ/dU0 (n_cells, 4) or None, dJ/dbc (MAX_BC + 1, 4),
+        gd_adjoint_stats)."""
         ap = L.gd_adjoint_params()
         ap.n_steps = int(n_steps)
         ap.checkpoint_every = int(checkpoint_every)
@@ -364,7 +366,7 @@ class GodunovSolver:
         ap.wall_code = int(wall_code)
         ap.force_weight[0], ap.force_weight[1] = float(force_weight[0]), float(force_weight[1])
         ap.record_budget = int(record_budget)
-        g0, gbc, st = self.state.adjoint(ap, dJ_dUN)
+        g0, gbc, st = self.state.adjoint(ap, dJ_dUN, to_host)
         self.flow_time = st.flow_time
         return g0, gbc, st
 

@@ -197,7 +197,8 @@ int gd_run(gd_state* st, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats
    faces of code wall_code, at a fixed dt.  Runs N steps forward from the
    state's U (which ends at U_N, as after gd_run), keeping U every
    checkpoint_every steps and recomputing the steps between checkpoints on the
-   way back (unless record_budget holds their records).  dJ_dU0 / dJ_dUN: (n_cells, 4) column-major, caller numbering.
+   way back (unless record_budget holds their records).  dJ_dU0 (NULL: kept
+   on the device only, see gd_state_device_adjoint) / dJ_dUN: (n_cells, 4) column-major, caller numbering.
    dJ_dbc: [(GD_MAX_BC + 1)][4], derivative with respect to the fixed states
    bc[-code].data[0..3] (rho, u, v, p) of GD_BC_FIXED codes, zero elsewhere. */
 typedef struct {
@@ -226,6 +227,10 @@ typedef struct {
 
 int gd_adjoint_run(gd_state* st, const gd_adjoint_params* ap, const double* dJ_dUN, double* dJ_dU0, double* dJ_dbc,
                    gd_adjoint_stats* stats);
+/* dJ/dU0 of the last gd_adjoint_run, on the device: 4 x n_cells SoA in device
+   order; valid until the next gd_adjoint_run on the state (dJ_dU0 may be NULL
+   for callers that keep the gradient on the device) */
+int gd_state_device_adjoint(gd_state* st, double** dev_grad_soa);
 int gd_sync(gd_state* st);
 
 /* device pointers for zero-copy callers; valid until the next gd_run/step */

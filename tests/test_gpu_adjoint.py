@@ -262,3 +262,24 @@ def test_adjoint_cylinder_against_central_differences():
         errs.append(abs(adj - fd) / max(abs(fd), 1e-8 * abs(J1)))
     print("max relative difference", max(errs))
     assert max(errs) <= 1e-4
+
+
+def test_adjoint_kept_on_device():
+    """dJ/dU0 can stay in HBM: the run without a host array gives the same
+    objective, and gd_state_device_adjoint hands out its device pointer (an
+    error before any adjoint has run)."""
+    import ctypes as C
+    S = pkg().stage
+    L = pkg()._lib
+    mesh, U = _box()
+    s = S.GodunovSolver(mesh, _dev_params())
+    ptr = C.POINTER(C.c_double)()
+    assert L.load().gd_state_device_adjoint(s.state.handle, C.byref(ptr)) == L.GD_ERR_INVALID
+    s.state.set_array(L.GD_ARR_U_OLD, U)
+    g_dev, _, st_dev = s.adjoint(4, DT, checkpoint_every=2, wall_code=-3, to_host=False)
+    assert g_dev is None
+    assert L.load().gd_state_device_adjoint(s.state.handle, C.byref(ptr)) == 0 and bool(ptr)
+    s.state.set_array(L.GD_ARR_U_OLD, U)
+    g, _, st = s.adjoint(4, DT, checkpoint_every=2, wall_code=-3)
+    s.close()
+    assert st.J == st_dev.J and np.abs(g).max() > 0
