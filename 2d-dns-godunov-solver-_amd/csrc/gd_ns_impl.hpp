@@ -364,6 +364,74 @@ __device__ __forceinline__ void viscous_face(const gd_params& P, T cp, T uf, T v
   gv[3] = uf * gv[1] + vf * gv[2] + kap * (gf[2][0] * nx + gf[2][1] * ny);
 }
 
+// LDS ordering between lanes of one wave (a wave's LDS operations complete in
+// issue order; the fences keep the compiler from moving them across)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The two records of each of a wave's 64 faces, gathered cooperatively: a
+// record is kP = sizeof(T) pieces of 16 B, and kP adjacent lanes load one
+// record's pieces in the same instruction, so an instruction touches 64 / kP
+// cache lines instead of 64 (the per-line cost of scattered 16 B lane loads
+// is what bounds a one-record-per-lane gather).  The pieces go through a
+// padded per-wave LDS slab (one piece of padding per record keeps the
+// read-back free of bank conflicts), owners first, then neighbours.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+struct FaceRecs {
+  static constexpr int kP = (int)sizeof(T);  // 16 B pieces per 16-value record
+  static constexpr int kRPI = 64 / kP;       // records per load instruction
+  static constexpr int kStride = kP + 1;     // LDS pieces per record
+  static constexpr int kWaves = kBlock / 64;
+  struct Shared {
+    u32x4 rec[kWaves][64 * kStride];
+    int32_t cell[kWaves][128];
+  };
+  __device__ __forceinline__ static void unpack(u32x4 x, T* o) {
+    if constexpr (sizeof(T) == 8) {
+      typedef double f64x2 __attribute__((ext_vector_type(2)));
+      const f64x2 d = __builtin_bit_cast(f64x2, x);
+      o[0] = d.x; o[1] = d.y;
+    } else {
+      typedef float f32x4 __attribute__((ext_vector_type(4)));
+      const f32x4 d = __builtin_bit_cast(f32x4, x);
+      o[0] = d.x; o[1] = d.y; o[2] = d.z; o[3] = d.w;
+    }
+  }
+  // cell0 / cell1: this lane's face's owner and the cell whose record stands
+  // on the other side (the owner again at a boundary)
+  __device__ __forceinline__ static void gather(Shared& sh, const T* rec, int32_t cell0, int32_t cell1, T* r0,
+                                                T* r1) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    sh.cell[w][lane] = cell0;
+    sh.cell[w][64 + lane] = cell1;
+    wave_lds_sync();
+    const u32x4* R = reinterpret_cast<const u32x4*>(rec);
+    const int sub = lane / kP, piece = lane % kP;
+    u32x4 v0[kP], v1[kP];
+#pragma unroll
+    for (int k = 0; k < kP; ++k) v0[k] = R[(int64_t)sh.cell[w][k * kRPI + sub] * kP + piece];
+#pragma unroll
+    for (int k = 0; k < kP; ++k) v1[k] = R[(int64_t)sh.cell[w][64 + k * kRPI + sub] * kP + piece];
+    u32x4* S = sh.rec[w];
+#pragma unroll
+    for (int k = 0; k < kP; ++k) S[(k * kRPI + sub) * kStride + piece] = v0[k];
+    wave_lds_sync();
+#pragma unroll
+    for (int p = 0; p < kP; ++p) unpack(S[lane * kStride + p], r0 + p * (16 / kP));
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < kP; ++k) S[(k * kRPI + sub) * kStride + piece] = v1[k];
+    wave_lds_sync();
+#pragma unroll
+    for (int p = 0; p < kP; ++p) unpack(S[lane * kStride + p], r1 + p * (16 / kP));
+  }
+};
+
 // ---- pass F1: one flux per face
 template <class T, bool VISC>
 __global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
@@ -371,13 +439,16 @@ __global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
   const MeshView& m = a.m;
   const gd_params& P = a.prm;
   const int64_t nF = m.n_faces;
-  const int64_t f = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (f >= nF) return;
+  const int64_t f0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  // lanes past the end gather the last face's records and store nothing, so
+  // that every lane of a wave takes part in the cooperative gather
+  const bool live = f0 < nF;
+  const int64_t f = live ? f0 : nF - 1;
   const int32_t c0 = m.fc0[f], c1 = m.fc1[f];
-  // loads first: both records (a boundary face reads the owner's twice), geometry
+  __shared__ typename FaceRecs<T>::Shared sh;
   T r0[16], r1[16];
-  ld16(a.rec + 16 * (int64_t)c0, r0);
-  ld16(a.rec + 16 * (int64_t)(c1 >= 0 ? c1 : c0), r1);
+  FaceRecs<T>::gather(sh, a.rec, c0, c1 >= 0 ? c1 : c0, r0, r1);
+  if (!live) return;
   const T nx = G::fnx(m, f), ny = G::fny(m, f);
   const T len = G::flen(m, f);
   const T rx = G::fr0(m, 0, f), ry = G::fr0(m, 1, f);
