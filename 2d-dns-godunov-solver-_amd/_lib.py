@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libgodunov.so")
 
 GD_OK = 0
 GD_ERR_INVALID, GD_ERR_HIP, GD_ERR_NOMEM, GD_ERR_UNSUPPORTED, GD_ERR_NONPHYSICAL, GD_ERR_NO_DEVICE = -1, -2, -3, -4, -5, -6
-GD_ORDER_AS_GIVEN, GD_ORDER_RCM, GD_ORDER_PERMUTATION = 0, 1, 2
+GD_ORDER_AS_GIVEN, GD_ORDER_RCM, GD_ORDER_PERMUTATION, GD_ORDER_HILBERT = 0, 1, 2, 3
 GD_SCHEME_REF, GD_SCHEME_NS = 0, 1
 GD_FP64, GD_FP32_MIXED = 0, 1
 GD_FLUX_AUSM_PLUS, GD_FLUX_ROE, GD_FLUX_AUSM_PLUS_M2020 = 0, 1, 2

@@ -106,6 +106,43 @@ std::vector<int64_t> rcm_order(int64_t n, const std::vector<int64_t>& xadj, cons
   return order;
 }
 
+// Hilbert-curve order of the cell centroids: the bounding square is divided
+// into 2^21 x 2^21 squares and cells are sorted by the position of theirs
+// along the curve (the iterative quadrant / rotate-and-reflect construction of
+// the curve's index; ties by caller id).  Consecutive cells then form compact
+// patches at every scale, so a block of cells and the faces it owns touch few
+// cache lines outside the block.
+std::vector<int64_t> hilbert_order(int64_t n, const double* xy /* n x 2, column-major */) {
+  double lo[2] = {xy[0], xy[n]}, hi[2] = {xy[0], xy[n]};
+  for (int q = 0; q < 2; ++q)
+    for (int64_t i = 0; i < n; ++i) {
+      lo[q] = std::min(lo[q], xy[q * n + i]);
+      hi[q] = std::max(hi[q], xy[q * n + i]);
+    }
+  constexpr uint64_t side = uint64_t(1) << 21;
+  const double span = std::max(hi[0] - lo[0], hi[1] - lo[1]);
+  const double scale = span > 0 ? (double)(side - 1) / span : 0.0;
+  std::vector<uint64_t> key(n);
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t x = (uint64_t)std::min((double)(side - 1), std::max(0.0, (xy[i] - lo[0]) * scale));
+    uint64_t y = (uint64_t)std::min((double)(side - 1), std::max(0.0, (xy[n + i] - lo[1]) * scale));
+    uint64_t d = 0;
+    for (uint64_t s = side >> 1; s > 0; s >>= 1) {
+      const uint64_t rx = (x & s) ? 1 : 0, ry = (y & s) ? 1 : 0;
+      d += s * s * ((3 * rx) ^ ry);
+      if (ry == 0) {  // enter the quadrant in the curve's orientation
+        if (rx == 1) { x = side - 1 - x; y = side - 1 - y; }
+        std::swap(x, y);
+      }
+    }
+    key[i] = d;
+  }
+  std::vector<int64_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] != key[b] ? key[a] < key[b] : a < b; });
+  return order;
+}
+
 }  // namespace
 
 struct gd_mesh : gd::Mesh {};
@@ -165,6 +202,8 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
       !d->node2cellsL2down || !d->cells2nodes || !d->cell_mid_points || !d->cell_nodes_X || !d->cell_nodes_Y ||
       !d->xNodes || !d->yNodes)
     return fail(GD_ERR_INVALID, "missing mesh array");
+  if (order != GD_ORDER_AS_GIVEN && order != GD_ORDER_RCM && order != GD_ORDER_PERMUTATION && order != GD_ORDER_HILBERT)
+    return fail(GD_ERR_INVALID, "unknown cell order");
   // --- validate and collect topology in the caller's numbering (0-based); all
   // of the host-side construction runs (and is checked) before any device call
   std::vector<int32_t> nv(nc);
@@ -207,6 +246,8 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
   std::vector<int64_t>& cp = M->cperm;
   if (order == GD_ORDER_RCM) {
     cp = rcm_order(nc, xadj, adj);
+  } else if (order == GD_ORDER_HILBERT) {
+    cp = hilbert_order(nc, d->cell_mid_points);
   } else if (order == GD_ORDER_PERMUTATION) {
     if (!perm) { delete M; return fail(GD_ERR_INVALID, "GD_ORDER_PERMUTATION without a permutation"); }
     cp.assign(perm, perm + nc);

@@ -197,11 +197,12 @@ def main():
     ap.add_argument("--n-r", type=int, default=1445)
     ap.add_argument("--dr0", type=float, default=2e-4)
     ap.add_argument("--order", choices=["random", "given"], default="random")
-    ap.add_argument("--device-order", choices=["given", "rcm"], default="rcm",
-                    help="cell order of the HBM layout; variants ending in -given keep the caller's order")
+    ap.add_argument("--device-order", choices=["given", "rcm", "hilbert"], default="hilbert",
+                    help="cell order of the HBM layout; variants ending in -given keep the caller's order, "
+                         "-rcm use reverse Cuthill-McKee")
     ap.add_argument("--flux", choices=["ausm", "roe"], default="ausm")
     ap.add_argument("--cfl", type=float, default=0.4)
-    ap.add_argument("--variants", default="fp64,mixed,fp64-given,mixed-given")
+    ap.add_argument("--variants", default="fp64,mixed,fp64-rcm,mixed-rcm,fp64-given,mixed-given")
     ap.add_argument("--cpu-sample-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--no-cpu", action="store_true")
@@ -223,11 +224,12 @@ def main():
     mesh = build_mesh(a.n_theta, a.n_r, a.dr0, a.order)
     U0 = initial_U(case, mesh.nCells)
     flux = L.GD_FLUX_ROE if a.flux == "roe" else L.GD_FLUX_AUSM_PLUS
-    dev_order = L.GD_ORDER_RCM if a.device_order == "rcm" else L.GD_ORDER_AS_GIVEN
+    orders = {"given": L.GD_ORDER_AS_GIVEN, "rcm": L.GD_ORDER_RCM, "hilbert": L.GD_ORDER_HILBERT}
+    dev_order = orders[a.device_order]
     res = {}
     for var in a.variants.split(","):
         prec = L.GD_FP64 if var.startswith("fp64") else L.GD_FP32_MIXED
-        order = L.GD_ORDER_AS_GIVEN if var.endswith("-given") else dev_order
+        order = orders[var.split("-", 1)[1]] if "-" in var else dev_order
         res[var] = run_variant(mesh, case, prec, flux, order, U0, a.warmup, a.steps, a.cfl)
         r = res[var]
         bstep = sum(k["bytes_per_launch"] for k in r["kernels"])

@@ -7,8 +7,8 @@
  * layout: Julia column-major (nCells, k) arrays, i.e. k contiguous column
  * vectors, and 1-based ids, so a Julia caller passes its arrays as they are.
  * The library keeps its own 0-based structure-of-arrays copy in HBM, in a cell
- * order of its choosing (as given, RCM, or a caller permutation); every host
- * transfer is in the caller's numbering.
+ * order of its choosing (as given, RCM, a Hilbert curve, or a caller
+ * permutation); every host transfer is in the caller's numbering.
  *
  * Reference interfaces replaced (dimaZloy/2D-DNS-godunov-solver-):
  *   mesh2d_Int32 / readMesh2dHDF5   primeObjects.jl:109-135, utilsIO.jl:107-316 -> gd_mesh_create
@@ -70,6 +70,7 @@ typedef struct {
 #define GD_ORDER_AS_GIVEN 0
 #define GD_ORDER_RCM 1            /* reverse Cuthill-McKee of the face graph */
 #define GD_ORDER_PERMUTATION 2    /* caller's new->old map (0-based) */
+#define GD_ORDER_HILBERT 3        /* Hilbert curve through the cell centroids */
 
 typedef struct {
   int64_t n_cells, n_nodes, n_boundary_faces;

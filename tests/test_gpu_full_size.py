@@ -39,9 +39,12 @@ def test_full_size_numbering_and_mixed_precision(cylinder):
     L = pkg()._lib
     U_rcm, r64, p64, d64 = _run(cylinder, L.GD_FP64, L.GD_ORDER_RCM)
     U_giv, _, _, d_giv = _run(cylinder, L.GD_FP64, L.GD_ORDER_AS_GIVEN)
+    U_hil, _, _, d_hil = _run(cylinder, L.GD_FP64, L.GD_ORDER_HILBERT)
     assert np.array_equal(U_rcm, U_giv)
-    assert abs(d64 - d_giv) <= 1e-12 * abs(d64)  # the wall sum's order follows the layout
-    _, r32, p32, d32 = _run(cylinder, L.GD_FP32_MIXED, L.GD_ORDER_RCM)
+    assert np.array_equal(U_rcm, U_hil)
+    # the wall sum's order follows the layout
+    assert abs(d64 - d_giv) <= 1e-12 * abs(d64) and abs(d64 - d_hil) <= 1e-12 * abs(d64)
+    _, r32, p32, d32 = _run(cylinder, L.GD_FP32_MIXED, L.GD_ORDER_HILBERT)
     er = np.abs(r32 - r64).max() / np.abs(r64).max()
     ep = np.abs(p32 - p64).max() / np.abs(p64).max()
     ed = abs(d32 - d64) / abs(d64)

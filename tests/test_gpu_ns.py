@@ -120,8 +120,10 @@ def test_ns_reordering_invisible_fp64():
     b, _ = _device_steps(mesh, U, 8, 1e-3, _dev_params(), order=L.GD_ORDER_RCM)
     c, _ = _device_steps(mesh, U, 8, 1e-3, _dev_params(), order=L.GD_ORDER_PERMUTATION,
                          perm=M.random_permutation(mesh.nCells, SEED))
+    d, _ = _device_steps(mesh, U, 8, 1e-3, _dev_params(), order=L.GD_ORDER_HILBERT)
     assert np.array_equal(a, b)
     assert np.array_equal(a, c)
+    assert np.array_equal(a, d)
 
 
 def test_ns_free_stream_preserved_periodic():

@@ -42,10 +42,17 @@ def _box():
     return pkg().mesh.make_box_mesh(6, 5, quads="mixed")
 
 
-def test_valid_mesh_reaches_the_device():
+@pytest.mark.parametrize("order", ["GD_ORDER_AS_GIVEN", "GD_ORDER_RCM", "GD_ORDER_HILBERT"])
+def test_valid_mesh_reaches_the_device(order):
     L = pkg()._lib
-    rc, msg = _create(_box(), order=L.GD_ORDER_RCM)
+    rc, msg = _create(_box(), order=getattr(L, order))
     assert rc in (0, L.GD_ERR_NO_DEVICE), msg
+
+
+def test_unknown_order_is_refused():
+    L = pkg()._lib
+    rc, msg = _create(_box(), order=7)
+    assert rc == L.GD_ERR_INVALID and "unknown cell order" in msg
 
 
 @pytest.mark.parametrize("case, expect", [
