@@ -42,6 +42,7 @@ struct AdjKArgs {
   T* lamF;             // [nc][4]  -dt/A lamU^{n+1}
   T* selfq;            // [nc][4] own part, conservative variables
   T* nrev;             // [nc][nf][4] part for each neighbour, its conservative variables
+  const T* sgeo;       // [7][nf][nc] each slot's face geometry (nx, ny, len, r0x, r0y, r1x, r1y)
   double* bcsum;       // [n_wall][4] fixed-state adjoint summed over steps
   const double* lamU_in;
   double* lamU_out;
@@ -59,6 +60,7 @@ AdjKArgs<T> kargs(const AdjStepArgs& a) {
   k.lamF = static_cast<T*>(a.lamF);
   k.selfq = static_cast<T*>(a.selfq);
   k.nrev = static_cast<T*>(a.nrev);
+  k.sgeo = static_cast<const T*>(a.sgeo);
   k.bcsum = a.bcsum;
   k.lamU_in = a.lamU_in;
   k.lamU_out = a.lamU_out;
@@ -230,18 +232,47 @@ struct FaceIn {
   bool on_wall;                      // its force enters J
 };
 
+// Each cell slot's face geometry copied out of the face list into the slot's
+// place, so that the cell pass reads it unit-stride instead of gathering it
+// through the face ids (the values are the flux pass's, bit for bit).
 template <class T>
-__device__ __forceinline__ FaceIn<T> load_face(const AdjKArgs<T>& a, int64_t f, int32_t c1) {
+__global__ __launch_bounds__(kBlock) void k_slot_geo(MeshView m, T* sg) {
   using G = FaceGeo<T>;
+  const int64_t nc = m.nc, ns = (int64_t)m.nf * nc;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= nc) return;
+  for (int k = 0; k < m.nf; ++k) {
+    const int32_t cf = m.cface[k * nc + i];
+    const int64_t f = cf >> 1;
+    const T v[7] = {cf < 0 ? (T)0 : G::fnx(m, f),     cf < 0 ? (T)0 : G::fny(m, f),
+                    cf < 0 ? (T)0 : G::flen(m, f),    cf < 0 ? (T)0 : G::fr0(m, 0, f),
+                    cf < 0 ? (T)0 : G::fr0(m, 1, f),  cf < 0 ? (T)0 : G::fr1(m, 0, f),
+                    cf < 0 ? (T)0 : G::fr1(m, 1, f)};
+#pragma unroll
+    for (int q = 0; q < 7; ++q) sg[q * ns + k * nc + i] = v[q];
+  }
+}
+
+template <class T>
+hipError_t launch_slot_geo(const MeshView& m, T* sg, hipStream_t s) {
+  hipLaunchKernelGGL((k_slot_geo<T>), dim3(ns::nblocks(m.nc)), dim3(kBlock), 0, s, m, sg);
+  return hipGetLastError();
+}
+
+// slot k of cell i; c1 < 0: the boundary code - 1; f: the face (wall flag)
+template <class T>
+__device__ __forceinline__ FaceIn<T> load_face(const AdjKArgs<T>& a, int k, int64_t i, int64_t f, int32_t c1) {
   const MeshView& m = a.m;
+  const int64_t ns = (int64_t)m.nf * m.nc, s = k * m.nc + i;
+  const T* g0 = a.sgeo + s;
   FaceIn<T> g;
-  g.nx = G::fnx(m, f); g.ny = G::fny(m, f);
-  g.len = G::flen(m, f);
-  g.rx = G::fr0(m, 0, f); g.ry = G::fr0(m, 1, f);
+  g.nx = ns::ntl(g0); g.ny = ns::ntl(g0 + ns);
+  g.len = ns::ntl(g0 + 2 * ns);
+  g.rx = ns::ntl(g0 + 3 * ns); g.ry = ns::ntl(g0 + 4 * ns);
   g.rxj = 0; g.ryj = 0;
   g.on_wall = false;
   if (c1 >= 0) {
-    g.rxj = G::fr1(m, 0, f); g.ryj = G::fr1(m, 1, f);
+    g.rxj = ns::ntl(g0 + 5 * ns); g.ryj = ns::ntl(g0 + 6 * ns);
   } else {
     g.on_wall = a.wall_code == c1 + 1 && m.fwall[f] >= 0;
   }
@@ -415,22 +446,35 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
   using G = FaceGeo<T>;
   const MeshView& m = a.m;
   const int64_t nc = m.nc;
-  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= nc) return;
+  const int64_t i0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  // lanes past the end stand in for the last cell in the cooperative gather
+  // and store nothing
+  const bool live = i0 < nc;
+  const int64_t i = live ? i0 : nc - 1;
   const int nf = QUADS ? m.nv[i] : 3;
   int32_t jn[NF], fc[NF];
 #pragma unroll
   for (int k = 0; k < NF; ++k) { jn[k] = m.nb[k * nc + i]; fc[k] = m.cface[k * nc + i]; }
-  T ri[16], li[4], rj[NF][16], lj[NF][4];
-  ld16(a.rec + 16 * i, ri);
+  // own and neighbour records (a boundary or absent slot reads the own)
+  __shared__ typename ns::RecGather<T, NF + 1>::Shared sh;
+  T rr[NF + 1][16];
+  {
+    int32_t cells[NF + 1];
+    cells[0] = (int32_t)i;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) cells[1 + k] = jn[k] >= 0 ? jn[k] : (int32_t)i;
+    ns::RecGather<T, NF + 1>::gather(sh, a.rec, cells, rr);
+  }
+  if (!live) return;
+  const T* ri = rr[0];
+  T li[4], lj[NF][4];
   ld4(a.lamF + 4 * i, li);
   FaceIn<T> fg[NF];
 #pragma unroll
   for (int k = 0; k < NF; ++k) {
     const int64_t jj = jn[k] >= 0 ? jn[k] : i;
-    ld16(a.rec + 16 * jj, rj[k]);
     ld4(a.lamF + 4 * jj, lj[k]);
-    if (k < nf) fg[k] = load_face(a, fc[k] >> 1, jn[k] >= 0 ? 0 : jn[k]);
+    if (k < nf) fg[k] = load_face(a, k, i, fc[k] >> 1, jn[k] >= 0 ? 0 : jn[k]);
   }
   T wx[NF], wy[NF], rx[NF], ry[NF];
 #pragma unroll
@@ -456,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
       const bool sd = (fc[k] & 1) != 0;
       T r0[16], r1[16], l0[4], l1[4];
 #pragma unroll
-      for (int t = 0; t < 16; ++t) { r0[t] = sd ? rj[k][t] : ri[t]; r1[t] = sd ? ri[t] : rj[k][t]; }
+      for (int t = 0; t < 16; ++t) { r0[t] = sd ? rr[1 + k][t] : ri[t]; r1[t] = sd ? ri[t] : rr[1 + k][t]; }
 #pragma unroll
       for (int v = 0; v < 4; ++v) { l0[v] = sd ? lj[k][v] : li[v]; l1[v] = sd ? li[v] : lj[k][v]; }
       face_rev<T, VISC>(a, fg[k], c1, sd ? 1 : 0, r0, r1, l0, l1, lr, bqk[k]);
@@ -467,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
 #pragma unroll
   for (int k = 0; k < NF; ++k) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) qk[k][v] = rj[k][v];
+    for (int v = 0; v < 4; ++v) qk[k][v] = rr[1 + k][v];
     if (k < nf && jn[k] < 0) {
       const int64_t s = k * nc + i;
       ns::ghost<T>(a.prm, jn[k] + 1, qi, G::nx(m, s), G::ny(m, s), qk[k]);
@@ -540,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
     if (k >= nf) continue;
     if (jn[k] >= 0) {
       T o[4];
-      prims_rev<T>(rj[k], gm1, lq[1 + k], o);  // in the neighbour's conservative variables
+      prims_rev<T>(rr[1 + k], gm1, lq[1 + k], o);  // in the neighbour's conservative variables
       st4(a.nrev + 4 * (i * nfs + k), o[0], o[1], o[2], o[3]);
     } else {
       const int idx = -(jn[k] + 1);

@@ -232,6 +232,7 @@ struct AdjStepArgs {
   void* lamF;           // [nc][4] -dt/A lamU^{n+1}, compute type
   void* selfq;          // [nc][4] primitive adjoint, own part
   void* nrev;           // [nc][nf][4] primitive adjoint sent to each neighbour
+  const void* sgeo;     // [7][nf][nc] geometry of each cell slot's face (see launch_adj_slot_geo)
   double* bcsum;        // [n_wall][4]
   const double* lamU_in;
   double* lamU_out;
@@ -239,6 +240,10 @@ struct AdjStepArgs {
   int32_t wall_code;
   double fw[2];
 };
+// per cell slot, the face-list geometry of the slot's face as the flux pass
+// reads it (owner's normal, length, both offsets), component-major [7][nf][nc]
+hipError_t launch_adj_slot_geo_f64(const MeshView& m, void* sgeo, hipStream_t s);
+hipError_t launch_adj_slot_geo_f32(const MeshView& m, void* sgeo, hipStream_t s);
 hipError_t launch_adj_cell_f64(const AdjStepArgs& a, hipStream_t s);
 hipError_t launch_adj_state_f64(const AdjStepArgs& a, bool gather, hipStream_t s);
 hipError_t launch_adj_cell_f32(const AdjStepArgs& a, hipStream_t s);

@@ -372,24 +372,25 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The two records of each of a wave's 64 faces, gathered cooperatively: a
+// R records per lane of a wave (R x 64 records), gathered cooperatively: a
 // record is kP = sizeof(T) pieces of 16 B, and kP adjacent lanes load one
 // record's pieces in the same instruction, so an instruction touches 64 / kP
 // cache lines instead of 64 (the per-line cost of scattered 16 B lane loads
-// is what bounds a one-record-per-lane gather).  The pieces go through a
-// padded per-wave LDS slab (one piece of padding per record keeps the
-// read-back free of bank conflicts), owners first, then neighbours.
+// is what bounds a one-record-per-lane gather).  Every load is issued first;
+// the pieces then pass through a padded per-wave LDS slab of 64 records (one
+// piece of padding per record keeps the read-back free of bank conflicts),
+// one round per record of the lane.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-template <class T>
-struct FaceRecs {
+template <class T, int R>
+struct RecGather {
   static constexpr int kP = (int)sizeof(T);  // 16 B pieces per 16-value record
   static constexpr int kRPI = 64 / kP;       // records per load instruction
   static constexpr int kStride = kP + 1;     // LDS pieces per record
   static constexpr int kWaves = kBlock / 64;
   struct Shared {
     u32x4 rec[kWaves][64 * kStride];
-    int32_t cell[kWaves][128];
+    int32_t cell[kWaves][R * 64];
   };
   __device__ __forceinline__ static void unpack(u32x4 x, T* o) {
     if constexpr (sizeof(T) == 8) {
@@ -402,33 +403,31 @@ struct FaceRecs {
       o[0] = d.x; o[1] = d.y; o[2] = d.z; o[3] = d.w;
     }
   }
-  // cell0 / cell1: this lane's face's owner and the cell whose record stands
-  // on the other side (the owner again at a boundary)
-  __device__ __forceinline__ static void gather(Shared& sh, const T* rec, int32_t cell0, int32_t cell1, T* r0,
-                                                T* r1) {
+  // cells[r]: the cell whose record this lane receives in out[r]; every lane
+  // of the wave must take part
+  __device__ __forceinline__ static void gather(Shared& sh, const T* rec, const int32_t (&cells)[R],
+                                                T (&out)[R][16]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    sh.cell[w][lane] = cell0;
-    sh.cell[w][64 + lane] = cell1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) sh.cell[w][r * 64 + lane] = cells[r];
     wave_lds_sync();
-    const u32x4* R = reinterpret_cast<const u32x4*>(rec);
+    const u32x4* G = reinterpret_cast<const u32x4*>(rec);
     const int sub = lane / kP, piece = lane % kP;
-    u32x4 v0[kP], v1[kP];
+    u32x4 v[R][kP];
 #pragma unroll
-    for (int k = 0; k < kP; ++k) v0[k] = R[(int64_t)sh.cell[w][k * kRPI + sub] * kP + piece];
+    for (int r = 0; r < R; ++r)
 #pragma unroll
-    for (int k = 0; k < kP; ++k) v1[k] = R[(int64_t)sh.cell[w][64 + k * kRPI + sub] * kP + piece];
+      for (int k = 0; k < kP; ++k) v[r][k] = G[(int64_t)sh.cell[w][r * 64 + k * kRPI + sub] * kP + piece];
     u32x4* S = sh.rec[w];
 #pragma unroll
-    for (int k = 0; k < kP; ++k) S[(k * kRPI + sub) * kStride + piece] = v0[k];
-    wave_lds_sync();
+    for (int r = 0; r < R; ++r) {
+      if (r > 0) wave_lds_sync();
 #pragma unroll
-    for (int p = 0; p < kP; ++p) unpack(S[lane * kStride + p], r0 + p * (16 / kP));
-    wave_lds_sync();
+      for (int k = 0; k < kP; ++k) S[(k * kRPI + sub) * kStride + piece] = v[r][k];
+      wave_lds_sync();
 #pragma unroll
-    for (int k = 0; k < kP; ++k) S[(k * kRPI + sub) * kStride + piece] = v1[k];
-    wave_lds_sync();
-#pragma unroll
-    for (int p = 0; p < kP; ++p) unpack(S[lane * kStride + p], r1 + p * (16 / kP));
+      for (int p = 0; p < kP; ++p) unpack(S[lane * kStride + p], out[r] + p * (16 / kP));
+    }
   }
 };
 
@@ -445,10 +444,13 @@ __global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
   const bool live = f0 < nF;
   const int64_t f = live ? f0 : nF - 1;
   const int32_t c0 = m.fc0[f], c1 = m.fc1[f];
-  __shared__ typename FaceRecs<T>::Shared sh;
-  T r0[16], r1[16];
-  FaceRecs<T>::gather(sh, a.rec, c0, c1 >= 0 ? c1 : c0, r0, r1);
+  // both records (a boundary face reads the owner's twice)
+  __shared__ typename RecGather<T, 2>::Shared sh;
+  T rr[2][16];
+  RecGather<T, 2>::gather(sh, a.rec, {c0, c1 >= 0 ? c1 : c0}, rr);
   if (!live) return;
+  const T* r0 = rr[0];
+  const T* r1 = rr[1];
   const T nx = G::fnx(m, f), ny = G::fny(m, f);
   const T len = G::flen(m, f);
   const T rx = G::fr0(m, 0, f), ry = G::fr0(m, 1, f);
