@@ -279,49 +279,58 @@ __device__ __forceinline__ FaceIn<T> load_face(const AdjKArgs<T>& a, int k, int6
   return g;
 }
 
-// Transposed face flux for one side of a face.  r0 / r1: owner's and
-// neighbour's records (a boundary face passes the owner's twice); l0 / l1:
-// their lamF (l1 = 0 at a boundary); c1 < 0: the boundary code - 1.  Adds the
-// adjoint of the record of the given side (0: owner, 1: neighbour) to lr; at
-// a boundary face returns the fixed-state adjoint in bq (zero unless the code
-// is GD_BC_FIXED).
+// Transposed face flux for one side of a face, from that side.  ri / rj:
+// this cell's and the other cell's records (a boundary face passes this
+// cell's twice; this cell then is the owner); li / lj: their lamF (lj = 0 at
+// a boundary); sd: this cell is the face's neighbour; c1 < 0: the boundary
+// code - 1.  The face is evaluated in the owner's orientation exactly as the
+// flux pass evaluates it: the owner-minus-neighbour differences are negated
+// when this cell is the neighbour (an exact operation) and the two-sided
+// averages are sums, which do not depend on the order of their operands.
+// Adds the adjoint of this cell's record to lr; at a boundary face returns the
+// fixed-state adjoint in bq (zero unless the code is GD_BC_FIXED).
 template <class T, bool VISC>
-__device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& fg, int32_t c1, int side,
-                                         const T* r0, const T* r1, const T* l0, const T* l1, T* lr, double* bq) {
+__device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& fg, int32_t c1, bool sd,
+                                         const T* ri, const T* rj, const T* li, const T* lj, T* lr, double* bq) {
   const gd_params& P = a.prm;
-  const T nx = fg.nx, ny = fg.ny, len = fg.len, rx = fg.rx, ry = fg.ry;
+  const T nx = fg.nx, ny = fg.ny, len = fg.len;
   const T g = (T)P.Gamma, Rg = (T)P.RGAS;
-  T le[4];
+  T le[4];  // lamF(owner) - lamF(neighbour)
 #pragma unroll
-  for (int v = 0; v < 4; ++v) le[v] = l0[v] - l1[v];
+  for (int v = 0; v < 4; ++v) {
+    const T d = li[v] - lj[v];
+    le[v] = sd ? -d : d;
+  }
   if (fg.on_wall) { le[1] += (T)a.fw[0]; le[2] += (T)a.fw[1]; }
 #pragma unroll
   for (int v = 0; v < 4; ++v) le[v] *= len;
-  // forward recompute of both face states
-  const T* gi = r0 + 4;
-  T qa[4];
+  // forward recompute of both face states, each from its own record and offset
+  const T sx = sd ? fg.rxj : fg.rx, sy = sd ? fg.ryj : fg.ry;
+  const T ox = sd ? fg.rx : fg.rxj, oy = sd ? fg.ry : fg.ryj;
+  const T* gs = ri + 4;
+  T qs[4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) qa[v] = r0[v] + r0[12 + v] * (gi[2 * v] * rx + gi[2 * v + 1] * ry);
-  const bool fa = !(qa[0] > 0) || !(qa[3] > 0);
-  if (fa) {
+  for (int v = 0; v < 4; ++v) qs[v] = ri[v] + ri[12 + v] * (gs[2 * v] * sx + gs[2 * v + 1] * sy);
+  const bool fs = !(qs[0] > 0) || !(qs[3] > 0);
+  if (fs) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) qa[v] = r0[v];
+    for (int v = 0; v < 4; ++v) qs[v] = ri[v];
   }
-  const T* gj = r1 + 4;
-  T qb[4];
-  bool fb = false;
-  const T rxj = fg.rxj, ryj = fg.ryj;
+  const T* go = rj + 4;
+  T qo[4];
   if (c1 >= 0) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) qb[v] = r1[v] + r1[12 + v] * (gj[2 * v] * rxj + gj[2 * v + 1] * ryj);
-    fb = !(qb[0] > 0) || !(qb[3] > 0);
-    if (fb) {
+    for (int v = 0; v < 4; ++v) qo[v] = rj[v] + rj[12 + v] * (go[2 * v] * ox + go[2 * v + 1] * oy);
+    if (!(qo[0] > 0) || !(qo[3] > 0)) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) qb[v] = r1[v];
+      for (int v = 0; v < 4; ++v) qo[v] = rj[v];
     }
   } else {
-    ns::ghost<T>(P, c1 + 1, qa, nx, ny, qb);
+    ns::ghost<T>(P, c1 + 1, qs, nx, ny, qo);
   }
+  T qa[4], qb[4];  // owner's and neighbour's
+#pragma unroll
+  for (int v = 0; v < 4; ++v) { qa[v] = sd ? qo[v] : qs[v]; qb[v] = sd ? qs[v] : qo[v]; }
   T lqa[4] = {0, 0, 0, 0}, lqb[4] = {0, 0, 0, 0};
   ausm_rev<T>(qb, qa, nx, ny, g, le, lqb, lqa);
   const int bkind = c1 < 0 ? (-(c1 + 1) <= GD_MAX_BC ? P.bc[-(c1 + 1)].kind : GD_BC_EXTRAPOLATE) : -1;
@@ -342,52 +351,48 @@ __device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& 
       for (int v = 0; v < 4; ++v) lqa[v] += lqb[v];
     }
   }
-  // this side's state and reconstruction (element-wise selects keep one code
-  // path for both sides and no dynamically indexed private arrays)
-  const bool sd = side != 0;
-  T rs[16], lqs[4];
-#pragma unroll
-  for (int t = 0; t < 16; ++t) rs[t] = sd ? r1[t] : r0[t];
+  T lqs[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) lqs[v] = sd ? lqb[v] : lqa[v];
-  const T* gs = rs + 4;
-  const bool fs = sd ? fb : fa;
-  const T sx = sd ? rxj : rx, sy = sd ? ryj : ry;
   if (VISC) {
     T lgv[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) lgv[v] = le[v];
     const T cp = g * Rg / (g - 1);
-    const T rRi = 1 / (r0[0] * Rg);
-    const T Ti = r0[3] * rRi;
+    const T rRs = 1 / (ri[0] * Rg);
+    const T Ts = ri[3] * rRs;
     if (c1 >= 0) {
-      const T rRj = 1 / (r1[0] * Rg);
-      const T Tj = r1[3] * rRj;
-      T g_i[3][2], g_j[3][2];
+      const T rRo = 1 / (rj[0] * Rg);
+      const T To = rj[3] * rRo;
+      T g_s[3][2], g_o[3][2];
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        g_i[0][c] = gi[2 + c]; g_i[1][c] = gi[4 + c];
-        g_i[2][c] = (gi[6 + c] - Ti * Rg * gi[c]) * rRi;
-        g_j[0][c] = gj[2 + c]; g_j[1][c] = gj[4 + c];
-        g_j[2][c] = (gj[6 + c] - Tj * Rg * gj[c]) * rRj;
+        g_s[0][c] = gs[2 + c]; g_s[1][c] = gs[4 + c];
+        g_s[2][c] = (gs[6 + c] - Ts * Rg * gs[c]) * rRs;
+        g_o[0][c] = go[2 + c]; g_o[1][c] = go[4 + c];
+        g_o[2][c] = (go[6 + c] - To * Rg * go[c]) * rRo;
       }
-      const T ddx = rx - rxj, ddy = ry - ryj;
+      const T ddx = fg.rx - fg.rxj, ddy = fg.ry - fg.ryj;  // centroid neighbour - centroid owner
       const T rdl = 1 / sqrt(ddx * ddx + ddy * ddy);
       const T tx = ddx * rdl, ty = ddy * rdl;
-      const T dphi[3] = {r1[1] - r0[1], r1[2] - r0[2], Tj - Ti};
+      // neighbour minus owner of u, v, T
+      const T d3[3] = {rj[1] - ri[1], rj[2] - ri[2], To - Ts};
+      T dphi[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) dphi[c] = sd ? -d3[c] : d3[c];
       T gf[3][2];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T ax = (T)0.5 * (g_i[c][0] + g_j[c][0]), ay = (T)0.5 * (g_i[c][1] + g_j[c][1]);
+        const T ax = (T)0.5 * (g_s[c][0] + g_o[c][0]), ay = (T)0.5 * (g_s[c][1] + g_o[c][1]);
         const T corr = dphi[c] * rdl - (ax * tx + ay * ty);
         gf[c][0] = ax + corr * tx; gf[c][1] = ay + corr * ty;
       }
       T luf, lvf, lTf, lgf[3][2];
-      viscous_face_rev<T>(P, cp, (T)0.5 * (r0[1] + r1[1]), (T)0.5 * (r0[2] + r1[2]), (T)0.5 * (Ti + Tj), gf, nx, ny,
+      viscous_face_rev<T>(P, cp, (T)0.5 * (ri[1] + rj[1]), (T)0.5 * (ri[2] + rj[2]), (T)0.5 * (Ts + To), gf, nx, ny,
                           lgv, &luf, &lvf, &lTf, lgf);
       // both sides take half the averaged-gradient adjoint; the difference
       // term has opposite signs
-      const T sg = side ? (T)1 : (T)-1;
+      const T sg = sd ? (T)1 : (T)-1;
       T lg[3][2], ldphi[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -404,24 +409,24 @@ __device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& 
         lr[8 + c] += lg[1][c];
       }
       const T lgT[2] = {lg[2][0], lg[2][1]};
-      temp_rev<T>(rs, Rg, (T)0.5 * lTf + sg * ldphi[2], lgT, lr);
+      temp_rev<T>(ri, Rg, (T)0.5 * lTf + sg * ldphi[2], lgT, lr);
     } else {
       const int idx = -(c1 + 1);
       if (idx <= GD_MAX_BC && P.bc[idx].kind == GD_BC_ISOTHERMAL_WALL) {
         const T Tw = (T)P.bc[idx].data[0];
-        const T rdp = -1 / (rx * nx + ry * ny);
-        const T dudn = (0 - r0[1]) * rdp, dvdn = (0 - r0[2]) * rdp, dTdn = (Tw - Ti) * rdp;
-        const T ox = -nx, oy = -ny;
-        const T gf[3][2] = {{dudn * ox, dudn * oy}, {dvdn * ox, dvdn * oy}, {dTdn * ox, dTdn * oy}};
+        const T rdp = -1 / (fg.rx * nx + fg.ry * ny);
+        const T dudn = (0 - ri[1]) * rdp, dvdn = (0 - ri[2]) * rdp, dTdn = (Tw - Ts) * rdp;
+        const T ex = -nx, ey = -ny;
+        const T gf[3][2] = {{dudn * ex, dudn * ey}, {dvdn * ex, dvdn * ey}, {dTdn * ex, dTdn * ey}};
         T luf, lvf, lTf, lgf[3][2];
         viscous_face_rev<T>(P, cp, (T)0, (T)0, Tw, gf, nx, ny, lgv, &luf, &lvf, &lTf, lgf);
-        const T ldn[3] = {lgf[0][0] * ox + lgf[0][1] * oy, lgf[1][0] * ox + lgf[1][1] * oy,
-                          lgf[2][0] * ox + lgf[2][1] * oy};
+        const T ldn[3] = {lgf[0][0] * ex + lgf[0][1] * ey, lgf[1][0] * ex + lgf[1][1] * ey,
+                          lgf[2][0] * ex + lgf[2][1] * ey};
         lr[1] -= ldn[0] * rdp;
         lr[2] -= ldn[1] * rdp;
         const T lT = -ldn[2] * rdp;  // T_i = p rR
-        lr[3] += lT * rRi;
-        lr[0] -= lT * Ti / r0[0];
+        lr[3] += lT * rRs;
+        lr[0] -= lT * Ts / ri[0];
       }
     }
   }
@@ -430,8 +435,8 @@ __device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& 
   for (int v = 0; v < 4; ++v) {
     lr[v] += lqs[v];
     if (!fs) {
-      lr[4 + 2 * v] += lqs[v] * rs[12 + v] * sx;
-      lr[5 + 2 * v] += lqs[v] * rs[12 + v] * sy;
+      lr[4 + 2 * v] += lqs[v] * ri[12 + v] * sx;
+      lr[5 + 2 * v] += lqs[v] * ri[12 + v] * sy;
       lr[12 + v] += lqs[v] * (gs[2 * v] * sx + gs[2 * v + 1] * sy);
     }
   }
@@ -497,13 +502,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) lj[k][v] = 0;
       }
-      const bool sd = (fc[k] & 1) != 0;
-      T r0[16], r1[16], l0[4], l1[4];
-#pragma unroll
-      for (int t = 0; t < 16; ++t) { r0[t] = sd ? rr[1 + k][t] : ri[t]; r1[t] = sd ? ri[t] : rr[1 + k][t]; }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) { l0[v] = sd ? lj[k][v] : li[v]; l1[v] = sd ? li[v] : lj[k][v]; }
-      face_rev<T, VISC>(a, fg[k], c1, sd ? 1 : 0, r0, r1, l0, l1, lr, bqk[k]);
+      face_rev<T, VISC>(a, fg[k], c1, (fc[k] & 1) != 0, ri, rr[1 + k], li, lj[k], lr, bqk[k]);
     }
   }
   const T* qi = ri;
