@@ -1,4 +1,4 @@
-// Discrete adjoint of one ns step (AUSM+), templated on the face arithmetic
+// Discrete adjoint of one ns step (AUSM+ or Roe), templated on the face arithmetic
 // type like the forward kernels (double: FP64, float: FP32-mixed).
 //
 // Forward step n, dt fixed:  Q = prims(U^n);  rec = (q, grad q, phi)(Q);
@@ -175,6 +175,108 @@ __device__ __forceinline__ void ausm_rev(const T* L, const T* R, T nx, T ny, T g
   lR[0] += lrR; lR[1] += luR; lR[2] += lvR; lR[3] += lpR;
 }
 
+// Transposed Jacobian of ns::roe(L, R, n) (Roe averages, wave strengths,
+// Harten's fix of width 0.2 a on the acoustic waves): lo = adjoint of the 4
+// outputs; adds to lL, lR.  The fix's branches are taken as the forward takes
+// them; fabs'(0) = +1.
+template <class T>
+__device__ __forceinline__ void roe_rev(const T* L, const T* R, T nx, T ny, T g, const T* lo, T* lL, T* lR) {
+  // forward recompute
+  const T rL = L[0], uL = L[1], vL = L[2], pL = L[3];
+  const T rR = R[0], uR = R[1], vR = R[2], pR = R[3];
+  const T mx = -ny, my = nx;
+  const T kg = g / (g - 1);
+  const T unL = uL * nx + vL * ny, umL = uL * mx + vL * my;
+  const T unR = uR * nx + vR * ny, umR = uR * mx + vR * my;
+  const T HL = kg * pL / rL + (T)0.5 * (uL * uL + vL * vL);
+  const T HR = kg * pR / rR + (T)0.5 * (uR * uR + vR * vR);
+  const T RT = sqrt(rR / rL);
+  const T rho = RT * rL;
+  const T den = 1 + RT;
+  const T u = (uL + RT * uR) / den, v = (vL + RT * vR) / den, H = (HL + RT * HR) / den;
+  const T q2 = u * u + v * v;
+  const T a = sqrt((g - 1) * (H - (T)0.5 * q2));
+  const T un = u * nx + v * ny, um = u * mx + v * my;
+  const T dp = pR - pL, dun = unR - unL, dum = umR - umL, drho = rR - rL;
+  const T A2 = 2 * a * a;
+  const T w1 = (dp - rho * a * dun) / A2, w2 = rho * dum, w3 = drho - dp / (a * a), w4 = (dp + rho * a * dun) / A2;
+  const T s1r = fabs(un - a), s4r = fabs(un + a), s2 = fabs(un);
+  const T dw = (T)0.2 * a;
+  const bool f1 = s1r < dw, f4 = s4r < dw;
+  const T s1 = f1 ? (T)0.5 * (s1r * s1r / dw + dw) : s1r;
+  const T s4 = f4 ? (T)0.5 * (s4r * s4r / dw + dw) : s4r;
+  const T t1 = s1 * w1, t2 = s2 * w2, t3 = s2 * w3, t4 = s4 * w4;
+  // out = -1/2 (central part - d)
+  const T c = (T)-0.5;
+  T lrL = c * (lo[0] * unL + lo[1] * unL * uL + lo[2] * unL * vL + lo[3] * unL * HL);
+  T lrR = c * (lo[0] * unR + lo[1] * unR * uR + lo[2] * unR * vR + lo[3] * unR * HR);
+  T lunL = c * rL * (lo[0] + lo[1] * uL + lo[2] * vL + lo[3] * HL);
+  T lunR = c * rR * (lo[0] + lo[1] * uR + lo[2] * vR + lo[3] * HR);
+  T luL = c * lo[1] * rL * unL, lvL = c * lo[2] * rL * unL, lHL = c * lo[3] * rL * unL;
+  T luR = c * lo[1] * rR * unR, lvR = c * lo[2] * rR * unR, lHR = c * lo[3] * rR * unR;
+  T lpL = c * (lo[1] * nx + lo[2] * ny), lpR = lpL;
+  const T l1 = (T)0.5 * lo[0], l2 = (T)0.5 * lo[1], l3 = (T)0.5 * lo[2], l4 = (T)0.5 * lo[3];
+  // d1..d4
+  const T lt1 = l1 + l2 * (u - a * nx) + l3 * (v - a * ny) + l4 * (H - un * a);
+  const T lt2 = l2 * mx + l3 * my + l4 * um;
+  const T lt3 = l1 + l2 * u + l3 * v + l4 * (T)0.5 * q2;
+  const T lt4 = l1 + l2 * (u + a * nx) + l3 * (v + a * ny) + l4 * (H + un * a);
+  T lu = l2 * (t1 + t3 + t4), lv = l3 * (t1 + t3 + t4);
+  T la = (t4 - t1) * (l2 * nx + l3 * ny + l4 * un);
+  T lH = l4 * (t1 + t4);
+  T lun = l4 * a * (t4 - t1);
+  T lum = l4 * t2;
+  T lq2 = (T)0.5 * l4 * t3;
+  // t = s w
+  T ls1 = lt1 * w1, ls4 = lt4 * w4;
+  const T ls2 = lt2 * w2 + lt3 * w3;
+  const T lw1 = lt1 * s1, lw2 = lt2 * s2, lw3 = lt3 * s2, lw4 = lt4 * s4;
+  // Harten's fix
+  T ldw = 0;
+  if (f1) { ldw += ls1 * (T)0.5 * (1 - s1r * s1r / (dw * dw)); ls1 = ls1 * s1r / dw; }
+  if (f4) { ldw += ls4 * (T)0.5 * (1 - s4r * s4r / (dw * dw)); ls4 = ls4 * s4r / dw; }
+  la += (T)0.2 * ldw;
+  const T g1 = ls1 * dsgn(un - a), g4 = ls4 * dsgn(un + a);
+  lun += g1 + g4 + ls2 * dsgn(un);
+  la += g4 - g1;
+  // wave strengths
+  const T ln1 = lw1 / A2, ln4 = lw4 / A2;
+  const T lA2 = -(lw1 * w1 + lw4 * w4) / A2;
+  T ldp = ln1 + ln4 - lw3 / (a * a);
+  T lrho = (ln4 - ln1) * a * dun + lw2 * dum;
+  la += (ln4 - ln1) * rho * dun + lA2 * 4 * a + lw3 * 2 * dp / (a * a * a);
+  const T ldun = (ln4 - ln1) * rho * a;
+  const T ldum = lw2 * rho;
+  const T ldrho = lw3;
+  lpR += ldp; lpL -= ldp;
+  lunR += ldun; lunL -= ldun;
+  T lumR = ldum, lumL = -ldum;
+  lrR += ldrho; lrL -= ldrho;
+  // averaged state
+  lu += lun * nx + lum * mx;
+  lv += lun * ny + lum * my;
+  const T lX = la * (g - 1) / (2 * a);  // a = sqrt((g - 1) X), X = H - q2 / 2
+  lH += lX;
+  lq2 -= (T)0.5 * lX;
+  lu += 2 * u * lq2;
+  lv += 2 * v * lq2;
+  T lRT = (lu * (uR - u) + lv * (vR - v) + lH * (HR - H)) / den + lrho * rL;
+  luL += lu / den; luR += lu * RT / den;
+  lvL += lv / den; lvR += lv * RT / den;
+  lHL += lH / den; lHR += lH * RT / den;
+  lrL += lrho * RT;
+  const T lrat = lRT / (2 * RT);  // RT = sqrt(rR / rL)
+  lrR += lrat / rL;
+  lrL -= lrat * rR / (rL * rL);
+  // H of each side, then the normal and tangential velocities
+  lpL += lHL * kg / rL; lrL -= lHL * kg * pL / (rL * rL); luL += lHL * uL; lvL += lHL * vL;
+  lpR += lHR * kg / rR; lrR -= lHR * kg * pR / (rR * rR); luR += lHR * uR; lvR += lHR * vR;
+  luL += lunL * nx + lumL * mx; lvL += lunL * ny + lumL * my;
+  luR += lunR * nx + lumR * mx; lvR += lunR * ny + lumR * my;
+  lL[0] += lrL; lL[1] += luL; lL[2] += lvL; lL[3] += lpL;
+  lR[0] += lrR; lR[1] += luR; lR[2] += lvR; lR[3] += lpR;
+}
+
 // Transposed Jacobian of ns::viscous_face with respect to (uf, vf, Tf, gf);
 // lg = adjoint of gv[1..3] (gv[0] = 0).
 template <class T>
@@ -289,7 +391,7 @@ __device__ __forceinline__ FaceIn<T> load_face(const AdjKArgs<T>& a, int k, int6
 // averages are sums, which do not depend on the order of their operands.
 // Adds the adjoint of this cell's record to lr; at a boundary face returns the
 // fixed-state adjoint in bq (zero unless the code is GD_BC_FIXED).
-template <class T, bool VISC>
+template <class T, bool VISC, bool ROE>
 __device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& fg, int32_t c1, bool sd,
                                          const T* ri, const T* rj, const T* li, const T* lj, T* lr, double* bq) {
   const gd_params& P = a.prm;
@@ -332,7 +434,8 @@ __device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& 
 #pragma unroll
   for (int v = 0; v < 4; ++v) { qa[v] = sd ? qo[v] : qs[v]; qb[v] = sd ? qs[v] : qo[v]; }
   T lqa[4] = {0, 0, 0, 0}, lqb[4] = {0, 0, 0, 0};
-  ausm_rev<T>(qb, qa, nx, ny, g, le, lqb, lqa);
+  if (ROE) roe_rev<T>(qb, qa, nx, ny, g, le, lqb, lqa);
+  else ausm_rev<T>(qb, qa, nx, ny, g, le, lqb, lqa);
   const int bkind = c1 < 0 ? (-(c1 + 1) <= GD_MAX_BC ? P.bc[-(c1 + 1)].kind : GD_BC_EXTRAPOLATE) : -1;
   // written unconditionally so that bq stays in registers
 #pragma unroll
@@ -445,7 +548,7 @@ __device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& 
 // ---- cell pass, one thread per cell: every load of the cell's faces is
 // issued before the first transposed flux, then the four variables' transposed
 // gradient + limiter run in the same thread.
-template <class T, bool QUADS, bool VISC>
+template <class T, bool QUADS, bool VISC, bool ROE>
 __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
   constexpr int NF = QUADS ? 4 : 3;
   using G = FaceGeo<T>;
@@ -502,7 +605,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) lj[k][v] = 0;
       }
-      face_rev<T, VISC>(a, fg[k], c1, (fc[k] & 1) != 0, ri, rr[1 + k], li, lj[k], lr, bqk[k]);
+      face_rev<T, VISC, ROE>(a, fg[k], c1, (fc[k] & 1) != 0, ri, rr[1 + k], li, lj[k], lr, bqk[k]);
     }
   }
   const T* qi = ri;
@@ -647,17 +750,23 @@ __global__ __launch_bounds__(kBlock) void k_adj_state(AdjKArgs<T> a) {
   st4(a.lamF + 4 * j, (T)(s * lU[0]), (T)(s * lU[1]), (T)(s * lU[2]), (T)(s * lU[3]));
 }
 
-template <class T>
-hipError_t launch_cell(const AdjKArgs<T>& a, hipStream_t s) {
+template <class T, bool ROE>
+void launch_cell_flux(const AdjKArgs<T>& a, hipStream_t s) {
   const dim3 g(ns::nblocks(a.m.nc)), b(kBlock);
   const bool visc = a.prm.viscous != 0;
   if (a.m.has_quads) {
-    if (visc) hipLaunchKernelGGL((k_adj_cell<T, true, true>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_adj_cell<T, true, false>), g, b, 0, s, a);
+    if (visc) hipLaunchKernelGGL((k_adj_cell<T, true, true, ROE>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_adj_cell<T, true, false, ROE>), g, b, 0, s, a);
   } else {
-    if (visc) hipLaunchKernelGGL((k_adj_cell<T, false, true>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_adj_cell<T, false, false>), g, b, 0, s, a);
+    if (visc) hipLaunchKernelGGL((k_adj_cell<T, false, true, ROE>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_adj_cell<T, false, false, ROE>), g, b, 0, s, a);
   }
+}
+
+template <class T>
+hipError_t launch_cell(const AdjKArgs<T>& a, hipStream_t s) {
+  if (a.prm.flux == GD_FLUX_ROE) launch_cell_flux<T, true>(a, s);
+  else launch_cell_flux<T, false>(a, s);
   return hipGetLastError();
 }
 

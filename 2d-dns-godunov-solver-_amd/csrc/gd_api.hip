@@ -1089,7 +1089,8 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
   if (!S || !ap) return fail(GD_ERR_INVALID, "null argument");
   const gd_params& P = S->prm;
   if (P.scheme != GD_SCHEME_NS) return fail(GD_ERR_UNSUPPORTED, "the adjoint covers the ns scheme");
-  if (P.flux != GD_FLUX_AUSM_PLUS) return fail(GD_ERR_UNSUPPORTED, "the adjoint covers the AUSM+ flux");
+  if (P.flux != GD_FLUX_AUSM_PLUS && P.flux != GD_FLUX_ROE)
+    return fail(GD_ERR_UNSUPPORTED, "the adjoint covers the AUSM+ and Roe fluxes");
   if (ap->n_steps < 1 || ap->checkpoint_every < 1) return fail(GD_ERR_INVALID, "n_steps and checkpoint_every >= 1");
   if (!(ap->dt > 0.0)) return fail(GD_ERR_INVALID, "the adjoint needs a fixed dt > 0");
   const MeshView& v = S->mesh->v;

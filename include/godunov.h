@@ -192,7 +192,7 @@ typedef struct {
 
 int gd_run(gd_state* st, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats);
 
-/* ---- discrete adjoint of the ns loop (AUSM+, FP64 or FP32-mixed) ----
+/* ---- discrete adjoint of the ns loop (AUSM+ or Roe, FP64 or FP32-mixed) ----
    Not in the reference, which has no adjoint; the objective is
    J = sum_{n<N} w . F_n + <dJ_dUN, U_N>, F_n the wall force of step n on the
    faces of code wall_code, at a fixed dt.  Runs N steps forward from the

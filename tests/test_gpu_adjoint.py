@@ -39,19 +39,19 @@ def _box(quads="mixed"):
     return mesh, U
 
 
-def _dev_params(mixed=False, viscous=True):
+def _dev_params(mixed=False, viscous=True, roe=False):
     S = pkg().stage
     L = pkg()._lib
     return S.make_params(scheme=L.GD_SCHEME_NS, precision=L.GD_FP32_MIXED if mixed else L.GD_FP64,
-                         viscous=viscous, venkat_K=GAS["venkat_K"], venkat_scale=GAS["venkat_scale"],
+                         flux=L.GD_FLUX_ROE if roe else L.GD_FLUX_AUSM_PLUS, viscous=viscous, venkat_K=GAS["venkat_K"], venkat_scale=GAS["venkat_scale"],
                          RGAS=GAS["R"], Gamma=1.4, mu_ref=GAS["mu_ref"], T_ref=GAS["T_ref"], S_suth=GAS["S"],
                          Pr=GAS["Pr"], bcs=BCS)
 
 
-def _tl_objective(mesh, U, n, dU, dbc, w, viscous=True):
+def _tl_objective(mesh, U, n, dU, dbc, w, viscous=True, roe=False):
     """Oracle dual-number run: J and dJ along (dU, dbc)."""
     mv = O.MeshView(mesh)
-    prm = O.ns_params(gamma=1.4, viscous=viscous, bcs=BCS, **GAS)
+    prm = O.ns_params(gamma=1.4, viscous=viscous, bcs=BCS, flux=O.FLUX_ROE if roe else O.FLUX_AUSM_PLUS, **GAS)
     u, du = U.copy(order="F"), dU.copy(order="F")
     J = dJ = 0.0
     for _ in range(n):
@@ -61,10 +61,10 @@ def _tl_objective(mesh, U, n, dU, dbc, w, viscous=True):
     return J + float((w * u).sum()), dJ + float((w * du).sum())
 
 
-def _adjoint(mesh, U, n, c, w, mixed=False, viscous=True, order=None, perm=None, budget=0):
+def _adjoint(mesh, U, n, c, w, mixed=False, viscous=True, order=None, perm=None, budget=0, roe=False):
     S = pkg().stage
     L = pkg()._lib
-    s = S.GodunovSolver(mesh, _dev_params(mixed, viscous), order=L.GD_ORDER_AS_GIVEN if order is None else order,
+    s = S.GodunovSolver(mesh, _dev_params(mixed, viscous, roe), order=L.GD_ORDER_AS_GIVEN if order is None else order,
                         perm=perm)
     s.state.set_array(L.GD_ARR_U_OLD, U)
     g0, gbc, st = s.adjoint(n, DT, checkpoint_every=c, wall_code=-3, force_weight=(1.0, 0.0), dJ_dUN=w,
@@ -74,19 +74,21 @@ def _adjoint(mesh, U, n, c, w, mixed=False, viscous=True, order=None, perm=None,
     return g0, gbc, st, UN
 
 
+@pytest.mark.parametrize("flux", ["ausm", "roe"])
 @pytest.mark.parametrize("quads", ["none", "mixed"])
 @pytest.mark.parametrize("viscous", [False, True])
-def test_adjoint_fp64_dot_product_with_tangent_linear_oracle(quads, viscous):
+def test_adjoint_fp64_dot_product_with_tangent_linear_oracle(quads, viscous, flux):
+    roe = flux == "roe"
     mesh, U = _box(quads)
     n = mesh.nCells
     M = pkg().mesh
     w = np.asfortranarray(1e-3 * M.normal(SEED + 1, 4 * n).reshape(n, 4, order="F"))
-    g0, gbc, st, UN = _adjoint(mesh, U, 6, 4, w, viscous=viscous)
-    print("adjoint", quads, viscous, "J", st.J, "recomputed", st.recomputed_steps)
+    g0, gbc, st, UN = _adjoint(mesh, U, 6, 4, w, viscous=viscous, roe=roe)
+    print("adjoint", flux, quads, viscous, "J", st.J, "recomputed", st.recomputed_steps)
     assert st.recomputed_steps == 4
     for k in range(3):
         d = np.asfortranarray(M.normal(SEED + 10 + k, 4 * n).reshape(n, 4, order="F") * np.abs(U).max(axis=0))
-        J, dJ = _tl_objective(mesh, U, 6, d, None, w, viscous)
+        J, dJ = _tl_objective(mesh, U, 6, d, None, w, viscous, roe)
         adj = float((g0 * d).sum())
         print("  <dJ/dU0, d>", adj, "tangent", dJ)
         assert abs(adj - dJ) <= 1e-9 * max(abs(dJ), 1e-300)
@@ -94,7 +96,7 @@ def test_adjoint_fp64_dot_product_with_tangent_linear_oracle(quads, viscous):
     # the fixed inflow state (code -1)
     dbc = np.zeros((O.MAX_BC + 1, 4))
     dbc[1] = (0.3, -0.5, 0.7, 0.2)
-    _, dJb = _tl_objective(mesh, U, 6, np.zeros_like(U), dbc, w, viscous)
+    _, dJb = _tl_objective(mesh, U, 6, np.zeros_like(U), dbc, w, viscous, roe)
     assert abs(float((gbc * dbc).sum()) - dJb) <= 1e-9 * abs(dJb)
 
 
@@ -138,11 +140,12 @@ def test_adjoint_independent_of_checkpointing_and_numbering():
     assert np.array_equal(a[3], b[3])
 
 
-def test_adjoint_mixed_close_to_fp64():
+@pytest.mark.parametrize("flux", ["ausm", "roe"])
+def test_adjoint_mixed_close_to_fp64(flux):
     mesh, U = _box()
     n = mesh.nCells
-    g64 = _adjoint(mesh, U, 6, 3, None)[0]
-    g32, _, st, _ = _adjoint(mesh, U, 6, 3, None, mixed=True)
+    g64 = _adjoint(mesh, U, 6, 3, None, roe=flux == "roe")[0]
+    g32, _, st, _ = _adjoint(mesh, U, 6, 3, None, mixed=True, roe=flux == "roe")
     M = pkg().mesh
     d = np.asfortranarray(M.normal(SEED + 5, 4 * n).reshape(n, 4, order="F") * np.abs(U).max(axis=0))
     a, b = float((g64 * d).sum()), float((g32 * d).sum())
@@ -150,15 +153,13 @@ def test_adjoint_mixed_close_to_fp64():
     assert abs(a - b) <= 1e-4 * abs(a)
 
 
-def test_adjoint_rejects_roe_and_ref_scheme():
+def test_adjoint_rejects_ref_scheme():
     S = pkg().stage
     L = pkg()._lib
     mesh, U = _box()
-    prm = _dev_params()
-    prm.flux = L.GD_FLUX_ROE
-    s = S.GodunovSolver(mesh, prm)
+    s = S.GodunovSolver(mesh, S.make_params(bcs={}))  # the reference scheme
     s.state.set_array(L.GD_ARR_U_OLD, U)
-    with pytest.raises(L.GodunovError):
+    with pytest.raises(L.GodunovError, match="ns scheme"):
         s.adjoint(2, DT)
     s.close()
 
