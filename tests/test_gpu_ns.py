@@ -202,3 +202,34 @@ def test_ns_small_cylinder_fp64_1000_steps():
     print("cylinder 1000 steps fp64", err, "time", st.flow_time, t)
     assert err <= 1e-9
     assert abs(st.flow_time - t) <= 1e-12 * t
+
+
+@pytest.mark.parametrize("order", ["given", "hilbert"])
+def test_ns_residual_norms(order):
+    """gd_run's residuals are updateResidualSA's (evaluate2d.jl:6-48): per
+    conservative variable, the sum over cells of (U_new - U_old)^2 of the
+    run's last step, independent of the device order."""
+    S = pkg().stage
+    L = pkg()._lib
+    mesh = _box("mixed", 30, 20)
+    U = _box_state(mesh)
+    s = S.GodunovSolver(mesh, _dev_params(), order=L.GD_ORDER_HILBERT if order == "hilbert" else L.GD_ORDER_AS_GIVEN)
+    s.state.set_array(L.GD_ARR_U_OLD, U)
+    s.run(3, dt=1e-3)
+    before = s.conservative()
+    st = s.run(1, dt=1e-3, residuals=True)
+    after = s.conservative()
+    st2 = s.run(2, dt=1e-3, residuals=True)
+    last = s.conservative()
+    s.close()
+    want = ((after - before) ** 2).sum(axis=0)
+    got = np.array(st.residuals[:])
+    assert np.all(want > 0)
+    assert np.allclose(got, want, rtol=1e-12, atol=0)
+    # a longer run reports its last step only
+    s = S.GodunovSolver(mesh, _dev_params())
+    s.state.set_array(L.GD_ARR_U_OLD, U)
+    s.run(5, dt=1e-3)
+    prev = s.conservative()
+    s.close()
+    assert np.allclose(np.array(st2.residuals[:]), ((last - prev) ** 2).sum(axis=0), rtol=1e-12, atol=0)
