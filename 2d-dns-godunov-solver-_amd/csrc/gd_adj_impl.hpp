@@ -42,7 +42,7 @@ struct AdjKArgs {
   T* lamF;             // [nc][4]  -dt/A lamU^{n+1}
   T* selfq;            // [nc][4] own part, conservative variables
   T* nrev;             // [nc][nf][4] part for each neighbour, its conservative variables
-  const T* sgeo;       // [7][nf][nc] each slot's face geometry (nx, ny, len, r0x, r0y, r1x, r1y)
+  const T* sgeo;       // [5][nf][nc] each slot's face geometry (nx, ny, len, other cell's offset x, y)
   double* bcsum;       // [n_wall][4] fixed-state adjoint summed over steps
   const double* lamU_in;
   double* lamU_out;
@@ -330,13 +330,17 @@ __device__ __forceinline__ void temp_rev(const T* r, T Rg, T lT, const T* lgT, T
 // Geometry of one face as the transposed flux needs it (loaded ahead of use)
 template <class T>
 struct FaceIn {
-  T nx, ny, len, rx, ry, rxj, ryj;  // owner's inward normal, length, both offsets
-  bool on_wall;                      // its force enters J
+  T nx, ny, len;  // owner's inward normal, length
+  T ox, oy;       // face midpoint - the other cell's centroid (0 at a boundary)
+  bool on_wall;   // its force enters J
 };
 
 // Each cell slot's face geometry copied out of the face list into the slot's
 // place, so that the cell pass reads it unit-stride instead of gathering it
-// through the face ids (the values are the flux pass's, bit for bit).
+// through the face ids (the values are the flux pass's, bit for bit): the
+// owner's normal and the length, and the other cell's offset to the face
+// midpoint.  The cell's own offset is its slot offset r_f, which the face list
+// holds as an exact copy (both are computed from the same cell's vertices).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_slot_geo(MeshView m, T* sg) {
   using G = FaceGeo<T>;
@@ -346,12 +350,15 @@ __global__ __launch_bounds__(kBlock) void k_slot_geo(MeshView m, T* sg) {
   for (int k = 0; k < m.nf; ++k) {
     const int32_t cf = m.cface[k * nc + i];
     const int64_t f = cf >> 1;
-    const T v[7] = {cf < 0 ? (T)0 : G::fnx(m, f),     cf < 0 ? (T)0 : G::fny(m, f),
-                    cf < 0 ? (T)0 : G::flen(m, f),    cf < 0 ? (T)0 : G::fr0(m, 0, f),
-                    cf < 0 ? (T)0 : G::fr0(m, 1, f),  cf < 0 ? (T)0 : G::fr1(m, 0, f),
-                    cf < 0 ? (T)0 : G::fr1(m, 1, f)};
+    const bool nbr = (cf & 1) != 0;  // this cell is the face's neighbour
+    T v[kAdjSlotGeo] = {0, 0, 0, 0, 0};
+    if (cf >= 0) {
+      v[0] = G::fnx(m, f); v[1] = G::fny(m, f); v[2] = G::flen(m, f);
+      v[3] = nbr ? G::fr0(m, 0, f) : G::fr1(m, 0, f);
+      v[4] = nbr ? G::fr0(m, 1, f) : G::fr1(m, 1, f);
+    }
 #pragma unroll
-    for (int q = 0; q < 7; ++q) sg[q * ns + k * nc + i] = v[q];
+    for (int q = 0; q < kAdjSlotGeo; ++q) sg[q * ns + k * nc + i] = v[q];
   }
 }
 
@@ -370,18 +377,18 @@ __device__ __forceinline__ FaceIn<T> load_face(const AdjKArgs<T>& a, int k, int6
   FaceIn<T> g;
   g.nx = ns::ntl(g0); g.ny = ns::ntl(g0 + ns);
   g.len = ns::ntl(g0 + 2 * ns);
-  g.rx = ns::ntl(g0 + 3 * ns); g.ry = ns::ntl(g0 + 4 * ns);
-  g.rxj = 0; g.ryj = 0;
+  g.ox = 0; g.oy = 0;
   g.on_wall = false;
   if (c1 >= 0) {
-    g.rxj = ns::ntl(g0 + 5 * ns); g.ryj = ns::ntl(g0 + 6 * ns);
+    g.ox = ns::ntl(g0 + 3 * ns); g.oy = ns::ntl(g0 + 4 * ns);
   } else {
     g.on_wall = a.wall_code == c1 + 1 && m.fwall[f] >= 0;
   }
   return g;
 }
 
-// Transposed face flux for one side of a face, from that side.  ri / rj:
+// Transposed face flux for one side of a face, from that side.  (sx, sy):
+// face midpoint - this cell's centroid; ri / rj:
 // this cell's and the other cell's records (a boundary face passes this
 // cell's twice; this cell then is the owner); li / lj: their lamF (lj = 0 at
 // a boundary); sd: this cell is the face's neighbour; c1 < 0: the boundary
@@ -392,7 +399,7 @@ __device__ __forceinline__ FaceIn<T> load_face(const AdjKArgs<T>& a, int k, int6
 // Adds the adjoint of this cell's record to lr; at a boundary face returns the
 // fixed-state adjoint in bq (zero unless the code is GD_BC_FIXED).
 template <class T, bool VISC, bool ROE>
-__device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& fg, int32_t c1, bool sd,
+__device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& fg, T sx, T sy, int32_t c1, bool sd,
                                          const T* ri, const T* rj, const T* li, const T* lj, T* lr, double* bq) {
   const gd_params& P = a.prm;
   const T nx = fg.nx, ny = fg.ny, len = fg.len;
@@ -407,8 +414,7 @@ __device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& 
 #pragma unroll
   for (int v = 0; v < 4; ++v) le[v] *= len;
   // forward recompute of both face states, each from its own record and offset
-  const T sx = sd ? fg.rxj : fg.rx, sy = sd ? fg.ryj : fg.ry;
-  const T ox = sd ? fg.rx : fg.rxj, oy = sd ? fg.ry : fg.ryj;
+  const T ox = fg.ox, oy = fg.oy;
   const T* gs = ri + 4;
   T qs[4];
 #pragma unroll
@@ -475,7 +481,9 @@ __device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& 
         g_o[0][c] = go[2 + c]; g_o[1][c] = go[4 + c];
         g_o[2][c] = (go[6 + c] - To * Rg * go[c]) * rRo;
       }
-      const T ddx = fg.rx - fg.rxj, ddy = fg.ry - fg.ryj;  // centroid neighbour - centroid owner
+      // centroid neighbour - centroid owner (an exact negation from the neighbour)
+      const T ex0 = sx - ox, ey0 = sy - oy;
+      const T ddx = sd ? -ex0 : ex0, ddy = sd ? -ey0 : ey0;
       const T rdl = 1 / sqrt(ddx * ddx + ddy * ddy);
       const T tx = ddx * rdl, ty = ddy * rdl;
       // neighbour minus owner of u, v, T
@@ -517,7 +525,7 @@ __device__ __forceinline__ void face_rev(const AdjKArgs<T>& a, const FaceIn<T>& 
       const int idx = -(c1 + 1);
       if (idx <= GD_MAX_BC && P.bc[idx].kind == GD_BC_ISOTHERMAL_WALL) {
         const T Tw = (T)P.bc[idx].data[0];
-        const T rdp = -1 / (fg.rx * nx + fg.ry * ny);
+        const T rdp = -1 / (sx * nx + sy * ny);
         const T dudn = (0 - ri[1]) * rdp, dvdn = (0 - ri[2]) * rdp, dTdn = (Tw - Ts) * rdp;
         const T ex = -nx, ey = -ny;
         const T gf[3][2] = {{dudn * ex, dudn * ey}, {dvdn * ex, dvdn * ey}, {dTdn * ex, dTdn * ey}};
@@ -605,7 +613,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) lj[k][v] = 0;
       }
-      face_rev<T, VISC, ROE>(a, fg[k], c1, (fc[k] & 1) != 0, ri, rr[1 + k], li, lj[k], lr, bqk[k]);
+      face_rev<T, VISC, ROE>(a, fg[k], rx[k], ry[k], c1, (fc[k] & 1) != 0, ri, rr[1 + k], li, lj[k], lr, bqk[k]);
     }
   }
   const T* qi = ri;

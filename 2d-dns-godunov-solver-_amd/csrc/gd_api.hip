@@ -1126,7 +1126,7 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
   rc |= dev_alloc(D.p, &selfq, 4 * nc * t);
   rc |= dev_alloc(D.p, &nrev, 4 * (size_t)v.nf * nc * t);
   char* sgeo = nullptr;
-  rc |= dev_alloc(D.p, &sgeo, 7 * (size_t)v.nf * nc * t);
+  rc |= dev_alloc(D.p, &sgeo, kAdjSlotGeo * (size_t)v.nf * nc * t);
   if (rc) return rc;
   GD_HIP((mixed ? launch_adj_slot_geo_f32 : launch_adj_slot_geo_f64)(v, sgeo, S->stream));
   // record slot of step s: kept segments have their own region; the others

@@ -225,6 +225,10 @@ hipError_t launch_ns_grad_f64(const NsStepArgs& a, hipStream_t s);
 hipError_t launch_ns_prims_f32(int64_t nc, double gamma, const double* U, void* Q, hipStream_t s);
 hipError_t launch_ns_grad_f32(const NsStepArgs& a, hipStream_t s);
 // ---- discrete adjoint of the ns step (gd_adj_fp64.hip / gd_adj_fp32.hip) ----
+// values per cell slot in the adjoint's slot geometry: the owner's normal
+// (2), the length, the other cell's offset to the face midpoint (2)
+constexpr int kAdjSlotGeo = 5;
+
 struct AdjStepArgs {
   MeshView m;
   gd_params prm;
@@ -232,7 +236,7 @@ struct AdjStepArgs {
   void* lamF;           // [nc][4] -dt/A lamU^{n+1}, compute type
   void* selfq;          // [nc][4] primitive adjoint, own part
   void* nrev;           // [nc][nf][4] primitive adjoint sent to each neighbour
-  const void* sgeo;     // [7][nf][nc] geometry of each cell slot's face (see launch_adj_slot_geo)
+  const void* sgeo;     // [kAdjSlotGeo][nf][nc] geometry of each cell slot's face (see launch_adj_slot_geo)
   double* bcsum;        // [n_wall][4]
   const double* lamU_in;
   double* lamU_out;
@@ -241,7 +245,7 @@ struct AdjStepArgs {
   double fw[2];
 };
 // per cell slot, the face-list geometry of the slot's face as the flux pass
-// reads it (owner's normal, length, both offsets), component-major [7][nf][nc]
+// reads it, component-major [kAdjSlotGeo][nf][nc]
 hipError_t launch_adj_slot_geo_f64(const MeshView& m, void* sgeo, hipStream_t s);
 hipError_t launch_adj_slot_geo_f32(const MeshView& m, void* sgeo, hipStream_t s);
 hipError_t launch_adj_cell_f64(const AdjStepArgs& a, hipStream_t s);
