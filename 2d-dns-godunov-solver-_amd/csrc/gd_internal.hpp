@@ -17,8 +17,9 @@ namespace gd {
 
 // MI355X dispatches workgroup b to XCD (b mod 8).  Remapping the launch order
 // so that each XCD receives one contiguous range of logical blocks keeps the
-// neighbours of a cell range (which, in RCM order, sit in nearby blocks) in the
-// same 4 MB L2 instead of spreading the halo over eight L2s.
+// neighbours of a cell range (which, in a locality-preserving order such as
+// Hilbert or RCM, sit in nearby blocks) in the same 4 MB L2 instead of
+// spreading the halo over eight L2s.
 constexpr unsigned kNumXcd = 8;
 // threads per block of every cell / face / node kernel (the per-block partial
 // reductions are sized by it)

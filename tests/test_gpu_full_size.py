@@ -1,6 +1,6 @@
 """The 2,079,360-triangle cylinder at full size, through size-independent
 properties: the FP64 step is bit-identical whatever the device numbering
-(RCM or the seeded random caller order), and the FP32-mixed step stays within
+(RCM, Hilbert or the seeded random caller order), and the FP32-mixed step stays within
 1e-5 of FP64 in density, pressure and wall drag after 100 steps."""
 import numpy as np
 import pytest

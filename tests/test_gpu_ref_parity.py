@@ -204,7 +204,7 @@ def test_device_loop_matches_oracle_20_steps():
     assert rel_linf(pC, fo.pressureCells) <= TOL_STEP * 20
 
 
-@pytest.mark.parametrize("order", ["rcm", "random"])
+@pytest.mark.parametrize("order", ["rcm", "hilbert", "random"])
 def test_reordering_is_bitwise_invisible(order):
     L = pkg()._lib
     M = pkg().mesh
@@ -213,6 +213,8 @@ def test_reordering_is_bitwise_invisible(order):
     a.run(15, dt=5e-4)
     if order == "rcm":
         b = _solver(mesh, th, f, bcs=BOX_BCS, order=L.GD_ORDER_RCM)
+    elif order == "hilbert":
+        b = _solver(mesh, th, f, bcs=BOX_BCS, order=L.GD_ORDER_HILBERT)
     else:
         b = _solver(mesh, th, f, bcs=BOX_BCS, order=L.GD_ORDER_PERMUTATION,
                     perm=M.random_permutation(mesh.nCells, SEED))
