@@ -221,7 +221,7 @@ typedef struct {
   double flow_time;
   int32_t recomputed_steps;
   int32_t n_checkpoints;
-  int64_t storage_bytes;     /* checkpoints + one segment of cell records */
+  int64_t storage_bytes;     /* checkpoints + the cell records of the kept segments */
   double ms_forward;         /* forward sweep, HIP events */
   double ms_reverse;         /* recomputation + transposed steps */
 } gd_adjoint_stats;
