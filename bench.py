@@ -173,7 +173,7 @@ def cpu_baseline(mesh, case, U0, sample_steps, threads):
     med = float(np.median(times))
     return {"value": round(mesh.nCells / med, 1), "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{sample_steps} FP64 ns steps of the same 2.08M-cell workload from the free stream "
-                      f"(oracle C++/OpenMP restatement, median step {med:.3f} s)"}
+                      f"(oracle C++/OpenMP restatement, {sum(times):.1f} s in all, median step {med:.3f} s)"}
 
 
 def copy_probe(nbytes):
@@ -203,7 +203,7 @@ def main():
     ap.add_argument("--flux", choices=["ausm", "roe"], default="ausm")
     ap.add_argument("--cfl", type=float, default=0.4)
     ap.add_argument("--variants", default="fp64,mixed,fp64-rcm,mixed-rcm,fp64-given,mixed-given")
-    ap.add_argument("--cpu-sample-steps", type=int, default=3)
+    ap.add_argument("--cpu-sample-steps", type=int, default=30, help="about 10 s of CPU work on 16 threads")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="bytes; run only a device copy of this size")
