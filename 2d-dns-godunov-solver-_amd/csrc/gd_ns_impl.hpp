@@ -241,18 +241,91 @@ struct FaceGeo {
   }
 };
 
-// ---- pass G: gradients + limiter
+// The face-reconstruction record of cell i: (q, d/dx d/dy of rho,u,v,p,
+// limiter of each), as 4 groups of 4 values in the order stored.  Every load
+// first: neighbour ids, then all neighbour states (boundary and absent faces
+// read the own cell, overwritten below), then geometry.
 template <class T, bool QUADS>
-__global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
+__device__ __forceinline__ void cell_record(const NsKArgs<T>& a, int64_t i, T* r) {
   constexpr int NF = QUADS ? 4 : 3;
   using G = FaceGeo<T>;
   const MeshView& m = a.m;
   const int64_t nc = m.nc;
-  // blocks 0 and 1 of a folding launch close the previous step's reductions
+  const int nf = QUADS ? m.nv[i] : 3;
+  int32_t jn[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) jn[k] = m.nb[k * nc + i];
+  T qi[4], qk[NF][4];
+  ld4(a.Q + 4 * i, qi);
+#pragma unroll
+  for (int k = 0; k < NF; ++k) ld4(a.Q + 4 * (int64_t)(jn[k] >= 0 ? jn[k] : i), qk[k]);
+  T wx[NF], wy[NF], rx[NF], ry[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    wx[k] = G::lsq(m, k, 0, i); wy[k] = G::lsq(m, k, 1, i);
+    rx[k] = G::rf(m, k, 0, i); ry[k] = G::rf(m, k, 1, i);
+  }
+  const T kh = (T)a.prm.venkat_K * G::hmin(m, i);
+  T gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0}, qmx[4], qmn[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) { qmx[v] = qi[v]; qmn[v] = qi[v]; }
+#pragma unroll
+  for (int k = 0; k < NF; ++k) {
+    if (k < nf) {
+      if (jn[k] < 0) {
+        const int64_t s = k * nc + i;
+        ghost<T>(a.prm, jn[k] + 1, qi, G::nx(m, s), G::ny(m, s), qk[k]);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const T d = qk[k][v] - qi[v];
+        gx[v] += wx[k] * d;
+        gy[v] += wy[k] * d;
+        qmx[v] = fmax(qmx[v], qk[k][v]);
+        qmn[v] = fmin(qmn[v], qk[k][v]);
+      }
+    }
+  }
+  const T kh3 = kh * kh * kh;
+  T ph[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const T e2 = kh3 * (T)(a.prm.venkat_scale[v] * a.prm.venkat_scale[v]);
+    T phi = 1;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      if (k < nf) {
+        const T d2 = gx[v] * rx[k] + gy[v] * ry[k];
+        if (d2 != 0) {
+          const T d1 = d2 > 0 ? qmx[v] - qi[v] : qmn[v] - qi[v];
+          const T psi = ((d1 * d1 + e2) * d2 + 2 * d2 * d2 * d1) / (d1 * d1 + 2 * d2 * d2 + d1 * d2 + e2) / d2;
+          phi = fmin(phi, psi);
+        }
+      }
+    }
+    ph[v] = fmax(phi, (T)0);
+  }
+  r[0] = qi[0]; r[1] = qi[1]; r[2] = qi[2]; r[3] = qi[3];
+  r[4] = gx[0]; r[5] = gy[0]; r[6] = gx[1]; r[7] = gy[1];
+  r[8] = gx[2]; r[9] = gy[2]; r[10] = gx[3]; r[11] = gy[3];
+  r[12] = ph[0]; r[13] = ph[1]; r[14] = ph[2]; r[15] = ph[3];
+}
+
+// blocks 0 and 1 of a folding launch close the previous step's reductions
+template <class T>
+__device__ __forceinline__ void fold_blocks(const NsKArgs<T>& a) {
+  if (blockIdx.x == 0) fold_acc_block(a.part, a.nparts, &a.acc[a.step % 3]);
+  else if (a.wall_f) wall_sum_block(a.wall_f, a.m.n_wall, a.drag);
+}
+
+// ---- pass G: gradients + limiter
+template <class T, bool QUADS>
+__global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
+  const MeshView& m = a.m;
+  const int64_t nc = m.nc;
   const unsigned nx = a.fold_prev ? 2u : 0u;
   if (blockIdx.x < nx) {
-    if (blockIdx.x == 0) fold_acc_block(a.part, a.nparts, &a.acc[a.step % 3]);
-    else if (a.wall_f) wall_sum_block(a.wall_f, m.n_wall, a.drag);
+    fold_blocks(a);
     return;
   }
   const int64_t base = (int64_t)xcd_block(blockIdx.x - nx, gridDim.x - nx) * kBlock;
@@ -262,67 +335,13 @@ __global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
   // leave every store instruction scattered over 64 partial lines)
   __shared__ T sRec[kBlock * 16];
   if (i < nc) {
-    const int nf = QUADS ? m.nv[i] : 3;
-    // every load first: neighbour ids, then all neighbour states (boundary and
-    // absent faces read the own cell, overwritten below), then geometry
-    int32_t jn[NF];
-#pragma unroll
-    for (int k = 0; k < NF; ++k) jn[k] = m.nb[k * nc + i];
-    T qi[4], qk[NF][4];
-    ld4(a.Q + 4 * i, qi);
-#pragma unroll
-    for (int k = 0; k < NF; ++k) ld4(a.Q + 4 * (int64_t)(jn[k] >= 0 ? jn[k] : i), qk[k]);
-    T wx[NF], wy[NF], rx[NF], ry[NF];
-#pragma unroll
-    for (int k = 0; k < NF; ++k) {
-      wx[k] = G::lsq(m, k, 0, i); wy[k] = G::lsq(m, k, 1, i);
-      rx[k] = G::rf(m, k, 0, i); ry[k] = G::rf(m, k, 1, i);
-    }
-    const T kh = (T)a.prm.venkat_K * G::hmin(m, i);
-    T gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0}, qmx[4], qmn[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) { qmx[v] = qi[v]; qmn[v] = qi[v]; }
-#pragma unroll
-    for (int k = 0; k < NF; ++k) {
-      if (k < nf) {
-        if (jn[k] < 0) {
-          const int64_t s = k * nc + i;
-          ghost<T>(a.prm, jn[k] + 1, qi, G::nx(m, s), G::ny(m, s), qk[k]);
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const T d = qk[k][v] - qi[v];
-          gx[v] += wx[k] * d;
-          gy[v] += wy[k] * d;
-          qmx[v] = fmax(qmx[v], qk[k][v]);
-          qmn[v] = fmin(qmn[v], qk[k][v]);
-        }
-      }
-    }
-    const T kh3 = kh * kh * kh;
-    T ph[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const T e2 = kh3 * (T)(a.prm.venkat_scale[v] * a.prm.venkat_scale[v]);
-      T phi = 1;
-#pragma unroll
-      for (int k = 0; k < NF; ++k) {
-        if (k < nf) {
-          const T d2 = gx[v] * rx[k] + gy[v] * ry[k];
-          if (d2 != 0) {
-            const T d1 = d2 > 0 ? qmx[v] - qi[v] : qmn[v] - qi[v];
-            const T psi = ((d1 * d1 + e2) * d2 + 2 * d2 * d2 * d1) / (d1 * d1 + 2 * d2 * d2 + d1 * d2 + e2) / d2;
-            phi = fmin(phi, psi);
-          }
-        }
-      }
-      ph[v] = fmax(phi, (T)0);
-    }
-    T* r = sRec + 16 * threadIdx.x;
-    st4(r, qi[0], qi[1], qi[2], qi[3]);
-    st4(r + 4, gx[0], gy[0], gx[1], gy[1]);
-    st4(r + 8, gx[2], gy[2], gx[3], gy[3]);
-    st4(r + 12, ph[0], ph[1], ph[2], ph[3]);
+    T r[16];
+    cell_record<T, QUADS>(a, i, r);
+    T* o = sRec + 16 * threadIdx.x;
+    st4(o, r[0], r[1], r[2], r[3]);
+    st4(o + 4, r[4], r[5], r[6], r[7]);
+    st4(o + 8, r[8], r[9], r[10], r[11]);
+    st4(o + 12, r[12], r[13], r[14], r[15]);
   }
   __syncthreads();
   const int64_t nvalid = nc - base < kBlock ? nc - base : kBlock;
@@ -431,26 +450,14 @@ struct RecGather {
   }
 };
 
-// ---- pass F1: one flux per face
+// The flux of face f from its owner's and neighbour's records (r1 = r0 at a
+// boundary, c1 = the neighbour or code - 1): reconstruction of both sides,
+// AUSM+ or Roe, the viscous flux, stored with the wall force.
 template <class T, bool VISC>
-__global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
+__device__ __forceinline__ void face_flux(const NsKArgs<T>& a, int64_t f, int32_t c1, const T* r0, const T* r1) {
   using G = FaceGeo<T>;
   const MeshView& m = a.m;
   const gd_params& P = a.prm;
-  const int64_t nF = m.n_faces;
-  const int64_t f0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  // lanes past the end gather the last face's records and store nothing, so
-  // that every lane of a wave takes part in the cooperative gather
-  const bool live = f0 < nF;
-  const int64_t f = live ? f0 : nF - 1;
-  const int32_t c0 = m.fc0[f], c1 = m.fc1[f];
-  // both records (a boundary face reads the owner's twice)
-  __shared__ typename RecGather<T, 2>::Shared sh;
-  T rr[2][16];
-  RecGather<T, 2>::gather(sh, a.rec, {c0, c1 >= 0 ? c1 : c0}, rr);
-  if (!live) return;
-  const T* r0 = rr[0];
-  const T* r1 = rr[1];
   const T nx = G::fnx(m, f), ny = G::fny(m, f);
   const T len = G::flen(m, f);
   const T rx = G::fr0(m, 0, f), ry = G::fr0(m, 1, f);
@@ -533,6 +540,25 @@ __global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
     const int32_t wid = m.fwall[f];
     if (wid >= 0) { a.wall_f[2 * wid] = fk[1]; a.wall_f[2 * wid + 1] = fk[2]; }
   }
+}
+
+// ---- pass F1: one flux per face
+template <class T, bool VISC>
+__global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
+  const MeshView& m = a.m;
+  const int64_t nF = m.n_faces;
+  const int64_t f0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  // lanes past the end gather the last face's records and store nothing, so
+  // that every lane of a wave takes part in the cooperative gather
+  const bool live = f0 < nF;
+  const int64_t f = live ? f0 : nF - 1;
+  const int32_t c0 = m.fc0[f], c1 = m.fc1[f];
+  // both records (a boundary face reads the owner's twice)
+  __shared__ typename RecGather<T, 2>::Shared sh;
+  T rr[2][16];
+  RecGather<T, 2>::gather(sh, a.rec, {c0, c1 >= 0 ? c1 : c0}, rr);
+  if (!live) return;
+  face_flux<T, VISC>(a, f, c1, rr[0], rr[1]);
 }
 
 // ---- pass F2: per-cell flux sum, update, primitives, dt inputs
