@@ -80,12 +80,12 @@ def run_variant(mesh, case, precision, flux, order_dev, U0, warmup, steps, cfl):
             "nonphysical": int(st.nonphysical)}
 
 
-def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl):
-    """Discrete adjoint of the summed wall drag over ``steps`` FP32-mixed
-    steps (fixed dt, checkpoints every ``ckpt``) against the same forward
+def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl, precision=L.GD_FP32_MIXED):
+    """Discrete adjoint of the summed wall drag over ``steps`` steps (FP32-mixed
+    by default; fixed dt, checkpoints every ``ckpt``) against the same forward
     steps alone, from the state after 20 CFL steps; then with the records
     of every segment kept in HBM."""
-    prm = case.params(precision=L.GD_FP32_MIXED)
+    prm = case.params(precision=precision)
     sol = S.GodunovSolver(mesh, prm, order=order_dev)
     sol.state.set_array(L.GD_ARR_U_OLD, U0)
     st = sol.run(20, cfl=cfl, dt_policy=L.GD_DT_LOCAL, wall_code=-3)
@@ -119,7 +119,8 @@ def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl):
     adj_h, g0, _, _ = timed(checkpoint_every=ckpt)
     _, g1, _, _ = timed(checkpoint_every=ckpt, record_budget=64 << 30)
     sol.close()
-    return {"steps": steps, "checkpoint_every": ckpt, "precision": "fp32-mixed", "dt": dt,
+    return {"steps": steps, "checkpoint_every": ckpt,
+            "precision": "fp32-mixed" if precision == L.GD_FP32_MIXED else "fp64", "dt": dt,
             "forward_ms": round(fwd * 1e3, 3), "adjoint_ms": round(adj * 1e3, 3),
             "cost_vs_forward": round(adj / fwd, 3),
             "adjoint_ms_with_host_copy": round(adj_h * 1e3, 3),
@@ -209,6 +210,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, help="bytes; run only a device copy of this size")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="0: skip the adjoint measurement")
     ap.add_argument("--adjoint-checkpoint", type=int, default=20)
+    ap.add_argument("--adjoint-precision", choices=["mixed", "fp64"], default="mixed")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU path is not built",
@@ -286,9 +288,10 @@ def main():
         out[var.replace("-", "_")] = d
     out["drag_fp64"] = r64["drag"]
     if a.adjoint_steps > 0:
-        out["adjoint"] = adjoint_cost(mesh, case, dev_order, U0, a.adjoint_steps, a.adjoint_checkpoint, a.cfl)
+        out["adjoint"] = adjoint_cost(mesh, case, dev_order, U0, a.adjoint_steps, a.adjoint_checkpoint, a.cfl,
+                                      L.GD_FP64 if a.adjoint_precision == "fp64" else L.GD_FP32_MIXED)
         ad = out["adjoint"]
-        log(f"adjoint: {ad['steps']} mixed steps, checkpoints every {ad['checkpoint_every']}: "
+        log(f"adjoint: {ad['steps']} {ad['precision']} steps, checkpoints every {ad['checkpoint_every']}: "
             f"{ad['adjoint_ms']:.1f} ms vs forward {ad['forward_ms']:.1f} ms -> {ad['cost_vs_forward']:.2f}x "
             f"(forward sweep {ad['adjoint_forward_sweep_ms']:.1f} ms, reverse {ad['adjoint_reverse_ms']:.1f} ms = "
             f"{ad['reverse_step_vs_forward_step']:.2f}x per step); "
