@@ -442,6 +442,10 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
     }
   }
   const int64_t nfaces = (int64_t)h_fc0.size();
+  if (2 * nfaces + 1 > (int64_t)INT32_MAX) {  // cface holds 2 f + side in int32
+    delete M;
+    return fail(GD_ERR_INVALID, "too many faces");
+  }
   for (int64_t c = 0; c < nc; ++c)
     for (int k = 0; k < h_nv[c]; ++k)
       if (h_cface[(int64_t)k * nc + c] < 0) {
