@@ -3,9 +3,10 @@
 // Geometry is read as FP64, the face-flux sum, the update and the p/T
 // recovery are FP64 in both forms.
 //
-// Two launches per step, both cell-centric gathers without atomics on the
-// state (deterministic, numbering independent):
-//   k_ns_grad    least-squares gradient of (rho,u,v,p) + Venkatakrishnan limiter
+// Three launches per step, gathers without atomics on the state
+// (deterministic, numbering independent):
+//   k_ns_grad    per cell: least-squares gradient of (rho,u,v,p) +
+//                Venkatakrishnan limiter, stored as the cell's record
 //   k_ns_face    one flux per face (owner = lower caller id): reconstruction of
 //                both sides, AUSM+/Roe, viscous stress/heat flux, wall force
 //   k_ns_update  per cell: sum of +-face fluxes in slot order (FP64), explicit
