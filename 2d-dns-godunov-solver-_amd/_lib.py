@@ -32,7 +32,7 @@ dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int32)
 
 EXPORTS = (
-    "gd_mesh_create", "gd_mesh_info_get", "gd_mesh_destroy", "gd_params_default", "gd_state_create",
+    "gd_mesh_create", "gd_mesh_info_get", "gd_mesh_cell_order", "gd_mesh_destroy", "gd_params_default", "gd_state_create",
     "gd_state_destroy", "gd_state_set_params", "gd_state_set_fields", "gd_state_get_fields", "gd_state_set_array",
     "gd_state_get_array", "gd_calc_one_stage", "gd_calc_diff_term", "gd_update_variables", "gd_cells2nodes",
     "gd_run", "gd_sync", "gd_state_device_u", "gd_device_count", "gd_last_error", "gd_version",
@@ -115,6 +115,7 @@ def load():
     vp = C.c_void_p
     L.gd_mesh_create.argtypes = [C.POINTER(gd_mesh_desc), C.c_int32, C.POINTER(C.c_int64), C.POINTER(vp)]
     L.gd_mesh_info_get.argtypes = [vp, C.POINTER(gd_mesh_info)]
+    L.gd_mesh_cell_order.argtypes = [vp, C.POINTER(C.c_int64)]
     L.gd_mesh_destroy.argtypes = [vp]
     L.gd_params_default.argtypes = [C.POINTER(gd_params)]
     L.gd_params_default.restype = None
@@ -216,6 +217,12 @@ class DeviceMesh:
         inf = gd_mesh_info()
         check(load().gd_mesh_info_get(self.handle, C.byref(inf)))
         return inf
+
+    def cell_order(self) -> np.ndarray:
+        """new_to_old: the caller's 0-based id of each device cell."""
+        out = np.zeros(self.info().n_cells, dtype=np.int64)
+        check(load().gd_mesh_cell_order(self.handle, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
 
     def close(self):
         if self.handle:

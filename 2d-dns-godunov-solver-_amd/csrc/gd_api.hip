@@ -558,6 +558,12 @@ int gd_mesh_info_get(const gd_mesh* m, gd_mesh_info* info) {
   return GD_OK;
 }
 
+int gd_mesh_cell_order(const gd_mesh* m, int64_t* new_to_old) {
+  if (!m || !new_to_old) return fail(GD_ERR_INVALID, "null argument");
+  std::copy(m->cperm.begin(), m->cperm.end(), new_to_old);
+  return GD_OK;
+}
+
 int gd_mesh_destroy(gd_mesh* m) {
   if (!m) return GD_OK;
   for (void* p : m->allocs) (void)hipFree(p);

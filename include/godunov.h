@@ -81,6 +81,11 @@ typedef struct {
 
 int gd_mesh_create(const gd_mesh_desc* desc, int32_t order, const int64_t* perm_new_to_old, gd_mesh** out);
 int gd_mesh_info_get(const gd_mesh* mesh, gd_mesh_info* info);
+/* The device cell order, for callers that work on the device arrays
+   (gd_state_device_u, gd_state_device_adjoint): new_to_old[d] = the caller's
+   0-based id of device cell d (n_cells entries).  No counterpart in the
+   reference, which has one numbering. */
+int gd_mesh_cell_order(const gd_mesh* mesh, int64_t* new_to_old);
 int gd_mesh_destroy(gd_mesh* mesh);
 
 /* ---- numerics ---- */

@@ -233,3 +233,21 @@ def test_ns_residual_norms(order):
     prev = s.conservative()
     s.close()
     assert np.allclose(np.array(st2.residuals[:]), ((last - prev) ** 2).sum(axis=0), rtol=1e-12, atol=0)
+
+
+def test_mesh_cell_order_reports_the_device_layout():
+    """gd_mesh_cell_order: identity as given, the caller's map for a caller
+    permutation, a permutation for RCM and Hilbert."""
+    L = pkg()._lib
+    M = pkg().mesh
+    mesh = _box("mixed", 30, 20)
+    n = mesh.nCells
+    perm = M.random_permutation(n, SEED)
+    for order, want in ((L.GD_ORDER_AS_GIVEN, np.arange(n)), (L.GD_ORDER_PERMUTATION, perm),
+                        (L.GD_ORDER_RCM, None), (L.GD_ORDER_HILBERT, None)):
+        dm = L.DeviceMesh(mesh, order, perm if order == L.GD_ORDER_PERMUTATION else None)
+        got = dm.cell_order()
+        dm.close()
+        assert np.array_equal(np.sort(got), np.arange(n))
+        if want is not None:
+            assert np.array_equal(got, want)
