@@ -419,3 +419,62 @@ def test_ns_isothermal_wall_at_rest():
     assert np.max(np.abs(UN - U)) < 1e-14
     assert F[0] == pytest.approx(0.0, abs=1e-13)
     assert F[1] == pytest.approx(-p0 * 2.0, rel=1e-13)
+
+
+# ------------------------------------------------------ mirror symmetry
+def _diagonal_mirror(mesh):
+    """Cell and node maps of the reflection (x, y) -> (y, x), which carries the
+    SW-NE split periodic square onto itself."""
+    c = np.asarray(mesh.cell_mid_points)
+    key = {(round(x, 9), round(y, 9)): i for i, (x, y) in enumerate(c)}
+    cmap = np.array([key[(round(y, 9), round(x, 9))] for x, y in c])
+    assert np.array_equal(np.sort(cmap), np.arange(mesh.nCells))
+    return cmap
+
+
+def _mirrored(cmap, r, u, v, p):
+    return r[cmap], v[cmap], u[cmap], p[cmap]
+
+
+def _mirror_U(U, cmap):
+    W = U[cmap].copy()
+    W[:, [1, 2]] = W[:, [2, 1]]
+    return W
+
+
+def test_steps_commute_with_the_diagonal_mirror():
+    """Reflecting the state across y = x (u <-> v) and stepping equals stepping
+    and reflecting, to rounding, for the reference step (MUSCL with node
+    stencils, the calcDiffTerm term) and the ns step with AUSM+ and Roe.  A
+    transposed normal component or velocity (p nx in the y-momentum, u for v)
+    keeps free-stream preservation and conservation but breaks this."""
+    mesh = M().make_tgv_periodic_mesh(16)
+    cmap = _diagonal_mirror(mesh)
+    x, y = mesh.cell_mid_points[:, 0], mesh.cell_mid_points[:, 1]
+    r = 1 + 0.2 * np.sin(x + 2 * y)
+    u = 0.3 * np.cos(2 * x - y)
+    v = 0.1 + 0.2 * np.sin(y)
+    p = 1 + 0.1 * np.cos(x) * np.sin(y)
+    mv = O.MeshView(mesh)
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+
+    def fields(q):
+        f = M().fields2d.zeros(mesh.nCells, mesh.nNodes)
+        f.densityCells[:], f.UxCells[:], f.UyCells[:], f.pressureCells[:] = q
+        return f
+
+    def ref_step(f):
+        U = O.phs2dcns(f, G, mesh.nCells)
+        O.cells2nodes(mv, f, U)
+        po = O.params(energy_law=O.ENERGY_CONST, mu_const=1e-2, k_const=1.0)
+        Ud = O.calc_diff_term(mv, f, th, po, U)[0]
+        return O.calc_one_stage_muscl_cpu_fast(mv, f, th, po, U, Ud, 1.0, 1e-3, 0.0)[1]
+
+    f1, f2 = fields((r, u, v, p)), fields(_mirrored(cmap, r, u, v, p))
+    a, b = ref_step(f1), ref_step(f2)
+    assert np.abs(b - _mirror_U(a, cmap)).max() <= 1e-13 * np.abs(a).max()
+    U1, U2 = O.phs2dcns(f1, G, mesh.nCells), O.phs2dcns(f2, G, mesh.nCells)
+    for flux in (O.FLUX_AUSM_PLUS, O.FLUX_ROE):
+        prm = O.ns_params(gamma=G, R=1.0, mu_ref=2e-3, T_ref=1.0, S=0.4, Pr=0.72, venkat_K=3.0, flux=flux)
+        a, b = O.ns_step(mv, prm, 1e-3, U1)[0], O.ns_step(mv, prm, 1e-3, U2)[0]
+        assert np.abs(b - _mirror_U(a, cmap)).max() <= 1e-13 * np.abs(a).max()
