@@ -478,3 +478,90 @@ def test_steps_commute_with_the_diagonal_mirror():
         prm = O.ns_params(gamma=G, R=1.0, mu_ref=2e-3, T_ref=1.0, S=0.4, Pr=0.72, venkat_K=3.0, flux=flux)
         a, b = O.ns_step(mv, prm, 1e-3, U1)[0], O.ns_step(mv, prm, 1e-3, U2)[0]
         assert np.abs(b - _mirror_U(a, cmap)).max() <= 1e-13 * np.abs(a).max()
+
+
+def test_ns_venkatakrishnan_known_answer_on_a_linear_field():
+    """K = 0 (eps = 0) and rho linear in x: the least-squares gradient is exact,
+    so for every interior cell phi_rho is the minimum over its faces of
+    Venkatakrishnan's phi(y) = (y^2 + 2y)/(y^2 + y + 2), y = D+/D-, with
+    D- = grad . (face midpoint - centroid) and D+ = max(neighbours) - q if
+    D- > 0 else min(neighbours) - q (Venkatakrishnan 1993), capped at 1."""
+    mesh = M().make_box_mesh(7, 6, 1.0, 1.0, quads="none")
+    n = mesh.nCells
+    c = np.asarray(mesh.cell_mid_points)
+    xn, yn = np.asarray(mesh.xNodes).ravel(), np.asarray(mesh.yNodes).ravel()
+    b = 0.1
+    rho = 1.0 + b * c[:, 0]
+    U = np.zeros((n, 4), order="F")
+    U[:, 0] = rho; U[:, 1] = 0.3 * rho; U[:, 2] = 0.1 * rho; U[:, 3] = 1.0 / (G - 1) + 0.5 * rho * 0.1
+    gas = {k: v for k, v in NS_GAS.items() if k != "venkat_K"}
+    _, g8, p4, _ = O.ns_step(O.MeshView(mesh), O.ns_params(venkat_K=0.0, **gas), 1e-4, U, want_grad=True)
+    nb = np.asarray(mesh.cell_stiffness)[:, :3].astype(np.int64)
+    c2n = np.asarray(mesh.cells2nodes).astype(np.int64)
+    checked = 0
+    for i in range(n):
+        if np.any(nb[i] <= 0):
+            continue  # boundary cell
+        assert abs(g8[i, 0] - b) < 1e-12 and abs(g8[i, 1]) < 1e-12  # exact gradient of rho
+        qn = rho[nb[i] - 1]
+        phi = 1.0
+        for k in range(3):
+            a, e = c2n[i, 2 * k] - 1, c2n[i, 2 * k + 1] - 1
+            dm = b * (0.5 * (xn[a] + xn[e]) - c[i, 0])
+            if dm == 0:
+                continue
+            dp = (qn.max() if dm > 0 else qn.min()) - rho[i]
+            y = dp / dm
+            phi = min(phi, (y * y + 2 * y) / (y * y + y + 2))
+        assert abs(p4[i, 0] - phi) <= 1e-12, (i, p4[i, 0], phi)
+        checked += 1
+    assert checked > 20
+
+
+def test_ns_viscous_stress_and_heat_flux_known_answers():
+    """Against the continuum operator on interior cells of a triangulated box:
+    heat conduction at rest with uniform p and T = 1 + c x^2 gives
+    dE/dt = d/dx(kappa dT/dx), kappa = mu(T) cp / Pr with Sutherland's mu,
+    and no mass or momentum change; at uniform T the viscous part of the
+    momentum change (the viscous step minus the inviscid one) is mu d2u/dy2 = 2 a mu for
+    u = a y^2 and d/dx(mu (2 du/dx - 2/3 div u)) = 8/3 a mu for u = a x^2."""
+    g, Rg, Pr, mu_ref, T_ref, S = 1.4, 1.0, 0.72, 2e-3, 1.0, 0.4
+    cp = g * Rg / (g - 1)
+
+    def mu(T):
+        return mu_ref * (T / T_ref) ** 1.5 * (T_ref + S) / (T + S)
+
+    mesh = M().make_box_mesh(32, 32, 1.0, 1.0, quads="none")
+    mv = O.MeshView(mesh)
+    x, y = mesh.cell_mid_points[:, 0], mesh.cell_mid_points[:, 1]
+    inner = (x > 0.2) & (x < 0.8) & (y > 0.2) & (y < 0.8)
+    bcs = {q: (O.BC_EXTRAPOLATE, ()) for q in (-1, -2, -3, -4)}
+    dt = 1e-6
+
+    def rate(U, viscous=True):
+        prm = O.ns_params(gamma=g, R=Rg, mu_ref=mu_ref, T_ref=T_ref, S=S, Pr=Pr, venkat_K=3.0, bcs=bcs,
+                          viscous=viscous)
+        return (O.ns_step(mv, prm, dt, U)[0] - U) / dt
+
+    # heat conduction
+    c = 0.5
+    T = 1.0 + c * x * x
+    U = np.zeros((mesh.nCells, 4), order="F")
+    U[:, 0] = 1.0 / (Rg * T)
+    U[:, 3] = 1.0 / (g - 1)
+    r = rate(U)
+    h = 1e-6
+    kap, dkap = mu(T) * cp / Pr, (mu(T + h) - mu(T - h)) / (2 * h) * cp / Pr
+    want = kap * 2 * c + dkap * (2 * c * x) ** 2
+    assert np.abs(r[inner, 3] - want[inner]).max() <= 2e-3 * np.abs(want[inner]).max()
+    assert np.abs(r[inner, :3]).max() == 0.0
+    # viscous stresses at T = T_ref (mu = mu_ref)
+    a = 0.2
+    for u, want_x in ((a * y * y, 2 * a * mu_ref), (a * x * x, 8.0 / 3.0 * a * mu_ref)):
+        U = np.zeros((mesh.nCells, 4), order="F")
+        U[:, 0] = 1.0
+        U[:, 1] = u
+        U[:, 3] = 1.0 / (g - 1) + 0.5 * u * u
+        dv = rate(U) - rate(U, viscous=False)
+        assert np.abs(dv[inner, 1] - want_x).max() <= 1e-6 * want_x
+        assert np.abs(dv[inner, 2]).max() <= 1e-9 * want_x
