@@ -85,6 +85,8 @@ def lib():
         for fn in (L.orc_ausm_plus_flux, L.orc_ausm_plus_m2020_flux, L.orc_roe_flux):
             fn.restype = None
             fn.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_double, dp]
+        L.orc_ns_ausm_flux.restype = None
+        L.orc_ns_ausm_flux.argtypes = [dp, C.c_double, C.c_double, C.c_double, dp]
         L.orc_air_conductivity.restype = C.c_double
         L.orc_air_conductivity.argtypes = [C.c_double]
         L.orc_sutherland_viscosity.restype = C.c_double
@@ -201,6 +203,14 @@ def ausm_plus_m2020_flux(qL, qR, nx, ny, side=1.0, gamma=1.4):
 
 def roe_flux(qL, qR, nx, ny, side=1.0, gamma=1.4):
     return _flux(lib().orc_roe_flux, qL, qR, nx, ny, side, gamma)
+
+
+def ns_ausm_flux(qL, qR, nx, ny, gamma=1.4):
+    """The ns scheme's AUSM+ per unit length (qL: the neighbour side)."""
+    q = np.ascontiguousarray(np.concatenate([qL, qR]), dtype=np.float64)
+    out = np.zeros(4)
+    lib().orc_ns_ausm_flux(_p(q), nx, ny, gamma, _p(out))
+    return out
 
 
 def air_conductivity(T):

@@ -71,6 +71,7 @@ double orc_limiter(int32_t kind, double x, double y, double c);
 void orc_ausm_plus_flux(const double* q8, double nx, double ny, double side, double gamma, double* flux4);
 void orc_ausm_plus_m2020_flux(const double* q8, double nx, double ny, double side, double gamma, double* flux4);
 void orc_roe_flux(const double* q8, double nx, double ny, double side, double gamma, double* flux4);
+void orc_ns_ausm_flux(const double* q8, double nx, double ny, double gamma, double* flux4);
 double orc_air_conductivity(double T);
 double orc_sutherland_viscosity(double T);
 

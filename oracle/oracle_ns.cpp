@@ -492,4 +492,10 @@ double orc_ns_dt(const orc_mesh* m, const orc_ns_params* p, const double* U, dou
   return cfl * best;
 }
 
+// the ns scheme's AUSM+ alone (q8 = L then R primitives, per unit length;
+// the neighbour side is "L"), for the flux's known-answer tests
+void orc_ns_ausm_flux(const double* q8, double nx, double ny, double gamma, double* flux4) {
+  ausm<double>(q8, q8 + 4, nx, ny, gamma, flux4);
+}
+
 }  // extern "C"

@@ -565,3 +565,27 @@ def test_ns_viscous_stress_and_heat_flux_known_answers():
         dv = rate(U) - rate(U, viscous=False)
         assert np.abs(dv[inner, 1] - want_x).max() <= 1e-6 * want_x
         assert np.abs(dv[inner, 2]).max() <= 1e-9 * want_x
+
+
+@pytest.mark.parametrize("which", ["reference", "ns"])
+def test_ausm_plus_subsonic_known_answer(which):
+    """Liou's split functions at M = 0.5 and 0.2 with beta = 1/8 and
+    alpha = 3/16, evaluated by hand:
+      M+(0.5) = (1.5)^2/4 + (0.25 - 1)^2/8            = 0.6328125
+      M-(0.2) = -(0.8)^2/4 - (0.04 - 1)^2/8           = -0.2752
+      P+(0.5) = (1.5)^2 (1.5)/4 + 3/16 (0.5)(0.5625)  = 0.896484375
+      P-(0.2) = (0.8)^2 (2.2)/4 - 3/16 (0.2)(0.9216)  = 0.31744
+    Both sides share the total enthalpy, so both interface speeds are the
+    critical speed a* and the face Mach numbers are exactly 0.5 and 0.2.
+    (Consistency, upwinding and the |M| = 1 continuity do not see beta or
+    alpha: their terms cancel in M+ + M- and P+ + P- and vanish at |M| = 1.)"""
+    ht = 3.5
+    astar = math.sqrt(2 * (G - 1) / (G + 1) * ht)
+    uL, uR = 0.5 * astar, 0.2 * astar
+    pL, pR = (ht - 0.5 * uL * uL) * (G - 1) / G, (ht - 0.5 * uR * uR) * (G - 1) / G
+    qL, qR = np.array([1.0, uL, 0.0, pL]), np.array([1.0, uR, 0.0, pR])
+    m = 0.6328125 - 0.2752
+    p12 = 0.896484375 * pL + 0.31744 * pR
+    want = -np.array([astar * m, astar * m * uL + p12, 0.0, astar * m * ht])
+    got = O.ausm_plus_flux(qL, qR, 1.0, 0.0) if which == "reference" else O.ns_ausm_flux(qL, qR, 1.0, 0.0)
+    assert np.allclose(got, want, rtol=1e-14, atol=1e-15)
