@@ -87,6 +87,8 @@ def lib():
             fn.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.orc_ns_ausm_flux.restype = None
         L.orc_ns_ausm_flux.argtypes = [dp, C.c_double, C.c_double, C.c_double, dp]
+        L.orc_ns_roe_flux.restype = None
+        L.orc_ns_roe_flux.argtypes = [dp, C.c_double, C.c_double, C.c_double, dp]
         L.orc_air_conductivity.restype = C.c_double
         L.orc_air_conductivity.argtypes = [C.c_double]
         L.orc_sutherland_viscosity.restype = C.c_double
@@ -205,12 +207,21 @@ def roe_flux(qL, qR, nx, ny, side=1.0, gamma=1.4):
     return _flux(lib().orc_roe_flux, qL, qR, nx, ny, side, gamma)
 
 
-def ns_ausm_flux(qL, qR, nx, ny, gamma=1.4):
-    """The ns scheme's AUSM+ per unit length (qL: the neighbour side)."""
+def _ns_flux(fn, qL, qR, nx, ny, gamma):
     q = np.ascontiguousarray(np.concatenate([qL, qR]), dtype=np.float64)
     out = np.zeros(4)
-    lib().orc_ns_ausm_flux(_p(q), nx, ny, gamma, _p(out))
+    fn(_p(q), nx, ny, gamma, _p(out))
     return out
+
+
+def ns_ausm_flux(qL, qR, nx, ny, gamma=1.4):
+    """The ns scheme's AUSM+ per unit length (qL: the neighbour side)."""
+    return _ns_flux(lib().orc_ns_ausm_flux, qL, qR, nx, ny, gamma)
+
+
+def ns_roe_flux(qL, qR, nx, ny, gamma=1.4):
+    """The ns scheme's Roe per unit length (qL: the neighbour side)."""
+    return _ns_flux(lib().orc_ns_roe_flux, qL, qR, nx, ny, gamma)
 
 
 def air_conductivity(T):

@@ -72,6 +72,7 @@ void orc_ausm_plus_flux(const double* q8, double nx, double ny, double side, dou
 void orc_ausm_plus_m2020_flux(const double* q8, double nx, double ny, double side, double gamma, double* flux4);
 void orc_roe_flux(const double* q8, double nx, double ny, double side, double gamma, double* flux4);
 void orc_ns_ausm_flux(const double* q8, double nx, double ny, double gamma, double* flux4);
+void orc_ns_roe_flux(const double* q8, double nx, double ny, double gamma, double* flux4);
 double orc_air_conductivity(double T);
 double orc_sutherland_viscosity(double T);
 

@@ -498,4 +498,9 @@ void orc_ns_ausm_flux(const double* q8, double nx, double ny, double gamma, doub
   ausm<double>(q8, q8 + 4, nx, ny, gamma, flux4);
 }
 
+// the ns scheme's Roe alone (same convention as orc_ns_ausm_flux)
+void orc_ns_roe_flux(const double* q8, double nx, double ny, double gamma, double* flux4) {
+  roe<double>(q8, q8 + 4, nx, ny, gamma, flux4);
+}
+
 }  // extern "C"

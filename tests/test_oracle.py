@@ -480,42 +480,66 @@ def test_steps_commute_with_the_diagonal_mirror():
         assert np.abs(b - _mirror_U(a, cmap)).max() <= 1e-13 * np.abs(a).max()
 
 
-def test_ns_venkatakrishnan_known_answer_on_a_linear_field():
-    """K = 0 (eps = 0) and rho linear in x: the least-squares gradient is exact,
-    so for every interior cell phi_rho is the minimum over its faces of
-    Venkatakrishnan's phi(y) = (y^2 + 2y)/(y^2 + y + 2), y = D+/D-, with
-    D- = grad . (face midpoint - centroid) and D+ = max(neighbours) - q if
-    D- > 0 else min(neighbours) - q (Venkatakrishnan 1993), capped at 1."""
+@pytest.mark.parametrize("field,K", [("linear", 0.0), ("wavy", 0.0), ("wavy", 0.6)])
+def test_ns_venkatakrishnan_known_answer(field, K):
+    """For every interior cell phi_rho is the minimum over its faces of
+    Venkatakrishnan's phi = (y^2 + 2y + e)/(y^2 + y + 2 + e), y = D+/D-,
+    e = eps^2/D-^2, with D- = grad . (face midpoint - centroid),
+    D+ = max(q, neighbours) - q if D- > 0 else min(q, neighbours) - q
+    (Venkatakrishnan 1993), capped at 1, and eps^2 = (K h)^3,
+    h = 2 area / perimeter of the triangle (its inscribed-circle radius),
+    taken here from its vertices.  rho linear in x: the least-squares
+    gradient is exact (and every interior cell is unlimited); a small wave
+    with local extrema limits many cells, D- then uses the oracle's gradient
+    (pinned on linear fields), and K = 0.6 makes e of order one there."""
     mesh = M().make_box_mesh(7, 6, 1.0, 1.0, quads="none")
     n = mesh.nCells
     c = np.asarray(mesh.cell_mid_points)
     xn, yn = np.asarray(mesh.xNodes).ravel(), np.asarray(mesh.yNodes).ravel()
     b = 0.1
-    rho = 1.0 + b * c[:, 0]
+    if field == "linear":
+        rho = 1.0 + b * c[:, 0]
+    else:
+        rho = 1.0 + 0.01 * np.sin(2 * math.pi * c[:, 0]) * np.cos(2 * math.pi * c[:, 1])
     U = np.zeros((n, 4), order="F")
     U[:, 0] = rho; U[:, 1] = 0.3 * rho; U[:, 2] = 0.1 * rho; U[:, 3] = 1.0 / (G - 1) + 0.5 * rho * 0.1
     gas = {k: v for k, v in NS_GAS.items() if k != "venkat_K"}
-    _, g8, p4, _ = O.ns_step(O.MeshView(mesh), O.ns_params(venkat_K=0.0, **gas), 1e-4, U, want_grad=True)
+    _, g8, p4, _ = O.ns_step(O.MeshView(mesh), O.ns_params(venkat_K=K, **gas), 1e-4, U, want_grad=True)
     nb = np.asarray(mesh.cell_stiffness)[:, :3].astype(np.int64)
     c2n = np.asarray(mesh.cells2nodes).astype(np.int64)
-    checked = 0
+    checked, limited, e_lim = 0, 0, 0.0
     for i in range(n):
         if np.any(nb[i] <= 0):
             continue  # boundary cell
-        assert abs(g8[i, 0] - b) < 1e-12 and abs(g8[i, 1]) < 1e-12  # exact gradient of rho
-        qn = rho[nb[i] - 1]
-        phi = 1.0
+        if field == "linear":
+            assert abs(g8[i, 0] - b) < 1e-12 and abs(g8[i, 1]) < 1e-12  # exact gradient of rho
+        qn = np.append(rho[nb[i] - 1], rho[i])
+        vert = np.unique(c2n[i, :6]) - 1
+        assert vert.size == 3
+        vx, vy = xn[vert], yn[vert]
+        area = 0.5 * abs((vx[1] - vx[0]) * (vy[2] - vy[0]) - (vx[2] - vx[0]) * (vy[1] - vy[0]))
+        perim = sum(math.hypot(vx[k] - vx[k - 1], vy[k] - vy[k - 1]) for k in range(3))
+        eps2 = (K * 2 * area / perim) ** 3
+        phi, e_at = 1.0, 0.0
         for k in range(3):
             a, e = c2n[i, 2 * k] - 1, c2n[i, 2 * k + 1] - 1
-            dm = b * (0.5 * (xn[a] + xn[e]) - c[i, 0])
+            dm = g8[i, 0] * (0.5 * (xn[a] + xn[e]) - c[i, 0]) + g8[i, 1] * (0.5 * (yn[a] + yn[e]) - c[i, 1])
             if dm == 0:
                 continue
             dp = (qn.max() if dm > 0 else qn.min()) - rho[i]
-            y = dp / dm
-            phi = min(phi, (y * y + 2 * y) / (y * y + y + 2))
+            y, ee = dp / dm, eps2 / (dm * dm)
+            f = (y * y + 2 * y + ee) / (y * y + y + 2 + ee)
+            if f < phi:
+                phi, e_at = f, ee
         assert abs(p4[i, 0] - phi) <= 1e-12, (i, p4[i, 0], phi)
         checked += 1
+        if phi < 0.99:
+            limited += 1
+            e_lim = max(e_lim, e_at)
     assert checked > 20
+    if field == "wavy":
+        assert limited >= 5
+        assert K == 0 or e_lim > 0.3
 
 
 def test_ns_viscous_stress_and_heat_flux_known_answers():
@@ -589,3 +613,182 @@ def test_ausm_plus_subsonic_known_answer(which):
     want = -np.array([astar * m, astar * m * uL + p12, 0.0, astar * m * ht])
     got = O.ausm_plus_flux(qL, qR, 1.0, 0.0) if which == "reference" else O.ns_ausm_flux(qL, qR, 1.0, 0.0)
     assert np.allclose(got, want, rtol=1e-14, atol=1e-15)
+
+
+# ------------------------------------------------ Roe in matrix form
+def _euler_flux_n(Uc, nx, ny, g=G):
+    """Exact flux F(U) . n of the conservative state U (complex-safe)."""
+    r, mx, my, E = Uc
+    u, v = mx / r, my / r
+    p = (g - 1) * (E - 0.5 * r * (u * u + v * v))
+    un = u * nx + v * ny
+    return np.array([r * un, mx * un + p * nx, my * un + p * ny, (E + p) * un])
+
+
+def _roe_matrix_flux(qL, qR, nx, ny, width, relative=True, g=G):
+    """0.5 (F_L + F_R) - 0.5 |A| (U_R - U_L) with A the flux Jacobian at Roe's
+    average state (u, v, H averaged with weights sqrt(rho); A depends on u, v,
+    H only) taken by a complex-step derivative of the exact flux, and |A| from
+    numpy's eigendecomposition, the two acoustic eigenvalues smoothed below
+    d = width (times Roe's sound speed if relative) by Harten's
+    0.5 (l^2/d + d).  Returns (flux, whether the smoothing was active)."""
+    def ht(q):
+        return g / (g - 1) * q[3] / q[0] + 0.5 * (q[1] ** 2 + q[2] ** 2)
+    w = math.sqrt(qR[0] / qL[0])
+    u, v, H = ((a + w * b) / (1 + w) for a, b in ((qL[1], qR[1]), (qL[2], qR[2]), (ht(qL), ht(qR))))
+    k2 = 0.5 * (u * u + v * v)
+    a = math.sqrt((g - 1) * (H - k2))
+    rho = math.sqrt(qL[0] * qR[0])
+    Ub = cons((rho, u, v, (g - 1) / g * rho * (H - k2)), g).astype(complex)
+    J = np.zeros((4, 4))
+    h = 1e-30
+    for k in range(4):
+        d = Ub.copy(); d[k] += 1j * h
+        J[:, k] = _euler_flux_n(d, nx, ny, g).imag / h
+    lam, V = np.linalg.eig(J)
+    un = u * nx + v * ny
+    mag, fixed = np.abs(lam.real), False
+    delta = width * a if relative else width
+    for k in range(4):
+        if abs(lam[k].real - un) > 0.5 * a and mag[k] < delta:
+            mag[k] = 0.5 * (mag[k] ** 2 / delta + delta)
+            fixed = True
+    absA = (V @ np.diag(mag) @ np.linalg.inv(V)).real
+    UL, UR = cons(qL, g), cons(qR, g)
+    F = 0.5 * (_euler_flux_n(UL, nx, ny, g) + _euler_flux_n(UR, nx, ny, g)) - 0.5 * absA @ (UR - UL)
+    return F, fixed
+
+
+def _roe_cases():
+    rng = np.random.default_rng(SEED)
+    out = []
+    for _ in range(12):
+        qL = np.array([rng.uniform(0.5, 2), rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(0.5, 3)])
+        qR = np.array([rng.uniform(0.5, 2), rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(0.5, 3)])
+        t = rng.uniform(0, 2 * math.pi)
+        out.append((qL, qR, (math.cos(t), math.sin(t))))
+    # near-sonic pair (un - a ~ 0.03 a): Harten's smoothing is active
+    out.append((np.array([1.0, 1.05, 0.2, 1 / 1.4]), np.array([0.9, 0.95, 0.25, 0.85 / 1.4]), (1.0, 0.0)))
+    qL, qR, n = oblique_shock_pair()
+    out.append((qL, qR, n))
+    return out
+
+
+def test_ns_roe_matches_the_matrix_form():
+    """The ns Roe (Harten width 0.2 of Roe's sound speed) against the
+    eigendecomposition of the Roe matrix, on random states, a near-sonic pair
+    and a stationary oblique shock."""
+    n_fixed = 0
+    for qL, qR, n in _roe_cases():
+        want, fixed = _roe_matrix_flux(qL, qR, n[0], n[1], 0.2)
+        n_fixed += fixed
+        got = O.ns_roe_flux(qL, qR, n[0], n[1], G)
+        scale = np.abs(want).max()
+        assert np.allclose(-got, want, rtol=1e-10, atol=1e-11 * scale), (qL, qR, n, got, want)
+    assert n_fixed >= 2
+
+
+def test_reference_roe_matches_the_matrix_form():
+    """The reference Roe is the same matrix form with an absolute entropy-fix
+    width 1/5 (RoeFlux2dFast.jl:85-96), scaled by the face length."""
+    n_fixed = 0
+    for qL, qR, n in _roe_cases():
+        want, fixed = _roe_matrix_flux(qL, qR, n[0], n[1], 0.2, relative=False)
+        n_fixed += fixed
+        got = O.roe_flux(qL, qR, n[0], n[1], 1.3, G)
+        scale = np.abs(want).max()
+        assert np.allclose(-got / 1.3, want, rtol=1e-10, atol=1e-11 * scale), (qL, qR, n, got, want)
+    assert n_fixed >= 2
+
+
+def test_ns_roe_stationary_shock_known_answer():
+    """Only the wave with ũn - ã = 0 carries an exact stationary jump, and
+    Harten's smoothing with width 0.2 ã turns |0| into 0.1 ã:
+    F = F(U_L) - 0.05 ã (U_R - U_L), ã from Roe's averages."""
+    qL, qR, n = oblique_shock_pair()
+    w = math.sqrt(qR[0] / qL[0])
+    hL = G / (G - 1) * qL[3] / qL[0] + 0.5 * (qL[1] ** 2 + qL[2] ** 2)
+    hR = G / (G - 1) * qR[3] / qR[0] + 0.5 * (qR[1] ** 2 + qR[2] ** 2)
+    u, v, H = (qL[1] + w * qR[1]) / (1 + w), (qL[2] + w * qR[2]) / (1 + w), (hL + w * hR) / (1 + w)
+    a = math.sqrt((G - 1) * (H - 0.5 * (u * u + v * v)))
+    assert abs(u * n[0] + v * n[1] - a) < 1e-9 * a
+    want = -(exact_flux(qL, *n) - 0.05 * a * (cons(qR) - cons(qL)))
+    assert np.allclose(O.ns_roe_flux(qL, qR, n[0], n[1], G), want, rtol=1e-11, atol=1e-9)
+
+
+# ------------------------------------------------ ns boundaries and time step
+@pytest.mark.parametrize("flux", ["ausm", "roe"])
+@pytest.mark.parametrize("viscous", [False, True])
+def test_ns_slip_walls_conserve_mass_and_energy(flux, viscous):
+    """A closed box of slip walls: the mirrored ghost makes the wall carry no
+    mass or energy (M+(M) + M-(-M) = 0 for AUSM+; for Roe the two acoustic
+    wave strengths cancel and ũn = 0), a slip wall takes no shear or heat,
+    and interior faces cancel pairwise."""
+    mesh = M().make_box_mesh(12, 10, 1.0, 1.0, quads="mixed")
+    f = random_fields(mesh)
+    U = O.phs2dcns(f, G, mesh.nCells)
+    prm = O.ns_params(flux=O.FLUX_ROE if flux == "roe" else O.FLUX_AUSM_PLUS, viscous=viscous,
+                      bcs={c: (O.BC_SLIP_WALL, ()) for c in (-1, -2, -3, -4)}, **NS_GAS)
+    UN = O.ns_step(O.MeshView(mesh), prm, 1e-3, U)[0]
+    A = mesh.cell_areas
+    assert np.abs(UN - U).max() > 1e-4  # the step moves the interior
+    for q in (0, 3):
+        d = abs((A * (UN[:, q] - U[:, q])).sum()) / (A * U[:, q]).sum()
+        assert d < 1e-14, (q, d)
+
+
+def test_ns_isothermal_wall_heat_flux_known_answer():
+    """Gas at rest at uniform p and T_i under a wall at T_w: Fourier's law with
+    the one-sided difference from the centroid to the wall (distance hy/2 on a
+    quad row) and the conductivity at the wall temperature,
+    kappa = mu(T_w) cp / Pr, puts dt kappa (T_w - T_i) hx / (hy/2) into each
+    wall cell's energy (divided by its area hx hy) and nothing else."""
+    g, Rg, Pr, mu_ref, T_ref, S = 1.4, NS_GAS["R"], NS_GAS["Pr"], NS_GAS["mu_ref"], NS_GAS["T_ref"], NS_GAS["S"]
+    mesh = M().make_box_mesh(6, 5, 1.5, 1.0, quads="all")
+    hx, hy = 0.25, 0.2
+    n = mesh.nCells
+    p0, r0 = 1.5, 1.2
+    Ti, Tw = p0 / (r0 * Rg), 1.6
+    U = np.zeros((n, 4), order="F"); U[:, 0] = r0; U[:, 3] = p0 / (g - 1)
+    bcs = {-1: (O.BC_EXTRAPOLATE, ()), -2: (O.BC_EXTRAPOLATE, ()), -3: (O.BC_ISOTHERMAL_WALL, (Tw,)),
+           -4: (O.BC_EXTRAPOLATE, ())}
+    dt = 1e-3
+    UN, _, _, F = O.ns_step(O.MeshView(mesh), O.ns_params(gamma=g, bcs=bcs, **NS_GAS), dt, U)
+    mu = mu_ref * (Tw / T_ref) ** 1.5 * (T_ref + S) / (Tw + S)
+    kappa = mu * g * Rg / (g - 1) / Pr
+    dE = dt * kappa * (Tw - Ti) * hx / (0.5 * hy) / (hx * hy)
+    wall = mesh.cell_mid_points[:, 1] < hy
+    assert wall.sum() == 6
+    assert np.allclose(UN[wall, 3] - U[wall, 3], dE, rtol=1e-9, atol=0)
+    assert np.abs(UN[~wall, 3] - U[~wall, 3]).max() < 1e-14
+    assert np.abs(UN[:, :3] - U[:, :3]).max() < 1e-14
+    assert F[1] == pytest.approx(-p0 * 1.5, rel=1e-13)
+
+
+def _tri_inradius(hx, hy):
+    return hx * hy / (hx + hy + math.hypot(hx, hy))
+
+
+@pytest.mark.parametrize("quads", ["all", "none"])
+def test_ns_time_step_known_answer(quads):
+    """Uniform state: every cell has the same length scale h = 2 area /
+    perimeter, hx hy / (hx + hy) for an hx x hy rectangle and
+    hx hy / (hx + hy + diagonal) (the inscribed-circle radius) for a right
+    triangle with legs hx, hy, and dt = cfl h / (|u| + a + 2 gamma mu(T) /
+    (Pr rho h)), or cfl h / (|u| + a) without viscosity."""
+    mesh = M().make_box_mesh(5, 4, 1.0, 1.2, quads=quads)
+    hx, hy = 0.2, 0.3
+    h = hx * hy / (hx + hy) if quads == "all" else _tri_inradius(hx, hy)
+    r0, u0, v0, p0 = 1.1, 0.3, -0.4, 0.9
+    n = mesh.nCells
+    U = np.zeros((n, 4), order="F")
+    U[:, 0] = r0; U[:, 1] = r0 * u0; U[:, 2] = r0 * v0; U[:, 3] = p0 / (G - 1) + 0.5 * r0 * (u0 * u0 + v0 * v0)
+    a = math.sqrt(G * p0 / r0)
+    T = p0 / (r0 * NS_GAS["R"])
+    mu = NS_GAS["mu_ref"] * (T / NS_GAS["T_ref"]) ** 1.5 * (NS_GAS["T_ref"] + NS_GAS["S"]) / (T + NS_GAS["S"])
+    mv = O.MeshView(mesh)
+    cfl = 0.7
+    lam = math.hypot(u0, v0) + a
+    assert O.ns_dt(mv, O.ns_params(viscous=False, **NS_GAS), U, cfl) == pytest.approx(cfl * h / lam, rel=1e-13)
+    lam += 2 * G * mu / (NS_GAS["Pr"] * r0 * h)
+    assert O.ns_dt(mv, O.ns_params(**NS_GAS), U, cfl) == pytest.approx(cfl * h / lam, rel=1e-13)
