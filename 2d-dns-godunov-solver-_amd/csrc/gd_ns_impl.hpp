@@ -423,10 +423,12 @@ struct RecGather {
       o[0] = d.x; o[1] = d.y; o[2] = d.z; o[3] = d.w;
     }
   }
-  // cells[r]: the cell whose record this lane receives in out[r]; every lane
-  // of the wave must take part
-  __device__ __forceinline__ static void gather(Shared& sh, const T* rec, const int32_t (&cells)[R],
-                                                T (&out)[R][16]) {
+  // cells[r]: the cell whose record this lane receives; fn(r, record) is
+  // called for r = 0 .. R-1 as soon as record r has arrived, so that the
+  // caller can reduce it before the next one is unpacked.  Every lane of the
+  // wave must take part.
+  template <class F>
+  __device__ __forceinline__ static void gather_each(Shared& sh, const T* rec, const int32_t (&cells)[R], F&& fn) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int r = 0; r < R; ++r) sh.cell[w][r * 64 + lane] = cells[r];
@@ -445,79 +447,90 @@ struct RecGather {
 #pragma unroll
       for (int k = 0; k < kP; ++k) S[(k * kRPI + sub) * kStride + piece] = v[r][k];
       wave_lds_sync();
+      T out[16];
 #pragma unroll
-      for (int p = 0; p < kP; ++p) unpack(S[lane * kStride + p], out[r] + p * (16 / kP));
+      for (int p = 0; p < kP; ++p) unpack(S[lane * kStride + p], out + p * (16 / kP));
+      fn(r, out);
     }
+  }
+  __device__ __forceinline__ static void gather(Shared& sh, const T* rec, const int32_t (&cells)[R],
+                                                T (&out)[R][16]) {
+    gather_each(sh, rec, cells, [&](int r, const T (&o)[16]) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) out[r][q] = o[q];
+    });
   }
 };
 
-// The flux of face f from its owner's and neighbour's records (r1 = r0 at a
-// boundary, c1 = the neighbour or code - 1): reconstruction of both sides,
-// AUSM+ or Roe, the viscous flux, stored with the wall force.
+// One side of a face, reduced from that side's record as soon as it arrives:
+// the limited reconstruction at the face (offset ox, oy from the centroid)
+// and, for the viscous flux, the cell's u, v, T and their gradients
+// (grad T = (grad p - T R grad rho) / (rho R)).
 template <class T, bool VISC>
-__device__ __forceinline__ void face_flux(const NsKArgs<T>& a, int64_t f, int32_t c1, const T* r0, const T* r1) {
-  using G = FaceGeo<T>;
+struct FaceSide {
+  T q[4];
+  T u, v, t;
+  T gr[3][2];
+};
+
+template <class T, bool VISC>
+__device__ __forceinline__ void face_side(const T (&r)[16], T ox, T oy, T Rg, FaceSide<T, VISC>& s) {
+  const T* gq = r + 4;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) s.q[v] = r[v] + r[12 + v] * (gq[2 * v] * ox + gq[2 * v + 1] * oy);
+  if (!(s.q[0] > 0) || !(s.q[3] > 0)) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) s.q[v] = r[v];
+  }
+  if (VISC) {
+    const T rR = 1 / (r[0] * Rg);
+    const T t = r[3] * rR;
+    s.u = r[1]; s.v = r[2]; s.t = t;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      s.gr[0][c] = gq[2 + c];
+      s.gr[1][c] = gq[4 + c];
+      s.gr[2][c] = (gq[6 + c] - t * Rg * gq[c]) * rR;
+    }
+  }
+}
+
+// A face's geometry: the owner's inward normal, the length, and the face
+// midpoint minus the owner's (rx, ry) and the neighbour's (rxj, ryj) centroid.
+template <class T>
+struct FaceGeoVals {
+  T nx, ny, len, rx, ry, rxj, ryj;
+};
+
+// The flux of face f from its owner's side s0 and neighbour's side s1 (c1 =
+// the neighbour, or code - 1 at a boundary, where s1's state is the ghost of
+// s0's): the viscous flux, AUSM+ or Roe, stored with the wall force.
+template <class T, bool VISC>
+__device__ __forceinline__ void face_flux(const NsKArgs<T>& a, int64_t f, int32_t c1, const FaceGeoVals<T>& fg,
+                                          const FaceSide<T, VISC>& s0, FaceSide<T, VISC>& s1) {
   const MeshView& m = a.m;
   const gd_params& P = a.prm;
-  const T nx = G::fnx(m, f), ny = G::fny(m, f);
-  const T len = G::flen(m, f);
-  const T rx = G::fr0(m, 0, f), ry = G::fr0(m, 1, f);
+  const T nx = fg.nx, ny = fg.ny, len = fg.len;
   const T g = (T)P.Gamma, Rg = (T)P.RGAS;
-  const T* qi = r0;
-  const T* gi = r0 + 4;
-  T qa[4];  // owner side at the face
-#pragma unroll
-  for (int v = 0; v < 4; ++v) qa[v] = qi[v] + r0[12 + v] * (gi[2 * v] * rx + gi[2 * v + 1] * ry);
-  if (!(qa[0] > 0) || !(qa[3] > 0)) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) qa[v] = qi[v];
-  }
-  const T* qj = r1;
-  const T* gj = r1 + 4;
-  T qb[4];  // neighbour side, from its own offset
-  if (c1 >= 0) {
-    const T rxj = G::fr1(m, 0, f), ryj = G::fr1(m, 1, f);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) qb[v] = qj[v] + r1[12 + v] * (gj[2 * v] * rxj + gj[2 * v + 1] * ryj);
-    if (!(qb[0] > 0) || !(qb[3] > 0)) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) qb[v] = qj[v];
-    }
-  } else {
-    ghost<T>(P, c1 + 1, qa, nx, ny, qb);
-  }
-  T e[4];
-  if (P.flux == GD_FLUX_ROE) roe<T>(qb, qa, nx, ny, g, e);
-  else ausm<T>(qb, qa, nx, ny, g, e);
+  if (c1 < 0) ghost<T>(P, c1 + 1, s0.q, nx, ny, s1.q);
   T gv[4] = {0, 0, 0, 0};
   if (VISC) {
     const T cp = g * Rg / (g - 1);
-    const T rRi = 1 / (qi[0] * Rg);
-    const T Ti = qi[3] * rRi;
+    const T rx = fg.rx, ry = fg.ry;
     if (c1 >= 0) {
       // edge-corrected average of the two cell gradients of u, v, T
-      const T rRj = 1 / (qj[0] * Rg);
-      const T Tj = qj[3] * rRj;
-      T g_i[3][2], g_j[3][2];
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        g_i[0][c] = gi[2 + c]; g_i[1][c] = gi[4 + c];
-        g_i[2][c] = (gi[6 + c] - Ti * Rg * gi[c]) * rRi;
-        g_j[0][c] = gj[2 + c]; g_j[1][c] = gj[4 + c];
-        g_j[2][c] = (gj[6 + c] - Tj * Rg * gj[c]) * rRj;
-      }
-      const T ddx = rx - G::fr1(m, 0, f), ddy = ry - G::fr1(m, 1, f);  // centroid j - centroid i
+      const T ddx = rx - fg.rxj, ddy = ry - fg.ryj;  // centroid j - centroid i
       const T rdl = 1 / sqrt(ddx * ddx + ddy * ddy);
       const T tx = ddx * rdl, ty = ddy * rdl;
-      const T dphi[3] = {qj[1] - qi[1], qj[2] - qi[2], Tj - Ti};
+      const T dphi[3] = {s1.u - s0.u, s1.v - s0.v, s1.t - s0.t};
       T gf[3][2];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T ax = (T)0.5 * (g_i[c][0] + g_j[c][0]), ay = (T)0.5 * (g_i[c][1] + g_j[c][1]);
+        const T ax = (T)0.5 * (s0.gr[c][0] + s1.gr[c][0]), ay = (T)0.5 * (s0.gr[c][1] + s1.gr[c][1]);
         const T corr = dphi[c] * rdl - (ax * tx + ay * ty);
         gf[c][0] = ax + corr * tx; gf[c][1] = ay + corr * ty;
       }
-      viscous_face<T>(P, cp, (T)0.5 * (qi[1] + qj[1]), (T)0.5 * (qi[2] + qj[2]), (T)0.5 * (Ti + Tj), gf, nx, ny,
+      viscous_face<T>(P, cp, (T)0.5 * (s0.u + s1.u), (T)0.5 * (s0.v + s1.v), (T)0.5 * (s0.t + s1.t), gf, nx, ny,
                       gv);
     } else {
       const int idx = -(c1 + 1);
@@ -526,13 +539,16 @@ __device__ __forceinline__ void face_flux(const NsKArgs<T>& a, int64_t f, int32_
         // outward normal is -n
         const T Tw = (T)P.bc[idx].data[0];
         const T rdp = -1 / (rx * nx + ry * ny);
-        const T dudn = (0 - qi[1]) * rdp, dvdn = (0 - qi[2]) * rdp, dTdn = (Tw - Ti) * rdp;
+        const T dudn = (0 - s0.u) * rdp, dvdn = (0 - s0.v) * rdp, dTdn = (Tw - s0.t) * rdp;
         const T ox = -nx, oy = -ny;
         const T gf[3][2] = {{dudn * ox, dudn * oy}, {dvdn * ox, dvdn * oy}, {dTdn * ox, dTdn * oy}};
         viscous_face<T>(P, cp, (T)0, (T)0, Tw, gf, nx, ny, gv);
       }
     }
   }
+  T e[4];
+  if (P.flux == GD_FLUX_ROE) roe<T>(s1.q, s0.q, nx, ny, g, e);
+  else ausm<T>(s1.q, s0.q, nx, ny, g, e);
   double fk[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) fk[v] = (double)((e[v] + gv[v]) * len);
@@ -546,6 +562,7 @@ __device__ __forceinline__ void face_flux(const NsKArgs<T>& a, int64_t f, int32_
 // ---- pass F1: one flux per face
 template <class T, bool VISC>
 __global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
+  using G = FaceGeo<T>;
   const MeshView& m = a.m;
   const int64_t nF = m.n_faces;
   const int64_t f0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -554,12 +571,21 @@ __global__ __launch_bounds__(kBlock) void k_ns_face(NsKArgs<T> a) {
   const bool live = f0 < nF;
   const int64_t f = live ? f0 : nF - 1;
   const int32_t c0 = m.fc0[f], c1 = m.fc1[f];
-  // both records (a boundary face reads the owner's twice)
+  const T Rg = (T)a.prm.RGAS;
+  // the face geometry is loaded first, so that it arrives during the gather
+  // (the f_r1 rows span every face; a boundary face's are not used)
+  const FaceGeoVals<T> fg = {G::fnx(m, f),    G::fny(m, f),    G::flen(m, f),  G::fr0(m, 0, f),
+                             G::fr0(m, 1, f), G::fr1(m, 0, f), G::fr1(m, 1, f)};
+  // both records (a boundary face reads the owner's twice and uses it once),
+  // each reduced to its side of the face as soon as it has arrived
   __shared__ typename RecGather<T, 2>::Shared sh;
-  T rr[2][16];
-  RecGather<T, 2>::gather(sh, a.rec, {c0, c1 >= 0 ? c1 : c0}, rr);
+  FaceSide<T, VISC> s[2];
+  RecGather<T, 2>::gather_each(sh, a.rec, {c0, c1 >= 0 ? c1 : c0}, [&](int r, const T (&rec)[16]) {
+    if (r == 0) face_side<T, VISC>(rec, fg.rx, fg.ry, Rg, s[0]);
+    else if (c1 >= 0) face_side<T, VISC>(rec, fg.rxj, fg.ryj, Rg, s[1]);
+  });
   if (!live) return;
-  face_flux<T, VISC>(a, f, c1, rr[0], rr[1]);
+  face_flux<T, VISC>(a, f, c1, fg, s[0], s[1]);
 }
 
 // ---- pass F2: per-cell flux sum, update, primitives, dt inputs
