@@ -571,18 +571,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
   int32_t jn[NF], fc[NF];
 #pragma unroll
   for (int k = 0; k < NF; ++k) { jn[k] = m.nb[k * nc + i]; fc[k] = m.cface[k * nc + i]; }
-  // own and neighbour records (a boundary or absent slot reads the own)
-  __shared__ typename ns::RecGather<T, NF + 1>::Shared sh;
-  T rr[NF + 1][16];
-  {
-    int32_t cells[NF + 1];
-    cells[0] = (int32_t)i;
-#pragma unroll
-    for (int k = 0; k < NF; ++k) cells[1 + k] = jn[k] >= 0 ? jn[k] : (int32_t)i;
-    ns::RecGather<T, NF + 1>::gather(sh, a.rec, cells, rr);
-  }
-  if (!live) return;
-  const T* ri = rr[0];
+  // the cell's adjoint flux inputs and geometry, loaded before the record
+  // gather so that they arrive during it (loads after its LDS fences would
+  // wait on their own)
   T li[4], lj[NF][4];
   ld4(a.lamF + 4 * i, li);
   FaceIn<T> fg[NF];
@@ -604,24 +595,43 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
   for (int t = 0; t < 16; ++t) lr[t] = 0;
   double bqk[NF][4];
 #pragma unroll
-  for (int k = 0; k < NF; ++k) {
+  for (int k = 0; k < NF; ++k)
 #pragma unroll
     for (int v = 0; v < 4; ++v) bqk[k][v] = 0.0;
-    if (k < nf) {
-      const int32_t c1 = jn[k] >= 0 ? 0 : jn[k];
-      if (c1 < 0) {
+  // own and neighbour records (a boundary or absent slot reads the own): the
+  // own record arrives first and is kept; each face is transposed as soon as
+  // its neighbour's record has arrived, after which only that neighbour's
+  // state is kept (faces in slot order, as the sums require)
+  __shared__ typename ns::RecGather<T, NF + 1>::Shared sh;
+  T ri[16], qk[NF][4];
+  {
+    int32_t cells[NF + 1];
+    cells[0] = (int32_t)i;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) lj[k][v] = 0;
+    for (int k = 0; k < NF; ++k) cells[1 + k] = jn[k] >= 0 ? jn[k] : (int32_t)i;
+    ns::RecGather<T, NF + 1>::gather_each(sh, a.rec, cells, [&](int r, const T (&rec)[16]) {
+      if (r == 0) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) ri[t] = rec[t];
+        return;
       }
-      face_rev<T, VISC, ROE>(a, fg[k], rx[k], ry[k], c1, (fc[k] & 1) != 0, ri, rr[1 + k], li, lj[k], lr, bqk[k]);
-    }
+      const int k = r - 1;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) qk[k][v] = rec[v];
+      if (live && k < nf) {
+        const int32_t c1 = jn[k] >= 0 ? 0 : jn[k];
+        if (c1 < 0) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) lj[k][v] = 0;
+        }
+        face_rev<T, VISC, ROE>(a, fg[k], rx[k], ry[k], c1, (fc[k] & 1) != 0, ri, rec, li, lj[k], lr, bqk[k]);
+      }
+    });
   }
+  if (!live) return;
   const T* qi = ri;
-  T qk[NF][4];
 #pragma unroll
   for (int k = 0; k < NF; ++k) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) qk[k][v] = rr[1 + k][v];
     if (k < nf && jn[k] < 0) {
       const int64_t s = k * nc + i;
       ns::ghost<T>(a.prm, jn[k] + 1, qi, G::nx(m, s), G::ny(m, s), qk[k]);
@@ -694,7 +704,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
     if (k >= nf) continue;
     if (jn[k] >= 0) {
       T o[4];
-      prims_rev<T>(rr[1 + k], gm1, lq[1 + k], o);  // in the neighbour's conservative variables
+      prims_rev<T>(qk[k], gm1, lq[1 + k], o);  // in the neighbour's conservative variables
       st4(a.nrev + 4 * (i * nfs + k), o[0], o[1], o[2], o[3]);
     } else {
       const int idx = -(jn[k] + 1);
