@@ -735,9 +735,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_cell(AdjKArgs<T> a) {
 }
 
 // ---- state pass: gather the primitive adjoint, transpose the primitive
-// recovery, add to lamU; seed the next step's lamF
-template <class T, bool GATHER>
+// recovery, add to lamU; seed the next step's lamF.  The slots are unrolled
+// (NF = 3 or 4), so that every slot's ids and then every neighbour part are
+// loaded together rather than one dependent pair per slot.
+template <class T, bool GATHER, bool QUADS>
 __global__ __launch_bounds__(kBlock) void k_adj_state(AdjKArgs<T> a) {
+  constexpr int NF = QUADS ? 4 : 3;
   const MeshView& m = a.m;
   const int64_t nc = m.nc;
   const int64_t j = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -746,19 +749,25 @@ __global__ __launch_bounds__(kBlock) void k_adj_state(AdjKArgs<T> a) {
 #pragma unroll
   for (int v = 0; v < 4; ++v) lU[v] = a.lamU_in[v * nc + j];
   if (GATHER) {
-    const int nf = m.has_quads ? m.nv[j] : 3;
-    T s4[4];
+    const int nf = QUADS ? m.nv[j] : 3;
+    int32_t jn[NF], kk[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      jn[k] = -1; kk[k] = 0;
+      if (k < nf) { jn[k] = m.nb[k * nc + j]; kk[k] = m.kn[k * nc + j]; }
+    }
+    T s4[4], b[NF][4];
     ld4(a.selfq + 4 * j, s4);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) lU[v] += (double)s4[v];
-    for (int k = 0; k < nf; ++k) {
-      const int32_t jn = m.nb[k * nc + j];
-      if (jn >= 0) {
-        const int32_t kk = m.kn[k * nc + j];
-        T b[4];
-        ld4(a.nrev + 4 * ((int64_t)jn * m.nf + kk), b);
+    for (int k = 0; k < NF; ++k)
+      if (jn[k] >= 0) ld4(a.nrev + 4 * ((int64_t)jn[k] * m.nf + kk[k]), b[k]);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) lU[v] += (double)b[v];
+    for (int v = 0; v < 4; ++v) lU[v] += (double)s4[v];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      if (jn[k] >= 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) lU[v] += (double)b[k][v];
       }
     }
 #pragma unroll
@@ -791,8 +800,9 @@ hipError_t launch_cell(const AdjKArgs<T>& a, hipStream_t s) {
 template <class T>
 hipError_t launch_state(const AdjKArgs<T>& a, bool gather, hipStream_t s) {
   const dim3 g(ns::nblocks(a.m.nc)), b(kBlock);
-  if (gather) hipLaunchKernelGGL((k_adj_state<T, true>), g, b, 0, s, a);
-  else hipLaunchKernelGGL((k_adj_state<T, false>), g, b, 0, s, a);
+  if (!gather) hipLaunchKernelGGL((k_adj_state<T, false, false>), g, b, 0, s, a);
+  else if (a.m.has_quads) hipLaunchKernelGGL((k_adj_state<T, true, true>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_adj_state<T, true, false>), g, b, 0, s, a);
   return hipGetLastError();
 }
 
