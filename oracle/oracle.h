@@ -8,6 +8,16 @@
  * All arrays use the reference layout: Julia column-major (nCells, k) arrays
  * (contiguous column vectors), 1-based ids, boundary codes <= 0 in
  * cell_stiffness.  Every function cites the reference lines it restates.
+ *
+ * Pinning (tests/test_oracle.py, test_mesh.py): every function is checked
+ * against hand-computed values, closed forms, invariants or an independent
+ * library computation.  Parity unpinned, since the Julia reference cannot run
+ * here: orc_calc_one_stage_muscl_cpu_fast / orc_calc_one_stage_muscl on a
+ * non-trivial state are pinned through their parts and whole-step properties
+ * (free stream, conservation, numbering independence, mirror symmetry), not
+ * value by value against a reference run; orc_air_conductivity does not
+ * reproduce the rounding of Julia's x^4 and pow; the reference's FP32 GPU
+ * accumulation order (AUSMflux2dCUDAx32.jl:88-91) is not restated.
  */
 #ifndef GODUNOV_ORACLE_H
 #define GODUNOV_ORACLE_H
