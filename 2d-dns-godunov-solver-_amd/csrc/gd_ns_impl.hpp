@@ -453,13 +453,6 @@ struct RecGather {
       fn(r, out);
     }
   }
-  __device__ __forceinline__ static void gather(Shared& sh, const T* rec, const int32_t (&cells)[R],
-                                                T (&out)[R][16]) {
-    gather_each(sh, rec, cells, [&](int r, const T (&o)[16]) {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) out[r][q] = o[q];
-    });
-  }
 };
 
 // One side of a face, reduced from that side's record as soon as it arrives:
