@@ -7,7 +7,9 @@ Sutherland viscous fluxes, isothermal wall) at CFL 0.4, starting from the
 uniform free stream.  A "step" is one explicit Euler step over all cells,
 every array resident in HBM.  The FP64 step is the headline value; the
 FP32-mixed step (same mesh, same start) is reported beside it with its density,
-pressure and wall-drag deviation from the FP64 run after the same steps.
+pressure and wall-drag deviation from the FP64 run after the same steps.  The
+reference scheme (the bit-faithful restatement of the reference's step) is
+timed beside them on a 2M-triangle periodic Taylor-Green square (``ref_scheme``).
 
 Prints one JSON line.  Only one GPU is supported (--gpus N>1 exits).
 """
@@ -143,6 +145,56 @@ def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl, precision=L.GD_FP3
                             "max_rel_diff_vs_checkpointed": float(np.abs(g1 - g0).max() / np.abs(g0).max())}}
 
 
+def ref_tgv(n, steps, warmup):
+    """The reference scheme, FP64 (vertex-based MUSCL with the reference's
+    Minmod_Limiter, AUSM+, the calcDiffTerm viscous term, updateVariablesSA and
+    the node interpolation, as calcOneStageMUSCLcpuFast and its driver run
+    them) on the periodic Taylor-Green square of n x n x 2 triangles: Re 1600,
+    Ma 0.1, seeded N(0, 1e-6) density noise, local dt at CFL 0.5.  This is the
+    reference's CPU case scaled up; the device layout is in Hilbert order."""
+    t0 = time.perf_counter()
+    mesh = M.make_tgv_periodic_mesh(n)
+    x, y = mesh.cell_mid_points[:, 0], mesh.cell_mid_points[:, 1]
+    g, ma = 1.4, 0.1
+    rho = 1.0 + 1e-6 * M.normal(SEED, mesh.nCells)
+    u, v = np.sin(x) * np.cos(y), -np.cos(x) * np.sin(y)
+    p = 1.0 / (g * ma * ma) + (np.cos(2 * x) + np.cos(2 * y)) / 4.0
+    prm = S.make_params(viscous=True, energy_law=L.GD_ENERGY_CONST, mu_const=1.0 / 1600, k_const=1.0 / 0.72,
+                        RGAS=1.0, Gamma=g, bcs={})
+    sol = S.GodunovSolver(mesh, prm, order=L.GD_ORDER_HILBERT)
+    sol.set_primitives(rho, u, v, p)
+    log(f"reference scheme, TGV {n}x{n}x2: {mesh.nCells} cells, {mesh.nNodes} nodes "
+        f"({time.perf_counter() - t0:.1f} s to set up)")
+    if warmup:
+        sol.run(warmup, cfl=0.5, dt_policy=L.GD_DT_LOCAL)
+    sol.state.sync()
+    t0 = time.perf_counter()
+    sol.run(steps, cfl=0.5, dt_policy=L.GD_DT_LOCAL)
+    sol.state.sync()
+    el = time.perf_counter() - t0
+    kt = L.kernel_table(sol.run(steps, cfl=0.5, dt_policy=L.GD_DT_LOCAL, time_kernels=True))
+    sol.state.sync()
+    sol.close()
+    loop = [k for k in kt if k["launches"] == steps]  # ref_phi runs once per call, before the loop
+    ms = 1e3 * el / steps
+    b = sum(k["bytes_per_launch"] for k in loop)
+    dom, roof = roofline(loop, steps)
+    for k in loop:
+        log(f"   {k['name']:16s} {k['ms_avg'] * 1e3:9.2f} us/launch  "
+            f"{k['bytes_per_launch'] / (k['ms_avg'] * 1e-3) / 1e9:8.1f} GB/s")
+    log(f"reference scheme: {ms:.4f} ms/step, {mesh.nCells / (ms * 1e-3):.4e} cell-updates/s, "
+        f"step {b / (ms * 1e-3) / 1e9:.0f} GB/s")
+    return {"workload": f"periodic Taylor-Green square, {n}x{n}x2 = {mesh.nCells} triangles, reference scheme "
+                        f"(MUSCL + Minmod_Limiter + AUSM+ + calcDiffTerm), FP64, CFL 0.5, Hilbert device order",
+            "cells": mesh.nCells, "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 5),
+            "value": round(mesh.nCells / (ms * 1e-3), 1), "unit": "cell-updates/s",
+            "kernels_us": {k["name"]: round(k["ms_avg"] * 1e3, 2) for k in loop},
+            "step_hbm": {"bytes_per_step": b, "achieved_GBs": round(b / (ms * 1e-3) / 1e9, 1),
+                         "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "roofline": {**roof, "kernel": dom["name"], "bytes_per_launch": dom["bytes_per_launch"],
+                         "avg_launch_us": round(dom["ms_avg"] * 1e3, 2)}}
+
+
 def roofline(kt, steps):
     dom = max(kt, key=lambda k: k["ms_total"])
     achieved = dom["bytes_per_launch"] / (dom["ms_avg"] * 1e-3) / 1e9
@@ -216,6 +268,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200, help="0: skip the adjoint measurement")
     ap.add_argument("--adjoint-checkpoint", type=int, default=20)
     ap.add_argument("--adjoint-precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--ref-tgv", type=int, default=1000,
+                    help="n of the n x n x 2 periodic TGV for the reference-scheme leg; 0 skips it")
+    ap.add_argument("--ref-steps", type=int, default=200)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU path is not built",
@@ -302,6 +357,8 @@ def main():
             f"(forward sweep {ad['adjoint_forward_sweep_ms']:.1f} ms, reverse {ad['adjoint_reverse_ms']:.1f} ms = "
             f"{ad['reverse_step_vs_forward_step']:.2f}x per step); "
             f"all records kept: {ad['all_records']['adjoint_ms']:.1f} ms -> {ad['all_records']['cost_vs_forward']:.2f}x")
+    if a.ref_tgv > 0:
+        out["ref_scheme"] = ref_tgv(a.ref_tgv, a.ref_steps, min(a.warmup, 10))
     if not a.no_cpu:
         threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
         try:
