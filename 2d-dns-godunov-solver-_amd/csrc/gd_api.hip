@@ -613,7 +613,7 @@ int gd_state_create(gd_mesh* m, const gd_params* p, gd_state** out) {
   rc |= dev_alloc(al, &S->Ud, 4 * nc);
   rc |= dev_alloc(al, &S->F, 4 * nc);
   rc |= dev_alloc(al, &S->Dl, 4 * nc);
-  rc |= dev_alloc(al, &S->phiN, 3 * nn);
+  rc |= dev_alloc(al, &S->phiN, 4 * nn);
   rc |= dev_alloc(al, &S->g6, 6 * nc);
   rc |= dev_alloc(al, &S->g6N, 6 * nn);
   double* rbuf = nullptr;

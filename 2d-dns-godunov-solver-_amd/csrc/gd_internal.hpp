@@ -135,9 +135,9 @@ struct State {
   double* Ud = nullptr;                 // [4][nc]
   double* F = nullptr;                  // [4][nc]
   double* Dl = nullptr;                 // [4][nc]
-  double* phiN = nullptr;               // [3][nn]
-  double* g6 = nullptr;                 // [6][nc]
-  double* g6N = nullptr;                // [6][nn]
+  double* phiN = nullptr;               // [nn][4]: rho*u, rho*v, E cluster means, 0
+  double* g6 = nullptr;                 // [nc][6]: Green-Gauss level-1 gradients per cell
+  double* g6N = nullptr;                // [nn][6]: their cluster means per node
   void* rec = nullptr;                  // ns: per-cell record [nc][16] (q, grad q, limiter), compute type
   void* fflux = nullptr;                // ns: [n_faces][4] face fluxes, compute type
   void* Qt[2] = {nullptr, nullptr};     // ns: primitives [nc][4], compute type

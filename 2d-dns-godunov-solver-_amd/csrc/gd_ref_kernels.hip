@@ -69,25 +69,47 @@ __device__ __forceinline__ void rotate_acc(Acc* acc, int step, double t, double 
   }
 }
 
-// GG face sum of one node-mean field: sum_k ((0.5 (phi_a + phi_b)) (-n_k)) len_k,
-// calcGradients.jl:116-129 (the x/y swap of the source is applied by the caller)
-template <bool QUADS>
-__device__ __forceinline__ void gg_sums(const MeshView& m, int64_t i, int nf, const double* fld, double* sx,
-                                        double* sy) {
+// 2K consecutive doubles as K 16 B loads / stores (p 16 B aligned)
+template <int K>
+__device__ __forceinline__ void ld2k(const double* p, double* o) {
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const double2 v = *reinterpret_cast<const double2*>(p + 2 * q);
+    o[2 * q] = v.x; o[2 * q + 1] = v.y;
+  }
+}
+template <int K>
+__device__ __forceinline__ void st2k(double* p, const double* o) {
+#pragma unroll
+  for (int q = 0; q < K; ++q) *reinterpret_cast<double2*>(p + 2 * q) = make_double2(o[2 * q], o[2 * q + 1]);
+}
+
+// GG face sums of the first W fields of node records of S doubles (S even):
+// per field, sum_k ((0.5 (phi_a + phi_b)) (-n_k)) len_k in face order,
+// calcGradients.jl:116-129 (the x/y swap of the source is applied by the
+// caller).  A node's fields are one record, so each face end is S/2 16 B
+// loads within one or two cache lines rather than one scattered load per
+// field.
+template <bool QUADS, int S, int W>
+__device__ __forceinline__ void gg_sums(const MeshView& m, int64_t i, int nf, const double* rec, double* ax,
+                                        double* ay) {
   const int64_t nc = m.nc;
-  double ax = 0.0, ay = 0.0;
 #pragma unroll
   for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
     if (k < nf) {
-      const double pa = fld[m.ea[k * nc + i]], pb = fld[m.eb[k * nc + i]];
-      const double hm = 0.5 * (pa + pb);
-      const double fx = hm * -m.fny[k * nc + i] * m.flen[k * nc + i];
-      const double fy = hm * -m.fnx[k * nc + i] * m.flen[k * nc + i];
-      if (k == 0) { ax = fx; ay = fy; } else { ax = ax + fx; ay = ay + fy; }
+      double pa[S], pb[S];
+      ld2k<S / 2>(rec + (int64_t)S * m.ea[k * nc + i], pa);
+      ld2k<S / 2>(rec + (int64_t)S * m.eb[k * nc + i], pb);
+      const double mny = -m.fny[k * nc + i], mnx = -m.fnx[k * nc + i], len = m.flen[k * nc + i];
+#pragma unroll
+      for (int f = 0; f < W; ++f) {
+        const double hm = 0.5 * (pa[f] + pb[f]);
+        const double fx = hm * mny * len;
+        const double fy = hm * mnx * len;
+        if (k == 0) { ax[f] = fx; ay[f] = fy; } else { ax[f] = ax[f] + fx; ay[f] = ay[f] + fy; }
+      }
     }
   }
-  *sx = ax;
-  *sy = ay;
 }
 
 // calcDiffTerm's per-cell tail (calcDiffTerm.jl:98-119): Laplacians from the
@@ -95,16 +117,12 @@ __device__ __forceinline__ void gg_sums(const MeshView& m, int64_t i, int nf, co
 template <bool QUADS>
 __device__ __forceinline__ void ref_udiff(const MeshView& m, const gd_params& p, const double* g6N, int64_t i, int nf,
                                           const double* q, double* ud) {
-  const int64_t nn = m.nn;
   const double A = m.area[i];
+  double ax[6], ay[6];
+  gg_sums<QUADS, 6, 6>(m, i, nf, g6N, ax, ay);
   double lap[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double sx, sy, tx, ty;
-    gg_sums<QUADS>(m, i, nf, g6N + (2 * c) * nn, &sx, &sy);      // "gradX" of the "gradX" field
-    gg_sums<QUADS>(m, i, nf, g6N + (2 * c + 1) * nn, &tx, &ty);  // "gradY" of the "gradY" field
-    lap[c] = sx / A + ty / A;
-  }
+  for (int c = 0; c < 3; ++c) lap[c] = ax[2 * c] / A + ay[2 * c + 1] / A;  // "gradX" of "gradX" + "gradY" of "gradY"
   const double T = q[3] / q[0] / p.RGAS;
   const double mu = p.mu_const;
   const double k = p.energy_law == GD_ENERGY_CONST ? p.k_const : refm::air_k(T);
@@ -268,9 +286,8 @@ __global__ __launch_bounds__(kBlock) void k_ref_nodes(MeshView m, const double* 
   if (Un) { Un[J] = s0; Un[nn + J] = s1; Un[2 * nn + J] = s2; Un[3 * nn + J] = s3; }
   if (phiN) {
     const double dc = (double)cnt;
-    phiN[J] = cnt ? f1 / dc : 0.0;
-    phiN[nn + J] = cnt ? f2 / dc : 0.0;
-    phiN[2 * nn + J] = cnt ? f3 / dc : 0.0;
+    const double o[4] = {cnt ? f1 / dc : 0.0, cnt ? f2 / dc : 0.0, cnt ? f3 / dc : 0.0, 0.0};
+    st2k<2>(phiN + 4 * J, o);
   }
 }
 
@@ -282,13 +299,11 @@ __global__ __launch_bounds__(kBlock) void k_ref_gg1(MeshView m, const double* ph
   if (i >= m.nc) return;
   const int nf = QUADS ? m.nv[i] : 3;
   const double A = m.area[i];
+  double ax[3], ay[3], o[6];
+  gg_sums<QUADS, 4, 3>(m, i, nf, phiN, ax, ay);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double sx, sy;
-    gg_sums<QUADS>(m, i, nf, phiN + c * m.nn, &sx, &sy);
-    g6[(2 * c) * m.nc + i] = sx / A;
-    g6[(2 * c + 1) * m.nc + i] = sy / A;
-  }
+  for (int c = 0; c < 3; ++c) { o[2 * c] = ax[c] / A; o[2 * c + 1] = ay[c] / A; }
+  st2k<3>(g6 + 6 * i, o);
 }
 
 // Cluster means of the six gradient fields (calcPhiInCellCluster applied
@@ -302,14 +317,18 @@ __global__ __launch_bounds__(kBlock) void k_ref_gg_nodes(MeshView m, const doubl
   for (int q = 0; q < m.ncl; ++q) {
     const int32_t c = m.cl[q * nn + J];
     if (c >= 0) {
+      double x[6];
+      ld2k<3>(g6 + 6 * (int64_t)c, x);
 #pragma unroll
-      for (int r = 0; r < 6; ++r) s[r] = s[r] + g6[r * nc + c];
+      for (int r = 0; r < 6; ++r) s[r] = s[r] + x[r];
       ++cnt;
     }
   }
   const double dc = (double)cnt;
+  double o[6];
 #pragma unroll
-  for (int r = 0; r < 6; ++r) g6N[r * nn + J] = cnt ? s[r] / dc : 0.0;
+  for (int r = 0; r < 6; ++r) o[r] = cnt ? s[r] / dc : 0.0;
+  st2k<3>(g6N + 6 * J, o);
 }
 
 template <bool QUADS>
