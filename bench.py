@@ -189,6 +189,7 @@ def ref_tgv(n, steps, warmup):
             "cells": mesh.nCells, "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 5),
             "value": round(mesh.nCells / (ms * 1e-3), 1), "unit": "cell-updates/s",
             "kernels_us": {k["name"]: round(k["ms_avg"] * 1e3, 2) for k in loop},
+            "kernel_bytes": {k["name"]: k["bytes_per_launch"] for k in loop},
             "step_hbm": {"bytes_per_step": b, "achieved_GBs": round(b / (ms * 1e-3) / 1e9, 1),
                          "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
             "roofline": {**roof, "kernel": dom["name"], "bytes_per_launch": dom["bytes_per_launch"],
