@@ -89,6 +89,9 @@ def lib():
         L.orc_ns_ausm_flux.argtypes = [dp, C.c_double, C.c_double, C.c_double, dp]
         L.orc_ns_roe_flux.restype = None
         L.orc_ns_roe_flux.argtypes = [dp, C.c_double, C.c_double, C.c_double, dp]
+        L.orc_ref_bc_ghost.restype = None
+        L.orc_ref_bc_ghost.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcThermo), C.POINTER(OrcParams), C.c_int64,
+                                       C.c_int32, dp, C.c_double, dp]
         L.orc_air_conductivity.restype = C.c_double
         L.orc_air_conductivity.argtypes = [C.c_double]
         L.orc_sutherland_viscosity.restype = C.c_double
@@ -174,6 +177,16 @@ def bc_table(bcs):
 def params(flux=FLUX_AUSM_PLUS, limiter=LIM_REF_HARMONIC, limiter_eps=1e-12, include_udiff=True,
            energy_law=ENERGY_AIR_POLY, mu_const=1e-2, k_const=0.0, bcs=None):
     return OrcParams(flux, limiter, limiter_eps, int(include_udiff), energy_law, mu_const, k_const, bc_table(bcs))
+
+
+def ref_bc_ghost(mv: MeshView, th, prm, i, k, cur, t=0.0):
+    """The reference scheme's ghost primitives of face k of cell i (both
+    1-based) from the cell's primitives ``cur`` at time ``t``."""
+    c = np.ascontiguousarray(cur, dtype=np.float64)
+    out = np.zeros(4)
+    lib().orc_ref_bc_ghost(C.byref(mv.c), C.byref(thermo_struct(th)), C.byref(prm), int(i), int(k), _p(c), float(t),
+                           _p(out))
+    return out
 
 
 def threads_table(n_threads, n):

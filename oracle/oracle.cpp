@@ -397,6 +397,13 @@ extern "C" {
 
 double orc_limiter(int32_t kind, double x, double y, double c) { return limiter(kind, x, y, c); }
 
+// the reference scheme's ghost state of face k of cell i (both 1-based) from
+// the cell's primitives, for the ghost's known-answer tests
+void orc_ref_bc_ghost(const orc_mesh* m, const orc_thermo* th, const orc_params* prm, int64_t i, int32_t k,
+                      const double* cur, double t, double* out) {
+  bc_ghost(m, th, prm->bc, i, k, cur, t, out);
+}
+
 void orc_ausm_plus_flux(const double* q, double nx, double ny, double side, double gamma, double* flux) {
   ausm_plus(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], nx, ny, side, gamma, flux);
 }
@@ -598,7 +605,7 @@ double orc_ref_tau(const orc_mesh* m, const orc_fields* f, double cfl) {
 }
 
 // CFL-limited step: cfl * min_i h_i / VMAX_i with h = 2 A / perimeter (the
-// inscribed-circle diameter of a triangle); the reference's own tau uses the
+// inscribed-circle radius of a triangle); the reference's own tau uses the
 // longest edge instead, which is not a stable bound on triangles.
 double orc_local_dt(const orc_mesh* m, const orc_fields* f, double cfl) {
   const int64_t n = m->nCells;

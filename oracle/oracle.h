@@ -87,6 +87,8 @@ double orc_air_conductivity(double T);
 double orc_sutherland_viscosity(double T);
 
 /* ---- reference operators (ref mode) ---- */
+void orc_ref_bc_ghost(const orc_mesh* m, const orc_thermo* th, const orc_params* prm, int64_t i, int32_t k,
+                      const double* cur, double t, double* out4);
 void orc_distribute_cells_in_threads(int32_t nThreads, int64_t nCells, int32_t* table /* nThreads x 2 */);
 void orc_phs2dcns(const orc_fields* f, int64_t nCells, double gamma, double* U);
 int orc_cells2nodes(const int32_t* nodesThreads, int32_t nThreads, const orc_mesh* m, const orc_fields* f,

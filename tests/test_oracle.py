@@ -792,3 +792,135 @@ def test_ns_time_step_known_answer(quads):
     assert O.ns_dt(mv, O.ns_params(viscous=False, **NS_GAS), U, cfl) == pytest.approx(cfl * h / lam, rel=1e-13)
     lam += 2 * G * mu / (NS_GAS["Pr"] * r0 * h)
     assert O.ns_dt(mv, O.ns_params(**NS_GAS), U, cfl) == pytest.approx(cfl * h / lam, rel=1e-13)
+
+
+# ------------------------------------------------ reference scheme: ghosts, weights, diff term
+def _boundary_faces(mesh):
+    st = np.asarray(mesh.cell_stiffness)
+    nv = np.asarray(mesh.mesh_connectivity)[:, 2]
+    return [(i + 1, k + 1, int(st[i, k])) for i in range(mesh.nCells) for k in range(nv[i]) if st[i, k] <= 0]
+
+
+def test_ref_slip_isothermal_fixed_and_zero_gradient_ghosts():
+    """On the oblique boundary faces of an O-grid: the slip-wall ghost
+    (bcInviscidWall.jl:21-38) reflects the velocity across the face,
+    g.n = -q.n and g.t = q.t with rho and p kept; the isothermal no-slip ghost
+    has u_g = -u, p_g = p and a mean of cell and ghost temperature equal to
+    T_w (T_g = 2 T_w - T_c, held at T_w / 2 from below); a fixed face returns
+    its table state and a zero-gradient face the cell's."""
+    mesh = M().make_cylinder_ogrid_mesh(16, 4, 0.5, 2.0, 0.1)
+    mv = O.MeshView(mesh)
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+    nxs, nys = np.asarray(mesh.cell_edges_Nx), np.asarray(mesh.cell_edges_Ny)
+    faces = _boundary_faces(mesh)
+    assert len(faces) == 32
+    rng = np.random.default_rng(SEED)
+    Tw, fixed = 1.3, (1.1, 0.4, -0.2, 0.9)
+    oblique = clamped = 0
+    for i, k, code in faces:
+        cur = np.array([rng.uniform(0.5, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.5, 3)])
+        n = np.array([nxs[i - 1, k - 1], nys[i - 1, k - 1]])
+        t = np.array([-n[1], n[0]])
+        oblique += abs(n[0] * n[1]) > 0.1
+        g = O.ref_bc_ghost(mv, th, O.params(bcs={code: (O.BC_SLIP_WALL, ())}), i, k, cur)
+        assert g[0] == cur[0] and g[3] == cur[3]
+        assert g[1:3] @ n == pytest.approx(-(cur[1:3] @ n), abs=1e-14)
+        assert g[1:3] @ t == pytest.approx(cur[1:3] @ t, abs=1e-14)
+        g = O.ref_bc_ghost(mv, th, O.params(bcs={code: (O.BC_ISOTHERMAL_WALL, (Tw,))}), i, k, cur)
+        assert g[1] == -cur[1] and g[2] == -cur[2] and g[3] == cur[3]
+        Tc, Tg = cur[3] / cur[0], g[3] / g[0]
+        if 2 * Tw - Tc >= 0.5 * Tw:
+            assert 0.5 * (Tc + Tg) == pytest.approx(Tw, rel=1e-14)
+        else:
+            clamped += 1
+            assert Tg == pytest.approx(0.5 * Tw, rel=1e-14)
+        assert np.array_equal(O.ref_bc_ghost(mv, th, O.params(bcs={code: (O.BC_FIXED, fixed)}), i, k, cur), fixed)
+        assert np.array_equal(O.ref_bc_ghost(mv, th, O.params(bcs={code: (O.BC_EXTRAPOLATE, ())}), i, k, cur), cur)
+    assert oblique >= 16 and clamped >= 1
+
+
+def test_ref_tgv_ghost_is_the_vortex_at_the_mirror_image():
+    """The Taylor-Green ghost (boundaryConditions_TGV2d.jl:16-30) is the
+    decaying vortex u = U sin(x/L) cos(y/L) F, v = -U cos(x/L) sin(y/L) F,
+    p = P + U^2/4 (cos 2x/L + cos 2y/L) F^2, F = exp(-2 mu t / L^2), rho = 1
+    (T = P/R), at the mirror image of the cell centroid across the face's line,
+    the image taken here by orthogonal projection."""
+    mesh = M().make_box_mesh(6, 5, 2 * math.pi, 2 * math.pi, quads="mixed")
+    mv = O.MeshView(mesh)
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+    P, U, L, mu, t = 100.0, 1.3, 1.2, 0.05, 0.7
+    prm = O.params(bcs={c: (O.BC_TGV_MIRROR, (P, U, L, mu)) for c in (-1, -2, -3, -4)})
+    c2n = np.asarray(mesh.cells2nodes)
+    xn, yn = np.asarray(mesh.xNodes).ravel(), np.asarray(mesh.yNodes).ravel()
+    cmid = np.asarray(mesh.cell_mid_points)
+    faces = _boundary_faces(mesh)
+    assert len(faces) == 22
+    F = math.exp(-2 * mu * t / L / L)
+    for i, k, _ in faces:
+        a = np.array([xn[c2n[i - 1, 2 * k - 2] - 1], yn[c2n[i - 1, 2 * k - 2] - 1]])
+        b = np.array([xn[c2n[i - 1, 2 * k - 1] - 1], yn[c2n[i - 1, 2 * k - 1] - 1]])
+        c = cmid[i - 1]
+        d = b - a
+        x, y = 2 * (a + ((c - a) @ d) / (d @ d) * d) - c
+        want = [1.0, U * math.sin(x / L) * math.cos(y / L) * F, -U * math.cos(x / L) * math.sin(y / L) * F,
+                P + 0.25 * U * U * (math.cos(2 * x / L) + math.cos(2 * y / L)) * F * F]
+        g = O.ref_bc_ghost(mv, th, prm, i, k, np.array([1.0, 0.0, 0.0, P]), t)
+        assert np.allclose(g, want, rtol=1e-13, atol=1e-13), (i, k, g, want)
+
+
+def test_node_interpolation_uses_the_stencil_weights():
+    """cells2nodesSolutionReconstructionWithStencils (utilsFVM2dp.jl:299-365)
+    is the weighted mean sum w_j q_j / sum w_j over the node's cluster: with the
+    weights of one node set to 1, 2, ..., n and its cluster cells' density to
+    10, 20, ..., 10 n, the node density is 10 (2n + 1) / 3 (the unweighted mean
+    would be 5 (n + 1))."""
+    mesh = M().make_box_mesh(6, 5, 1.0, 1.0, quads="none")
+    cl = np.asarray(mesh.cell_clusters)
+    J = int(np.argmax((cl != 0).sum(1)))
+    slots = np.nonzero(cl[J])[0]
+    n = len(slots)
+    assert n == 6
+    f = M().fields2d.zeros(mesh.nCells, mesh.nNodes)
+    f.densityCells[:] = 1.0; f.UxCells[:] = 0.2; f.UyCells[:] = 0.1; f.pressureCells[:] = 1.0
+    for r, s in enumerate(slots):
+        mesh.node_stencils[J, s] = r + 1.0
+        f.densityCells[cl[J, s] - 1] = 10.0 * (r + 1)
+    O.cells2nodes(O.MeshView(mesh), f)
+    assert f.densityNodes[J] == pytest.approx(10.0 * (2 * n + 1) / 3.0, rel=1e-14)
+    assert f.UxNodes[J] == pytest.approx(0.2, rel=1e-14)
+
+
+def test_diff_term_scales_with_mu_and_k():
+    """calcDiffTerm.jl:98-119: the momentum rows are mu times the velocity
+    Laplacians and the energy row mu k times the energy's: doubling mu doubles
+    every row, doubling a constant k doubles only the energy row (both exactly,
+    a power-of-two scaling)."""
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+    mesh = M().make_tgv_periodic_mesh(16)
+    f = tgv_fields(mesh, noise=1e-2)
+    mv, U = _setup(mesh, f)
+
+    def ud(mu, k):
+        return O.calc_diff_term(mv, f, th, O.params(energy_law=O.ENERGY_CONST, mu_const=mu, k_const=k), U)[0]
+    base, m2, k2 = ud(0.05, 1.4), ud(0.1, 1.4), ud(0.05, 2.8)
+    assert np.abs(base[:, 1:]).min(0).max() > 0 and np.abs(base[:, 3]).max() > 1e-6
+    assert np.array_equal(m2[:, 1:], 2 * base[:, 1:])
+    assert np.array_equal(k2[:, 1:3], base[:, 1:3])
+    assert np.array_equal(k2[:, 3], 2 * base[:, 3])
+
+
+def test_slip_annulus_conserves_mass_and_energy():
+    """The reference step in a closed annulus of slip walls (the O-grid's
+    cylinder and outer circle): the walls' normals are oblique, so a wrong sign
+    in the reflection's n_x n_y term would carry mass; interior faces cancel
+    pairwise on this all-triangle mesh."""
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+    mesh = M().make_cylinder_ogrid_mesh(24, 6, 0.5, 2.0, 0.1)
+    f = random_fields(mesh)
+    mv, U = _setup(mesh, f)
+    prm = O.params(bcs={c: (O.BC_SLIP_WALL, ()) for c in (-1, -2, -3)})
+    _, UN = O.calc_one_stage_muscl_cpu_fast(mv, f, th, prm, U, None, 1.0, 1e-3, 0.0)
+    A = mesh.cell_areas
+    assert np.abs(UN - U).max() > 1e-4
+    for q in (0, 3):
+        assert abs((A * (UN[:, q] - U[:, q])).sum()) / (A * U[:, q]).sum() < 1e-14
