@@ -924,3 +924,84 @@ def test_slip_annulus_conserves_mass_and_energy():
     assert np.abs(UN - U).max() > 1e-4
     for q in (0, 3):
         assert abs((A * (UN[:, q] - U[:, q])).sum()) / (A * U[:, q]).sum() < 1e-14
+
+
+def test_ausm_plus_m2020_subsonic_known_answer():
+    """AUSMplusM2020Flux2dFast (AUSMflux2dFast.jl:234-284) on a pair chosen so
+    that every factor is exact: equal total enthalpies (both interface speeds
+    a*, a12 = a*), M_L = 1/3 and M_R = 0.2 along n = (1, 0), p_R = 3 p_L, no
+    tangential velocity.  By hand: M+(1/3) = (4/3)^2/4 + (1/9 - 1)^2/8 = 44/81,
+    P+(1/3) = (4/3)^2 (5/3)/4 + 3/16 (1/3)(1/9 - 1)^2 = 64/81, M-(0.2) = -0.2752,
+    P-(0.2) = 0.31744; f = (1 - cos(pi max|M|))/2 = 1/4 and
+    g = (1 + cos(pi min(p ratio)))/2 = 3/4 (cos(pi/3) = 1/2).  Then
+    M12 = M+ + M- - (1 - f)/2 (p_R - p_L) 2/(rho_L + rho_R)/a*^2 (1 - g),
+    p_u = -g gamma (p_L + p_R)/2/a* P+ P- (u_R - u_L), and the pressure
+    p_s = (p_L + p_R)/2 - (rho_L + rho_R)/4 a* (u_R - u_L); the mass flux
+    M12 a* rho_L is upwinded from L.  (F(U, U) sees none of f, g, p_u or the
+    diffusion in p_s.)"""
+    ht = 3.5
+    a = math.sqrt(2 * (G - 1) / (G + 1) * ht)
+    uL, uR = a / 3.0, 0.2 * a
+    rL = 1.0
+    pL = (ht - 0.5 * uL * uL) * (G - 1) / G * rL
+    pR = 3.0 * pL
+    rR = pR / ((ht - 0.5 * uR * uR) * (G - 1) / G)
+    Mp, Mm, Pp, Pm = 44.0 / 81.0, -0.2752, 64.0 / 81.0, 0.31744
+    f, g = 0.25, 0.75
+    M12 = Mp + Mm - 0.5 * (1 - f) * (pR - pL) * 2.0 / (rL + rR) / (a * a) * (1 - g)
+    mdot = M12 * a * rL
+    assert mdot > 0
+    pu = -g * G * (pL + pR) * 0.5 / a * Pp * Pm * (uR - uL)
+    ps = 0.5 * (pL + pR) - 0.25 * (rL + rR) * a * (uR - uL)
+    want = -np.array([mdot, mdot * uL + ps + pu, 0.0, mdot * ht]) * 1.7
+    got = O.ausm_plus_m2020_flux(np.array([rL, uL, 0.0, pL]), np.array([rR, uR, 0.0, pR]), 1.0, 0.0, 1.7, G)
+    assert np.allclose(got, want, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("quads", ["all", "none"])
+def test_reference_local_dt_known_answer(quads):
+    """The reference loop's CFL step (local policy): cfl min_i h_i / VMAX_i
+    with h = 2 area / perimeter, hx hy / (hx + hy) for an hx x hy rectangle and
+    the inscribed-circle radius hx hy / (hx + hy + diagonal) for a right
+    triangle; the cell with the largest VMAX sets it."""
+    mesh = M().make_box_mesh(5, 4, 1.0, 1.2, quads=quads)
+    hx, hy = 0.2, 0.3
+    h = hx * hy / (hx + hy) if quads == "all" else _tri_inradius(hx, hy)
+    f = M().fields2d.zeros(mesh.nCells, mesh.nNodes)
+    f.VMAXCells[:] = 2.5
+    f.VMAXCells[7] = 4.0
+    assert O.local_dt(O.MeshView(mesh), f, 0.7) == pytest.approx(0.7 * h / 4.0, rel=1e-14)
+
+
+def test_two_phase_stage_equals_the_fused_stage():
+    """calcOneStageMUSCL (every face's two states stored first, then the
+    fluxes summed per cell, calcOneStageMUSCL.jl:4-106) and
+    calcOneStageMUSCLcpuFast without the diff term compute the same update and
+    the same per-cell flux sums (FLUXES)."""
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+    mesh = M().make_box_mesh(12, 10, 1.0, 1.0, quads="mixed")
+    f = random_fields(mesh)
+    mv, U = _setup(mesh, f)
+    prm = O.params(bcs={-1: (O.BC_FIXED, (1.2, 0.3, -0.2, 2.0)), -2: (O.BC_EXTRAPOLATE, ()),
+                        -3: (O.BC_SLIP_WALL, ()), -4: (O.BC_EXTRAPOLATE, ())})
+    dt = 1e-3
+    FL, UN2 = O.calc_one_stage_muscl(mv, f, th, prm, U, 1.0, dt, 0.0)
+    FL1, UN1 = O.calc_one_stage_muscl_cpu_fast(mv, f, th, prm, U, None, 1.0, dt, 0.0)
+    assert np.abs(UN1 - U).max() > 1e-4
+    assert np.allclose(UN2, UN1, rtol=1e-14, atol=1e-15 * np.abs(U).max())
+    assert np.allclose(FL, FL1, rtol=1e-13, atol=1e-13 * np.abs(FL1).max())
+
+
+def test_reference_tau_known_answer():
+    """The reference driver's step, tau = CFL max(HX) / max(max(VMAX), 1e-6)
+    (testViscousFluxesTGV2d.jl:420-423): the longest edge over the fastest
+    cell, and the 1e-6 floor for a gas at rest."""
+    mesh = M().make_box_mesh(5, 4, 1.0, 1.2, quads="none")
+    f = M().fields2d.zeros(mesh.nCells, mesh.nNodes)
+    f.VMAXCells[:] = 2.5
+    f.VMAXCells[3] = 4.0
+    mv = O.MeshView(mesh)
+    hmax = float(np.max(mesh.HX))
+    assert O.ref_tau(mv, f, 0.5) == pytest.approx(0.5 * hmax / 4.0, rel=1e-15)
+    f.VMAXCells[:] = 0.0
+    assert O.ref_tau(mv, f, 0.5) == pytest.approx(0.5 * hmax / 1e-6, rel=1e-15)
