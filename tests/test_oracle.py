@@ -1005,3 +1005,34 @@ def test_reference_tau_known_answer():
     assert O.ref_tau(mv, f, 0.5) == pytest.approx(0.5 * hmax / 4.0, rel=1e-15)
     f.VMAXCells[:] = 0.0
     assert O.ref_tau(mv, f, 0.5) == pytest.approx(0.5 * hmax / 1e-6, rel=1e-15)
+
+
+def test_ns_isothermal_wall_shear_known_answer():
+    """Uniform flow u0 along a no-slip wall at the gas temperature: the
+    one-sided gradient du/dy = u0 / (hy/2) gives the wall shear
+    tau = mu(T_w) 2 u0 / hy, so each wall cell loses dt tau hx / (hx hy) of
+    x-momentum; the wall does no work (the face velocity is the wall's, 0) and
+    conducts no heat, so no energy, density or y-momentum changes anywhere, and
+    the force on the wall is (tau Lx, -p Lx)."""
+    g, Rg, mu_ref, T_ref, S = 1.4, NS_GAS["R"], NS_GAS["mu_ref"], NS_GAS["T_ref"], NS_GAS["S"]
+    mesh = M().make_box_mesh(6, 5, 1.5, 1.0, quads="all")
+    hx, hy, Lx = 0.25, 0.2, 1.5
+    n = mesh.nCells
+    p0, r0, u0 = 1.5, 1.2, 0.3
+    Tw = p0 / (r0 * Rg)
+    U = np.zeros((n, 4), order="F")
+    U[:, 0] = r0; U[:, 1] = r0 * u0; U[:, 3] = p0 / (g - 1) + 0.5 * r0 * u0 * u0
+    bcs = {-1: (O.BC_EXTRAPOLATE, ()), -2: (O.BC_EXTRAPOLATE, ()), -3: (O.BC_ISOTHERMAL_WALL, (Tw,)),
+           -4: (O.BC_EXTRAPOLATE, ())}
+    dt = 1e-3
+    UN, _, _, F = O.ns_step(O.MeshView(mesh), O.ns_params(gamma=g, bcs=bcs, **NS_GAS), dt, U)
+    mu = mu_ref * (Tw / T_ref) ** 1.5 * (T_ref + S) / (Tw + S)
+    tau = mu * 2 * u0 / hy
+    wall = mesh.cell_mid_points[:, 1] < hy
+    assert wall.sum() == 6
+    d = UN - U
+    assert np.allclose(d[wall, 1], -dt * tau * hx / (hx * hy), rtol=1e-9, atol=0)
+    assert np.abs(d[~wall, 1]).max() < 1e-14
+    assert np.abs(d[:, [0, 2, 3]]).max() < 1e-13
+    assert F[0] == pytest.approx(tau * Lx, rel=1e-12)
+    assert F[1] == pytest.approx(-p0 * Lx, rel=1e-12)
