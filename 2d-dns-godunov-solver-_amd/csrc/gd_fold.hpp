@@ -9,6 +9,9 @@
 
 namespace gd {
 
+// Whole-wave max, min and count by xor butterflies (every lane of the wave
+// active): max, min and an integer sum are exact, so every lane ends with the
+// same value whatever the order.
 __device__ __forceinline__ double wmax64(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
@@ -17,6 +20,11 @@ __device__ __forceinline__ double wmin64(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
   return v;
 }
+__device__ __forceinline__ unsigned wsum_u32(unsigned v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// whole-wave floating-point sum in a fixed butterfly order
 __device__ __forceinline__ double wsum64(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;

@@ -353,18 +353,6 @@ __global__ __launch_bounds__(kBlock) void k_ns_grad(NsKArgs<T> a) {
   }
 }
 
-__device__ __forceinline__ double wave_max64(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ double wave_min64(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 // viscous flux per unit length through a face with inward normal n, from the
 // face gradients of u, v, T (gf[var][x|y]) and face values u, v, T:
@@ -651,9 +639,9 @@ __global__ __launch_bounds__(kBlock) void k_ns_update(NsKArgs<T> a) {
   // block reduction of the next step's dt inputs into per-block partials
   __shared__ double s_max[kBlock / 64], s_min[kBlock / 64];
   __shared__ unsigned s_bad[kBlock / 64];
-  vmx = wave_max64(vmx);
-  hdt = wave_min64(hdt);
-  bad = wave_sum_u32(bad);
+  vmx = wmax64(vmx);
+  hdt = wmin64(hdt);
+  bad = wsum_u32(bad);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) { s_max[wid] = vmx; s_min[wid] = hdt; s_bad[wid] = bad; }
   __syncthreads();

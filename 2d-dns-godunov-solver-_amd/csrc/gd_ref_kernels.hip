@@ -19,18 +19,6 @@ namespace {
 
 constexpr int kBlock = kThreads;
 
-__device__ __forceinline__ double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ double wave_min(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 // Block-reduce the step's dt inputs and non-physical count and store the
 // block's partial (max |u|+a, min local dt, count) in part[3 b .. 3 b + 2];
@@ -39,9 +27,9 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
 __device__ __forceinline__ void reduce_acc(double vmax, double hdt, unsigned bad, double* part) {
   __shared__ double s_max[kBlock / 64], s_min[kBlock / 64];
   __shared__ unsigned s_bad[kBlock / 64];
-  vmax = wave_max(vmax);
-  hdt = wave_min(hdt);
-  bad = wave_sum_u(bad);
+  vmax = wmax64(vmax);
+  hdt = wmin64(hdt);
+  bad = wsum_u32(bad);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) { s_max[wid] = vmax; s_min[wid] = hdt; s_bad[wid] = bad; }
   __syncthreads();
