@@ -34,7 +34,7 @@ ip = C.POINTER(C.c_int32)
 EXPORTS = (
     "gd_mesh_create", "gd_mesh_info_get", "gd_mesh_cell_order", "gd_mesh_destroy", "gd_params_default", "gd_state_create",
     "gd_state_destroy", "gd_state_set_params", "gd_state_set_fields", "gd_state_get_fields", "gd_state_set_array",
-    "gd_state_get_array", "gd_calc_one_stage", "gd_calc_diff_term", "gd_update_variables", "gd_cells2nodes",
+    "gd_state_get_array", "gd_calc_one_stage", "gd_calc_face_states", "gd_calc_diff_term", "gd_update_variables", "gd_cells2nodes",
     "gd_run", "gd_sync", "gd_state_device_u", "gd_device_count", "gd_last_error", "gd_version",
     "gd_adjoint_run", "gd_state_device_adjoint",
 )
@@ -127,6 +127,7 @@ def load():
     L.gd_state_set_array.argtypes = [vp, C.c_int32, dp]
     L.gd_state_get_array.argtypes = [vp, C.c_int32, dp]
     L.gd_calc_one_stage.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_double]
+    L.gd_calc_face_states.argtypes = [vp, C.c_double, dp, dp]
     for f in (L.gd_calc_diff_term, L.gd_update_variables, L.gd_cells2nodes, L.gd_sync):
         f.argtypes = [vp]
     L.gd_run.argtypes = [vp, C.c_int32, C.POINTER(gd_run_params), C.POINTER(gd_run_stats)]
@@ -273,6 +274,19 @@ class DeviceState:
 
     def calc_one_stage(self, variant, betta, dt, flow_time):
         check(load().gd_calc_one_stage(self.handle, variant, betta, dt, flow_time))
+
+    def calc_face_states(self, flow_time, ULeft=None, URight=None):
+        """Face states of the two-phase stage: (4 slots, 4, n_cells) arrays
+        (slot k is the reference's (4, n_cells) ULeft_k / URight_k); slots past
+        a cell's faces keep what the given arrays hold (zeros when new)."""
+        nc = self.mesh.n_cells
+        UL = np.zeros((4, 4, nc)) if ULeft is None else ULeft
+        UR = np.zeros((4, 4, nc)) if URight is None else URight
+        # slot k's (4, n) column-major array is element [v, i] at 4 i + v
+        hl = np.ascontiguousarray(UL.transpose(0, 2, 1), dtype=np.float64)
+        hr = np.ascontiguousarray(UR.transpose(0, 2, 1), dtype=np.float64)
+        check(load().gd_calc_face_states(self.handle, flow_time, ptr(hl), ptr(hr)))
+        return hl.transpose(0, 2, 1), hr.transpose(0, 2, 1)
 
     def calc_diff_term(self):
         check(load().gd_calc_diff_term(self.handle))

@@ -242,6 +242,7 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
 
   gd_mesh* M = new gd_mesh();
   (void)hipGetDevice(&M->device);
+  M->nv_caller = nv;
   // --- cell order
   std::vector<int64_t>& cp = M->cperm;
   if (order == GD_ORDER_RCM) {
@@ -816,6 +817,37 @@ int gd_calc_one_stage(gd_state* S, int32_t variant, double betta, double dt, dou
   a.flow_time = flow_time;
   GD_HIP(launch_ref_step(a, S->stream));
   GD_HIP(hipStreamSynchronize(S->stream));
+  return GD_OK;
+}
+
+int gd_calc_face_states(gd_state* S, double flow_time, double* ULeft, double* URight) {
+  if (!S || !ULeft || !URight) return fail(GD_ERR_INVALID, "null argument");
+  int rc = need_ref(S);
+  if (rc) return rc;
+  const Mesh& M = *S->mesh;
+  const int64_t nc = M.v.nc;
+  if (!S->fstate && (rc = dev_alloc(S->allocs, &S->fstate, 2 * 16 * (size_t)nc))) return rc;
+  RefStepArgs a{};
+  a.m = M.v;
+  a.prm = S->prm;
+  a.Q = S->Q[S->qc];
+  a.Qn = S->Qn;
+  a.flow_time = flow_time;
+  double* dUL = S->fstate;
+  double* dUR = S->fstate + 16 * nc;
+  GD_HIP(launch_ref_face_states(a, dUL, dUR, S->stream));
+  // [slot][4][nc] device order -> per slot a (4, n_cells) column-major array
+  // in the caller's numbering; slots past a cell's faces keep their contents
+  std::vector<double> h(16 * (size_t)nc);
+  for (int side = 0; side < 2; ++side) {
+    if ((rc = get_cols(S, side ? dUR : dUL, h.data(), 16, false))) return rc;
+    double* out = side ? URight : ULeft;
+    for (int k = 0; k < 4; ++k)
+      for (int64_t i = 0; i < nc; ++i) {
+        if (k >= M.nv_caller[i]) continue;
+        for (int v = 0; v < 4; ++v) out[(int64_t)k * 4 * nc + 4 * i + v] = h[((int64_t)k * 4 + v) * nc + i];
+      }
+  }
   return GD_OK;
 }
 

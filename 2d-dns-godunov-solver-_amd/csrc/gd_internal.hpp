@@ -103,6 +103,7 @@ struct Mesh {
   int32_t max_cluster = 0;
   int device = 0;
   std::vector<int32_t> wall_codes;  // boundary code of each wall-list entry
+  std::vector<int32_t> nv_caller;   // vertices per cell in the caller's numbering
   // the same permutations on the device, for column transfers; the identity
   // is flagged and skipped
   const int32_t *d_cperm = nullptr, *d_ciperm = nullptr, *d_nperm = nullptr, *d_niperm = nullptr;
@@ -148,6 +149,7 @@ struct State {
   double* red = nullptr;                // reduction scratch
   double* xfer = nullptr;               // [4][max(nc, nn)] staging of host transfers
   double* grad0 = nullptr;              // [4][nc] dJ/dU0 of the last adjoint run (allocated by the first)
+  double* fstate = nullptr;             // [2][4 slots][4][nc] two-phase face states (allocated by the first request)
   std::vector<void*> allocs;
 };
 
@@ -176,6 +178,7 @@ struct RefStepArgs {
 };
 
 hipError_t launch_ref_step(const RefStepArgs& a, hipStream_t s);
+hipError_t launch_ref_face_states(const RefStepArgs& a, double* UL, double* UR, hipStream_t s);
 hipError_t launch_ref_nodes(const MeshView& m, const double* Q, const double* U, double* Qn, double* Un,
                             double* phiN, hipStream_t s);
 hipError_t launch_ref_phi(const MeshView& m, const double* U, double* phiN, hipStream_t s);

@@ -119,6 +119,49 @@ __device__ __forceinline__ void ref_udiff(const MeshView& m, const gd_params& p,
   ud[3] = mu * k * lap[2];
 }
 
+// The two reconstructed states of face k of cell i, computeInterfaceSlope
+// (computeslope2d.jl:44-126): fL = UpLeft (own side), fR = UpRight (the
+// neighbour's side, or the boundary ghost's with a first-order slope).
+// Returns the neighbour (>= 0) or code - 1.
+template <bool QUADS>
+__device__ __forceinline__ int32_t ref_face_states(const RefStepArgs& a, int64_t i, int k, int nf, const double* qL,
+                                                   double t, double* fL, double* fR) {
+  const MeshView& m = a.m;
+  const int64_t nc = m.nc;
+  const int64_t s = k * nc + i;
+  const int32_t j = m.nb[s];
+  double up[4], dn[4], qR[4];
+  if (j >= 0) {
+    const int32_t d1 = m.dn1[s], u1 = m.up1[s];
+    if (QUADS && nf == 4) {
+      const int32_t d2 = m.dn2[s], u2 = m.up2[s];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        up[v] = 0.5 * (a.Qn[v * m.nn + u1] + a.Qn[v * m.nn + u2]);
+        dn[v] = 0.5 * (a.Qn[v * m.nn + d1] + a.Qn[v * m.nn + d2]);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { up[v] = a.Qn[v * m.nn + u1]; dn[v] = a.Qn[v * m.nn + d1]; }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) qR[v] = a.Q[v * nc + j];
+  } else {
+    const int32_t na = m.ea[s], nbn = m.eb[s];
+    refm::bc_ghost(a.prm.bc, j + 1, qL, m.fnx[s], m.fny[s], m.xc[i], m.yc[i], m.xn[na], m.yn[na], m.xn[nbn],
+                   m.yn[nbn], t, a.prm.RGAS, qR);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dn[v] = qL[v]; up[v] = qR[v]; }
+  }
+  const double eps = a.prm.limiter_eps;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    fL[v] = qL[v] + 0.5 * refm::limiter(a.prm.limiter, qL[v] - dn[v], qR[v] - qL[v], eps);
+    fR[v] = qR[v] - 0.5 * refm::limiter(a.prm.limiter, qR[v] - qL[v], up[v] - qR[v], eps);
+  }
+  return j;
+}
+
 // One cell of computeInterfaceSlope (computeslope2d.jl:3-136) summed over its
 // faces as in SecondOrderUpwindM2 (SOUscheme.jl:56-92).
 template <bool QUADS>
@@ -131,40 +174,10 @@ __device__ __forceinline__ void ref_cell_flux(const RefStepArgs& a, int64_t i, i
   for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
     if (k < nf) {
       const int64_t s = k * nc + i;
-      const int32_t j = m.nb[s];
-      const double nx = m.fnx[s], ny = m.fny[s], len = m.flen[s];
-      double up[4], dn[4], qR[4];
-      if (j >= 0) {
-        const int32_t d1 = m.dn1[s], u1 = m.up1[s];
-        if (QUADS && nf == 4) {
-          const int32_t d2 = m.dn2[s], u2 = m.up2[s];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            up[v] = 0.5 * (a.Qn[v * m.nn + u1] + a.Qn[v * m.nn + u2]);
-            dn[v] = 0.5 * (a.Qn[v * m.nn + d1] + a.Qn[v * m.nn + d2]);
-          }
-        } else {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) { up[v] = a.Qn[v * m.nn + u1]; dn[v] = a.Qn[v * m.nn + d1]; }
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) qR[v] = a.Q[v * nc + j];
-      } else {
-        const int32_t na = m.ea[s], nbn = m.eb[s];
-        refm::bc_ghost(a.prm.bc, j + 1, qL, nx, ny, m.xc[i], m.yc[i], m.xn[na], m.yn[na], m.xn[nbn], m.yn[nbn], t,
-                       a.prm.RGAS, qR);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) { dn[v] = qL[v]; up[v] = qR[v]; }
-      }
-      const double eps = a.prm.limiter_eps;
       double fL[4], fR[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        fL[v] = qL[v] + 0.5 * refm::limiter(a.prm.limiter, qL[v] - dn[v], qR[v] - qL[v], eps);
-        fR[v] = qR[v] - 0.5 * refm::limiter(a.prm.limiter, qR[v] - qL[v], up[v] - qR[v], eps);
-      }
+      const int32_t j = ref_face_states<QUADS>(a, i, k, nf, qL, t, fL, fR);
       double e[4];
-      refm::face_flux(a.prm.flux, fR, fL, nx, ny, len, g, e);  // neighbour side is "L"
+      refm::face_flux(a.prm.flux, fR, fL, m.fnx[s], m.fny[s], m.flen[s], g, e);  // neighbour side is "L"
       if (j < 0 && wallF && m.wall_id && a.wall_code == j + 1) {
         const int32_t wid = m.wall_id[s];
         if (wid >= 0) { wallF[2 * wid] = e[1]; wallF[2 * wid + 1] = e[2]; }
@@ -176,6 +189,28 @@ __device__ __forceinline__ void ref_cell_flux(const RefStepArgs& a, int64_t i, i
 #pragma unroll
         for (int v = 0; v < 4; ++v) F[v] = F[v] + e[v];
       }
+    }
+  }
+}
+
+// Phase one of calcOneStageMUSCL (computeMUSCLStencilsCUDA.jl:3-28 with
+// computeInterfaceSlopeCUDA, computeslope2d.jl:139-269): every face slot's
+// two states, UL/UR = [slot][4][nc] (slots past a cell's faces untouched).
+template <bool QUADS>
+__global__ __launch_bounds__(kBlock) void k_ref_face_states(RefStepArgs a, double* UL, double* UR) {
+  const MeshView& m = a.m;
+  const int64_t nc = m.nc;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nc) return;
+  const double qL[4] = {a.Q[i], a.Q[nc + i], a.Q[2 * nc + i], a.Q[3 * nc + i]};
+  const int nf = QUADS ? m.nv[i] : 3;
+  for (int k = 0; k < nf; ++k) {
+    double fL[4], fR[4];
+    (void)ref_face_states<QUADS>(a, i, k, nf, qL, a.flow_time, fL, fR);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      UL[((int64_t)k * 4 + v) * nc + i] = fL[v];
+      UR[((int64_t)k * 4 + v) * nc + i] = fR[v];
     }
   }
 }
@@ -519,6 +554,14 @@ hipError_t launch_ref_step(const RefStepArgs& a, hipStream_t s) {
     if (q) GD_LAUNCH(true, false, false); else GD_LAUNCH(false, false, false);
   }
 #undef GD_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_face_states(const RefStepArgs& a, double* UL, double* UR, hipStream_t s) {
+  if (a.m.has_quads)
+    hipLaunchKernelGGL(k_ref_face_states<true>, dim3(nblocks(a.m.nc)), dim3(kBlock), 0, s, a, UL, UR);
+  else
+    hipLaunchKernelGGL(k_ref_face_states<false>, dim3(nblocks(a.m.nc)), dim3(kBlock), 0, s, a, UL, UR);
   return hipGetLastError();
 }
 

@@ -164,9 +164,11 @@ def calcOneStageMUSCL(betta, dtX, flowTime, testMesh, testfields2dX, thermo, cel
                       config: SchemeConfig = None):
     """Two-phase stage without the viscous term (the source comments it out).
 
-    The face-state work arrays ``ULeft*/URight*`` are scratch in the
-    reference; the device kernel keeps face states in registers and leaves
-    them untouched."""
+    The update itself runs as one fused kernel (face states in registers);
+    when the caller passes the face-state arrays ``ULeft1..4`` /
+    ``URight1..4`` (each ``(4, nCells)``), they are filled as the reference's
+    phase one fills them (computeMUSCLStencilsCUDA.jl:3-28): every slot of a
+    cell's faces, slot 4 of triangles left as given."""
     mask = _ranges(cellsThreadsX, testMesh.nCells)
     st = _ctx(testMesh).state_for((config or _config).params(thermo))
     _set_cells(st, testfields2dX)
@@ -175,6 +177,16 @@ def calcOneStageMUSCL(betta, dtX, flowTime, testMesh, testfields2dX, thermo, cel
     _copy_rows(UconsCellsNewX, st.get_array(L.GD_ARR_U_NEW), mask)
     if FLUXES is not None:
         _copy_rows(FLUXES, st.get_array(L.GD_ARR_FLUXES), mask)
+    lefts, rights = (ULeft1, ULeft2, ULeft3, ULeft4), (URight1, URight2, URight3, URight4)
+    if any(a is not None for a in lefts + rights):
+        nc = testMesh.nCells
+        cur = lambda arrs: np.stack([np.zeros((4, nc)) if a is None else np.asarray(a, dtype=np.float64)  # noqa: E731
+                                     for a in arrs])
+        UL, UR = st.calc_face_states(flowTime, cur(lefts), cur(rights))
+        for arrs, full in ((lefts, UL), (rights, UR)):
+            for k, a in enumerate(arrs):
+                if a is not None:
+                    a[:, mask] = full[k][:, mask]
 
 
 def calcOneStageCUDA(betta, dtX, flowTime, testMeshX, testfields2dX, thermoX, cellsThreadsX,  # noqa: N802

@@ -163,6 +163,12 @@ int gd_state_get_array(gd_state* st, int32_t which, double* host);
 #define GD_STAGE_MUSCL_FAST 0  /* calcOneStageMUSCLcpuFast / SecondOrderUpwindM2: adds betta*dt*Udiff */
 #define GD_STAGE_MUSCL 1       /* calcOneStageMUSCL / calcOneStageCUDA: no Udiff */
 int gd_calc_one_stage(gd_state* st, int32_t variant, double betta, double dt, double flow_time);
+/* phase one of calcOneStageMUSCL (calcOneStageMUSCL.jl:4-106 with
+   computeMUSCLStencilsCUDA, computeMUSCLStencilsCUDA.jl:3-28): the face
+   states ULeft1..4 / URight1..4 from the state's cell and node primitives at
+   flow_time; ULeft, URight: four (4, n_cells) column-major arrays each, slot
+   after slot, caller numbering; slots past a cell's faces are left as given */
+int gd_calc_face_states(gd_state* st, double flow_time, double* ULeft, double* URight);
 int gd_calc_diff_term(gd_state* st);
 int gd_update_variables(gd_state* st);
 int gd_cells2nodes(gd_state* st);

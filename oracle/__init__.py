@@ -104,7 +104,7 @@ def lib():
             C.POINTER(OrcThermo), C.POINTER(OrcParams), dp, dp, dp, dp]
         L.orc_calc_one_stage_muscl.argtypes = [
             ip, C.c_int32, C.c_double, C.c_double, C.c_double, C.POINTER(OrcMesh), C.POINTER(OrcFields),
-            C.POINTER(OrcThermo), C.POINTER(OrcParams), dp, dp, dp]
+            C.POINTER(OrcThermo), C.POINTER(OrcParams), dp, dp, dp, dp, dp]
         L.orc_calc_diff_term.argtypes = [ip, C.c_int32, C.POINTER(OrcMesh), C.POINTER(OrcFields),
                                          C.POINTER(OrcThermo), C.POINTER(OrcParams), dp, dp, dp]
         L.orc_green_gauss_gradient.argtypes = [ip, C.c_int32, C.POINTER(OrcMesh), dp, dp, dp]
@@ -283,15 +283,26 @@ def calc_one_stage_muscl_cpu_fast(mv: MeshView, fields, thermo, prm, UOld, Udiff
     return FL, UN
 
 
-def calc_one_stage_muscl(mv: MeshView, fields, thermo, prm, UOld, betta, dt, flow_time, n_threads=2):
+def calc_one_stage_muscl(mv: MeshView, fields, thermo, prm, UOld, betta, dt, flow_time, n_threads=2,
+                         face_states=False):
+    """calcOneStageMUSCL; with ``face_states`` also returns its phase-1
+    arrays ULeft1..4 and URight1..4, each ``(4, nCells)`` column-major, as
+    ``(4 slots, 4, nCells)`` numpy arrays."""
     m = mv.mesh
     t = threads_table(n_threads, m.nCells)
     FL = np.zeros((m.nCells, 4), order="F")
     UN = np.zeros((m.nCells, 4), order="F")
+    UL = np.zeros(16 * m.nCells) if face_states else None
+    UR = np.zeros(16 * m.nCells) if face_states else None
     f = fields_struct(fields); th = thermo_struct(thermo)
     _rc(lib().orc_calc_one_stage_muscl(_p(t, C.c_int32), n_threads, betta, dt, flow_time, C.byref(mv.c), C.byref(f),
-                                       C.byref(th), C.byref(prm), _p(_chk_f64(UOld)), _p(UN), _p(FL)), "two-phase step")
-    return FL, UN
+                                       C.byref(th), C.byref(prm), _p(_chk_f64(UOld)), _p(UN), _p(FL), _p(UL), _p(UR)),
+        "two-phase step")
+    if not face_states:
+        return FL, UN
+    # slot k's (4, n) column-major array: element [v, i] at 4 i + v
+    shape = lambda a: a.reshape(4, m.nCells, 4).transpose(0, 2, 1)  # noqa: E731
+    return FL, UN, shape(UL), shape(UR)
 
 
 def calc_diff_term(mv: MeshView, fields, thermo, prm, UC, Udiff_in=None, n_threads=2):

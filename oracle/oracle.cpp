@@ -10,6 +10,7 @@
 // mesh2d_Int32 (primeObjects.jl:109-135) and fields2d (primeObjects.jl:85-106).
 #include "oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -508,9 +509,9 @@ int orc_calc_one_stage_muscl_cpu_fast(const int32_t* cellsThreads, int32_t nThre
 // computeslope2d.jl:139-269); no Udiff (commented out in the source).
 int orc_calc_one_stage_muscl(const int32_t* cellsThreads, int32_t nThreads, double betta, double dt, double flowTime,
                              const orc_mesh* m, const orc_fields* f, const orc_thermo* th, const orc_params* prm,
-                             const double* UOld, double* UNew, double* FLUXES) {
+                             const double* UOld, double* UNew, double* FLUXES, double* ULeftOut, double* URightOut) {
   const int64_t n = m->nCells;
-  std::vector<double> UL(16 * n, 0.0), UR(16 * n, 0.0);  // (4, n) per face slot
+  std::vector<double> UL(16 * n, 0.0), UR(16 * n, 0.0);  // (4, n) per face slot: ULeft1..4, URight1..4
   int rc = for_partition(cellsThreads, nThreads, [&](int64_t i) {
     const double uLeftp[4] = {f->densityCells[i - 1], f->UxCells[i - 1], f->UyCells[i - 1], f->pressureCells[i - 1]};
     const int nf = I(m->mesh_connectivity, n, i, 3) == 4 ? 4 : 3;
@@ -519,6 +520,8 @@ int orc_calc_one_stage_muscl(const int32_t* cellsThreads, int32_t nThreads, doub
                      &UR[(k - 1) * 4 * n + 4 * (i - 1)]);
   });
   if (rc) return rc;
+  if (ULeftOut) std::copy(UL.begin(), UL.end(), ULeftOut);
+  if (URightOut) std::copy(UR.begin(), UR.end(), URightOut);
   return for_partition(cellsThreads, nThreads, [&](int64_t C) {
     double e[4][4] = {{0}};
     const int nf = I(m->mesh_connectivity, n, C, 3) == 4 ? 4 : 3;

@@ -11,13 +11,15 @@
  *
  * Pinning (tests/test_oracle.py, test_mesh.py): every function is checked
  * against hand-computed values, closed forms, invariants or an independent
- * library computation.  Parity unpinned, since the Julia reference cannot run
- * here: orc_calc_one_stage_muscl_cpu_fast / orc_calc_one_stage_muscl on a
- * non-trivial state are pinned through their parts and whole-step properties
- * (free stream, conservation, numbering independence, mirror symmetry), not
- * value by value against a reference run; orc_air_conductivity does not
- * reproduce the rounding of Julia's x^4 and pow; the reference's FP32 GPU
- * accumulation order (AUSMflux2dCUDAx32.jl:88-91) is not restated.
+ * library computation.  orc_calc_one_stage_muscl_cpu_fast /
+ * orc_calc_one_stage_muscl are pinned value by value by the closed form on a
+ * linear field (face states and flux sums), by the order of accuracy of their
+ * residual on a smooth field, and by whole-step properties (free stream,
+ * conservation, numbering independence, mirror symmetry).  Parity unpinned,
+ * since the Julia reference cannot run here: the rounding of Julia's x^4 and
+ * pow in orc_air_conductivity, and of its @fastmath node interpolation
+ * (utilsFVM2dp.jl:305), which may contract or reorder; the reference's FP32
+ * GPU accumulation order (AUSMflux2dCUDAx32.jl:88-91) is not restated.
  */
 #ifndef GODUNOV_ORACLE_H
 #define GODUNOV_ORACLE_H
@@ -99,7 +101,9 @@ int orc_calc_one_stage_muscl_cpu_fast(const int32_t* cellsThreads, int32_t nThre
                                       double* UNew);
 int orc_calc_one_stage_muscl(const int32_t* cellsThreads, int32_t nThreads, double betta, double dt, double flowTime,
                              const orc_mesh* m, const orc_fields* f, const orc_thermo* th, const orc_params* prm,
-                             const double* UOld, double* UNew, double* FLUXES);
+                             const double* UOld, double* UNew, double* FLUXES,
+                             double* ULeft /* optional: ULeft1..4, four (4, nCells) arrays */,
+                             double* URight /* optional: URight1..4 */);
 int orc_calc_diff_term(const int32_t* cellsThreads, int32_t nThreads, const orc_mesh* m, const orc_fields* f,
                        const orc_thermo* th, const orc_params* prm, const double* UC, double* Udiff,
                        double* lap3 /* optional nCells x 3: laplasUCu, laplasUCv, laplasUCe */);

@@ -249,3 +249,58 @@ def test_tgv_1000_steps_config0():
     print("1000-step rel Linf", err, "t", t, "last dt", st.last_dt, dt)
     assert err <= TOL_1000
     assert abs(s.flow_time - t) <= 1e-12 * t
+
+
+@pytest.mark.parametrize("quads", ["none", "mixed", "all"])
+def test_two_phase_face_states(quads):
+    """Phase one of calcOneStageMUSCL: the mirror fills ULeft1..4 / URight1..4
+    as the reference does (computeMUSCLStencilsCUDA.jl:3-28), bit for bit
+    with the oracle's face states, slot 4 of triangles untouched."""
+    S = pkg().stage
+    mesh, mv, th, f, U = _box_case(quads)
+    prm = _oparams(bcs=BOX_BCS, udiff=False)
+    _, UNo, ULo, URo = O.calc_one_stage_muscl(mv, f, th, prm, U, 1.0, 1e-3, 0.25, face_states=True)
+    n = mesh.nCells
+    ct = pkg().mesh.distibuteCellsInThreadsSA(3, n)
+    UL = [np.full((4, n), 7.0, order="F") for _ in range(4)]
+    UR = [np.full((4, n), -7.0, order="F") for _ in range(4)]
+    UN = np.zeros((n, 4), order="F")
+    S.calcOneStageMUSCL(1.0, 1e-3, 0.25, mesh, f, th, ct, U, None, UN, *UL, *UR, config=_cfg(S, bcs=BOX_BCS))
+    assert rel_linf(UN, UNo) <= TOL_STEP
+    nv = mesh.mesh_connectivity[:, 2]
+    for k in range(4):
+        has = nv > k
+        assert np.array_equal(UL[k][:, has], ULo[k][:, has])
+        assert np.array_equal(UR[k][:, has], URo[k][:, has])
+        assert np.all(UL[k][:, ~has] == 7.0) and np.all(UR[k][:, ~has] == -7.0)
+
+
+def test_linear_field_known_answer_on_device():
+    """The device step on a linear field equals the closed form that the
+    mathematics fixes (face states q(m) at every face midpoint, FLUXES the
+    exact Euler flux summed over the faces), on the interior cells where the
+    node means are exact; see test_oracle.linear_field_known_answer."""
+    import test_oracle as TO
+    S = pkg().stage
+    th = _thermo()
+    mesh, f, mv, U, sel = TO.linear_field_case(40, 32, 1.0, 0.8)
+    idx, qm, Fx = TO.linear_field_known_answer(mesh, sel)
+    assert len(idx) > 1000
+    n = mesh.nCells
+    bcs = {c: (O.BC_EXTRAPOLATE, ()) for c in (-1, -2, -3, -4)}
+    ct = pkg().mesh.distibuteCellsInThreadsSA(2, n)
+    UL = [np.zeros((4, n), order="F") for _ in range(4)]
+    UR = [np.zeros((4, n), order="F") for _ in range(4)]
+    UN = np.zeros((n, 4), order="F"); FL = np.zeros((n, 4), order="F")
+    dt = 1e-3
+    S.calcOneStageMUSCL(1.0, dt, 0.0, mesh, f, th, ct, U, None, UN, *UL, *UR, FLUXES=FL, config=_cfg(S, bcs=bcs))
+    for k in range(3):
+        assert np.abs(UL[k][:, idx] - qm[k]).max() < 1e-12
+        assert np.abs(UR[k][:, idx] - qm[k]).max() < 1e-12
+    scale = np.abs(Fx).max()
+    assert np.abs(FL[idx] - Fx).max() < 1e-10 * scale
+    A = mesh.cell_areas[idx, None]
+    assert np.abs(UN[idx] - (U[idx] - dt * Fx / A)).max() < 1e-12 * np.abs(U).max()
+    # the oracle agrees bit for bit on the whole mesh
+    FLo, _ = O.calc_one_stage_muscl(mv, f, th, _oparams(bcs=bcs, udiff=False), U, 1.0, dt, 0.0)
+    assert rel_linf(FL, FLo) <= TOL_STEP

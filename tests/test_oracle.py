@@ -4,7 +4,11 @@ Each check is chosen so that a plausible restatement mistake (a dropped
 term, a swapped L/R, a wrong sign or index, the wrong limiter branch) fails:
 hand-computed values, flux consistency, upwinding and symmetry identities,
 Rankine-Hugoniot jumps, free-stream preservation, conservation on periodic
-meshes, exactness on linear fields and second-order convergence.
+meshes, exactness on linear fields (the vertex-MUSCL step's face states and
+flux sums in closed form), the order of accuracy of the reference step's
+residual (first order per triangle, second order per triangle pair, as the
+stencil geometry fixes) and exact-arithmetic values of the source's c_p
+polynomials.
 """
 import math
 
@@ -1036,3 +1040,185 @@ def test_ns_isothermal_wall_shear_known_answer():
     assert np.abs(d[:, [0, 2, 3]]).max() < 1e-13
     assert F[0] == pytest.approx(tau * Lx, rel=1e-12)
     assert F[1] == pytest.approx(-p0 * Lx, rel=1e-12)
+
+
+# ------------------------------------------- the composite reference step, value by value
+
+LIN_COEF = np.array([[1.0, 0.30, 0.20],      # rho = a + b x + c y
+                     [0.4, 0.25, -0.15],     # u
+                     [-0.2, 0.10, 0.35],     # v
+                     [2.0, -0.40, 0.30]])    # p
+
+
+def lin_prims(x, y):
+    return LIN_COEF[:, :1] + LIN_COEF[:, 1:2] * x + LIN_COEF[:, 2:3] * y
+
+
+def linear_field_case(nx=10, ny=8, lx=1.0, ly=0.8):
+    """Non-periodic SW-NE triangle box with linear primitives at the
+    centroids, node values from the oracle's node interpolation.  Returns the
+    mesh, fields, MeshView, U and the cells whose three faces are interior and
+    whose six stencil nodes (own and neighbour opposite vertices) are interior
+    nodes with a full 6-cell cluster."""
+    mesh = M().make_box_mesh(nx, ny, lx, ly)
+    x, y = mesh.cell_mid_points[:, 0], mesh.cell_mid_points[:, 1]
+    f = M().fields2d.zeros(mesh.nCells, mesh.nNodes)
+    f.densityCells[:], f.UxCells[:], f.UyCells[:], f.pressureCells[:] = lin_prims(x, y)
+    mv, U = _setup(mesh, f)
+    full = (mesh.cell_clusters > 0).sum(1) == 6
+    interior = (mesh.cell_stiffness[:, :3] > 0).all(1)
+    dn = np.where(interior[:, None], mesh.node2cellsL2down[:, [0, 2, 4]] - 1, 0)
+    up = np.where(interior[:, None], mesh.node2cellsL2up[:, [0, 2, 4]] - 1, 0)
+    sel = interior & full[dn].all(1) & full[up].all(1)
+    return mesh, f, mv, U, sel
+
+
+def linear_field_known_answer(mesh, sel):
+    """Closed form of the reference MUSCL step on a linear field, per selected
+    cell: face states q(m_f) at each face midpoint and
+    FLUXES = sum_f -len_f F(q(m_f)) . n_f (n_f inward).
+
+    Why: on interior nodes of the SW-NE split the 6-cell cluster is centrally
+    symmetric and its inverse-distance weights come in equal pairs, so the node
+    mean of a linear field is exact.  Across face f of cell i, with v_i the
+    own opposite vertex and v_j the neighbour's, the two triangles form a
+    parallelogram (v_j = 2 m - v_i), so v_i, c_i, m, c_j, v_j lie on one line
+    at -1, -1/3, 0, 1/3, 1 times (m - v_i): uL - uDown = uR - uL = uUp - uR = d.
+    lim(d, d) = d up to the 1e-12 ksi (computeslope2d.jl:116), so
+    UpLeft = uL + d/2 = q(m) and UpRight = uR - d/2 = q(m); AUSM+ of equal
+    states is the exact Euler flux."""
+    c2n = mesh.cells2nodes - 1
+    xn, yn = mesh.xNodes, mesh.yNodes
+    idx = np.nonzero(sel)[0]
+    qm = np.zeros((3, 4, len(idx)))
+    F = np.zeros((len(idx), 4))
+    for k in range(3):
+        a, b = c2n[idx, 2 * k], c2n[idx, 2 * k + 1]
+        qm[k] = lin_prims(0.5 * (xn[a] + xn[b]), 0.5 * (yn[a] + yn[b]))
+        nx, ny, L = (mesh.cell_edges_Nx[idx, k], mesh.cell_edges_Ny[idx, k], mesh.cell_edges_length[idx, k])
+        F += (-L * exact_flux(qm[k], nx, ny)).T
+    return idx, qm, F
+
+
+def test_reference_muscl_linear_field_known_answer():
+    """The vertex-MUSCL reconstruction (computeslope2d.jl:116-126) and the
+    step (calcOneStageMUSCL, calcOneStageMUSCLcpuFast) on a linear field equal
+    the closed form of linear_field_known_answer.  A reconstruction mistake
+    made alike on both sides of a face keeps the scheme conservative and
+    passes the whole-step properties; it fails here: a 1.0 in place of the
+    0.5 puts each face state at q(m) + d/2, uUp and uDown swapped turn the
+    limiter's arguments to -2d and d (limited to 0: first order), a ksi of
+    1e-6 moves the face states by 2.5e-7."""
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+    mesh, f, mv, U, sel = linear_field_case()
+    assert sel.sum() >= 60
+    idx, qm, Fx = linear_field_known_answer(mesh, sel)
+    prm = O.params(bcs={c: (O.BC_EXTRAPOLATE, ()) for c in (-1, -2, -3, -4)})
+    dt = 1e-3
+    FL, UN, UL, UR = O.calc_one_stage_muscl(mv, f, th, prm, U, 1.0, dt, 0.0, face_states=True)
+    for k in range(3):
+        # the ksi term moves each state by d ksi / (2 (2|d| + ksi)) < 2.6e-13 here
+        assert np.abs(UL[k][:, idx] - qm[k]).max() < 1e-12
+        assert np.abs(UR[k][:, idx] - qm[k]).max() < 1e-12
+    scale = np.abs(Fx).max()
+    assert scale > 1e-3
+    assert np.abs(FL[idx] - Fx).max() < 1e-10 * scale
+    A = mesh.cell_areas[idx, None]
+    assert np.abs(UN[idx] - (U[idx] - dt * Fx / A)).max() < 1e-12 * np.abs(U).max()
+    # the fused variant with its Udiff term: same fluxes, update + betta dt Udiff
+    Ud = np.asfortranarray(np.outer(np.linspace(0.1, 0.3, mesh.nCells), [0.0, 1.0, -2.0, 3.0]))
+    FL2, UN2 = O.calc_one_stage_muscl_cpu_fast(mv, f, th, prm, U, Ud, 0.5, dt, 0.0)
+    assert np.abs(FL2[idx] - Fx).max() < 1e-10 * scale
+    exp2 = U[idx] - 0.5 * dt * Fx / A + 0.5 * dt * Ud[idx]
+    assert np.abs(UN2[idx] - exp2).max() < 1e-12 * np.abs(U).max()
+
+
+def _smooth_prims(x, y):
+    """Smooth primitives with no extremum in [0,1]^2 along the stencil
+    directions, so the harmonic limiter never clips."""
+    return np.array([1.0 + 0.3 * np.sin(0.9 * x + 0.4 * y) + 0.5 * x, 0.5 + 0.3 * x + 0.2 * y * y,
+                     -0.2 + 0.25 * y + 0.1 * np.sin(x), 2.0 + 0.4 * x + 0.3 * y + 0.1 * x * y])
+
+
+def test_reference_residual_order_of_accuracy():
+    """FLUXES of the reference step against the exact boundary integral of
+    the Euler flux of a smooth field (6-point Gauss-Legendre per edge), on
+    cells two rings inside a box refined 16 -> 32 -> 64.
+
+    On a quadratic field the face states are q(m) - (1/18) d2q/ds2 |m - v|^2
+    on both sides (the points -1, -1/3, 0, 1/3, 1 along the line v_i..v_j and
+    the harmonic limiter's (a-b)^2/(a+b) term of higher order), an O(h^2) face
+    error that depends on the face's direction.  A triangle's three faces have
+    three directions, so its residual error per area is O(h) and alternates in
+    sign between the two triangle orientations: first order per cell.  Summed
+    over the two triangles of a quad (the shared face cancels exactly) the
+    four outer faces come in opposite pairs of one direction each and the
+    O(h) parts cancel: second order.  A reconstruction made first order
+    (uUp/uDown swapped) or inconsistent (1.0 for 0.5) fails the pair ratio."""
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+    gx, gw = np.polynomial.legendre.leggauss(6)
+    e_cell, e_pair = [], []
+    for nq in (16, 32, 64):
+        mesh = M().make_box_mesh(nq, nq, 1.0, 1.0)
+        n = mesh.nCells
+        x, y = mesh.cell_mid_points[:, 0], mesh.cell_mid_points[:, 1]
+        f = M().fields2d.zeros(n, mesh.nNodes)
+        f.densityCells[:], f.UxCells[:], f.UyCells[:], f.pressureCells[:] = _smooth_prims(x, y)
+        mv, U = _setup(mesh, f)
+        prm = O.params(bcs={c: (O.BC_EXTRAPOLATE, ()) for c in (-1, -2, -3, -4)})
+        FL, _ = O.calc_one_stage_muscl(mv, f, th, prm, U, 1.0, 1e-3, 0.0)
+        c2n = mesh.cells2nodes - 1
+        xn, yn = mesh.xNodes, mesh.yNodes
+        ex = np.zeros((n, 4))
+        for k in range(3):
+            a, b = c2n[:, 2 * k], c2n[:, 2 * k + 1]
+            L, nx, ny = mesh.cell_edges_length[:, k], mesh.cell_edges_Nx[:, k], mesh.cell_edges_Ny[:, k]
+            for t, w in zip(gx, gw):
+                s = 0.5 * (t + 1.0)
+                xs, ys = xn[a] + s * (xn[b] - xn[a]), yn[a] + s * (yn[b] - yn[a])
+                ex += (-L * 0.5 * w * exact_flux(_smooth_prims(xs, ys), nx, ny)).T
+        h = 1.0 / nq
+        inner = (x > 2 * h) & (x < 1 - 2 * h) & (y > 2 * h) & (y < 1 - 2 * h)
+        A = mesh.cell_areas
+        err = FL - ex
+        e_cell.append(np.abs(err[inner] / A[inner, None]).mean(0))
+        pin = inner[0::2] & inner[1::2]          # the two triangles of one quad are consecutive
+        pair = (err[0::2] + err[1::2])[pin] / (A[0::2] + A[1::2])[pin, None]
+        e_pair.append(np.abs(pair).mean(0))
+    for a, b in zip(e_cell, e_cell[1:]):
+        assert np.all((a / b > 1.8) & (a / b < 2.2)), a / b
+    for a, b in zip(e_pair, e_pair[1:]):
+        assert np.all((a / b > 3.4) & (a / b < 4.6)), a / b
+    assert np.all(e_pair[-1] < 0.05 * e_cell[-1])
+
+
+def test_air_conductivity_known_answers_per_branch():
+    """k(T) of calcAirConductivityFromT (thermo.jl:57-60) with c_p from each
+    branch of calcAirHeatCapacityFromT (thermo.jl:43-54), against the
+    source's polynomials evaluated here in exact rational arithmetic with the
+    coefficients as printed in thermo.jl:17-40 (a slipped digit or exponent in
+    any coefficient moves c_p by far more than the 1e-12 allowed)."""
+    from fractions import Fraction as Fr
+    low = [Fr("2898903.0"), Fr("-56496.26"), Fr("1437.799"), Fr("-1.653609"), Fr("0.003062254"),
+           Fr("-2.279138e-6"), Fr("6.272365e-10")]
+    high = [Fr("6.932494e+7"), Fr("-361053.2"), Fr("1476.665"), Fr("-0.06138349"), Fr("2.027963e-5"),
+            Fr("-3.075525e-9"), Fr("1.888054e-13")]
+
+    def poly(c, T):
+        T = Fr(T)
+        return float(sum(ci * T ** (e - 2) for e, ci in enumerate(c)))
+
+    def k_of(cp, T):
+        mu = 1.716e-5 * (T / 273.11) ** (2.0 / 3.0)
+        return 15.0 / 4.0 * 8.314462 / 28.9647 * mu * (4.0 / 15.0 * cp * 28.9647 / 8.314462 + 1.0 / 3.0)
+    cases = [(150.0, poly(low, 200.0)),     # below 200 K: frozen at c_p(200)
+             (450.0, poly(low, 450.0)),     # 200 < T < 1000: low polynomial
+             (999.5, poly(low, 999.5)),
+             (1000.0, poly(high, 1000.0)),  # 1000 <= T < 6000: high polynomial
+             (3200.0, poly(high, 3200.0)),
+             (7000.0, poly(high, 6000.0)),  # from 6000 K: frozen at c_p(6000)
+             (200.0, poly(high, 6000.0))]   # T == 200 exactly falls through to the last branch
+    for T, cp in cases:
+        assert O.air_conductivity(T) == pytest.approx(k_of(cp, T), rel=1e-12), T
+    # and the numbers themselves: c_p of air, J/(kg K) (low branch at 300 K)
+    assert poly(low, 300.0) == pytest.approx(1004.75, abs=0.01)
