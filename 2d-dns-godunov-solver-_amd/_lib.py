@@ -36,7 +36,7 @@ EXPORTS = (
     "gd_state_destroy", "gd_state_set_params", "gd_state_set_fields", "gd_state_get_fields", "gd_state_set_array",
     "gd_state_get_array", "gd_calc_one_stage", "gd_calc_face_states", "gd_calc_diff_term", "gd_update_variables", "gd_cells2nodes",
     "gd_run", "gd_sync", "gd_state_device_u", "gd_device_count", "gd_last_error", "gd_version",
-    "gd_adjoint_run", "gd_state_device_adjoint",
+    "gd_adjoint_run", "gd_state_device_adjoint", "gd_diff_term_scratch_bytes", "gd_calc_diff_term_noalloc",
 )
 
 
@@ -134,12 +134,15 @@ def load():
     L.gd_state_device_u.argtypes = [vp, C.POINTER(dp)]
     L.gd_adjoint_run.argtypes = [vp, C.POINTER(gd_adjoint_params), dp, dp, dp, C.POINTER(gd_adjoint_stats)]
     L.gd_state_device_adjoint.argtypes = [vp, C.POINTER(dp)]
+    L.gd_diff_term_scratch_bytes.restype = C.c_size_t
+    L.gd_diff_term_scratch_bytes.argtypes = [C.c_int64, C.c_int64]
+    L.gd_calc_diff_term_noalloc.argtypes = [vp, C.POINTER(gd_params), vp, vp, vp, vp]
     L.gd_device_count.argtypes = [C.POINTER(C.c_int32)]
     L.gd_last_error.argtypes = [C.c_char_p, C.c_size_t]
     L.gd_version.restype = C.c_char_p
     for name in EXPORTS:
         fn = getattr(L, name)
-        if name not in ("gd_params_default", "gd_version"):
+        if name not in ("gd_params_default", "gd_version", "gd_diff_term_scratch_bytes"):
             fn.restype = C.c_int
     _lib = L
     return L

@@ -23,6 +23,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-t
 UNITS = {
     "gd_api.hip": [],
     "gd_ref_kernels.hip": ["-ffp-contract=off"],
+    "gd_noalloc.hip": [],
     "gd_ns_fp64.hip": ["-ffp-contract=off"],
     # FP32-mixed: hardware reciprocal / square root for float division and
     # sqrt; float denormals flushed, which drops the range-scaling steps from

@@ -572,7 +572,12 @@ int gd_mesh_destroy(gd_mesh* m) {
   return GD_OK;
 }
 
-static int check_params(const gd::Mesh* m, const gd_params* p) {
+namespace gd {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+int check_params(const Mesh* m, const gd_params* p);
+}  // namespace gd
+
+int gd::check_params(const gd::Mesh* m, const gd_params* p) {
   if (p->scheme != GD_SCHEME_REF && p->scheme != GD_SCHEME_NS) return fail(GD_ERR_INVALID, "unknown scheme");
   if (p->precision != GD_FP64 && p->precision != GD_FP32_MIXED) return fail(GD_ERR_INVALID, "unknown precision");
   if (p->scheme == GD_SCHEME_REF && p->precision != GD_FP64)

@@ -153,6 +153,10 @@ struct State {
   std::vector<void*> allocs;
 };
 
+// ---- shared by the C ABI's translation units (gd_api.hip) ----
+int set_error(int code, const std::string& msg);  // records gd_last_error's message, returns code
+int check_params(const Mesh* m, const gd_params* p);
+
 // ---- kernel launchers (gd_ref_kernels.hip) ----
 struct RefStepArgs {
   MeshView m;

@@ -170,6 +170,21 @@ int gd_calc_one_stage(gd_state* st, int32_t variant, double betta, double dt, do
    after slot, caller numbering; slots past a cell's faces are left as given */
 int gd_calc_face_states(gd_state* st, double flow_time, double* ULeft, double* URight);
 int gd_calc_diff_term(gd_state* st);
+/* calcDiffTerm on the caller's device arrays, allocating nothing.  No
+   counterpart in the reference: for callers that cannot allocate device
+   memory (work recorded into HIP graphs, tight time loops).  d_U (conservative
+   state) and d_Q (cell primitives rho, u, v, p) are 4 x n_cells SoA device
+   arrays in the mesh's device order (gd_mesh_cell_order); columns 1..3 of
+   d_Udiff (same layout) are written, column 0 is left as it is.  Enqueued on
+   `stream` (a hipStream_t, NULL for the null stream) without waiting; the
+   current device must be the mesh's.  The intermediates live in one static
+   device array of 256 MiB per device (held until the process ends), so a mesh
+   may need at most that: gd_diff_term_scratch_bytes(n_cells, n_nodes) <= 256 MiB
+   (about 3M triangles), else GD_ERR_INVALID.  Calls that overlap on one
+   device share that array and race. */
+size_t gd_diff_term_scratch_bytes(int64_t n_cells, int64_t n_nodes);
+int gd_calc_diff_term_noalloc(const gd_mesh* mesh, const gd_params* params, const double* d_U, const double* d_Q,
+                              double* d_Udiff, void* stream);
 int gd_update_variables(gd_state* st);
 int gd_cells2nodes(gd_state* st);
 
