@@ -163,6 +163,7 @@ def ref_tgv(n, steps, warmup):
                         RGAS=1.0, Gamma=g, bcs={})
     sol = S.GodunovSolver(mesh, prm, order=L.GD_ORDER_HILBERT)
     sol.set_primitives(rho, u, v, p)
+    init = (mesh, rho, u, v, p)
     log(f"reference scheme, TGV {n}x{n}x2: {mesh.nCells} cells, {mesh.nNodes} nodes "
         f"({time.perf_counter() - t0:.1f} s to set up)")
     if warmup:
@@ -184,7 +185,7 @@ def ref_tgv(n, steps, warmup):
             f"{k['bytes_per_launch'] / (k['ms_avg'] * 1e-3) / 1e9:8.1f} GB/s")
     log(f"reference scheme: {ms:.4f} ms/step, {mesh.nCells / (ms * 1e-3):.4e} cell-updates/s, "
         f"step {b / (ms * 1e-3) / 1e9:.0f} GB/s")
-    return {"workload": f"periodic Taylor-Green square, {n}x{n}x2 = {mesh.nCells} triangles, reference scheme "
+    return init, {"workload": f"periodic Taylor-Green square, {n}x{n}x2 = {mesh.nCells} triangles, reference scheme "
                         f"(MUSCL + Minmod_Limiter + AUSM+ + calcDiffTerm), FP64, CFL 0.5, Hilbert device order",
             "cells": mesh.nCells, "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 5),
             "value": round(mesh.nCells / (ms * 1e-3), 1), "unit": "cell-updates/s",
@@ -213,6 +214,60 @@ def measured_traffic(kernel_name):
         return None
 
 
+def host_info():
+    """The host the CPU baselines ran on: logical CPUs, the CPUs this process
+    may use, the OpenMP threads it was given, and the CPU model."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
+
+
+def ref_cpu_baseline(init, threads, warm=3, steps=20):
+    """The reference's own CPU time-loop body, as the oracle restates it
+    (C++/OpenMP, the reference's static partition of distibuteCellsInThreadsSA
+    over ``threads`` threads): calcDiffTerm + calcOneStageMUSCLcpuFast +
+    updateVariablesSA + the node interpolation, plus the local CFL step, on the
+    same TGV as the GPU reference leg; median of ``steps`` steps after
+    ``warm``."""
+    import oracle as O
+    mesh, rho, u, v, p = init
+    g = 1.4
+    n = mesh.nCells
+    mv = O.MeshView(mesh)
+    th = M.THERMOPHYSICS(RGAS=1.0, Gamma=g)
+    prm = O.params(energy_law=O.ENERGY_CONST, mu_const=1.0 / 1600, k_const=1.0 / 0.72)
+    f = M.fields2d.zeros(n, mesh.nNodes)
+    f.densityCells[:], f.UxCells[:], f.UyCells[:], f.pressureCells[:] = rho, u, v, p
+    U = O.phs2dcns(f, g, n)
+    O.update_variables(f, g, U.copy(order="F"), U, n_threads=threads)  # aSound, VMAX
+    O.cells2nodes(mv, f, U, n_threads=threads)
+    times, t = [], 0.0
+    for s_ in range(warm + steps):
+        t0 = time.perf_counter()
+        Ud, _ = O.calc_diff_term(mv, f, th, prm, U, n_threads=threads)
+        dt = O.local_dt(mv, f, 0.5)
+        _, UN = O.calc_one_stage_muscl_cpu_fast(mv, f, th, prm, U, Ud, 1.0, dt, t, n_threads=threads)
+        O.update_variables(f, g, UN, U, n_threads=threads)
+        O.cells2nodes(mv, f, U, n_threads=threads)
+        t += dt
+        if s_ >= warm:
+            times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": round(n / med, 1), "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "host": host_info(),
+            "sample": f"{steps} steps after {warm} of the reference's CPU loop body (calcDiffTerm + "
+                      f"calcOneStageMUSCLcpuFast + updateVariablesSA + node interpolation, oracle C++/OpenMP "
+                      f"restatement, the reference's static partition over {threads} threads) on the same "
+                      f"{n}-triangle TGV, {sum(times):.1f} s in all, median step {med:.3f} s"}
+
+
 def cpu_baseline(mesh, case, U0, sample_steps, threads):
     """The oracle's FP64 ns step (C++/OpenMP restatement) on the same mesh."""
     import oracle as O
@@ -231,20 +286,9 @@ def cpu_baseline(mesh, case, U0, sample_steps, threads):
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
     return {"value": round(mesh.nCells / med, 1), "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "host": host_info(),
             "sample": f"{sample_steps} FP64 ns steps of the same 2.08M-cell workload from the free stream "
                       f"(oracle C++/OpenMP restatement, {sum(times):.1f} s in all, median step {med:.3f} s)"}
-
-
-def copy_probe(nbytes):
-    """A plain device copy of known size, for scaling rocprofv3 FETCH/WRITE counters."""
-    import torch
-    n = nbytes // 4
-    x = torch.ones(n, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    for _ in range(3):
-        y.copy_(x)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes": n * 4}))
 
 
 def main():
@@ -263,9 +307,8 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.4)
     ap.add_argument("--variants", default="fp64,mixed,fp64-rcm,mixed-rcm,fp64-given,mixed-given")
     ap.add_argument("--cpu-sample-steps", type=int, default=30, help="about 10 s of CPU work on 16 threads")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: OMP_NUM_THREADS, else every CPU available")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-probe", type=int, default=0, help="bytes; run only a device copy of this size")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="0: skip the adjoint measurement")
     ap.add_argument("--adjoint-checkpoint", type=int, default=20)
     ap.add_argument("--adjoint-precision", choices=["mixed", "fp64"], default="mixed")
@@ -277,9 +320,6 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU path is not built",
               file=sys.stderr)
         sys.exit(2)
-    if a.copy_probe:
-        copy_probe(a.copy_probe)
-        return
     if L.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
@@ -358,10 +398,21 @@ def main():
             f"(forward sweep {ad['adjoint_forward_sweep_ms']:.1f} ms, reverse {ad['adjoint_reverse_ms']:.1f} ms = "
             f"{ad['reverse_step_vs_forward_step']:.2f}x per step); "
             f"all records kept: {ad['all_records']['adjoint_ms']:.1f} ms -> {ad['all_records']['cost_vs_forward']:.2f}x")
+    # CPU baselines: the OpenMP threads of the host's share for this GPU
+    # (OMP_NUM_THREADS where the launcher sets it, else every CPU this process
+    # may run on)
+    threads = a.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
     if a.ref_tgv > 0:
-        out["ref_scheme"] = ref_tgv(a.ref_tgv, a.ref_steps, min(a.warmup, 10))
+        init, out["ref_scheme"] = ref_tgv(a.ref_tgv, a.ref_steps, min(a.warmup, 10))
+        if not a.no_cpu:
+            try:
+                rc = ref_cpu_baseline(init, threads)
+                out["ref_scheme"]["cpu_baseline"] = rc
+                log(f"reference scheme CPU baseline: {rc['value']:.4e} cell-updates/s on {threads} threads")
+            except Exception as e:  # the baseline is reported, never required
+                out["ref_scheme"]["cpu_baseline"] = {"value": None, "error": str(e)}
+        del init
     if not a.no_cpu:
-        threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
         try:
             out["cpu_baseline"] = cpu_baseline(mesh, case, U0, a.cpu_sample_steps, threads)
         except Exception as e:  # the baseline is reported, never required

@@ -44,3 +44,36 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(L.gd_mesh_desc) == 8 + 8 + 4 + 4 + 17 * 8
     assert ctypes.sizeof(L.gd_adjoint_params) == 4 + 4 + 8 + 8 + 4 + 4 + 2 * 8 + 8
     assert ctypes.sizeof(L.gd_adjoint_stats) == 8 + 2 * 8 + 8 + 4 + 4 + 8 + 8 + 8
+
+
+def _c_layout(structs):
+    """sizeof and offsetof of every field of the given ctypes structures, as
+    the C compiler lays out the header's structs (gcc on a generated file)."""
+    import subprocess
+    import tempfile
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "godunov.h"', "int main(void) {"]
+    for st in structs:
+        name = st.__name__
+        lines.append(f'  printf("{name} sizeof %zu\\n", sizeof({name}));')
+        for f in st._fields_:
+            lines.append(f'  printf("{name} {f[0]} %zu\\n", offsetof({name}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "layout.c"), os.path.join(d, "layout")
+        open(src, "w").write("\n".join(lines) + "\n")
+        subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    return {tuple(line.split()[:2]): int(line.split()[2]) for line in out.splitlines()}
+
+
+def test_ctypes_layouts_match_the_c_compiler():
+    """Every ctypes mirror of a header struct has the C compiler's size and
+    field offsets (a reordering of equal-size fields fails here)."""
+    L = pkg()._lib
+    structs = [L.gd_mesh_desc, L.gd_mesh_info, L.gd_bc, L.gd_params, L.gd_run_params, L.gd_run_stats,
+               L.gd_adjoint_params, L.gd_adjoint_stats]
+    c = _c_layout(structs)
+    for st in structs:
+        assert ctypes.sizeof(st) == c[(st.__name__, "sizeof")], st.__name__
+        for f in st._fields_:
+            assert getattr(st, f[0]).offset == c[(st.__name__, f[0])], (st.__name__, f[0])
