@@ -149,6 +149,7 @@ struct gd_mesh : gd::Mesh {};
 struct gd_state : gd::State {
   std::vector<hipEvent_t> ev_pool;
 };
+static int materialize(gd_state* S);
 
 // The definitions below take C linkage from their declarations in godunov.h.
 
@@ -677,6 +678,8 @@ int gd_state_destroy(gd_state* S) {
 
 int gd_state_set_params(gd_state* S, const gd_params* p) {
   if (!S || !p) return fail(GD_ERR_INVALID, "null argument");
+  int rcm = materialize(S);
+  if (rcm) return rcm;
   int rc = check_params(S->mesh, p);
   if (rc) return rc;
   S->prm = *p;
@@ -713,9 +716,37 @@ static int get_cols(gd_state* S, const double* dev, double* host, int ncols, boo
   return GD_OK;
 }
 
+// The bookkeeping a run leaves pending: the reference loop's DeltaX of the
+// last step and UconsCellsNewX = UconsCellsOldX (evaluate2d.jl:100-104), and
+// for the ns scheme the FP64 fields2d primitives, aSoundCells and VMAXCells
+// of U (evaluate2d.jl:87-93).  Every entry point that reads or writes those
+// arrays calls this first; the time loop itself never needs them.
+static int materialize(gd_state* S) {
+  if (!S->pend_delta && !S->pend_derived) return GD_OK;
+  const MeshView& v = S->mesh->v;
+  if (S->pend_delta) {
+    GD_HIP(launch_delta(v.nc, S->U[S->cur], S->U[1 - S->cur], S->Dl, S->stream));
+    GD_HIP(launch_copy(S->U[1 - S->cur], S->U[S->cur], 4 * v.nc, S->stream));
+    S->pend_delta = false;
+  }
+  if (S->pend_derived) {
+    GD_HIP(launch_update_variables(v, S->prm.Gamma, S->U[S->cur], S->U[S->cur], S->Q[S->qc], S->aS, S->vmax, nullptr,
+                                   S->stream));
+    S->pend_derived = false;
+  }
+  return GD_OK;
+}
+
+#define GD_MATERIALIZE(S)                   \
+  do {                                      \
+    const int rc_m_ = materialize(S);       \
+    if (rc_m_) return rc_m_;                \
+  } while (0)
+
 int gd_state_set_fields(gd_state* S, const double* rC, const double* uC, const double* vC, const double* pC,
                         const double* rN, const double* uN, const double* vN, const double* pN) {
   if (!S) return fail(GD_ERR_INVALID, "null state");
+  GD_MATERIALIZE(S);
   const int64_t nc = S->mesh->v.nc, nn = S->mesh->v.nn;
   const double* cs[4] = {rC, uC, vC, pC};
   const double* ns[4] = {rN, uN, vN, pN};
@@ -733,6 +764,7 @@ int gd_state_set_fields(gd_state* S, const double* rC, const double* uC, const d
 int gd_state_get_fields(gd_state* S, double* rC, double* uC, double* vC, double* pC, double* aC, double* vmC,
                         double* rN, double* uN, double* vN, double* pN) {
   if (!S) return fail(GD_ERR_INVALID, "null state");
+  GD_MATERIALIZE(S);
   const int64_t nc = S->mesh->v.nc, nn = S->mesh->v.nn;
   double* cs[4] = {rC, uC, vC, pC};
   double* ns[4] = {rN, uN, vN, pN};
@@ -764,6 +796,7 @@ static double* array_ptr(gd_state* S, int32_t which, bool* nodes) {
 
 int gd_state_set_array(gd_state* S, int32_t which, const double* host) {
   if (!S || !host) return fail(GD_ERR_INVALID, "null argument");
+  GD_MATERIALIZE(S);
   bool nodes;
   double* p = array_ptr(S, which, &nodes);
   if (!p) return fail(GD_ERR_INVALID, "unknown array");
@@ -772,6 +805,7 @@ int gd_state_set_array(gd_state* S, int32_t which, const double* host) {
 
 int gd_state_get_array(gd_state* S, int32_t which, double* host) {
   if (!S || !host) return fail(GD_ERR_INVALID, "null argument");
+  GD_MATERIALIZE(S);
   bool nodes;
   double* p = array_ptr(S, which, &nodes);
   if (!p) return fail(GD_ERR_INVALID, "unknown array");
@@ -804,6 +838,7 @@ static int need_ref(gd_state* S) {
 
 int gd_calc_one_stage(gd_state* S, int32_t variant, double betta, double dt, double flow_time) {
   if (!S) return fail(GD_ERR_INVALID, "null state");
+  GD_MATERIALIZE(S);
   int rc = need_ref(S);
   if (rc) return rc;
   if (variant != GD_STAGE_MUSCL_FAST && variant != GD_STAGE_MUSCL) return fail(GD_ERR_INVALID, "unknown variant");
@@ -827,6 +862,7 @@ int gd_calc_one_stage(gd_state* S, int32_t variant, double betta, double dt, dou
 
 int gd_calc_face_states(gd_state* S, double flow_time, double* ULeft, double* URight) {
   if (!S || !ULeft || !URight) return fail(GD_ERR_INVALID, "null argument");
+  GD_MATERIALIZE(S);
   int rc = need_ref(S);
   if (rc) return rc;
   const Mesh& M = *S->mesh;
@@ -858,6 +894,7 @@ int gd_calc_face_states(gd_state* S, double flow_time, double* ULeft, double* UR
 
 int gd_calc_diff_term(gd_state* S) {
   if (!S) return fail(GD_ERR_INVALID, "null state");
+  GD_MATERIALIZE(S);
   int rc = need_ref(S);
   if (rc) return rc;
   const MeshView& v = S->mesh->v;
@@ -871,6 +908,7 @@ int gd_calc_diff_term(gd_state* S) {
 
 int gd_update_variables(gd_state* S) {
   if (!S) return fail(GD_ERR_INVALID, "null state");
+  GD_MATERIALIZE(S);
   const MeshView& v = S->mesh->v;
   GD_HIP(launch_update_variables(v, S->prm.Gamma, S->U[1 - S->cur], S->U[S->cur], S->Q[S->qc], S->aS, S->vmax, S->Dl,
                                  S->stream));
@@ -880,6 +918,7 @@ int gd_update_variables(gd_state* S) {
 
 int gd_cells2nodes(gd_state* S) {
   if (!S) return fail(GD_ERR_INVALID, "null state");
+  GD_MATERIALIZE(S);
   const MeshView& v = S->mesh->v;
   GD_HIP(launch_ref_nodes(v, S->Q[S->qc], S->U[S->cur], S->Qn, S->Un, nullptr, S->stream));
   GD_HIP(hipStreamSynchronize(S->stream));
@@ -988,17 +1027,24 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
   Timer T{S, rp->time_kernels != 0, stats};
   const bool wall = rp->wall_code != 0 && v.n_wall > 0;
   const bool ns = P.scheme == GD_SCHEME_NS;
-  if (ns)  // the ns scheme advances U; its double primitives are recovered from U
-    GD_HIP(launch_update_variables(v, P.Gamma, S->U[S->cur], S->U[S->cur], S->Q[S->qc], S->aS, S->vmax, nullptr,
-                                   S->stream));
-  else
-    GD_HIP(launch_fields_derived(v.nc, P.Gamma, S->Q[S->qc], S->aS, S->vmax, S->stream));
-  GD_HIP(launch_prime_acc(v, P, S->Q[S->qc], S->vmax, S->acc, S->part, rp->flow_time, rp->dt_policy,
-                          ns && P.viscous, S->stream));
+  const bool mixed = P.precision == GD_FP32_MIXED;
   const int64_t nparts = (v.nc + kThreads - 1) / kThreads;
+  // a run that steps supersedes the previous run's pending Delta / U_NEW
+  if (n_steps > 0) S->pend_delta = false;
   if (wall) {
     GD_HIP(launch_fill(S->wall_f, 2 * (int64_t)v.n_wall, 0.0, S->stream));
     GD_HIP(launch_fill(S->drag, 4, 0.0, S->stream));
+  }
+  if (ns) {
+    // the ns scheme advances U: its compute-type primitives and the first dt
+    // inputs come from U in one pass (folded by the first gradient launch)
+    GD_HIP(launch_ns_prime(v, P, S->U[S->cur], S->Qt[0], mixed, S->acc, S->part, rp->flow_time, P.viscous,
+                           S->stream));
+    if (n_steps == 0) GD_HIP(launch_acc_fold(S->part, nparts, &S->acc[0], S->stream));
+  } else {
+    GD_HIP(launch_fields_derived(v.nc, P.Gamma, S->Q[S->qc], S->aS, S->vmax, S->stream));
+    GD_HIP(launch_prime_acc(v, P, S->Q[S->qc], S->vmax, S->acc, S->part, rp->flow_time, rp->dt_policy, 0,
+                            S->stream));
   }
   if (!ns) {
     const bool visc = P.viscous != 0;
@@ -1029,23 +1075,28 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       a.wall_f = wall ? S->wall_f : nullptr;
       a.wall_code = rp->wall_code;
       GD_HIP(T.run("ref_step", bytes_ref_step(v, visc), [&] { return launch_ref_step(a, S->stream); }));
-      GD_HIP(launch_acc_fold(S->part, nparts, &S->acc[(s + 1) % 3], S->stream));
-      if (wall) GD_HIP(launch_wall_sum(S->wall_f, v.n_wall, S->drag, S->stream));
       S->cur ^= 1;
       S->qc ^= 1;
+      // the node pass also closes the step: dt partials into the next slot,
+      // the wall faces' forces into drag
+      FoldArgs fa;
+      fa.part = S->part;
+      fa.nparts = nparts;
+      fa.slot = &S->acc[(s + 1) % 3];
+      fa.wall_f = wall ? S->wall_f : nullptr;
+      fa.n_wall = v.n_wall;
+      fa.drag = S->drag;
       GD_HIP(T.run("ref_nodes", bytes_ref_nodes(v, visc), [&] {
-        return launch_ref_nodes(v, S->Q[S->qc], S->U[S->cur], S->Qn, nullptr, visc ? S->phiN : nullptr, S->stream);
+        return launch_ref_nodes(v, S->Q[S->qc], S->U[S->cur], S->Qn, nullptr, visc ? S->phiN : nullptr, S->stream,
+                                fa);
       }));
     }
   } else {
-    const bool mixed = P.precision == GD_FP32_MIXED;
     const double t = mixed ? 4.0 : 8.0;
-    auto prims = mixed ? launch_ns_prims_f32 : launch_ns_prims_f64;
     auto grad = mixed ? launch_ns_grad_f32 : launch_ns_grad_f64;
     auto face = mixed ? launch_ns_face_f32 : launch_ns_face_f64;
     auto update = mixed ? launch_ns_update_f32 : launch_ns_update_f64;
     int tq = 0;
-    GD_HIP(prims(v.nc, P.Gamma, S->U[S->cur], S->Qt[tq], S->stream));
     for (int32_t s = 0; s < n_steps; ++s) {
       NsStepArgs a{};
       a.m = v;
@@ -1064,7 +1115,7 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       a.cfl = rp->cfl;
       a.wall_f = wall ? S->wall_f : nullptr;
       a.wall_code = rp->wall_code;
-      a.fold_prev = s > 0;
+      a.fold_prev = 1;  // step 0 folds the prime pass's partials (wall_f is still zero then)
       a.nparts = nparts;
       a.drag = S->drag;
       GD_HIP(T.run(mixed ? "ns_grad_fp32" : "ns_grad_fp64", bytes_ns_grad(v, t), [&] { return grad(a, S->stream); }));
@@ -1078,14 +1129,13 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       GD_HIP(launch_acc_fold(S->part, nparts, &S->acc[n_steps % 3], S->stream));
       if (wall) GD_HIP(launch_wall_sum(S->wall_f, v.n_wall, S->drag, S->stream));
     }
-    GD_HIP(launch_update_variables(v, P.Gamma, S->U[S->cur], S->U[S->cur], S->Q[S->qc], S->aS, S->vmax, nullptr,
-                                   S->stream));
+    S->pend_derived = true;  // fields2d's FP64 primitives of U, when someone asks
   }
-  // reference bookkeeping after the loop: Delta of the last step, U_NEW == U_OLD
+  // the reference's bookkeeping after the step (Delta of the last step,
+  // U_NEW == U_OLD) is left pending for the caller that reads it
   if (n_steps > 0) {
-    GD_HIP(launch_delta(v.nc, S->U[S->cur], S->U[1 - S->cur], S->Dl, S->stream));
     if (rp->residuals) GD_HIP(launch_residuals(v, S->U[S->cur], S->U[1 - S->cur], S->red, S->stream));
-    GD_HIP(launch_copy(S->U[1 - S->cur], S->U[S->cur], 4 * v.nc, S->stream));
+    S->pend_delta = true;
   }
   Acc acc[3];
   GD_HIP(hipMemcpyAsync(acc, S->acc, sizeof(acc), hipMemcpyDeviceToHost, S->stream));
@@ -1140,6 +1190,7 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
     return fail(GD_ERR_UNSUPPORTED, "the adjoint covers the AUSM+ and Roe fluxes");
   if (ap->n_steps < 1 || ap->checkpoint_every < 1) return fail(GD_ERR_INVALID, "n_steps and checkpoint_every >= 1");
   if (!(ap->dt > 0.0)) return fail(GD_ERR_INVALID, "the adjoint needs a fixed dt > 0");
+  GD_MATERIALIZE(S);
   const MeshView& v = S->mesh->v;
   const int64_t nc = v.nc;
   const bool mixed = P.precision == GD_FP32_MIXED;
@@ -1192,8 +1243,11 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
   auto acell = mixed ? launch_adj_cell_f32 : launch_adj_cell_f64;
   auto astate = mixed ? launch_adj_state_f32 : launch_adj_state_f64;
 
-  // one forward step s from Uo into Un, the record into its segment slot
-  auto fstep = [&](int32_t s, const double* Uo, double* Un, int tq, bool first, bool with_wall) -> int {
+  // one forward step s from Uo into Un, the record into its segment slot;
+  // records_only: the record alone (the last step of a recomputed segment,
+  // whose U_{s+1} is the next checkpoint)
+  auto fstep = [&](int32_t s, const double* Uo, double* Un, int tq, bool fold, bool with_wall,
+                   bool records_only = false) -> int {
     NsStepArgs a{};
     a.m = v;
     a.prm = P;
@@ -1211,10 +1265,11 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
     a.cfl = 0.0;
     a.wall_f = with_wall ? S->wall_f : nullptr;
     a.wall_code = ap->wall_code;
-    a.fold_prev = first ? 0 : 1;
+    a.fold_prev = fold ? 1 : 0;
     a.nparts = nparts;
     a.drag = S->drag;
     GD_HIP(grad(a, S->stream));
+    if (records_only) return GD_OK;
     GD_HIP(face(a, S->stream));
     GD_HIP(update(a, S->stream));
     return GD_OK;
@@ -1229,19 +1284,17 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
     }
   } evg{ev};
 
-  // ---- forward sweep
+  // ---- forward sweep (as gd_run: the prime pass, each gradient launch
+  // folding the step before it)
   GD_HIP(hipEventRecord(ev[0], S->stream));
-  GD_HIP(launch_update_variables(v, P.Gamma, S->U[S->cur], S->U[S->cur], S->Q[S->qc], S->aS, S->vmax, nullptr,
-                                 S->stream));
-  GD_HIP(launch_prime_acc(v, P, S->Q[S->qc], S->vmax, S->acc, S->part, ap->flow_time, GD_DT_FIXED, P.viscous,
-                          S->stream));
   GD_HIP(launch_fill(S->wall_f, 2 * nw, 0.0, S->stream));
   GD_HIP(launch_fill(S->drag, 4, 0.0, S->stream));
   int tq = 0;
-  GD_HIP(prims(nc, P.Gamma, S->U[S->cur], S->Qt[tq], S->stream));
+  GD_HIP(launch_ns_prime(v, P, S->U[S->cur], S->Qt[tq], mixed, S->acc, S->part, ap->flow_time, P.viscous,
+                         S->stream));
   for (int32_t s = 0; s < N; ++s) {
     if (s % c == 0) GD_HIP(launch_copy(ck + (size_t)(s / c) * 4 * nc, S->U[S->cur], 4 * nc, S->stream));
-    if ((rc = fstep(s, S->U[S->cur], S->U[1 - S->cur], tq, s == 0, wall))) return rc;
+    if ((rc = fstep(s, S->U[S->cur], S->U[1 - S->cur], tq, true, wall))) return rc;
     S->cur ^= 1;
     tq ^= 1;
   }
@@ -1296,9 +1349,14 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
       GD_HIP(launch_copy(Uw0, ck + (size_t)seg * 4 * nc, 4 * nc, S->stream));
       double* Uw[2] = {Uw0, Uw1};
       int wc = 0, rq = 0;
-      GD_HIP(prims(nc, P.Gamma, Uw[wc], S->Qt[rq], S->stream));
+      // the primitives of the checkpoint exactly as the forward sweep had them:
+      // from the prime pass at step 0, from the update pass after
+      if (s0 == 0)
+        GD_HIP(launch_ns_prime(v, P, Uw[wc], S->Qt[rq], mixed, S->acc, S->part, ap->flow_time, P.viscous, S->stream));
+      else
+        GD_HIP(prims(nc, P.Gamma, Uw[wc], S->Qt[rq], S->stream));
       for (int32_t s = s0; s < s1; ++s) {
-        if ((rc = fstep(s, Uw[wc], Uw[1 - wc], rq, true, false))) return rc;
+        if ((rc = fstep(s, Uw[wc], Uw[1 - wc], rq, false, false, s == s1 - 1))) return rc;
         wc ^= 1;
         rq ^= 1;
         ++recomputed;
@@ -1329,10 +1387,9 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
       for (int q = 0; q < 4; ++q) dJ_dbc[4 * idx + q] += bs[4 * (size_t)w + q];
     }
   }
-  // the state holds U_N, as after gd_run
-  GD_HIP(launch_update_variables(v, P.Gamma, S->U[S->cur], S->U[S->cur], S->Q[S->qc], S->aS, S->vmax, nullptr,
-                                 S->stream));
-  GD_HIP(launch_copy(S->U[1 - S->cur], S->U[S->cur], 4 * nc, S->stream));
+  // the state holds U_N, as after gd_run (its bookkeeping pending likewise)
+  S->pend_delta = N > 0;
+  S->pend_derived = true;
   GD_HIP(hipStreamSynchronize(S->stream));
   if (stats) {
     float f_ms = 0.f, r_ms = 0.f;

@@ -150,6 +150,9 @@ struct State {
   double* xfer = nullptr;               // [4][max(nc, nn)] staging of host transfers
   double* grad0 = nullptr;              // [4][nc] dJ/dU0 of the last adjoint run (allocated by the first)
   double* fstate = nullptr;             // [2][4 slots][4][nc] two-phase face states (allocated by the first request)
+  // bookkeeping a run leaves to whoever reads it (done by the next call that does):
+  bool pend_delta = false;    // Dl = U[cur] - U[1-cur] of the last step, then U[1-cur] = U[cur]
+  bool pend_derived = false;  // Q[qc], aS, vmax from U[cur] (updateVariablesSA)
   std::vector<void*> allocs;
 };
 
@@ -181,10 +184,25 @@ struct RefStepArgs {
   int32_t wall_code;
 };
 
+// what a launch closes of the step before it (see k_ref_nodes); slot null: nothing
+struct FoldArgs {
+  const double* part = nullptr;   // per-block dt partials
+  int64_t nparts = 0;
+  Acc* slot = nullptr;            // the accumulator slot they fold into
+  const double* wall_f = nullptr; // wall-face forces of the step (nullptr: no wall)
+  int32_t n_wall = 0;
+  double* drag = nullptr;         // {fx, fy, running sums}
+};
+
 hipError_t launch_ref_step(const RefStepArgs& a, hipStream_t s);
 hipError_t launch_ref_face_states(const RefStepArgs& a, double* UL, double* UR, hipStream_t s);
 hipError_t launch_ref_nodes(const MeshView& m, const double* Q, const double* U, double* Qn, double* Un,
-                            double* phiN, hipStream_t s);
+                            double* phiN, hipStream_t s, const FoldArgs& fold = FoldArgs{});
+// start of the ns loop: resets the accumulators to flow_time, writes the
+// compute-type primitives of U into Qt and the first step's dt partials into
+// part (folded by the first gradient launch)
+hipError_t launch_ns_prime(const MeshView& m, const gd_params& prm, const double* U, void* Qt, bool f32, Acc* acc,
+                           double* part, double flow_time, int32_t viscous_ns, hipStream_t s);
 hipError_t launch_ref_phi(const MeshView& m, const double* U, double* phiN, hipStream_t s);
 hipError_t launch_ref_gg1(const MeshView& m, const double* phiN, double* g6, hipStream_t s);
 hipError_t launch_ref_gg_nodes(const MeshView& m, const double* g6, double* g6N, hipStream_t s);

@@ -276,10 +276,18 @@ __global__ __launch_bounds__(kBlock) void k_ref_step(RefStepArgs a) {
 
 // cells2nodesSolutionReconstructionWithStencils (utilsFVM2dp.jl:299-365) plus,
 // when phiN is given, the unweighted cluster means of calcPhiInCellCluster
-// (calcGradients.jl:3-18) for rho*u, rho*v, E.
+// (calcGradients.jl:3-18) for rho*u, rho*v, E.  With fa.slot set, two leading
+// blocks close the step that ran before: the dt partials into fa.slot and the
+// wall faces' forces into fa.drag.
 __global__ __launch_bounds__(kBlock) void k_ref_nodes(MeshView m, const double* Q, const double* U, double* Qn,
-                                                      double* Un, double* phiN) {
-  const int64_t J = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+                                                      double* Un, double* phiN, FoldArgs fa) {
+  const unsigned nfold = fa.slot ? 2u : 0u;
+  if (blockIdx.x < nfold) {
+    if (blockIdx.x == 0) fold_acc_block(fa.part, fa.nparts, fa.slot);
+    else if (fa.wall_f) wall_sum_block(fa.wall_f, fa.n_wall, fa.drag);
+    return;
+  }
+  const int64_t J = (int64_t)xcd_block(blockIdx.x - nfold, gridDim.x - nfold) * kBlock + threadIdx.x;
   if (J >= m.nn) return;
   const int64_t nc = m.nc, nn = m.nn;
   double det = 0.0, r = 0.0, u = 0.0, v = 0.0, p = 0.0;
@@ -416,6 +424,51 @@ __global__ __launch_bounds__(kBlock) void k_prime_acc(MeshView m, gd_params p, c
       const double T = pr / (rho * p.RGAS);
       const double t = T / p.T_ref;
       const double mu = p.mu_ref * (t * sqrt(t)) * (p.T_ref + p.S_suth) / (T + p.S_suth);
+      lam = vm + 2.0 * p.Gamma * mu / (p.Pr * rho * m.hmin[i]);
+    }
+    hdt = m.hmin[i] / lam;
+    bad = (!(rho > 0.0) || !(pr > 0.0) || vm != vm) ? 1u : 0u;
+    if (vm != vm) { vm = 0.0; hdt = __longlong_as_double(0x7ff0000000000000ll); }
+  }
+  reduce_acc(vm, hdt, bad, part);
+}
+
+// The start of the ns loop from U alone: the primitives in the compute type
+// (the gradient pass's input) and the first step's dt inputs, computed as
+// updateVariablesSA's primitives and k_prime_acc's reduction would be
+// (evaluate2d.jl:87-93), without storing the FP64 fields.
+template <class T>
+__device__ __forceinline__ void st_prims(T* q, double r, double u, double v, double p);
+template <>
+__device__ __forceinline__ void st_prims<double>(double* q, double r, double u, double v, double p) {
+  const double o[4] = {r, u, v, p};
+  st2k<2>(q, o);
+}
+template <>
+__device__ __forceinline__ void st_prims<float>(float* q, double r, double u, double v, double p) {
+  *reinterpret_cast<float4*>(q) = make_float4((float)r, (float)u, (float)v, (float)p);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_ns_prime(MeshView m, gd_params p, const double* U, T* Qt, double* part,
+                                                     int32_t viscous_ns) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double vm = 0.0, hdt = __longlong_as_double(0x7ff0000000000000ll);
+  unsigned bad = 0;
+  if (i < m.nc) {
+    const int64_t nc = m.nc;
+    const double G = p.Gamma;
+    const double u0 = U[i], u1 = U[nc + i], u2 = U[2 * nc + i], u3 = U[3 * nc + i];
+    const double rho = u0, ux = u1 / u0, uy = u2 / u0;
+    const double pr = (G - 1.0) * (u3 - 0.5 * (u1 * u1 + u2 * u2) / u0);
+    st_prims<T>(Qt + 4 * i, rho, ux, uy, pr);
+    const double as = sqrt(G * pr / rho);
+    vm = sqrt(ux * ux + uy * uy) + as;
+    double lam = vm;
+    if (viscous_ns) {
+      const double Tc = pr / (rho * p.RGAS);
+      const double t = Tc / p.T_ref;
+      const double mu = p.mu_ref * (t * sqrt(t)) * (p.T_ref + p.S_suth) / (Tc + p.S_suth);
       lam = vm + 2.0 * p.Gamma * mu / (p.Pr * rho * m.hmin[i]);
     }
     hdt = m.hmin[i] / lam;
@@ -566,13 +619,27 @@ hipError_t launch_ref_face_states(const RefStepArgs& a, double* UL, double* UR, 
 }
 
 hipError_t launch_ref_nodes(const MeshView& m, const double* Q, const double* U, double* Qn, double* Un, double* phiN,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(k_ref_nodes, dim3(nblocks(m.nn)), dim3(kBlock), 0, s, m, Q, U, Qn, Un, phiN);
+                            hipStream_t s, const FoldArgs& fold) {
+  const unsigned nfold = fold.slot ? 2u : 0u;
+  hipLaunchKernelGGL(k_ref_nodes, dim3(nblocks(m.nn) + nfold), dim3(kBlock), 0, s, m, Q, U, Qn, Un, phiN, fold);
   return hipGetLastError();
 }
 
 hipError_t launch_ref_phi(const MeshView& m, const double* U, double* phiN, hipStream_t s) {
-  hipLaunchKernelGGL(k_ref_nodes, dim3(nblocks(m.nn)), dim3(kBlock), 0, s, m, nullptr, U, nullptr, nullptr, phiN);
+  hipLaunchKernelGGL(k_ref_nodes, dim3(nblocks(m.nn)), dim3(kBlock), 0, s, m, nullptr, U, nullptr, nullptr, phiN,
+                     FoldArgs{});
+  return hipGetLastError();
+}
+
+hipError_t launch_ns_prime(const MeshView& m, const gd_params& prm, const double* U, void* Qt, bool f32, Acc* acc,
+                           double* part, double flow_time, int32_t viscous_ns, hipStream_t s) {
+  hipLaunchKernelGGL(k_acc_reset, dim3(1), dim3(64), 0, s, acc, flow_time);
+  if (f32)
+    hipLaunchKernelGGL(k_ns_prime<float>, dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m, prm, U, static_cast<float*>(Qt),
+                       part, viscous_ns);
+  else
+    hipLaunchKernelGGL(k_ns_prime<double>, dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m, prm, U,
+                       static_cast<double*>(Qt), part, viscous_ns);
   return hipGetLastError();
 }
 

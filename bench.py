@@ -87,11 +87,14 @@ def run_variant(mesh, case, precision, flux, order_dev, U0, warmup, steps, cfl):
             "nonphysical": int(st.nonphysical)}
 
 
-def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl, precision=L.GD_FP32_MIXED):
+def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl, precision=L.GD_FP32_MIXED, budget_gib=16):
     """Discrete adjoint of the summed wall drag over ``steps`` steps (FP32-mixed
     by default; fixed dt, checkpoints every ``ckpt``) against the same forward
-    steps alone, from the state after 20 CFL steps; then with the records
-    of every segment kept in HBM."""
+    steps alone, from the state after 20 CFL steps.  The headline run keeps
+    the cell records of as many trailing checkpoint segments as fit in
+    ``budget_gib`` GiB of HBM and recomputes the others from their
+    checkpoints; beside it the run that keeps one segment only (the least
+    memory) and the one that keeps every segment (nothing recomputed)."""
     prm = case.params(precision=precision)
     sol = S.GodunovSolver(mesh, prm, order=order_dev)
     sol.state.set_array(L.GD_ARR_U_OLD, U0)
@@ -117,17 +120,27 @@ def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl, precision=L.GD_FP3
         sol.state.sync()
         return time.perf_counter() - t0, g, gb, st
 
+    budget = int(budget_gib) << 30
     # dJ/dU0 left in HBM, as the forward leaves U (the host copy is timed apart)
-    adj, _, gbc, ast = timed(checkpoint_every=ckpt, to_host=False)
-    # the same checkpoints with every segment's records kept in HBM (nothing
-    # recomputed; 27 GB of the 288 GB)
+    adj, _, gbc, ast = timed(checkpoint_every=ckpt, record_budget=budget, to_host=False)
+    # one segment's records only (the least memory: every other segment recomputed)
+    adj0, _, _, ast0 = timed(checkpoint_every=ckpt, to_host=False)
+    # every segment's records kept in HBM (nothing recomputed)
     adj1, _, _, ast1 = timed(checkpoint_every=ckpt, record_budget=64 << 30, to_host=False)
     # with dJ/dU0 copied to the caller's host array (66.5 MB over PCIe)
-    adj_h, g0, _, _ = timed(checkpoint_every=ckpt)
+    adj_h, g, _, _ = timed(checkpoint_every=ckpt, record_budget=budget)
+    _, g0, _, _ = timed(checkpoint_every=ckpt)
     _, g1, _, _ = timed(checkpoint_every=ckpt, record_budget=64 << 30)
     sol.close()
+
+    def variant(a_s, st, gv, gib):
+        return {"record_budget_GiB": gib, "recomputed_steps": st.recomputed_steps, "adjoint_ms": round(a_s * 1e3, 3),
+                "cost_vs_forward": round(a_s / fwd, 3), "storage_GB": round(st.storage_bytes / 1e9, 3),
+                "reverse_step_vs_forward_step": round(st.ms_reverse / (fwd * 1e3), 3),
+                "gradient_bit_identical": bool(np.array_equal(gv, g))}
     return {"steps": steps, "checkpoint_every": ckpt,
             "precision": "fp32-mixed" if precision == L.GD_FP32_MIXED else "fp64", "dt": dt,
+            "record_budget_GiB": budget_gib,
             "forward_ms": round(fwd * 1e3, 3), "adjoint_ms": round(adj * 1e3, 3),
             "cost_vs_forward": round(adj / fwd, 3),
             "adjoint_ms_with_host_copy": round(adj_h * 1e3, 3),
@@ -137,12 +150,9 @@ def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl, precision=L.GD_FP3
             "adjoint_forward_sweep_ms": round(ast.ms_forward, 3), "adjoint_reverse_ms": round(ast.ms_reverse, 3),
             "recomputed_steps": ast.recomputed_steps, "storage_GB": round(ast.storage_bytes / 1e9, 3),
             "J_sum_drag": ast.J, "J_rel_diff_vs_forward": abs(ast.J - J) / abs(J),
-            "dJ_dU0_linf": float(np.abs(g0).max()), "dJ_dinflow": [float(x) for x in gbc[1]],
-            "all_records": {"record_budget_GiB": 64, "recomputed_steps": ast1.recomputed_steps,
-                            "adjoint_ms": round(adj1 * 1e3, 3),
-                            "cost_vs_forward": round(adj1 / fwd, 3), "storage_GB": round(ast1.storage_bytes / 1e9, 3),
-                            "reverse_step_vs_forward_step": round(ast1.ms_reverse / (fwd * 1e3), 3),
-                            "max_rel_diff_vs_checkpointed": float(np.abs(g1 - g0).max() / np.abs(g0).max())}}
+            "dJ_dU0_linf": float(np.abs(g).max()), "dJ_dinflow": [float(x) for x in gbc[1]],
+            "one_segment": variant(adj0, ast0, g0, 0),
+            "all_records": variant(adj1, ast1, g1, 64)}
 
 
 def ref_tgv(n, steps, warmup):
@@ -312,6 +322,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200, help="0: skip the adjoint measurement")
     ap.add_argument("--adjoint-checkpoint", type=int, default=20)
     ap.add_argument("--adjoint-precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--adjoint-record-budget", type=int, default=16,
+                    help="GiB of HBM for the cell records of trailing checkpoint segments")
     ap.add_argument("--ref-tgv", type=int, default=1000,
                     help="n of the n x n x 2 periodic TGV for the reference-scheme leg; 0 skips it")
     ap.add_argument("--ref-steps", type=int, default=200)
@@ -391,12 +403,14 @@ def main():
     out["drag_fp64"] = r64["drag"]
     if a.adjoint_steps > 0:
         out["adjoint"] = adjoint_cost(mesh, case, dev_order, U0, a.adjoint_steps, a.adjoint_checkpoint, a.cfl,
-                                      L.GD_FP64 if a.adjoint_precision == "fp64" else L.GD_FP32_MIXED)
+                                      L.GD_FP64 if a.adjoint_precision == "fp64" else L.GD_FP32_MIXED,
+                                      a.adjoint_record_budget)
         ad = out["adjoint"]
         log(f"adjoint: {ad['steps']} {ad['precision']} steps, checkpoints every {ad['checkpoint_every']}: "
             f"{ad['adjoint_ms']:.1f} ms vs forward {ad['forward_ms']:.1f} ms -> {ad['cost_vs_forward']:.2f}x "
             f"(forward sweep {ad['adjoint_forward_sweep_ms']:.1f} ms, reverse {ad['adjoint_reverse_ms']:.1f} ms = "
             f"{ad['reverse_step_vs_forward_step']:.2f}x per step); "
+            f"one segment kept: {ad['one_segment']['adjoint_ms']:.1f} ms -> {ad['one_segment']['cost_vs_forward']:.2f}x; "
             f"all records kept: {ad['all_records']['adjoint_ms']:.1f} ms -> {ad['all_records']['cost_vs_forward']:.2f}x")
     # CPU baselines: the OpenMP threads of the host's share for this GPU
     # (OMP_NUM_THREADS where the launcher sets it, else every CPU this process
