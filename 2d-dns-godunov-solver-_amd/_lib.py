@@ -34,7 +34,7 @@ ip = C.POINTER(C.c_int32)
 EXPORTS = (
     "gd_mesh_create", "gd_mesh_info_get", "gd_mesh_cell_order", "gd_mesh_destroy", "gd_params_default", "gd_state_create",
     "gd_state_destroy", "gd_state_set_params", "gd_state_set_fields", "gd_state_get_fields", "gd_state_set_array",
-    "gd_state_get_array", "gd_calc_one_stage", "gd_calc_face_states", "gd_calc_diff_term", "gd_update_variables", "gd_cells2nodes",
+    "gd_state_get_array", "gd_calc_one_stage", "gd_calc_one_stage_cuda", "gd_calc_face_states", "gd_calc_diff_term", "gd_update_variables", "gd_cells2nodes",
     "gd_run", "gd_sync", "gd_state_device_u", "gd_device_count", "gd_last_error", "gd_version",
     "gd_adjoint_run", "gd_state_device_adjoint", "gd_diff_term_scratch_bytes", "gd_calc_diff_term_noalloc",
 )
@@ -128,6 +128,7 @@ def load():
     L.gd_state_get_array.argtypes = [vp, C.c_int32, dp]
     L.gd_calc_one_stage.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_double]
     L.gd_calc_face_states.argtypes = [vp, C.c_double, dp, dp]
+    L.gd_calc_one_stage_cuda.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_double, dp, dp, dp]
     for f in (L.gd_calc_diff_term, L.gd_update_variables, L.gd_cells2nodes, L.gd_sync):
         f.argtypes = [vp]
     L.gd_run.argtypes = [vp, C.c_int32, C.POINTER(gd_run_params), C.POINTER(gd_run_stats)]
@@ -290,6 +291,17 @@ class DeviceState:
         hr = np.ascontiguousarray(UR.transpose(0, 2, 1), dtype=np.float64)
         check(load().gd_calc_face_states(self.handle, flow_time, ptr(hl), ptr(hr)))
         return hl.transpose(0, 2, 1), hr.transpose(0, 2, 1)
+
+    def calc_one_stage_cuda(self, precision, betta, dt, flow_time=0.0, ULeft=None, URight=None):
+        """calcOneStageCUDA / x32 from face states (4 slots, 4, n_cells) or,
+        when None, from the state's primitives; returns iFlux (n_cells, 4)."""
+        nc = self.mesh.n_cells
+        raw = lambda a: None if a is None else np.ascontiguousarray(  # noqa: E731
+            np.asarray(a, dtype=np.float64).transpose(0, 2, 1))
+        hl, hr = raw(ULeft), raw(URight)
+        out = np.zeros((nc, 4), order="F")
+        check(load().gd_calc_one_stage_cuda(self.handle, precision, betta, dt, flow_time, ptr(hl), ptr(hr), ptr(out)))
+        return out
 
     def calc_diff_term(self):
         check(load().gd_calc_diff_term(self.handle))

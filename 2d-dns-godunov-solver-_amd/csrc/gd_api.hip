@@ -671,6 +671,7 @@ int gd_state_destroy(gd_state* S) {
   if (S->stream) (void)hipStreamSynchronize(S->stream);
   for (hipEvent_t ev : S->ev_pool) (void)hipEventDestroy(ev);
   for (void* p : S->allocs) (void)hipFree(p);
+  if (S->adj_ws) (void)hipFree(S->adj_ws);
   if (S->stream) (void)hipStreamDestroy(S->stream);
   delete S;
   return GD_OK;
@@ -888,6 +889,49 @@ int gd_calc_face_states(gd_state* S, double flow_time, double* ULeft, double* UR
         if (k >= M.nv_caller[i]) continue;
         for (int v = 0; v < 4; ++v) out[(int64_t)k * 4 * nc + 4 * i + v] = h[((int64_t)k * 4 + v) * nc + i];
       }
+  }
+  return GD_OK;
+}
+
+int gd_calc_one_stage_cuda(gd_state* S, int32_t precision, double betta, double dt, double flow_time,
+                           const double* ULeft, const double* URight, double* iFlux) {
+  if (!S) return fail(GD_ERR_INVALID, "null state");
+  GD_MATERIALIZE(S);
+  int rc = need_ref(S);
+  if (rc) return rc;
+  if (precision != GD_FP64 && precision != GD_FP32_MIXED) return fail(GD_ERR_INVALID, "unknown precision");
+  if ((ULeft == nullptr) != (URight == nullptr)) return fail(GD_ERR_INVALID, "give both face-state arrays or neither");
+  const Mesh& M = *S->mesh;
+  const int64_t nc = M.v.nc;
+  if (!S->fstate && (rc = dev_alloc(S->allocs, &S->fstate, 2 * 16 * (size_t)nc))) return rc;
+  double* dUL = S->fstate;
+  double* dUR = S->fstate + 16 * nc;
+  if (ULeft) {
+    // the caller's slots 1..3, each (4, n_cells) column-major, to [slot][4][nc] in device order
+    std::vector<double> h(4 * (size_t)nc);
+    for (int side = 0; side < 2; ++side) {
+      const double* src = side ? URight : ULeft;
+      for (int k = 0; k < 3; ++k) {
+        for (int64_t i = 0; i < nc; ++i)
+          for (int v = 0; v < 4; ++v) h[(size_t)v * nc + i] = src[(int64_t)k * 4 * nc + 4 * i + v];
+        if ((rc = put_cols(S, (side ? dUR : dUL) + (int64_t)k * 4 * nc, h.data(), 4, false))) return rc;
+      }
+    }
+  } else {  // phase one on the device from the state's cell and node primitives
+    RefStepArgs a{};
+    a.m = M.v;
+    a.prm = S->prm;
+    a.Q = S->Q[S->qc];
+    a.Qn = S->Qn;
+    a.flow_time = flow_time;
+    GD_HIP(launch_ref_face_states(a, dUL, dUR, S->stream));
+  }
+  GD_HIP(launch_ref_cuda_stage(M.v, S->prm.Gamma, precision == GD_FP32_MIXED, dUL, dUR, S->U[S->cur], betta, dt, S->F,
+                               S->U[1 - S->cur], S->stream));
+  if (iFlux) {
+    if ((rc = get_cols(S, S->F, iFlux, 4, false))) return rc;
+  } else {
+    GD_HIP(hipStreamSynchronize(S->stream));
   }
   return GD_OK;
 }
@@ -1172,15 +1216,6 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
 // cell records of the current segment; the reverse sweep walks the segments
 // backwards, recomputing each (except the last) from its checkpoint into the
 // segment's record slots, then applies the transposed steps.
-namespace {
-struct DevScratch {
-  std::vector<void*> p;
-  ~DevScratch() {
-    for (void* q : p) (void)hipFree(q);
-  }
-};
-}  // namespace
-
 int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dUN, double* dJ_dU0, double* dJ_dbc,
                    gd_adjoint_stats* stats) {
   if (!S || !ap) return fail(GD_ERR_INVALID, "null argument");
@@ -1206,27 +1241,50 @@ int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dU
   // records kept for the last `keep` segments; earlier segments are recomputed
   const size_t seg_bytes = (size_t)c * 16 * nc * t;
   const int32_t keep = (int32_t)std::max<int64_t>(1, std::min<int64_t>(nseg, ap->record_budget / (int64_t)seg_bytes));
-  DevScratch D;
-  int64_t bytes = 0;
+  // one workspace per state, kept between calls and grown when a call needs
+  // more (an optimisation loop calls the adjoint again and again): the slot
+  // geometry first (valid across calls of one precision), then the rest
+  const size_t geo_bytes = kAdjSlotGeo * (size_t)v.nf * nc * t;
+  const size_t ck_bytes = (size_t)nseg * 4 * nc * sizeof(double);
+  const size_t rec_bytes = (size_t)keep * seg_bytes;
+  const size_t parts[] = {geo_bytes, ck_bytes, rec_bytes, 4 * nc * sizeof(double), 4 * nc * sizeof(double),
+                          4 * nc * sizeof(double), 4 * nc * sizeof(double), 4 * nw * sizeof(double),
+                          4 * nc * t, 4 * nc * t, 4 * (size_t)v.nf * nc * t};
+  constexpr int kParts = (int)(sizeof(parts) / sizeof(parts[0]));
+  size_t offs[kParts], need = 0;
+  for (int q = 0; q < kParts; ++q) {
+    offs[q] = need;
+    need += (parts[q] + 255) & ~size_t(255);
+  }
+  if (need > S->adj_ws_bytes) {
+    if (S->adj_ws) (void)hipFree(S->adj_ws);
+    S->adj_ws = nullptr;
+    S->adj_ws_bytes = 0;
+    S->adj_geo_precision = -1;
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, need);
+    if (e != hipSuccess) return fail(GD_ERR_NOMEM, std::string("hipMalloc (adjoint workspace): ") + hipGetErrorString(e));
+    S->adj_ws = p;
+    S->adj_ws_bytes = need;
+  }
+  char* W = static_cast<char*>(S->adj_ws);
+  char* sgeo = W + offs[0];
+  double* ck = reinterpret_cast<double*>(W + offs[1]);
+  char* recs = W + offs[2];
+  double* Uw0 = reinterpret_cast<double*>(W + offs[3]);
+  double* Uw1 = reinterpret_cast<double*>(W + offs[4]);
+  double* lU0 = reinterpret_cast<double*>(W + offs[5]);
+  double* lU1 = reinterpret_cast<double*>(W + offs[6]);
+  double* bcsum = reinterpret_cast<double*>(W + offs[7]);
+  char* lamF = W + offs[8];
+  char* selfq = W + offs[9];
+  char* nrev = W + offs[10];
+  const int64_t bytes = (int64_t)(ck_bytes + rec_bytes);  // checkpoints + kept records
   int rc = 0;
-  double* ck = nullptr;
-  rc |= dev_alloc(D.p, &ck, (size_t)nseg * 4 * nc, &bytes);
-  char* recs = nullptr;
-  rc |= dev_alloc(D.p, &recs, (size_t)keep * seg_bytes, &bytes);
-  double *Uw0 = nullptr, *Uw1 = nullptr, *lU0 = nullptr, *lU1 = nullptr, *bcsum = nullptr;
-  rc |= dev_alloc(D.p, &Uw0, 4 * nc);
-  rc |= dev_alloc(D.p, &Uw1, 4 * nc);
-  rc |= dev_alloc(D.p, &lU0, 4 * nc);
-  rc |= dev_alloc(D.p, &lU1, 4 * nc);
-  rc |= dev_alloc(D.p, &bcsum, 4 * nw);
-  char *lamF = nullptr, *selfq = nullptr, *nrev = nullptr;
-  rc |= dev_alloc(D.p, &lamF, 4 * nc * t);
-  rc |= dev_alloc(D.p, &selfq, 4 * nc * t);
-  rc |= dev_alloc(D.p, &nrev, 4 * (size_t)v.nf * nc * t);
-  char* sgeo = nullptr;
-  rc |= dev_alloc(D.p, &sgeo, kAdjSlotGeo * (size_t)v.nf * nc * t);
-  if (rc) return rc;
-  GD_HIP((mixed ? launch_adj_slot_geo_f32 : launch_adj_slot_geo_f64)(v, sgeo, S->stream));
+  if (S->adj_geo_precision != P.precision) {
+    GD_HIP((mixed ? launch_adj_slot_geo_f32 : launch_adj_slot_geo_f64)(v, sgeo, S->stream));
+    S->adj_geo_precision = P.precision;
+  }
   // record slot of step s: kept segments have their own region; the others
   // (recomputed later) share the first kept region, which only later steps
   // of the forward sweep overwrite and which the reverse sweep no longer needs

@@ -150,6 +150,9 @@ struct State {
   double* xfer = nullptr;               // [4][max(nc, nn)] staging of host transfers
   double* grad0 = nullptr;              // [4][nc] dJ/dU0 of the last adjoint run (allocated by the first)
   double* fstate = nullptr;             // [2][4 slots][4][nc] two-phase face states (allocated by the first request)
+  void* adj_ws = nullptr;                // adjoint workspace, kept between calls (grown when needed)
+  size_t adj_ws_bytes = 0;
+  int32_t adj_geo_precision = -1;       // precision of the slot geometry held in it, -1: none
   // bookkeeping a run leaves to whoever reads it (done by the next call that does):
   bool pend_delta = false;    // Dl = U[cur] - U[1-cur] of the last step, then U[1-cur] = U[cur]
   bool pend_derived = false;  // Q[qc], aS, vmax from U[cur] (updateVariablesSA)
@@ -196,6 +199,10 @@ struct FoldArgs {
 
 hipError_t launch_ref_step(const RefStepArgs& a, hipStream_t s);
 hipError_t launch_ref_face_states(const RefStepArgs& a, double* UL, double* UR, hipStream_t s);
+// calcOneStageCUDA (x32 false) / calcOneStageCUDAx32 from face states UL/UR
+// [slot][4][nc]: F = iFlux [4][nc], Un = Uo - F betta dt / A
+hipError_t launch_ref_cuda_stage(const MeshView& m, double gamma, bool x32, const double* UL, const double* UR,
+                                 const double* Uo, double betta, double dt, double* F, double* Un, hipStream_t s);
 hipError_t launch_ref_nodes(const MeshView& m, const double* Q, const double* U, double* Qn, double* Un,
                             double* phiN, hipStream_t s, const FoldArgs& fold = FoldArgs{});
 // start of the ns loop: resets the accumulators to flow_time, writes the

@@ -193,6 +193,112 @@ __device__ __forceinline__ void ref_cell_flux(const RefStepArgs& a, int64_t i, i
   }
 }
 
+// The reference's GPU stages, calcOneStageCUDA (calcOneStageCUDA.jl:4-135)
+// and calcOneStageCUDAx32 (calcOneStageCUDAx32.jl:5-155): kernel_AUSM2d of
+// AUSMflux2dCUDA.jl:56-98 / AUSMflux2dCUDAx32.jl:56-98 on face slots 1..3
+// (the 4th slot's launch is commented out in the source), accumulated per
+// cell in the kernel's type, then the Float64 update without Udiff.  The
+// x32 kernel's arrays are Float32, but the literals of its split functions
+// (AUSMflux2dCUDAx32.jl:19-48) are Float64, so Julia evaluates M + 1.0 and
+// the products with them in Float64 (M*M and AUSM_ALFA*M stay Float32) and
+// each function returns Float32.  Built without contraction, like the source
+// expressions.
+struct CudaSplitF64 {
+  __device__ static double M_p(double M, double B) {
+    return fabs(M) >= 1.0 ? 0.5 * (M + fabs(M)) : 0.25 * (M + 1.0) * (M + 1.0) + B * (M * M - 1.0) * (M * M - 1.0);
+  }
+  __device__ static double M_m(double M, double B) {
+    return fabs(M) >= 1.0 ? 0.5 * (M - fabs(M)) : -0.25 * (M - 1.0) * (M - 1.0) - B * (M * M - 1.0) * (M * M - 1.0);
+  }
+  __device__ static double P_p(double M, double A) {
+    return fabs(M) >= 1.0 ? 0.5 * (1.0 + refm::jsign(M))
+                          : 0.25 * (M + 1.0) * (M + 1.0) * (2.0 - M) + A * M * (M * M - 1.0) * (M * M - 1.0);
+  }
+  __device__ static double P_m(double M, double A) {
+    return fabs(M) >= 1.0 ? 0.5 * (1.0 - refm::jsign(M))
+                          : 0.25 * (M - 1.0) * (M - 1.0) * (2.0 + M) - A * M * (M * M - 1.0) * (M * M - 1.0);
+  }
+};
+__device__ __forceinline__ float fsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : x); }
+struct CudaSplitX32 {
+  __device__ static float M_p(float M, float B) {
+    if (fabsf(M) >= 1.0) return (float)(0.5 * (double)(M + fabsf(M)));
+    const double a = (double)M + 1.0, d = (double)(M * M) - 1.0;
+    return (float)(0.25 * a * a + (double)B * d * d);
+  }
+  __device__ static float M_m(float M, float B) {
+    if (fabsf(M) >= 1.0) return (float)(0.5 * (double)(M - fabsf(M)));
+    const double a = (double)M - 1.0, d = (double)(M * M) - 1.0;
+    return (float)(-0.25 * a * a - (double)B * d * d);
+  }
+  __device__ static float P_p(float M, float A) {
+    if (fabsf(M) >= 1.0) return (float)(0.5 * (1.0 + (double)fsign(M)));
+    const double a = (double)M + 1.0, d = (double)(M * M) - 1.0;
+    return (float)(0.25 * a * a * (2.0 - (double)M) + (double)(A * M) * d * d);
+  }
+  __device__ static float P_m(float M, float A) {
+    if (fabsf(M) >= 1.0) return (float)(0.5 * (1.0 - (double)fsign(M)));
+    const double a = (double)M - 1.0, d = (double)(M * M) - 1.0;
+    return (float)(0.25 * a * a * (2.0 + (double)M) - (double)(A * M) * d * d);
+  }
+};
+
+template <class T>
+__device__ __forceinline__ T tmin(T x, T y) { return y < x ? y : x; }
+
+template <class T, class Split>
+__device__ __forceinline__ void cuda_ausm(const T* L, const T* R, T nx, T ny, T side, T g, T* acc) {
+  const T rL = L[0], uL = L[1], vL = L[2], pL = L[3];
+  const T rR = R[0], uR = R[1], vR = R[2], pR = R[3];
+  const T one = (T)1.0, two = (T)2.0, half = (T)0.5;
+  const T htL = pL / rL / (g - one) + half * (uL * uL + vL * vL) + pL / rL;
+  const T htR = pR / rR / (g - one) + half * (uR * uR + vR * vR) + pR / rR;
+  const T aL = sqrt(two * (g - one) / (g + one) * htL) *
+               tmin(one, sqrt(two * (g - one) / (g + one) * htL) / fabs(uL * nx + vL * ny));
+  const T aR = sqrt(two * (g - one) / (g + one) * htR) *
+               tmin(one, sqrt(two * (g - one) / (g + one) * htR) / fabs(uR * nx + vR * ny));
+  const T a12 = tmin(aL, aR);
+  const T ML = (uL * nx + vL * ny) / a12, MR = (uR * nx + vR * ny) / a12;
+  const T B = (T)(1.0 / 8.0), A = (T)(3.0 / 16.0);
+  const T m = Split::M_p(ML, B) + Split::M_m(MR, B);
+  const T p12 = Split::P_p(ML, A) * pL + Split::P_m(MR, A) * pR;
+  const T mp = half * (m + fabs(m)), mm = half * (m - fabs(m));
+  acc[0] = acc[0] - (a12 * (mp * rL + mm * rR)) * side;
+  acc[1] = acc[1] - (a12 * (mp * rL * uL + mm * rR * uR) + p12 * nx) * side;
+  acc[2] = acc[2] - (a12 * (mp * rL * vL + mm * rR * vR) + p12 * ny) * side;
+  acc[3] = acc[3] - (a12 * (mp * rL * htL + mm * rR * htR)) * side;
+}
+
+// UL/UR: the face states, [slot][4][nc] (slots 0..2 read); F: iFlux [4][nc]
+template <class T, class Split>
+__global__ __launch_bounds__(kBlock) void k_ref_cuda_stage(MeshView m, double gamma, const double* UL,
+                                                           const double* UR, const double* Uo, double betta,
+                                                           double dt, double* F, double* Un) {
+  const int64_t nc = m.nc;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nc) return;
+  const T g = (T)gamma;
+  T acc[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    T L[4], R[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      L[v] = (T)UR[((int64_t)k * 4 + v) * nc + i];  // the neighbour side is "L"
+      R[v] = (T)UL[((int64_t)k * 4 + v) * nc + i];
+    }
+    const int64_t s = (int64_t)k * nc + i;
+    cuda_ausm<T, Split>(L, R, (T)m.fnx[s], (T)m.fny[s], (T)m.flen[s], g, acc);
+  }
+  const double Rarea = m.rarea[i];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const double fl = (double)acc[v];
+    F[v * nc + i] = fl;
+    Un[v * nc + i] = Uo[v * nc + i] - fl * betta * dt * Rarea;
+  }
+}
+
 // Phase one of calcOneStageMUSCL (computeMUSCLStencilsCUDA.jl:3-28 with
 // computeInterfaceSlopeCUDA, computeslope2d.jl:139-269): every face slot's
 // two states, UL/UR = [slot][4][nc] (slots past a cell's faces untouched).
@@ -615,6 +721,17 @@ hipError_t launch_ref_face_states(const RefStepArgs& a, double* UL, double* UR, 
     hipLaunchKernelGGL(k_ref_face_states<true>, dim3(nblocks(a.m.nc)), dim3(kBlock), 0, s, a, UL, UR);
   else
     hipLaunchKernelGGL(k_ref_face_states<false>, dim3(nblocks(a.m.nc)), dim3(kBlock), 0, s, a, UL, UR);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_cuda_stage(const MeshView& m, double gamma, bool x32, const double* UL, const double* UR,
+                                 const double* Uo, double betta, double dt, double* F, double* Un, hipStream_t s) {
+  if (x32)
+    hipLaunchKernelGGL((k_ref_cuda_stage<float, CudaSplitX32>), dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m, gamma, UL,
+                       UR, Uo, betta, dt, F, Un);
+  else
+    hipLaunchKernelGGL((k_ref_cuda_stage<double, CudaSplitF64>), dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m, gamma,
+                       UL, UR, Uo, betta, dt, F, Un);
   return hipGetLastError();
 }
 

@@ -189,14 +189,50 @@ def calcOneStageMUSCL(betta, dtX, flowTime, testMesh, testfields2dX, thermo, cel
                     a[:, mask] = full[k][:, mask]
 
 
+def _cuda_stage(x32, betta, dtX, flowTime, testMeshX, testfields2dX, thermoX, cellsThreadsX, UconsCellsOldX,
+                UconsCellsNewX, lefts, rights, fluxes, config):
+    mask = _ranges(cellsThreadsX, testMeshX.nCells)
+    st = _ctx(testMeshX).state_for((config or _config).params(thermoX))
+    st.set_array(L.GD_ARR_U_OLD, UconsCellsOldX)
+    nc = testMeshX.nCells
+    have = all(a is not None for a in lefts[:3] + rights[:3])
+    if not have:  # no face states handed in: phase one on the device from the fields
+        _set_cells(st, testfields2dX)
+    UL = np.stack([np.asarray(a, dtype=np.float64) if a is not None else np.zeros((4, nc)) for a in lefts])
+    UR = np.stack([np.asarray(a, dtype=np.float64) if a is not None else np.zeros((4, nc)) for a in rights])
+    iflux = st.calc_one_stage_cuda(L.GD_FP32_MIXED if x32 else L.GD_FP64, betta, dtX, flowTime,
+                                   UL if have else None, UR if have else None)
+    _copy_rows(UconsCellsNewX, st.get_array(L.GD_ARR_U_NEW), mask)
+    # copyto!(iFluxV_k, cuFluxV_k) copies every cell (calcOneStageCUDAx32.jl:126-129)
+    for k, a in enumerate(fluxes):
+        if a is not None:
+            a[:] = iflux[:, k]
+
+
 def calcOneStageCUDA(betta, dtX, flowTime, testMeshX, testfields2dX, thermoX, cellsThreadsX,  # noqa: N802
-                     UconsCellsOldX, UconsDiffTermX, UconsCellsNewX, *device_buffers, config: SchemeConfig = None):
-    """GPU stage of the reference (no viscous term); its CuVector work buffers are not needed."""
-    calcOneStageMUSCL(betta, dtX, flowTime, testMeshX, testfields2dX, thermoX, cellsThreadsX, UconsCellsOldX,
-                      UconsDiffTermX, UconsCellsNewX, config=config)
+                     UconsCellsOldX, UconsDiffTermX, UconsCellsNewX, uLeft1=None, uLeft2=None, uLeft3=None,
+                     uLeft4=None, uRight1=None, uRight2=None, uRight3=None, uRight4=None, iFluxV1=None,
+                     iFluxV2=None, iFluxV3=None, iFluxV4=None, *device_buffers, config: SchemeConfig = None):
+    """The reference's FP64 GPU stage: AUSM+ of the given face states
+    (computeMUSCLStencilsCUDA's uLeft/uRight) on face slots 1..3, summed per
+    cell on the device, then the update without the viscous term.  The CuVector
+    work buffers of the source are not needed (the library owns its device
+    memory); without face states, phase one runs on the device."""
+    _cuda_stage(False, betta, dtX, flowTime, testMeshX, testfields2dX, thermoX, cellsThreadsX, UconsCellsOldX,
+                UconsCellsNewX, (uLeft1, uLeft2, uLeft3, uLeft4), (uRight1, uRight2, uRight3, uRight4),
+                (iFluxV1, iFluxV2, iFluxV3, iFluxV4), config)
 
 
-calcOneStageCUDAx32 = calcOneStageCUDA
+def calcOneStageCUDAx32(betta, dtX, flowTime, testMeshX, testfields2dX, thermoX, cellsThreadsX,  # noqa: N802
+                        UconsCellsOldX, UconsDiffTermX, UconsCellsNewX, uLeft1=None, uLeft2=None, uLeft3=None,
+                        uLeft4=None, uRight1=None, uRight2=None, uRight3=None, uRight4=None, iFluxV1=None,
+                        iFluxV2=None, iFluxV3=None, iFluxV4=None, *device_buffers, config: SchemeConfig = None):
+    """The reference's mixed GPU stage: the face states rounded to Float32,
+    AUSM+ and the per-cell sums in Float32 (with the Float64 literals of its
+    split functions), the update in Float64 without the viscous term."""
+    _cuda_stage(True, betta, dtX, flowTime, testMeshX, testfields2dX, thermoX, cellsThreadsX, UconsCellsOldX,
+                UconsCellsNewX, (uLeft1, uLeft2, uLeft3, uLeft4), (uRight1, uRight2, uRight3, uRight4),
+                (iFluxV1, iFluxV2, iFluxV3, iFluxV4), config)
 
 
 def calcDiffTerm(cellsThreads, nodesThreads, testMesh, testfields2d, viscousFields2d, thermo,  # noqa: N802

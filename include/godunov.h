@@ -17,7 +17,7 @@
  *   SecondOrderUpwindM2             SOUscheme.jl:3-118                          -> same kernel
  *   calcOneStageMUSCL               calcOneStageMUSCL.jl:4-106                  -> gd_calc_one_stage(GD_STAGE_MUSCL)
  *   calcOneStageCUDA / ...x32       calcOneStageCUDA.jl:4-135, calcOneStageCUDAx32.jl:5-155
- *                                                                               -> gd_calc_one_stage(GD_STAGE_MUSCL)
+ *                                                                               -> gd_calc_one_stage_cuda
  *   calcDiffTerm                    calcDiffTerm.jl:31-121                      -> gd_calc_diff_term
  *   updateVariablesSA               evaluate2d.jl:52-110                        -> gd_update_variables
  *   cells2nodesSolutionReconstructionWithStencils utilsFVM2dp.jl:299-365       -> gd_cells2nodes
@@ -169,6 +169,18 @@ int gd_calc_one_stage(gd_state* st, int32_t variant, double betta, double dt, do
    flow_time; ULeft, URight: four (4, n_cells) column-major arrays each, slot
    after slot, caller numbering; slots past a cell's faces are left as given */
 int gd_calc_face_states(gd_state* st, double flow_time, double* ULeft, double* URight);
+/* calcOneStageCUDA (precision GD_FP64, calcOneStageCUDA.jl:4-135) and
+   calcOneStageCUDAx32 (GD_FP32_MIXED, calcOneStageCUDAx32.jl:5-155): AUSM+ as
+   the reference's GPU kernel writes it (AUSMflux2dCUDA*.jl:56-98) on face
+   slots 1..3 of the face states ULeft1..3 / URight1..3 (each (4, n_cells)
+   column-major, slot after slot as for gd_calc_face_states; slot 4 is not
+   read: the source leaves the 4th face's launch commented out), the per-cell
+   sums accumulated in Float64 or Float32, then U_NEW = U_OLD - iFlux betta dt
+   / A in Float64 without Udiff.  ULeft = URight = NULL: the face states of
+   the state's primitives at flow_time (phase one on the device).  iFlux
+   (n_cells, 4) or NULL: the accumulated sums (FLUXES holds them too). */
+int gd_calc_one_stage_cuda(gd_state* st, int32_t precision, double betta, double dt, double flow_time,
+                           const double* ULeft, const double* URight, double* iFlux);
 int gd_calc_diff_term(gd_state* st);
 /* calcDiffTerm on the caller's device arrays, allocating nothing.  No
    counterpart in the reference: for callers that cannot allocate device
@@ -227,7 +239,10 @@ int gd_run(gd_state* st, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats
    way back (unless record_budget holds their records).  dJ_dU0 (NULL: kept
    on the device only, see gd_state_device_adjoint) / dJ_dUN: (n_cells, 4) column-major, caller numbering.
    dJ_dbc: [(GD_MAX_BC + 1)][4], derivative with respect to the fixed states
-   bc[-code].data[0..3] (rho, u, v, p) of GD_BC_FIXED codes, zero elsewhere. */
+   bc[-code].data[0..3] (rho, u, v, p) of GD_BC_FIXED codes, zero elsewhere.
+   The device workspace (checkpoints, kept records, adjoint fields) stays with
+   the state for the next call, grown when a call needs more, and is freed by
+   gd_state_destroy. */
 typedef struct {
   int32_t n_steps;
   int32_t checkpoint_every;

@@ -109,6 +109,10 @@ def lib():
                                          C.POINTER(OrcThermo), C.POINTER(OrcParams), dp, dp, dp]
         L.orc_green_gauss_gradient.argtypes = [ip, C.c_int32, C.POINTER(OrcMesh), dp, dp, dp]
         L.orc_update_variables.argtypes = [ip, C.c_int32, C.c_int64, C.c_double, dp, dp, dp, C.POINTER(OrcFields)]
+        L.orc_calc_one_stage_cuda.argtypes = [C.c_int32, C.POINTER(OrcMesh), C.POINTER(OrcThermo), C.c_double,
+                                              C.c_double, dp, dp, dp, dp, dp]
+        L.orc_cuda_split.restype = C.c_double
+        L.orc_cuda_split.argtypes = [C.c_int32, C.c_int32, C.c_double]
         L.orc_ref_tau.restype = C.c_double
         L.orc_ref_tau.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcFields), C.c_double]
         L.orc_local_dt.restype = C.c_double
@@ -303,6 +307,26 @@ def calc_one_stage_muscl(mv: MeshView, fields, thermo, prm, UOld, betta, dt, flo
     # slot k's (4, n) column-major array: element [v, i] at 4 i + v
     shape = lambda a: a.reshape(4, m.nCells, 4).transpose(0, 2, 1)  # noqa: E731
     return FL, UN, shape(UL), shape(UR)
+
+
+def calc_one_stage_cuda(mv: MeshView, thermo, betta, dt, UOld, ULeft, URight, x32=False):
+    """calcOneStageCUDA / calcOneStageCUDAx32 from the face states ``ULeft``,
+    ``URight`` ((4 slots, 4, nCells) as calc_one_stage_muscl returns them):
+    returns (iFlux (nCells, 4), UNew)."""
+    m = mv.mesh
+    raw = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64).transpose(0, 2, 1))  # noqa: E731
+    UL, UR = raw(ULeft), raw(URight)
+    FL = np.zeros((m.nCells, 4), order="F")
+    UN = np.zeros((m.nCells, 4), order="F")
+    th = thermo_struct(thermo)
+    _rc(lib().orc_calc_one_stage_cuda(int(bool(x32)), C.byref(mv.c), C.byref(th), betta, dt, _p(_chk_f64(UOld)),
+                                      _p(UL), _p(UR), _p(UN), _p(FL)), "calcOneStageCUDA")
+    return FL, UN
+
+
+def cuda_split(x32, which, M):
+    """The GPU kernels' AUSM+ split functions (0 M+, 1 M-, 2 P+, 3 P-)."""
+    return lib().orc_cuda_split(int(bool(x32)), int(which), float(M))
 
 
 def calc_diff_term(mv: MeshView, fields, thermo, prm, UC, Udiff_in=None, n_threads=2):

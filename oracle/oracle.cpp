@@ -625,3 +625,141 @@ double orc_local_dt(const orc_mesh* m, const orc_fields* f, double cfl) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------ the reference's GPU stages
+// kernel_AUSM2d of AUSMflux2dCUDA.jl:56-98 (FP64) and of AUSMflux2dCUDAx32.jl:56-98
+// (Float32 arrays), restated at the level of the source expressions with
+// Julia's promotion rules: in the Float32 kernel the split functions
+// (AUSMflux2dCUDAx32.jl:19-48) take Float32 arguments but their literals
+// (0.5, 0.25, 1.0, 2.0) are Float64, so M + 1.0 and the products with them are
+// Float64, M*M and AUSM_ALFA*M stay Float32, and each function returns Float32;
+// the kernel body is Float32 throughout.  L = the neighbour side (uRight).
+namespace {
+
+struct CudaSplitF64 {
+  static double M_p(double M, double B) {
+    return std::fabs(M) >= 1.0 ? 0.5 * (M + std::fabs(M)) : 0.25 * (M + 1.0) * (M + 1.0) + B * (M * M - 1.0) * (M * M - 1.0);
+  }
+  static double M_m(double M, double B) {
+    return std::fabs(M) >= 1.0 ? 0.5 * (M - std::fabs(M))
+                               : -0.25 * (M - 1.0) * (M - 1.0) - B * (M * M - 1.0) * (M * M - 1.0);
+  }
+  static double P_p(double M, double A) {
+    return std::fabs(M) >= 1.0 ? 0.5 * (1.0 + jsign(M))
+                               : 0.25 * (M + 1.0) * (M + 1.0) * (2.0 - M) + A * M * (M * M - 1.0) * (M * M - 1.0);
+  }
+  static double P_m(double M, double A) {
+    return std::fabs(M) >= 1.0 ? 0.5 * (1.0 - jsign(M))
+                               : 0.25 * (M - 1.0) * (M - 1.0) * (2.0 + M) - A * M * (M * M - 1.0) * (M * M - 1.0);
+  }
+};
+
+inline float fsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : x); }
+
+struct CudaSplitX32 {  // Float32 in and out, Float64 where a literal meets the argument
+  static float M_p(float M, float B) {
+    if (std::fabs(M) >= 1.0) return (float)(0.5 * (double)(M + std::fabs(M)));
+    const double a = (double)M + 1.0, d = (double)(M * M) - 1.0;
+    return (float)(0.25 * a * a + (double)B * d * d);
+  }
+  static float M_m(float M, float B) {
+    if (std::fabs(M) >= 1.0) return (float)(0.5 * (double)(M - std::fabs(M)));
+    const double a = (double)M - 1.0, d = (double)(M * M) - 1.0;
+    return (float)(-0.25 * a * a - (double)B * d * d);
+  }
+  static float P_p(float M, float A) {
+    if (std::fabs(M) >= 1.0) return (float)(0.5 * (1.0 + (double)fsign(M)));
+    const double a = (double)M + 1.0, d = (double)(M * M) - 1.0;
+    return (float)(0.25 * a * a * (2.0 - (double)M) + (double)(A * M) * d * d);
+  }
+  static float P_m(float M, float A) {
+    if (std::fabs(M) >= 1.0) return (float)(0.5 * (1.0 - (double)fsign(M)));
+    const double a = (double)M - 1.0, d = (double)(M * M) - 1.0;
+    return (float)(0.25 * a * a * (2.0 + (double)M) - (double)(A * M) * d * d);
+  }
+};
+
+// one kernel_AUSM2d evaluation added into acc (flux_k[i] = flux_k[i] - (...) * side)
+template <class T, class Split>
+void cuda_ausm(const T* L, const T* R, T nx, T ny, T side, T g, T* acc) {
+  const T rL = L[0], uL = L[1], vL = L[2], pL = L[3];
+  const T rR = R[0], uR = R[1], vR = R[2], pR = R[3];
+  const T one = (T)1.0, two = (T)2.0, half = (T)0.5;
+  const T htL = pL / rL / (g - one) + half * (uL * uL + vL * vL) + pL / rL;
+  const T htR = pR / rR / (g - one) + half * (uR * uR + vR * vR) + pR / rR;
+  const T aL = std::sqrt(two * (g - one) / (g + one) * htL) *
+               std::min(one, std::sqrt(two * (g - one) / (g + one) * htL) / std::fabs(uL * nx + vL * ny));
+  const T aR = std::sqrt(two * (g - one) / (g + one) * htR) *
+               std::min(one, std::sqrt(two * (g - one) / (g + one) * htR) / std::fabs(uR * nx + vR * ny));
+  const T a12 = std::min(aL, aR);
+  const T ML = (uL * nx + vL * ny) / a12, MR = (uR * nx + vR * ny) / a12;
+  const T B = (T)(1.0 / 8.0), A = (T)(3.0 / 16.0);
+  const T m = Split::M_p(ML, B) + Split::M_m(MR, B);
+  const T p12 = Split::P_p(ML, A) * pL + Split::P_m(MR, A) * pR;
+  const T mp = half * (m + std::fabs(m)), mm = half * (m - std::fabs(m));
+  acc[0] = acc[0] - (a12 * (mp * rL + mm * rR)) * side;
+  acc[1] = acc[1] - (a12 * (mp * rL * uL + mm * rR * uR) + p12 * nx) * side;
+  acc[2] = acc[2] - (a12 * (mp * rL * vL + mm * rR * vR) + p12 * ny) * side;
+  acc[3] = acc[3] - (a12 * (mp * rL * htL + mm * rR * htR)) * side;
+}
+
+template <class T, class Split>
+void cuda_stage(const orc_mesh* m, const orc_thermo* th, double betta, double dt, const double* UOld,
+                const double* ULeft, const double* URight, double* UNew, double* iFlux) {
+  const int64_t n = m->nCells;
+  const T g = (T)th->Gamma;
+  for (int64_t i = 0; i < n; ++i) {
+    T acc[4] = {0, 0, 0, 0};
+    // slots 1..3 only: the 4th face's launch is commented out
+    // (calcOneStageCUDA.jl:89-103, calcOneStageCUDAx32.jl:111-124)
+    for (int k = 0; k < 3; ++k) {
+      T L[4], R[4];
+      for (int v = 0; v < 4; ++v) {
+        L[v] = (T)URight[(int64_t)k * 4 * n + 4 * i + v];  // copyto! rounds Float64 to T
+        R[v] = (T)ULeft[(int64_t)k * 4 * n + 4 * i + v];
+      }
+      cuda_ausm<T, Split>(L, R, (T)m->cell_edges_Nx[(int64_t)k * n + i], (T)m->cell_edges_Ny[(int64_t)k * n + i],
+                          (T)m->cell_edges_length[(int64_t)k * n + i], g, acc);
+    }
+    const double Rarea = 1.0 / m->cell_areas[i];
+    for (int v = 0; v < 4; ++v) {
+      const double fl = (double)acc[v];
+      if (iFlux) iFlux[v * n + i] = fl;
+      // calcOneStageCUDAx32.jl:138-153: the update on the host in Float64, no Udiff
+      UNew[v * n + i] = UOld[v * n + i] - fl * betta * dt * Rarea;
+    }
+  }
+}
+
+}  // namespace
+
+// calcOneStageCUDA (calcOneStageCUDA.jl:4-135) / calcOneStageCUDAx32
+// (calcOneStageCUDAx32.jl:5-155) from the caller's face states ULeft1..4 /
+// URight1..4 (four (4, nCells) column-major arrays each, slot after slot).
+int orc_calc_one_stage_cuda(int32_t x32, const orc_mesh* m, const orc_thermo* th, double betta, double dt,
+                            const double* UOld, const double* ULeft, const double* URight, double* UNew,
+                            double* iFlux) {
+  if (!m || !th || !UOld || !ULeft || !URight || !UNew) return -1;
+  if (x32) cuda_stage<float, CudaSplitX32>(m, th, betta, dt, UOld, ULeft, URight, UNew, iFlux);
+  else cuda_stage<double, CudaSplitF64>(m, th, betta, dt, UOld, ULeft, URight, UNew, iFlux);
+  return 0;
+}
+
+// the split functions alone, for the tests: which = 0 M+, 1 M-, 2 P+, 3 P-
+double orc_cuda_split(int32_t x32, int32_t which, double M) {
+  if (x32) {
+    const float f = (float)M, B = 0.125f, A = 0.1875f;
+    switch (which) {
+      case 0: return CudaSplitX32::M_p(f, B);
+      case 1: return CudaSplitX32::M_m(f, B);
+      case 2: return CudaSplitX32::P_p(f, A);
+      default: return CudaSplitX32::P_m(f, A);
+    }
+  }
+  switch (which) {
+    case 0: return CudaSplitF64::M_p(M, 0.125);
+    case 1: return CudaSplitF64::M_m(M, 0.125);
+    case 2: return CudaSplitF64::P_p(M, 0.1875);
+    default: return CudaSplitF64::P_m(M, 0.1875);
+  }
+}

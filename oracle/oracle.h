@@ -18,8 +18,10 @@
  * conservation, numbering independence, mirror symmetry).  Parity unpinned,
  * since the Julia reference cannot run here: the rounding of Julia's x^4 and
  * pow in orc_air_conductivity, and of its @fastmath node interpolation
- * (utilsFVM2dp.jl:305), which may contract or reorder; the reference's FP32
- * GPU accumulation order (AUSMflux2dCUDAx32.jl:88-91) is not restated.
+ * (utilsFVM2dp.jl:305), which may contract or reorder; orc_calc_one_stage_cuda
+ * follows the reference's GPU kernels (AUSMflux2dCUDA*.jl:56-98) at the level
+ * of their source expressions and Julia's type promotion, not of the
+ * CUDA.jl/LLVM code generation, which may contract multiply-adds.
  */
 #ifndef GODUNOV_ORACLE_H
 #define GODUNOV_ORACLE_H
@@ -112,6 +114,13 @@ int orc_green_gauss_gradient(const int32_t* cellsThreads, int32_t nThreads, cons
 int orc_update_variables(const int32_t* cellsThreads, int32_t nThreads, int64_t nCells, double gamma,
                          const double* UNew, double* UOld, double* Delta, const orc_fields* f);
 double orc_ref_tau(const orc_mesh* m, const orc_fields* f, double cfl);
+/* calcOneStageCUDA (x32 = 0) / calcOneStageCUDAx32 (x32 = 1): AUSM+ of the
+   caller's face states on face slots 1..3 with device-side accumulation in
+   Float64 / Float32, then the Float64 update; iFlux: (nCells, 4) or NULL */
+int orc_calc_one_stage_cuda(int32_t x32, const orc_mesh* m, const orc_thermo* th, double betta, double dt,
+                            const double* UOld, const double* ULeft, const double* URight, double* UNew,
+                            double* iFlux);
+double orc_cuda_split(int32_t x32, int32_t which, double M);
 double orc_local_dt(const orc_mesh* m, const orc_fields* f, double cfl);
 
 /* ---- Navier-Stokes (ns) scheme: cell gradients, Venkatakrishnan, Sutherland ---- */

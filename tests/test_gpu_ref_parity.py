@@ -304,3 +304,35 @@ def test_linear_field_known_answer_on_device():
     # the oracle agrees bit for bit on the whole mesh
     FLo, _ = O.calc_one_stage_muscl(mv, f, th, _oparams(bcs=bcs, udiff=False), U, 1.0, dt, 0.0)
     assert rel_linf(FL, FLo) <= TOL_STEP
+
+
+@pytest.mark.parametrize("x32", [False, True])
+@pytest.mark.parametrize("quads", ["none", "mixed"])
+def test_reference_gpu_stages(quads, x32):
+    """calcOneStageCUDA / calcOneStageCUDAx32 through the mirror: given the
+    oracle's face states (as computeMUSCLStencilsCUDA hands them over), the
+    update and the accumulated iFlux sums equal the oracle's restatement of the
+    GPU kernel bit for bit (Float32 arithmetic included); without face states
+    the device's own phase one gives the same result."""
+    S = pkg().stage
+    mesh, mv, th, f, U = _box_case(quads)
+    prm = _oparams(bcs=BOX_BCS, udiff=False)
+    _, _, ULo, URo = O.calc_one_stage_muscl(mv, f, th, prm, U, 1.0, 1e-3, 0.25, face_states=True)
+    Fo, UNo = O.calc_one_stage_cuda(mv, th, 1.0, 1e-3, U, ULo, URo, x32=x32)
+    n = mesh.nCells
+    ct = pkg().mesh.distibuteCellsInThreadsSA(3, n)
+    stage = S.calcOneStageCUDAx32 if x32 else S.calcOneStageCUDA
+    UN = np.zeros((n, 4), order="F")
+    fl = [np.zeros(n) for _ in range(4)]
+    UL = [np.asfortranarray(ULo[k]) for k in range(4)]
+    UR = [np.asfortranarray(URo[k]) for k in range(4)]
+    stage(1.0, 1e-3, 0.25, mesh, f, th, ct, U, None, UN, *UL, *UR, *fl, config=_cfg(S, bcs=BOX_BCS))
+    F = np.stack(fl, 1)
+    print("x32" if x32 else "fp64", quads, "iFlux", rel_linf(F, Fo), "U", rel_linf(UN, UNo))
+    assert np.array_equal(F, Fo)
+    assert np.array_equal(UN, UNo)
+    UN2 = np.zeros((n, 4), order="F")
+    fl2 = [np.zeros(n) for _ in range(4)]
+    stage(1.0, 1e-3, 0.25, mesh, f, th, ct, U, None, UN2, *([None] * 8), *fl2, config=_cfg(S, bcs=BOX_BCS))
+    assert np.array_equal(UN2, UN)
+    assert np.array_equal(np.stack(fl2, 1), F)

@@ -1222,3 +1222,90 @@ def test_air_conductivity_known_answers_per_branch():
         assert O.air_conductivity(T) == pytest.approx(k_of(cp, T), rel=1e-12), T
     # and the numbers themselves: c_p of air, J/(kg K) (low branch at 300 K)
     assert poly(low, 300.0) == pytest.approx(1004.75, abs=0.01)
+
+
+# ------------------------------------------------ the reference's GPU stages
+
+def _gpu_stage_case(quads):
+    th = M().THERMOPHYSICS(RGAS=1.0, Gamma=G)
+    mesh = M().make_box_mesh(12, 9, 1.0, 1.0, quads=quads)
+    f = random_fields(mesh)
+    mv, U = _setup(mesh, f)
+    prm = O.params(bcs={-1: (O.BC_FIXED, (1.2, 0.3, -0.2, 2.0)), -2: (O.BC_EXTRAPOLATE, ()),
+                        -3: (O.BC_SLIP_WALL, ()), -4: (O.BC_EXTRAPOLATE, ())}, include_udiff=False)
+    FL, UN, UL, UR = O.calc_one_stage_muscl(mv, f, th, prm, U, 1.0, 1e-3, 0.0, face_states=True)
+    return mesh, th, mv, U, FL, UN, UL, UR
+
+
+@pytest.mark.parametrize("quads", ["none", "mixed"])
+def test_reference_gpu_stage_fp64(quads):
+    """calcOneStageCUDA (calcOneStageCUDA.jl:4-135): on triangles the GPU
+    kernel's AUSM+ (plain h_t where the CPU writes muladd) summed over the
+    three faces equals the CPU two-phase stage to rounding; on quads the 4th
+    face is missing (its launch is commented out), so FLUXES differ by exactly
+    that face's CPU flux."""
+    mesh, th, mv, U, FL, UN, UL, UR = _gpu_stage_case(quads)
+    F64, UN64 = O.calc_one_stage_cuda(mv, th, 1.0, 1e-3, U, UL, UR, x32=False)
+    sc = np.abs(FL).max(axis=0)
+    tri = mesh.mesh_connectivity[:, 2] == 3
+    assert np.all(np.abs(F64 - FL)[tri] <= 1e-14 * sc)
+    if (~tri).any():
+        q = np.nonzero(~tri)[0]
+        e4 = np.array([O.ausm_plus_flux(UR[3][:, i], UL[3][:, i], mesh.cell_edges_Nx[i, 3], mesh.cell_edges_Ny[i, 3],
+                                        mesh.cell_edges_length[i, 3], G) for i in q])
+        assert np.abs(e4).max() > 1e-3 * sc.max()
+        assert np.all(np.abs(F64[q] - (FL[q] - e4)) <= 1e-13 * sc)
+    A = mesh.cell_areas[:, None]
+    assert np.allclose(UN64, U - F64 * 1e-3 / A, rtol=1e-15, atol=1e-15 * np.abs(U).max())
+
+
+def _x32_split_numpy(which, M):
+    """The x32 split functions (AUSMflux2dCUDAx32.jl:19-48) emulated with
+    numpy scalars under Julia's promotion: Float32 argument, Float64 literals."""
+    f32, f64 = np.float32, np.float64
+    m = f32(M)
+    B, A = f32(1.0 / 8.0), f32(3.0 / 16.0)
+    mm = f32(m * m)                    # Float32 * Float32
+    d = f64(mm) - 1.0                  # Float32 - Float64 literal
+    if abs(m) >= 1.0:
+        if which == 0: return f32(0.5 * f64(f32(m + abs(m))))
+        if which == 1: return f32(0.5 * f64(f32(m - abs(m))))
+        s = f32(np.sign(m))
+        return f32(0.5 * (1.0 + f64(s))) if which == 2 else f32(0.5 * (1.0 - f64(s)))
+    if which == 0: return f32(0.25 * (f64(m) + 1.0) * (f64(m) + 1.0) + f64(B) * d * d)
+    if which == 1: return f32(-0.25 * (f64(m) - 1.0) * (f64(m) - 1.0) - f64(B) * d * d)
+    am = f64(f32(A * m))               # AUSM_ALFA * M stays Float32
+    if which == 2: return f32(0.25 * (f64(m) + 1.0) * (f64(m) + 1.0) * (2.0 - f64(m)) + am * d * d)
+    return f32(0.25 * (f64(m) - 1.0) * (f64(m) - 1.0) * (2.0 + f64(m)) - am * d * d)
+
+
+def test_reference_gpu_stage_x32():
+    """calcOneStageCUDAx32: the split functions follow Julia's promotion (an
+    independent numpy emulation agrees bit for bit, and the pure-Float32
+    evaluation differs somewhere on the sample); M+ + M- = M and P+ + P- = 1 to
+    Float32 rounding; the stage equals the FP64 GPU stage to Float32 rounding
+    of the face states and sums, and is not the FP64 result."""
+    Ms = np.concatenate([np.linspace(-1.5, 1.5, 301), [0.9999999, -0.9999999, 1.0, -1.0, 0.0]])
+    differs = False
+    for which in range(4):
+        for m in Ms:
+            got = O.cuda_split(True, which, m)
+            assert got == float(_x32_split_numpy(which, m)), (which, m)
+            f = np.float32(m)
+            if abs(f) < 1 and which == 0:
+                pure = np.float32(np.float32(0.25) * (f + np.float32(1)) * (f + np.float32(1)) +
+                                  np.float32(0.125) * (f * f - np.float32(1)) * (f * f - np.float32(1)))
+                differs = differs or pure != np.float32(got)
+    assert differs
+    for m in Ms:
+        mf = float(np.float32(m))
+        assert abs(O.cuda_split(True, 0, m) + O.cuda_split(True, 1, m) - mf) <= 4e-7 * max(1.0, abs(mf))
+        assert abs(O.cuda_split(True, 2, m) + O.cuda_split(True, 3, m) - 1.0) <= 4e-7
+        assert abs(O.cuda_split(False, 0, m) + O.cuda_split(False, 1, m) - m) <= 1e-15 * max(1.0, abs(m))
+    mesh, th, mv, U, FL, UN, UL, UR = _gpu_stage_case("mixed")
+    F64, _ = O.calc_one_stage_cuda(mv, th, 1.0, 1e-3, U, UL, UR, x32=False)
+    F32, UN32 = O.calc_one_stage_cuda(mv, th, 1.0, 1e-3, U, UL, UR, x32=True)
+    sc = np.abs(F64).max(axis=0)
+    err = (np.abs(F32 - F64) / sc).max()
+    assert 1e-9 < err < 2e-6
+    assert np.all(F32 == F32.astype(np.float32))       # the sums are Float32 values
