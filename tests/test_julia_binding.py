@@ -99,3 +99,17 @@ def test_chain_rules_file_uses_the_binding():
     src = _src("LibGodunovChainRules.jl")
     assert "ChainRulesCore.rrule(::typeof(drag_sum)" in src
     assert "drag_gradient" in src and "using ..LibGodunov" in src
+
+
+def test_snapshot_io_reads_the_datasets_this_package_writes(tmp_path):
+    """LibGodunovIO.jl reads and writes exactly the datasets of
+    hdf5io.save_solution_hdf5 (the solutionCellsT fields)."""
+    g = pkg()
+    import numpy as np
+    path = str(tmp_path / "snap.h5")
+    g.hdf5io.save_solution_hdf5(path, 1e-3, 0.25, np.ones(3), np.zeros(3), np.zeros(3), np.ones(3))
+    written = set(g.hdf5io.read_hdf5(path))
+    src = _src("LibGodunovIO.jl")
+    read = set(re.findall(r'read\(io, "(\w+)"\)', src))
+    wrote = set(re.findall(r'write\(io, "(\w+)"', src))
+    assert read == written - {"dt"} and wrote == written
