@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <numeric>
@@ -454,6 +455,25 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
         delete M;
         return fail(GD_ERR_INVALID, "a cell lists the same neighbour on two faces");
       }
+  // --- face list of the reference scheme's face pass (see MeshView::rcf)
+  std::vector<int32_t> h_rfc, h_rfs, h_rcf(nf * nc, -1);
+  for (int64_t c = 0; c < nc; ++c) {
+    if (h_nv[c] != 3) continue;
+    for (int k = 0; k < 3; ++k) {
+      const int64_t s = (int64_t)k * nc + c;
+      const int32_t j = h_nb[s];
+      if (j < 0 || h_nv[j] != 3 || cp[j] < cp[c]) continue;  // boundary, quad, or the neighbour owns it
+      const int64_t sj = (int64_t)h_kn[s] * nc + j;
+      const bool mirror = h_nx[sj] == -h_nx[s] && h_ny[sj] == -h_ny[s] && h_len[sj] == h_len[s] &&
+                          h_dn1[sj] == h_up1[s] && h_up1[sj] == h_dn1[s];
+      if (!mirror) continue;
+      const int32_t f = (int32_t)h_rfc.size();
+      h_rfc.push_back((int32_t)c);
+      h_rfs.push_back(k);
+      h_rcf[s] = 2 * f;
+      h_rcf[sj] = 2 * f + 1;
+    }
+  }
   auto pair =[&](const std::vector<double>* a) {
     std::vector<double> o(2 * nfaces);
     std::copy(a[0].begin(), a[0].end(), o.begin());
@@ -532,6 +552,10 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
   rc |= upload(al, &v.f_len32, to_float(h_flen), B);
   rc |= upload(al, &v.hmin32, to_float(h_hmin), B);
   rc |= upload(al, &v.cface, h_cface, B);
+  v.n_rfaces = (int64_t)h_rfc.size();
+  rc |= upload(al, &v.rf_cell, h_rfc, B);
+  rc |= upload(al, &v.rf_slot, h_rfs, B);
+  rc |= upload(al, &v.rcf, h_rcf, B);
   rc |= upload(al, &v.cl, h_cl, B);
   rc |= upload(al, &v.w, h_w, B);
   rc |= upload(al, &v.xn, h_xn, B);
@@ -1036,6 +1060,22 @@ static double bytes_ref_step(const MeshView& v, bool visc) {
   if (visc) b += nc * (24 /*ea,eb*/ + 8 /*area*/) + nn * 48 /*g6N*/;
   return b;
 }
+// the face pass: per listed face its ids, the owner slot's geometry and node
+// ids, the flux written; cell and node primitives gathered at unique size
+static double bytes_ref_faces(const MeshView& v) {
+  return (double)v.n_rfaces * (8 /*cell, slot*/ + 12 /*nb, dn1, up1*/ + 24 /*nx, ny, len*/ + 32 /*flux*/) +
+         (double)v.nc * 32 + (double)v.nn * 32;
+}
+// the step with the face pass: listed faces' fluxes read instead of evaluated
+// (the per-side faces' data is counted as in bytes_ref_step, in the few
+// boundary and non-mirrored faces it is not)
+static double bytes_ref_step_faces(const MeshView& v, bool visc) {
+  const double nc = (double)v.nc, nn = (double)v.nn;
+  double b = nc * (32 /*Q own*/ + 12 /*rcf*/ + 8 /*rarea*/ + 8 /*hmin*/ + 32 /*U old*/ + 32 /*U new*/ + 32 /*Q new*/) +
+             (double)v.n_rfaces * 32 /*face fluxes, unique*/;
+  if (visc) b += nc * (24 /*ea,eb*/ + 72 /*nx,ny,len*/ + 8 /*area*/) + nn * 48 /*g6N*/;
+  return b;
+}
 static double bytes_ref_nodes(const MeshView& v, bool visc) {
   const double nc = (double)v.nc, nn = (double)v.nn;
   const double inc = 3.0 * nc;  // cell-node incidences of a triangle mesh
@@ -1092,6 +1132,15 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
   }
   if (!ns) {
     const bool visc = P.viscous != 0;
+    // the face pass is exact where the two sides' evaluations are exact
+    // negatives: AUSM+ with an odd, symmetric limiter (GODUNOV_REF_PER_SIDE=1
+    // evaluates every face from each of its cells, for comparison)
+    const char* per_side = std::getenv("GODUNOV_REF_PER_SIDE");
+    const bool face_pass = v.n_rfaces > 0 && P.flux == GD_FLUX_AUSM_PLUS &&
+                           (P.limiter == GD_LIM_REF_HARMONIC || P.limiter == GD_LIM_VAN_ALBADA_A ||
+                            P.limiter == GD_LIM_MINMOD) &&
+                           !(per_side && per_side[0] == '1');
+    double* rflux = static_cast<double*>(S->fflux);  // n_rfaces <= the ns face list's count
     if (visc) GD_HIP(T.run("ref_phi", 0.0, [&] { return launch_ref_phi(v, S->U[S->cur], S->phiN, S->stream); }));
     for (int32_t s = 0; s < n_steps; ++s) {
       if (visc) {
@@ -1118,7 +1167,12 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       a.include_udiff = 1;
       a.wall_f = wall ? S->wall_f : nullptr;
       a.wall_code = rp->wall_code;
-      GD_HIP(T.run("ref_step", bytes_ref_step(v, visc), [&] { return launch_ref_step(a, S->stream); }));
+      if (face_pass) {
+        GD_HIP(T.run("ref_faces", bytes_ref_faces(v), [&] { return launch_ref_faces(a, rflux, S->stream); }));
+        a.rflux = rflux;
+      }
+      GD_HIP(T.run("ref_step", face_pass ? bytes_ref_step_faces(v, visc) : bytes_ref_step(v, visc),
+                   [&] { return launch_ref_step(a, S->stream); }));
       S->cur ^= 1;
       S->qc ^= 1;
       // the node pass also closes the step: dt partials into the next slot,

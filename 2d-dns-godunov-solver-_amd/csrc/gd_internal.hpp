@@ -84,6 +84,14 @@ struct MeshView {
   const float* f_r132;
   const float* f_len32;
   const int32_t* cface; // [nf][nc]: 2 f + (1 if the cell is the face's neighbour), -1 for no face
+  // reference-scheme face list: the interior faces between two triangles
+  // whose two sides mirror each other exactly (inward normals exact
+  // negatives, equal lengths, each side's far vertex the other's), evaluated
+  // once from the owner (lower caller id); faces numbered in owner order
+  int64_t n_rfaces;
+  const int32_t* rf_cell;  // owner cell
+  const int32_t* rf_slot;  // the owner's slot of the face
+  const int32_t* rcf;      // [nf][nc]: 2 f + (1 if the cell is not the owner), -1 when evaluated per side
   // per node
   const int32_t* cl;   // [ncl][nn] cluster cell ids, -1 padded
   const double* w;     // [ncl][nn] interpolation weights
@@ -185,6 +193,9 @@ struct RefStepArgs {
   int32_t include_udiff;
   double* wall_f;
   int32_t wall_code;
+  // face pass: the fluxes of the listed faces (MeshView::rcf), or nullptr:
+  // every face evaluated from each of its cells
+  const double* rflux;
 };
 
 // what a launch closes of the step before it (see k_ref_nodes); slot null: nothing
@@ -199,6 +210,8 @@ struct FoldArgs {
 
 hipError_t launch_ref_step(const RefStepArgs& a, hipStream_t s);
 hipError_t launch_ref_face_states(const RefStepArgs& a, double* UL, double* UR, hipStream_t s);
+// the face pass: the flux of every listed face from its owner, into rflux [n_rfaces][4]
+hipError_t launch_ref_faces(const RefStepArgs& a, double* rflux, hipStream_t s);
 // calcOneStageCUDA (x32 false) / calcOneStageCUDAx32 from face states UL/UR
 // [slot][4][nc]: F = iFlux [4][nc], Un = Uo - F betta dt / A
 hipError_t launch_ref_cuda_stage(const MeshView& m, double gamma, bool x32, const double* UL, const double* UR,

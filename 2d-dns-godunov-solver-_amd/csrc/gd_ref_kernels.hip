@@ -174,13 +174,24 @@ __device__ __forceinline__ void ref_cell_flux(const RefStepArgs& a, int64_t i, i
   for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
     if (k < nf) {
       const int64_t s = k * nc + i;
-      double fL[4], fR[4];
-      const int32_t j = ref_face_states<QUADS>(a, i, k, nf, qL, t, fL, fR);
       double e[4];
-      refm::face_flux(a.prm.flux, fR, fL, m.fnx[s], m.fny[s], m.flen[s], g, e);  // neighbour side is "L"
-      if (j < 0 && wallF && m.wall_id && a.wall_code == j + 1) {
-        const int32_t wid = m.wall_id[s];
-        if (wid >= 0) { wallF[2 * wid] = e[1]; wallF[2 * wid + 1] = e[2]; }
+      const int32_t cf = a.rflux ? m.rcf[s] : -1;
+      if (cf >= 0) {
+        // evaluated once by the face pass from the owner's side; the other
+        // side's evaluation is its exact negative (mirrored stencils, odd and
+        // symmetric limiter, AUSM+ antisymmetric under L <-> R, n -> -n)
+        double x[4];
+        ld2k<2>(a.rflux + 4 * (int64_t)(cf >> 1), x);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) e[v] = (cf & 1) ? -x[v] : x[v];
+      } else {
+        double fL[4], fR[4];
+        const int32_t j = ref_face_states<QUADS>(a, i, k, nf, qL, t, fL, fR);
+        refm::face_flux(a.prm.flux, fR, fL, m.fnx[s], m.fny[s], m.flen[s], g, e);  // neighbour side is "L"
+        if (j < 0 && wallF && m.wall_id && a.wall_code == j + 1) {
+          const int32_t wid = m.wall_id[s];
+          if (wid >= 0) { wallF[2 * wid] = e[1]; wallF[2 * wid + 1] = e[2]; }
+        }
       }
       if (k == 0) {
 #pragma unroll
@@ -191,6 +202,25 @@ __device__ __forceinline__ void ref_cell_flux(const RefStepArgs& a, int64_t i, i
       }
     }
   }
+}
+
+// The face pass of the reference loop: the flux of each listed face (an
+// interior face between two triangles with mirrored stencils) from its
+// owner's side, computed exactly as ref_cell_flux computes it there.
+__global__ __launch_bounds__(kBlock) void k_ref_faces(RefStepArgs a, double* rflux) {
+  const MeshView& m = a.m;
+  const int64_t f = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (f >= m.n_rfaces) return;
+  const int64_t nc = m.nc;
+  const int64_t i = m.rf_cell[f];
+  const int k = m.rf_slot[f];
+  const double qL[4] = {a.Q[i], a.Q[nc + i], a.Q[2 * nc + i], a.Q[3 * nc + i]};
+  double fL[4], fR[4];
+  (void)ref_face_states<false>(a, i, k, 3, qL, a.flow_time, fL, fR);
+  const int64_t s = (int64_t)k * nc + i;
+  double e[4];
+  refm::face_flux(a.prm.flux, fR, fL, m.fnx[s], m.fny[s], m.flen[s], a.prm.Gamma, e);
+  st2k<2>(rflux + 4 * f, e);
 }
 
 // The reference's GPU stages, calcOneStageCUDA (calcOneStageCUDA.jl:4-135)
@@ -721,6 +751,12 @@ hipError_t launch_ref_face_states(const RefStepArgs& a, double* UL, double* UR, 
     hipLaunchKernelGGL(k_ref_face_states<true>, dim3(nblocks(a.m.nc)), dim3(kBlock), 0, s, a, UL, UR);
   else
     hipLaunchKernelGGL(k_ref_face_states<false>, dim3(nblocks(a.m.nc)), dim3(kBlock), 0, s, a, UL, UR);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_faces(const RefStepArgs& a, double* rflux, hipStream_t s) {
+  if (a.m.n_rfaces == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ref_faces, dim3(nblocks(a.m.n_rfaces)), dim3(kBlock), 0, s, a, rflux);
   return hipGetLastError();
 }
 
