@@ -214,14 +214,18 @@ def roofline(kt, steps):
                  "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
 
 
-def measured_traffic(kernel_name):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+def measured_traffic(kernel_name, n_cells, device_order):
+    """HBM bytes per launch of `kernel_name` from the committed rocprofv3 PMC
+    summary, with the build it was measured on, or (None, None) when that
+    summary is of another mesh size or device order (traffic depends on both)."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         d = json.load(open(path))
-        return d.get(kernel_name, {}).get("bytes_per_launch")
     except Exception:
-        return None
+        return None, None
+    if d.get("cells") != n_cells or d.get("device_order") != device_order:
+        return None, None
+    return d.get(kernel_name, {}).get("bytes_per_launch"), d.get("commit")
 
 
 def host_info():
@@ -358,7 +362,7 @@ def main():
     r64 = res[head]
     value = mesh.nCells / (r64["ms_per_step"] * 1e-3)
     dom, roof = roofline(r64["kernels"], a.steps)
-    roof["traffic"] = measured_traffic(dom["name"])
+    roof["traffic"], roof["traffic_commit"] = measured_traffic(dom["name"], mesh.nCells, a.device_order)
 
     def step_frac(r):  # algorithmic bytes of one whole step over its time
         b = sum(k["bytes_per_launch"] for k in r["kernels"])
