@@ -456,7 +456,8 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
         return fail(GD_ERR_INVALID, "a cell lists the same neighbour on two faces");
       }
   // --- face list of the reference scheme's face pass (see MeshView::rcf)
-  std::vector<int32_t> h_rfc, h_rfs, h_rcf(nf * nc, -1);
+  std::vector<int32_t> h_rfc, h_rfs, h_rcf(nf * nc, -1), h_rfnb, h_rfdn1, h_rfup1;
+  std::vector<double> h_rfnx, h_rfny, h_rflen;
   for (int64_t c = 0; c < nc; ++c) {
     if (h_nv[c] != 3) continue;
     for (int k = 0; k < 3; ++k) {
@@ -472,8 +473,27 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
       h_rfs.push_back(k);
       h_rcf[s] = 2 * f;
       h_rcf[sj] = 2 * f + 1;
+      h_rfnb.push_back(j);
+      h_rfdn1.push_back(h_dn1[s]);
+      h_rfup1.push_back(h_up1[s]);
+      h_rfnx.push_back(h_nx[s]);
+      h_rfny.push_back(h_ny[s]);
+      h_rflen.push_back(h_len[s]);
     }
   }
+  const int64_t n_rmirror = (int64_t)h_rfc.size();
+  if (2 * (int64_t)nf * nc + 1 > (int64_t)INT32_MAX) {  // rcf holds 2 f + side in int32
+    delete M;
+    return fail(GD_ERR_INVALID, "too many face slots");
+  }
+  for (int64_t c = 0; c < nc; ++c)  // one-sided entries: every other face slot, in cell order
+    for (int k = 0; k < h_nv[c]; ++k) {
+      const int64_t s = (int64_t)k * nc + c;
+      if (h_rcf[s] >= 0) continue;
+      h_rcf[s] = 2 * (int32_t)h_rfc.size();
+      h_rfc.push_back((int32_t)c);
+      h_rfs.push_back(k);
+    }
   auto pair =[&](const std::vector<double>* a) {
     std::vector<double> o(2 * nfaces);
     std::copy(a[0].begin(), a[0].end(), o.begin());
@@ -553,9 +573,16 @@ int gd_mesh_create(const gd_mesh_desc* d, int32_t order, const int64_t* perm, gd
   rc |= upload(al, &v.hmin32, to_float(h_hmin), B);
   rc |= upload(al, &v.cface, h_cface, B);
   v.n_rfaces = (int64_t)h_rfc.size();
+  v.n_rmirror = n_rmirror;
   rc |= upload(al, &v.rf_cell, h_rfc, B);
   rc |= upload(al, &v.rf_slot, h_rfs, B);
   rc |= upload(al, &v.rcf, h_rcf, B);
+  rc |= upload(al, &v.rf_nb, h_rfnb, B);
+  rc |= upload(al, &v.rf_dn1, h_rfdn1, B);
+  rc |= upload(al, &v.rf_up1, h_rfup1, B);
+  rc |= upload(al, &v.rf_nx, h_rfnx, B);
+  rc |= upload(al, &v.rf_ny, h_rfny, B);
+  rc |= upload(al, &v.rf_len, h_rflen, B);
   rc |= upload(al, &v.cl, h_cl, B);
   rc |= upload(al, &v.w, h_w, B);
   rc |= upload(al, &v.xn, h_xn, B);
@@ -651,7 +678,8 @@ int gd_state_create(gd_mesh* m, const gd_params* p, gd_state** out) {
   rc |= dev_alloc(al, &rbuf, 16 * nc);
   S->rec = rbuf;
   double* fbuf = nullptr;
-  rc |= dev_alloc(al, &fbuf, 4 * (size_t)std::max<int64_t>(m->v.n_faces, 1));
+  // the ns face fluxes, or the reference loop's face-pass fluxes
+  rc |= dev_alloc(al, &fbuf, 4 * (size_t)std::max<int64_t>(std::max(m->v.n_faces, m->v.n_rfaces), 1));
   S->fflux = fbuf;
   double* qt0 = nullptr;
   double* qt1 = nullptr;
@@ -1060,19 +1088,20 @@ static double bytes_ref_step(const MeshView& v, bool visc) {
   if (visc) b += nc * (24 /*ea,eb*/ + 8 /*area*/) + nn * 48 /*g6N*/;
   return b;
 }
-// the face pass: per listed face its ids, the owner slot's geometry and node
-// ids, the flux written; cell and node primitives gathered at unique size
+// the face pass: per mirrored face its owner, neighbour and far-node ids, the
+// owner slot's geometry and the flux written; per one-sided entry its cell and
+// slot besides; cell and node primitives gathered at unique size
 static double bytes_ref_faces(const MeshView& v) {
-  return (double)v.n_rfaces * (8 /*cell, slot*/ + 12 /*nb, dn1, up1*/ + 24 /*nx, ny, len*/ + 32 /*flux*/) +
-         (double)v.nc * 32 + (double)v.nn * 32;
+  const double n1 = (double)(v.n_rfaces - v.n_rmirror);
+  return (double)v.n_rmirror * (4 /*cell*/ + 12 /*nb, dn1, up1*/ + 24 /*nx, ny, len*/ + 32 /*flux*/) +
+         n1 * (8 /*cell, slot*/ + 12 /*nb, dn1, up1*/ + 24 /*nx, ny, len*/ + 32 /*flux*/) + (double)v.nc * 32 +
+         (double)v.nn * 32;
 }
-// the step with the face pass: listed faces' fluxes read instead of evaluated
-// (the per-side faces' data is counted as in bytes_ref_step, in the few
-// boundary and non-mirrored faces it is not)
+// the step with the face pass: every face slot's flux read (unique per entry)
 static double bytes_ref_step_faces(const MeshView& v, bool visc) {
   const double nc = (double)v.nc, nn = (double)v.nn;
   double b = nc * (32 /*Q own*/ + 12 /*rcf*/ + 8 /*rarea*/ + 8 /*hmin*/ + 32 /*U old*/ + 32 /*U new*/ + 32 /*Q new*/) +
-             (double)v.n_rfaces * 32 /*face fluxes, unique*/;
+             (double)v.n_rfaces * 32 /*face fluxes*/;
   if (visc) b += nc * (24 /*ea,eb*/ + 72 /*nx,ny,len*/ + 8 /*area*/) + nn * 48 /*g6N*/;
   return b;
 }
@@ -1140,7 +1169,7 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
                            (P.limiter == GD_LIM_REF_HARMONIC || P.limiter == GD_LIM_VAN_ALBADA_A ||
                             P.limiter == GD_LIM_MINMOD) &&
                            !(per_side && per_side[0] == '1');
-    double* rflux = static_cast<double*>(S->fflux);  // n_rfaces <= the ns face list's count
+    double* rflux = static_cast<double*>(S->fflux);  // sized for n_rfaces entries
     if (visc) GD_HIP(T.run("ref_phi", 0.0, [&] { return launch_ref_phi(v, S->U[S->cur], S->phiN, S->stream); }));
     for (int32_t s = 0; s < n_steps; ++s) {
       if (visc) {

@@ -84,14 +84,26 @@ struct MeshView {
   const float* f_r132;
   const float* f_len32;
   const int32_t* cface; // [nf][nc]: 2 f + (1 if the cell is the face's neighbour), -1 for no face
-  // reference-scheme face list: the interior faces between two triangles
-  // whose two sides mirror each other exactly (inward normals exact
-  // negatives, equal lengths, each side's far vertex the other's), evaluated
-  // once from the owner (lower caller id); faces numbered in owner order
+  // reference-scheme face list, the entries of the loop's face pass.  First
+  // the n_rmirror interior faces between two triangles whose two sides
+  // mirror each other exactly (inward normals exact negatives, equal lengths,
+  // each side's far vertex the other's), evaluated once from the owner (lower
+  // caller id) and numbered in owner order; then one entry per remaining
+  // (cell, slot) — boundary faces, faces of quads, faces whose two sides do
+  // not mirror bit for bit — evaluated from that cell's side.
   int64_t n_rfaces;
-  const int32_t* rf_cell;  // owner cell
-  const int32_t* rf_slot;  // the owner's slot of the face
-  const int32_t* rcf;      // [nf][nc]: 2 f + (1 if the cell is not the owner), -1 when evaluated per side
+  int64_t n_rmirror;
+  const int32_t* rf_cell;  // owner cell / the cell of a one-sided entry
+  const int32_t* rf_slot;  // its slot of the face
+  const int32_t* rcf;      // [nf][nc]: 2 f + (1 if the cell is a mirrored face's non-owner), -1 for no face
+  // the mirrored faces' owner-slot data in face order: neighbour, far nodes
+  // of the owner and of the neighbour side, owner's inward normal and length
+  const int32_t* rf_nb;
+  const int32_t* rf_dn1;
+  const int32_t* rf_up1;
+  const double* rf_nx;
+  const double* rf_ny;
+  const double* rf_len;
   // per node
   const int32_t* cl;   // [ncl][nn] cluster cell ids, -1 padded
   const double* w;     // [ncl][nn] interpolation weights

@@ -611,6 +611,8 @@ __global__ __launch_bounds__(kBlock) void k_ns_update(NsKArgs<T> a) {
       }
     }
     const double ra = m.rarea[i];
+    // read before the stores below, which the compiler may not move a load past
+    const double h = (double)FaceGeo<T>::hmin(m, i);
     double Un[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -623,7 +625,6 @@ __global__ __launch_bounds__(kBlock) void k_ns_update(NsKArgs<T> a) {
     st4(a.Qw + 4 * i, (T)rho, (T)ux, (T)uy, (T)p);
     const double as = sqrt(Gm * p / rho);
     const double vm = sqrt(ux * ux + uy * uy) + as;
-    const double h = (double)FaceGeo<T>::hmin(m, i);
     double lam = vm;
     if (VISC) {
       const double Tc = p / (rho * P.RGAS);

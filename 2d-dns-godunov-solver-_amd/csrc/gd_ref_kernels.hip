@@ -174,24 +174,12 @@ __device__ __forceinline__ void ref_cell_flux(const RefStepArgs& a, int64_t i, i
   for (int k = 0; k < (QUADS ? 4 : 3); ++k) {
     if (k < nf) {
       const int64_t s = k * nc + i;
-      double e[4];
-      const int32_t cf = a.rflux ? m.rcf[s] : -1;
-      if (cf >= 0) {
-        // evaluated once by the face pass from the owner's side; the other
-        // side's evaluation is its exact negative (mirrored stencils, odd and
-        // symmetric limiter, AUSM+ antisymmetric under L <-> R, n -> -n)
-        double x[4];
-        ld2k<2>(a.rflux + 4 * (int64_t)(cf >> 1), x);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) e[v] = (cf & 1) ? -x[v] : x[v];
-      } else {
-        double fL[4], fR[4];
-        const int32_t j = ref_face_states<QUADS>(a, i, k, nf, qL, t, fL, fR);
-        refm::face_flux(a.prm.flux, fR, fL, m.fnx[s], m.fny[s], m.flen[s], g, e);  // neighbour side is "L"
-        if (j < 0 && wallF && m.wall_id && a.wall_code == j + 1) {
-          const int32_t wid = m.wall_id[s];
-          if (wid >= 0) { wallF[2 * wid] = e[1]; wallF[2 * wid + 1] = e[2]; }
-        }
+      double e[4], fL[4], fR[4];
+      const int32_t j = ref_face_states<QUADS>(a, i, k, nf, qL, t, fL, fR);
+      refm::face_flux(a.prm.flux, fR, fL, m.fnx[s], m.fny[s], m.flen[s], g, e);  // neighbour side is "L"
+      if (j < 0 && wallF && m.wall_id && a.wall_code == j + 1) {
+        const int32_t wid = m.wall_id[s];
+        if (wid >= 0) { wallF[2 * wid] = e[1]; wallF[2 * wid + 1] = e[2]; }
       }
       if (k == 0) {
 #pragma unroll
@@ -204,22 +192,86 @@ __device__ __forceinline__ void ref_cell_flux(const RefStepArgs& a, int64_t i, i
   }
 }
 
-// The face pass of the reference loop: the flux of each listed face (an
-// interior face between two triangles with mirrored stencils) from its
-// owner's side, computed exactly as ref_cell_flux computes it there.
+// ref_cell_flux from the face pass's fluxes (MeshView::rcf covers every face
+// slot of every cell): a mirrored face's flux evaluated once from its owner's
+// side enters the other cell negated — that side's own evaluation is its exact
+// negative (mirrored stencils, odd and symmetric limiter, AUSM+ antisymmetric
+// under L <-> R, n -> -n) — and the others as evaluated from this cell's side.
+// All slots' fluxes are loaded before the sum, which keeps the slot order.
+template <bool QUADS>
+__device__ __forceinline__ void ref_cell_flux_gathered(const RefStepArgs& a, int64_t i, int nf, double* F) {
+  constexpr int NS = QUADS ? 4 : 3;
+  const int64_t nc = a.m.nc;
+  int32_t cf[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) cf[k] = (k < nf) ? a.m.rcf[k * nc + i] : 0;
+  double x[NS][4];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) ld2k<2>(a.rflux + 4 * (int64_t)(cf[k] >> 1), x[k]);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    if (k < nf) {
+      double e[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) e[v] = (cf[k] & 1) ? -x[k][v] : x[k][v];
+      if (k == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) F[v] = e[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) F[v] = F[v] + e[v];
+      }
+    }
+  }
+}
+
+// The face pass of the reference loop (the accumulator slot of the step gives
+// its flow time, as in k_ref_step).  Entries below n_rmirror: the flux of an
+// interior face between two triangles with mirrored stencils from its owner's
+// side, from face-ordered copies of the owner slot's data; the rest: the flux
+// of one cell's face slot from that cell's side, with its boundary ghost and
+// wall force.  Both are computed exactly as ref_cell_flux computes them there.
+template <bool QUADS>
 __global__ __launch_bounds__(kBlock) void k_ref_faces(RefStepArgs a, double* rflux) {
   const MeshView& m = a.m;
   const int64_t f = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (f >= m.n_rfaces) return;
-  const int64_t nc = m.nc;
+  const int64_t nc = m.nc, nn = m.nn;
   const int64_t i = m.rf_cell[f];
-  const int k = m.rf_slot[f];
-  const double qL[4] = {a.Q[i], a.Q[nc + i], a.Q[2 * nc + i], a.Q[3 * nc + i]};
-  double fL[4], fR[4];
-  (void)ref_face_states<false>(a, i, k, 3, qL, a.flow_time, fL, fR);
-  const int64_t s = (int64_t)k * nc + i;
   double e[4];
-  refm::face_flux(a.prm.flux, fR, fL, m.fnx[s], m.fny[s], m.flen[s], a.prm.Gamma, e);
+  if (f < m.n_rmirror) {
+    const int32_t j = m.rf_nb[f], d1 = m.rf_dn1[f], u1 = m.rf_up1[f];
+    const double nx = m.rf_nx[f], ny = m.rf_ny[f], len = m.rf_len[f];
+    double qL[4], qR[4], up[4], dn[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      qL[v] = a.Q[v * nc + i];
+      qR[v] = a.Q[v * nc + j];
+      up[v] = a.Qn[v * nn + u1];
+      dn[v] = a.Qn[v * nn + d1];
+    }
+    const double eps = a.prm.limiter_eps;
+    double fL[4], fR[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {  // ref_face_states on an interior face
+      fL[v] = qL[v] + 0.5 * refm::limiter(a.prm.limiter, qL[v] - dn[v], qR[v] - qL[v], eps);
+      fR[v] = qR[v] - 0.5 * refm::limiter(a.prm.limiter, qR[v] - qL[v], up[v] - qR[v], eps);
+    }
+    refm::face_flux(a.prm.flux, fR, fL, nx, ny, len, a.prm.Gamma, e);
+  } else {
+    const int k = m.rf_slot[f];
+    const double t = a.acc[a.step % 3].time;
+    const double qL[4] = {a.Q[i], a.Q[nc + i], a.Q[2 * nc + i], a.Q[3 * nc + i]};
+    const int nf = QUADS ? m.nv[i] : 3;
+    double fL[4], fR[4];
+    const int32_t j = ref_face_states<QUADS>(a, i, k, nf, qL, t, fL, fR);
+    const int64_t s = (int64_t)k * nc + i;
+    refm::face_flux(a.prm.flux, fR, fL, m.fnx[s], m.fny[s], m.flen[s], a.prm.Gamma, e);  // neighbour side is "L"
+    if (j < 0 && a.wall_f && m.wall_id && a.wall_code == j + 1) {
+      const int32_t wid = m.wall_id[s];
+      if (wid >= 0) { a.wall_f[2 * wid] = e[1]; a.wall_f[2 * wid + 1] = e[2]; }
+    }
+  }
   st2k<2>(rflux + 4 * f, e);
 }
 
@@ -351,7 +403,8 @@ __global__ __launch_bounds__(kBlock) void k_ref_face_states(RefStepArgs a, doubl
   }
 }
 
-template <bool QUADS, bool VISC, bool FUSED>
+// FP: the loop with the face pass (a.rflux holds every face slot's flux)
+template <bool QUADS, bool VISC, bool FUSED, bool FP>
 __global__ __launch_bounds__(kBlock) void k_ref_step(RefStepArgs a) {
   const MeshView& m = a.m;
   const int64_t nc = m.nc;
@@ -366,9 +419,12 @@ __global__ __launch_bounds__(kBlock) void k_ref_step(RefStepArgs a) {
   unsigned bad = 0;
   if (i < nc) {
     const double qL[4] = {a.Q[i], a.Q[nc + i], a.Q[2 * nc + i], a.Q[3 * nc + i]};
+    // everything of the cell itself is read before the first store (the
+    // compiler does not move a load past a store it cannot prove disjoint)
     const int nf = QUADS ? m.nv[i] : 3;
     double F[4];
-    ref_cell_flux<QUADS>(a, i, nf, qL, t, F, FUSED ? a.wall_f : nullptr);
+    if (FP) ref_cell_flux_gathered<QUADS>(a, i, nf, F);
+    else ref_cell_flux<QUADS>(a, i, nf, qL, t, F, FUSED ? a.wall_f : nullptr);
     double ud[4] = {0.0, 0.0, 0.0, 0.0};
     bool with_ud = FUSED ? true : (a.include_udiff && a.Ud);
     if (VISC) ref_udiff<QUADS>(m, a.prm, a.g6N, i, nf, qL, ud);
@@ -411,10 +467,17 @@ __global__ __launch_bounds__(kBlock) void k_ref_step(RefStepArgs a) {
 }
 
 // cells2nodesSolutionReconstructionWithStencils (utilsFVM2dp.jl:299-365) plus,
-// when phiN is given, the unweighted cluster means of calcPhiInCellCluster
-// (calcGradients.jl:3-18) for rho*u, rho*v, E.  With fa.slot set, two leading
-// blocks close the step that ran before: the dt partials into fa.slot and the
-// wall faces' forces into fa.drag.
+// when PHI, the unweighted cluster means of calcPhiInCellCluster
+// (calcGradients.jl:3-18) for rho*u, rho*v, E; QN / UN / PHI select the outputs
+// (Qn, Un, phiN).  With fa.slot set, two leading blocks close the step that ran
+// before: the dt partials into fa.slot and the wall faces' forces into fa.drag.
+//
+// NCL > 0 is the cluster width of the mesh, known at compile time: every
+// slot's cell id, weight and cell values are loaded before the sums, so a
+// node waits on two rounds of loads instead of two per slot; the sums then run
+// in slot order, skipping empty slots, exactly as the one-slot-at-a-time loop
+// (NCL == 0, any width) does, so both give the same bits.
+template <int NCL, bool QN, bool UN, bool PHI>
 __global__ __launch_bounds__(kBlock) void k_ref_nodes(MeshView m, const double* Q, const double* U, double* Qn,
                                                       double* Un, double* phiN, FoldArgs fa) {
   const unsigned nfold = fa.slot ? 2u : 0u;
@@ -430,28 +493,54 @@ __global__ __launch_bounds__(kBlock) void k_ref_nodes(MeshView m, const double* 
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   double f1 = 0.0, f2 = 0.0, f3 = 0.0;
   int cnt = 0;
-  for (int q = 0; q < m.ncl; ++q) {
-    const int32_t c = m.cl[q * nn + J];
-    if (c >= 0) {
-      const double wi = m.w[q * nn + J];
-      if (Qn) {
-        r = r + Q[c] * wi; u = u + Q[nc + c] * wi; v = v + Q[2 * nc + c] * wi; p = p + Q[3 * nc + c] * wi;
+  // one slot's contribution, in the source's order of operations
+  auto add = [&](double wi, const double* qc, const double* uc) {
+    if (QN) { r = r + qc[0] * wi; u = u + qc[1] * wi; v = v + qc[2] * wi; p = p + qc[3] * wi; }
+    if (UN) { s0 = s0 + uc[0] * wi; s1 = s1 + uc[1] * wi; s2 = s2 + uc[2] * wi; s3 = s3 + uc[3] * wi; }
+    if (PHI) { f1 = f1 + uc[1]; f2 = f2 + uc[2]; f3 = f3 + uc[3]; }
+    det = det + wi;
+    ++cnt;
+  };
+  if constexpr (NCL > 0) {
+    int32_t c[NCL];
+#pragma unroll
+    for (int q = 0; q < NCL; ++q) c[q] = m.cl[q * nn + J];
+    double wv[NCL], qv[NCL][4], uv[NCL][4];
+#pragma unroll
+    for (int q = 0; q < NCL; ++q) {
+      const int64_t k = c[q] >= 0 ? c[q] : 0;  // an empty slot loads cell 0 and is skipped below
+      wv[q] = m.w[q * nn + J];
+      if (QN) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) qv[q][t] = Q[t * nc + k];
       }
-      if (Un) {
-        s0 = s0 + U[c] * wi; s1 = s1 + U[nc + c] * wi; s2 = s2 + U[2 * nc + c] * wi; s3 = s3 + U[3 * nc + c] * wi;
+      if (UN || PHI) {
+#pragma unroll
+        for (int t = UN ? 0 : 1; t < 4; ++t) uv[q][t] = U[t * nc + k];
       }
-      if (phiN) { f1 = f1 + U[nc + c]; f2 = f2 + U[2 * nc + c]; f3 = f3 + U[3 * nc + c]; }
-      det = det + wi;
-      ++cnt;
+    }
+#pragma unroll
+    for (int q = 0; q < NCL; ++q)
+      if (c[q] >= 0) add(wv[q], qv[q], uv[q]);
+  } else {
+    for (int q = 0; q < m.ncl; ++q) {
+      const int32_t c = m.cl[q * nn + J];
+      if (c >= 0) {
+        double qc[4] = {0, 0, 0, 0}, uc[4] = {0, 0, 0, 0};
+        if (QN) { qc[0] = Q[c]; qc[1] = Q[nc + c]; qc[2] = Q[2 * nc + c]; qc[3] = Q[3 * nc + c]; }
+        if (UN) uc[0] = U[c];
+        if (UN || PHI) { uc[1] = U[nc + c]; uc[2] = U[2 * nc + c]; uc[3] = U[3 * nc + c]; }
+        add(m.w[q * nn + J], qc, uc);
+      }
     }
   }
   if (det != 0) {
     r = r / det; u = u / det; v = v / det; p = p / det;
     s0 = s0 / det; s1 = s1 / det; s2 = s2 / det; s3 = s3 / det;
   }
-  if (Qn) { Qn[J] = r; Qn[nn + J] = u; Qn[2 * nn + J] = v; Qn[3 * nn + J] = p; }
-  if (Un) { Un[J] = s0; Un[nn + J] = s1; Un[2 * nn + J] = s2; Un[3 * nn + J] = s3; }
-  if (phiN) {
+  if (QN) { Qn[J] = r; Qn[nn + J] = u; Qn[2 * nn + J] = v; Qn[3 * nn + J] = p; }
+  if (UN) { Un[J] = s0; Un[nn + J] = s1; Un[2 * nn + J] = s2; Un[3 * nn + J] = s3; }
+  if (PHI) {
     const double dc = (double)cnt;
     const double o[4] = {cnt ? f1 / dc : 0.0, cnt ? f2 / dc : 0.0, cnt ? f3 / dc : 0.0, 0.0};
     st2k<2>(phiN + 4 * J, o);
@@ -474,27 +563,46 @@ __global__ __launch_bounds__(kBlock) void k_ref_gg1(MeshView m, const double* ph
 }
 
 // Cluster means of the six gradient fields (calcPhiInCellCluster applied
-// inside the second gradient level, calcDiffTerm.jl:88-95).
+// inside the second gradient level, calcDiffTerm.jl:88-95); NCL as in
+// k_ref_nodes (every slot's record loaded before the sums, which keep the
+// slot order).
+template <int NCL>
 __global__ __launch_bounds__(kBlock) void k_ref_gg_nodes(MeshView m, const double* g6, double* g6N) {
   const int64_t J = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (J >= m.nn) return;
-  const int64_t nc = m.nc, nn = m.nn;
+  const int64_t nn = m.nn;
   double s[6] = {0, 0, 0, 0, 0, 0};
   int cnt = 0;
-  for (int q = 0; q < m.ncl; ++q) {
-    const int32_t c = m.cl[q * nn + J];
-    if (c >= 0) {
-      double x[6];
-      ld2k<3>(g6 + 6 * (int64_t)c, x);
+  if constexpr (NCL > 0) {
+    int32_t c[NCL];
 #pragma unroll
-      for (int r = 0; r < 6; ++r) s[r] = s[r] + x[r];
-      ++cnt;
+    for (int q = 0; q < NCL; ++q) c[q] = m.cl[q * nn + J];
+    double x[NCL][6];
+#pragma unroll
+    for (int q = 0; q < NCL; ++q) ld2k<3>(g6 + 6 * (int64_t)(c[q] >= 0 ? c[q] : 0), x[q]);
+#pragma unroll
+    for (int q = 0; q < NCL; ++q)
+      if (c[q] >= 0) {
+#pragma unroll
+        for (int t = 0; t < 6; ++t) s[t] = s[t] + x[q][t];
+        ++cnt;
+      }
+  } else {
+    for (int q = 0; q < m.ncl; ++q) {
+      const int32_t c = m.cl[q * nn + J];
+      if (c >= 0) {
+        double x[6];
+        ld2k<3>(g6 + 6 * (int64_t)c, x);
+#pragma unroll
+        for (int t = 0; t < 6; ++t) s[t] = s[t] + x[t];
+        ++cnt;
+      }
     }
   }
   const double dc = (double)cnt;
   double o[6];
 #pragma unroll
-  for (int r = 0; r < 6; ++r) o[r] = cnt ? s[r] / dc : 0.0;
+  for (int t = 0; t < 6; ++t) o[t] = cnt ? s[t] / dc : 0.0;
   st2k<3>(g6N + 6 * J, o);
 }
 
@@ -735,12 +843,15 @@ hipError_t launch_ref_step(const RefStepArgs& a, hipStream_t s) {
   const bool fused = a.acc != nullptr;
   const bool visc = a.g6N != nullptr;
   const dim3 g(nblocks(a.m.nc)), b(kBlock);
-#define GD_LAUNCH(Q_, V_, F_) hipLaunchKernelGGL((k_ref_step<Q_, V_, F_>), g, b, 0, s, a)
-  if (fused) {
-    if (q) { if (visc) GD_LAUNCH(true, true, true); else GD_LAUNCH(true, false, true); }
-    else { if (visc) GD_LAUNCH(false, true, true); else GD_LAUNCH(false, false, true); }
+#define GD_LAUNCH(Q_, V_, F_, P_) hipLaunchKernelGGL((k_ref_step<Q_, V_, F_, P_>), g, b, 0, s, a)
+  if (fused && a.rflux) {
+    if (q) { if (visc) GD_LAUNCH(true, true, true, true); else GD_LAUNCH(true, false, true, true); }
+    else { if (visc) GD_LAUNCH(false, true, true, true); else GD_LAUNCH(false, false, true, true); }
+  } else if (fused) {
+    if (q) { if (visc) GD_LAUNCH(true, true, true, false); else GD_LAUNCH(true, false, true, false); }
+    else { if (visc) GD_LAUNCH(false, true, true, false); else GD_LAUNCH(false, false, true, false); }
   } else {
-    if (q) GD_LAUNCH(true, false, false); else GD_LAUNCH(false, false, false);
+    if (q) GD_LAUNCH(true, false, false, false); else GD_LAUNCH(false, false, false, false);
   }
 #undef GD_LAUNCH
   return hipGetLastError();
@@ -756,7 +867,10 @@ hipError_t launch_ref_face_states(const RefStepArgs& a, double* UL, double* UR, 
 
 hipError_t launch_ref_faces(const RefStepArgs& a, double* rflux, hipStream_t s) {
   if (a.m.n_rfaces == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_ref_faces, dim3(nblocks(a.m.n_rfaces)), dim3(kBlock), 0, s, a, rflux);
+  if (a.m.has_quads)
+    hipLaunchKernelGGL(k_ref_faces<true>, dim3(nblocks(a.m.n_rfaces)), dim3(kBlock), 0, s, a, rflux);
+  else
+    hipLaunchKernelGGL(k_ref_faces<false>, dim3(nblocks(a.m.n_rfaces)), dim3(kBlock), 0, s, a, rflux);
   return hipGetLastError();
 }
 
@@ -771,17 +885,56 @@ hipError_t launch_ref_cuda_stage(const MeshView& m, double gamma, bool x32, cons
   return hipGetLastError();
 }
 
+// the cluster widths with a kernel of their own; others take the slot loop
+template <template <int> class L, class... A>
+static void by_width(int ncl, A&&... args) {
+  switch (ncl) {
+    case 3: L<3>::run(args...); break;
+    case 4: L<4>::run(args...); break;
+    case 5: L<5>::run(args...); break;
+    case 6: L<6>::run(args...); break;
+    case 7: L<7>::run(args...); break;
+    case 8: L<8>::run(args...); break;
+    default: L<0>::run(args...); break;
+  }
+}
+template <bool QN, bool UN, bool PHI>
+struct NodesLaunch {
+  template <int NCL>
+  struct W {
+    static void run(const MeshView& m, dim3 g, hipStream_t s, const double* Q, const double* U, double* Qn, double* Un,
+                    double* phiN, const FoldArgs& fa) {
+      hipLaunchKernelGGL((k_ref_nodes<NCL, QN, UN, PHI>), g, dim3(kBlock), 0, s, m, Q, U, Qn, Un, phiN, fa);
+    }
+  };
+};
+template <int NCL>
+struct GgNodesLaunch {
+  static void run(const MeshView& m, hipStream_t s, const double* g6, double* g6N) {
+    hipLaunchKernelGGL(k_ref_gg_nodes<NCL>, dim3(nblocks(m.nn)), dim3(kBlock), 0, s, m, g6, g6N);
+  }
+};
+
 hipError_t launch_ref_nodes(const MeshView& m, const double* Q, const double* U, double* Qn, double* Un, double* phiN,
                             hipStream_t s, const FoldArgs& fold) {
-  const unsigned nfold = fold.slot ? 2u : 0u;
-  hipLaunchKernelGGL(k_ref_nodes, dim3(nblocks(m.nn) + nfold), dim3(kBlock), 0, s, m, Q, U, Qn, Un, phiN, fold);
+  const dim3 g(nblocks(m.nn) + (fold.slot ? 2u : 0u));
+  // the loop's node pass, gd_cells2nodes and the cluster means of calcDiffTerm
+  // have kernels per cluster width; other output sets take the slot loop
+  switch ((Qn ? 4 : 0) | (Un ? 2 : 0) | (phiN ? 1 : 0)) {
+    case 5: by_width<NodesLaunch<true, false, true>::W>(m.ncl, m, g, s, Q, U, Qn, Un, phiN, fold); break;
+    case 6: by_width<NodesLaunch<true, true, false>::W>(m.ncl, m, g, s, Q, U, Qn, Un, phiN, fold); break;
+    case 1: by_width<NodesLaunch<false, false, true>::W>(m.ncl, m, g, s, Q, U, Qn, Un, phiN, fold); break;
+    case 4: NodesLaunch<true, false, false>::W<0>::run(m, g, s, Q, U, Qn, Un, phiN, fold); break;
+    case 2: NodesLaunch<false, true, false>::W<0>::run(m, g, s, Q, U, Qn, Un, phiN, fold); break;
+    case 3: NodesLaunch<false, true, true>::W<0>::run(m, g, s, Q, U, Qn, Un, phiN, fold); break;
+    case 7: NodesLaunch<true, true, true>::W<0>::run(m, g, s, Q, U, Qn, Un, phiN, fold); break;
+    default: NodesLaunch<false, false, false>::W<0>::run(m, g, s, Q, U, Qn, Un, phiN, fold); break;
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_ref_phi(const MeshView& m, const double* U, double* phiN, hipStream_t s) {
-  hipLaunchKernelGGL(k_ref_nodes, dim3(nblocks(m.nn)), dim3(kBlock), 0, s, m, nullptr, U, nullptr, nullptr, phiN,
-                     FoldArgs{});
-  return hipGetLastError();
+  return launch_ref_nodes(m, nullptr, U, nullptr, nullptr, phiN, s, FoldArgs{});
 }
 
 hipError_t launch_ns_prime(const MeshView& m, const gd_params& prm, const double* U, void* Qt, bool f32, Acc* acc,
@@ -803,7 +956,7 @@ hipError_t launch_ref_gg1(const MeshView& m, const double* phiN, double* g6, hip
 }
 
 hipError_t launch_ref_gg_nodes(const MeshView& m, const double* g6, double* g6N, hipStream_t s) {
-  hipLaunchKernelGGL(k_ref_gg_nodes, dim3(nblocks(m.nn)), dim3(kBlock), 0, s, m, g6, g6N);
+  by_width<GgNodesLaunch>(m.ncl, m, s, g6, g6N);
   return hipGetLastError();
 }
 
