@@ -339,19 +339,31 @@ def test_reference_gpu_stages(quads, x32):
 
 
 @pytest.mark.parametrize("limiter", [O.LIM_REF_HARMONIC, O.LIM_VAN_ALBADA_A, O.LIM_MINMOD])
-@pytest.mark.parametrize("case", ["tgv", "box"])
+@pytest.mark.parametrize("case", ["tgv", "box", "cylinder"])
 def test_face_pass_equals_per_side_evaluation(case, limiter, monkeypatch):
     """The reference loop evaluates each interior triangle-triangle face with
-    mirrored stencils once (the face pass) and the rest from each side.  With
-    AUSM+ and an odd, symmetric limiter the two sides' evaluations are exact
-    negatives, so 20 steps give the same state as evaluating every face from
-    both of its cells (GODUNOV_REF_PER_SIDE=1), on a periodic square and on a
-    box with quads, walls, inflow and outflow."""
+    mirrored stencils once and every other face slot (boundary faces, faces of
+    quads, faces whose sides do not mirror bit for bit) from its cell's side,
+    in the face pass; the step then sums the stored fluxes.  With AUSM+ and an
+    odd, symmetric limiter the two sides' evaluations of a mirrored face are
+    exact negatives, so 20 steps give the same state, wall force and time as
+    evaluating every face from both of its cells in the step
+    (GODUNOV_REF_PER_SIDE=1): on a periodic square, on a box with quads,
+    walls, inflow and outflow, and on an O-grid cylinder with an oblique slip
+    wall."""
     S, L, M = pkg().stage, pkg()._lib, pkg().mesh
     if case == "tgv":
         mesh = M.make_tgv_periodic_mesh(64)
         f = tgv_fields(mesh, noise=1e-3)
         bcs, wall = {}, 0
+    elif case == "cylinder":
+        mesh = M.make_cylinder_ogrid_mesh(48, 25, r0=0.5, r1=5.0, dr0=0.02)
+        f = M.fields2d.zeros(mesh.nCells, mesh.nNodes)
+        f.densityCells[:] = 1.0 + 1e-3 * M.normal(SEED + 5, mesh.nCells)
+        f.UxCells[:], f.UyCells[:], f.pressureCells[:] = 0.3, 0.0, 1.0 / 1.4
+        bcs = {-1: (L.GD_BC_FIXED, (1.0, 0.3, 0.0, 1.0 / 1.4)), -2: (L.GD_BC_EXTRAPOLATE, ()),
+               -3: (L.GD_BC_SLIP_WALL, ())}
+        wall = -3
     else:
         mesh = M.make_box_mesh(40, 24, 2.0, 1.0, quads="mixed")
         f = random_fields(mesh, seed=SEED + 3)
