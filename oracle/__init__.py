@@ -122,6 +122,8 @@ def lib():
         L.orc_ns_step_tl.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcNsParams), C.c_double, dp, dp, dp, dp, dp,
                                      dp, dp]
         L.orc_set_threads.restype = None
+        L.orc_ref_loop_tl.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcThermo), C.POINTER(OrcParams), C.c_int32,
+                                      C.c_double, C.c_double, C.c_int32, dp, dp, dp, dp]
         L.orc_ns_dt.restype = C.c_double
         L.orc_ns_dt.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcNsParams), dp, C.c_double]
         _lib = L
@@ -407,6 +409,22 @@ def ns_step_tl(mv: MeshView, prm, dt, U, dU, dbc=None):
     _rc(lib().orc_ns_step_tl(C.byref(mv.c), C.byref(prm), dt, _p(_chk_f64(U)), _p(_chk_f64(dU)), _p(db), _p(UN),
                              _p(dUN), _p(f2), _p(df2)), "ns_tl")
     return UN, dUN, f2, df2
+
+
+def ref_loop_tl(mv: MeshView, thermo, prm, U0, dU0, dt, n_steps, viscous=True, betta=1.0):
+    """N steps of the reference loop (calcDiffTerm, calcOneStageMUSCLcpuFast,
+    updateVariablesSA, node interpolation) from U0 at a fixed dt, starting from
+    the primitives of U0 and their node means, with the tangent dU0 carried in
+    dual numbers: returns (U_N, dU_N).  Periodic triangle meshes, AUSM+,
+    constant mu and k."""
+    n = mv.mesh.nCells
+    UN = np.zeros((n, 4), order="F")
+    dUN = np.zeros((n, 4), order="F")
+    th = thermo_struct(thermo)
+    d = None if dU0 is None else _chk_f64(dU0)
+    _rc(lib().orc_ref_loop_tl(C.byref(mv.c), C.byref(th), C.byref(prm), int(bool(viscous)), betta, dt, int(n_steps),
+                              _p(_chk_f64(U0)), _p(d), _p(UN), _p(dUN)), "ref_loop_tl")
+    return UN, dUN
 
 
 def ns_dt(mv: MeshView, prm, U, cfl):

@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "dual.hpp"
 #include "oracle.h"
 
 namespace {
@@ -39,40 +40,12 @@ using std::fmax;
 using std::fmin;
 using std::sqrt;
 
-// Forward-mode dual number (value, one directional derivative): the ns step
-// instantiated with it gives the exact tangent-linear step, the reference
-// against which a discrete adjoint is checked (<adjoint, dU0> = dJ[dU0]).
-struct Dual {
-  double v, d;
-  Dual() : v(0.0), d(0.0) {}
-  Dual(double x) : v(x), d(0.0) {}  // NOLINT: constants enter the scheme as duals
-  Dual(double x, double dx) : v(x), d(dx) {}
-};
-inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
-inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
-inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
-inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
-inline Dual operator/(Dual a, Dual b) {
-  const double q = a.v / b.v;
-  return {q, (a.d - q * b.d) / b.v};
-}
-inline Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
-inline Dual& operator-=(Dual& a, Dual b) { a = a - b; return a; }
-inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
-inline bool operator>(Dual a, Dual b) { return a.v > b.v; }
-inline bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
-inline bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
-inline bool operator!=(Dual a, Dual b) { return a.v != b.v; }
-inline Dual sqrt(Dual a) {
-  const double r = std::sqrt(a.v);
-  return {r, a.d / (2.0 * r)};
-}
-inline Dual fabs(Dual a) { return a.v < 0.0 ? -a : a; }
-inline Dual fmin(Dual a, Dual b) { return b.v < a.v ? b : a; }
-inline Dual fmax(Dual a, Dual b) { return b.v > a.v ? b : a; }
-inline double val(double x) { return x; }
-inline double val(float x) { return x; }
-inline double val(Dual x) { return x.v; }
+using orc_ad::Dual;
+using orc_ad::val;
+using orc_ad::fabs;
+using orc_ad::fmax;
+using orc_ad::fmin;
+using orc_ad::sqrt;
 
 inline int32_t I(const int32_t* a, int64_t n, int64_t i, int k) { return a[(int64_t)(k - 1) * n + (i - 1)]; }
 inline double D(const double* a, int64_t n, int64_t i, int k) { return a[(int64_t)(k - 1) * n + (i - 1)]; }

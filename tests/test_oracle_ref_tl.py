@@ -1,0 +1,126 @@
+"""The tangent-linear reference loop of the oracle (the reference scheme's
+time loop — calcDiffTerm, calcOneStageMUSCLcpuFast, updateVariablesSA, the
+node interpolation — instantiated with forward-mode dual numbers), which the
+reference-scheme adjoint is checked against.
+
+Pinned by what the mathematics fixes: its value part is the oracle's own
+loop of the pointwise functions, bit for bit; its derivative is linear in the
+seed (doubling the seed doubles it exactly); and it matches central
+differences of the loop, viscous and inviscid, with each limiter."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import pkg
+from helpers import SEED, tgv_fields
+
+DT = 2e-3
+
+
+def _thermo():
+    return pkg().mesh.THERMOPHYSICS(RGAS=1.0, Gamma=1.4)
+
+
+def _case(n=24, noise=1e-2):
+    """Periodic TGV with seeded noise on every primitive: the symmetric TGV has
+    exactly equal neighbour values along its symmetry lines, where the limiters
+    have a kink; there the dual number takes one side's derivative and central
+    differences the average of both, so the comparison needs a state without
+    such ties."""
+    M = pkg().mesh
+    mesh = M.make_tgv_periodic_mesh(n)
+    th = _thermo()
+    f = tgv_fields(mesh, noise=noise)
+    r = M.uniform01(SEED + 3, 3 * mesh.nCells).reshape(3, -1) - 0.5
+    f.UxCells[:] += noise * r[0]
+    f.UyCells[:] += noise * r[1]
+    f.pressureCells[:] *= 1.0 + noise * r[2]
+    U = O.phs2dcns(f, th.Gamma, mesh.nCells)
+    return mesh, O.MeshView(mesh), th, U
+
+
+def _oprm(limiter=O.LIM_REF_HARMONIC):
+    return O.params(limiter=limiter, include_udiff=True, energy_law=O.ENERGY_CONST, mu_const=1.0 / 1600,
+                    k_const=1.0 / 0.72)
+
+
+def _oracle_loop(mesh, mv, th, U0, n_steps, prm, viscous):
+    """The same loop from the oracle's double-precision pointwise functions:
+    primitives of U0 and their node means first, then per step the diff term,
+    the fast stage, updateVariablesSA and the node interpolation."""
+    M = pkg().mesh
+    f = M.fields2d.zeros(mesh.nCells, mesh.nNodes)
+    U = np.array(U0, order="F", copy=True)
+    O.update_variables(f, th.Gamma, np.array(U0, order="F", copy=True), U, n_threads=2)
+    O.cells2nodes(mv, f, None, n_threads=2)
+    for _ in range(n_steps):
+        Ud = O.calc_diff_term(mv, f, th, prm, U, n_threads=2)[0] if viscous else None
+        _, UN = O.calc_one_stage_muscl_cpu_fast(mv, f, th, prm, U, Ud, 1.0, DT, 0.0, n_threads=2)
+        O.update_variables(f, th.Gamma, UN, U, n_threads=2)
+        O.cells2nodes(mv, f, None, n_threads=2)
+    return U
+
+
+def _direction(mesh, U, seed):
+    M = pkg().mesh
+    x, y = mesh.cell_mid_points[:, 0], mesh.cell_mid_points[:, 1]
+    r = M.uniform01(seed, 12).reshape(4, 3)
+    scale = np.abs(U).max(axis=0)
+    d = np.stack([scale[v] * np.sin((1.0 + 2.0 * r[v, 0]) * (np.cos(6.3 * r[v, 1]) * x + np.sin(6.3 * r[v, 1]) * y)
+                                    + 6.3 * r[v, 2]) for v in range(4)], 1)
+    return np.asfortranarray(d)
+
+
+@pytest.mark.parametrize("viscous", [True, False])
+def test_value_part_is_the_oracle_loop(viscous):
+    mesh, mv, th, U0 = _case()
+    prm = _oprm()
+    UN, _ = O.ref_loop_tl(mv, th, prm, U0, None, DT, 6, viscous=viscous)
+    Uo = _oracle_loop(mesh, mv, th, U0, 6, prm, viscous)
+    assert np.abs(UN - U0).max() > 0
+    assert np.array_equal(UN, Uo)
+
+
+def test_derivative_is_linear_in_the_seed():
+    mesh, mv, th, U0 = _case()
+    d = _direction(mesh, U0, SEED + 1)
+    _, a = O.ref_loop_tl(mv, th, _oprm(), U0, d, DT, 4)
+    _, b = O.ref_loop_tl(mv, th, _oprm(), U0, np.asfortranarray(2.0 * d), DT, 4)
+    assert np.abs(a).max() > 0
+    assert np.array_equal(b, 2.0 * a)
+
+
+@pytest.mark.parametrize("viscous", [True, False])
+@pytest.mark.parametrize("limiter", [O.LIM_REF_HARMONIC, O.LIM_VAN_LEER_A, O.LIM_VAN_ALBADA_A, O.LIM_MINMOD])
+def test_derivative_matches_central_differences(limiter, viscous):
+    """<w, dU_N> against (J(U0 + h d) - J(U0 - h d)) / 2h for J = <w, U_N>,
+    h = 1e-7 relative to the state's scale (noisy TGV: the limiters' branch
+    points are off the sample points)."""
+    M = pkg().mesh
+    mesh, mv, th, U0 = _case()
+    prm = _oprm(limiter)
+    n = mesh.nCells
+    w = np.asfortranarray(M.uniform01(SEED + 7, 4 * n).reshape(n, 4, order="F") - 0.5)
+    errs = []
+    for k in range(2):
+        d = _direction(mesh, U0, SEED + 11 + k)
+        _, dUN = O.ref_loop_tl(mv, th, prm, U0, d, DT, 5, viscous=viscous)
+        tl = float((w * dUN).sum())
+        h = 1e-7
+        Jp = float((w * O.ref_loop_tl(mv, th, prm, np.asfortranarray(U0 + h * d), None, DT, 5, viscous=viscous)[0]).sum())
+        Jm = float((w * O.ref_loop_tl(mv, th, prm, np.asfortranarray(U0 - h * d), None, DT, 5, viscous=viscous)[0]).sum())
+        fd = (Jp - Jm) / (2 * h)
+        scale = float(np.abs(w * dUN).sum())
+        errs.append(abs(tl - fd) / scale)
+        print(limiter, viscous, k, tl, fd, errs[-1])
+    assert max(errs) < 1e-8
+
+
+def test_unsupported_meshes_are_refused():
+    M = pkg().mesh
+    mesh = M.make_box_mesh(6, 5, 1.0, 1.0)      # boundary faces
+    th = _thermo()
+    f = tgv_fields(mesh, noise=1e-3)
+    U = O.phs2dcns(f, th.Gamma, mesh.nCells)
+    with pytest.raises(RuntimeError):
+        O.ref_loop_tl(O.MeshView(mesh), th, _oprm(), U, None, DT, 1)
