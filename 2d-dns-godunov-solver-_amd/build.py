@@ -23,6 +23,9 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-t
 UNITS = {
     "gd_api.hip": [],
     "gd_ref_kernels.hip": ["-ffp-contract=off"],
+    # the reference-scheme adjoint: its recomputed forward values are the
+    # forward kernels' to the bit, so every branch is taken the same way
+    "gd_ref_adj.hip": ["-ffp-contract=off"],
     "gd_noalloc.hip": [],
     "gd_ns_fp64.hip": ["-ffp-contract=off"],
     # FP32-mixed: hardware reciprocal / square root for float division and

@@ -1299,11 +1299,315 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
 // cell records of the current segment; the reverse sweep walks the segments
 // backwards, recomputing each (except the last) from its checkpoint into the
 // segment's record slots, then applies the transposed steps.
+// The reference-scheme adjoint's incidence lists (RefAdjMaps), built once per
+// mesh from its device arrays; flags meshes outside what the adjoint covers.
+template <class T>
+static int download(const T* dev, int64_t n, std::vector<T>& host) {
+  host.resize((size_t)std::max<int64_t>(n, 0));
+  if (n > 0) GD_HIP(hipMemcpy(host.data(), dev, (size_t)n * sizeof(T), hipMemcpyDeviceToHost));
+  return GD_OK;
+}
+template <class T>
+static void csr(int64_t rows, const std::vector<std::pair<int64_t, T>>& items, std::vector<int64_t>& off,
+                std::vector<T>& ref) {
+  off.assign((size_t)rows + 1, 0);
+  for (const auto& it : items) ++off[(size_t)it.first + 1];
+  for (int64_t r = 0; r < rows; ++r) off[(size_t)r + 1] += off[(size_t)r];
+  ref.resize(items.size());
+  std::vector<int64_t> pos(off.begin(), off.end() - 1);
+  for (const auto& it : items) ref[(size_t)pos[(size_t)it.first]++] = it.second;  // stable: items' order per row
+}
+
+static int ref_adj_maps(Mesh* M) {
+  RefAdjMaps& a = M->radj;
+  if (a.built) return GD_OK;
+  const MeshView& v = M->v;
+  const int64_t nc = v.nc, nn = v.nn, nf = v.nf, nr = v.n_rfaces, nm = v.n_rmirror;
+  std::vector<int32_t> nb, dn1, up1, ea, eb, rfc, rfs, rfnb, rfdn, rfup, cl;
+  std::vector<double> w;
+  int rc = 0;
+  if ((rc = download(v.nb, nf * nc, nb)) || (rc = download(v.dn1, nf * nc, dn1)) || (rc = download(v.up1, nf * nc, up1)) ||
+      (rc = download(v.ea, nf * nc, ea)) || (rc = download(v.eb, nf * nc, eb)) || (rc = download(v.rf_cell, nr, rfc)) ||
+      (rc = download(v.rf_slot, nr, rfs)) || (rc = download(v.rf_nb, nm, rfnb)) || (rc = download(v.rf_dn1, nm, rfdn)) ||
+      (rc = download(v.rf_up1, nm, rfup)) || (rc = download(v.cl, (int64_t)v.ncl * nn, cl)) ||
+      (rc = download(v.w, (int64_t)v.ncl * nn, w)))
+    return rc;
+  // every list index below is checked here, so the lists address only rows
+  // that exist
+  auto in = [](int64_t x, int64_t n) { return x >= 0 && x < n; };
+  a.supported = !v.has_quads;
+  for (int64_t q = 0; q < 3 * nc && a.supported; ++q)
+    a.supported = in(nb[(size_t)q], nc) && in(dn1[(size_t)q], nn) && in(up1[(size_t)q], nn) && in(ea[(size_t)q], nn) &&
+                  in(eb[(size_t)q], nn);
+  for (int64_t f = 0; f < nm && a.supported; ++f)
+    a.supported = in(rfnb[(size_t)f], nc) && in(rfdn[(size_t)f], nn) && in(rfup[(size_t)f], nn);
+  for (int64_t f = 0; f < nr && a.supported; ++f) a.supported = in(rfc[(size_t)f], nc) && in(rfs[(size_t)f], 3);
+  for (size_t q = 0; q < cl.size() && a.supported; ++q) a.supported = cl[q] >= -1 && cl[q] < nc;
+  a.built = true;
+  if (!a.supported) return GD_OK;
+  std::vector<std::pair<int64_t, int32_t>> nq, ne, ce;
+  nq.reserve((size_t)2 * nr); ne.reserve((size_t)6 * nc); ce.reserve((size_t)2 * nr);
+  for (int64_t f = 0; f < nr; ++f) {
+    const int64_t i = rfc[(size_t)f];
+    int64_t j, d, u;
+    if (f < nm) {
+      j = rfnb[(size_t)f]; d = rfdn[(size_t)f]; u = rfup[(size_t)f];
+    } else {
+      const size_t sl = (size_t)rfs[(size_t)f] * nc + i;
+      j = nb[sl]; d = dn1[sl]; u = up1[sl];
+    }
+    nq.push_back({d, (int32_t)(2 * f)});
+    nq.push_back({u, (int32_t)(2 * f + 1)});
+    ce.push_back({i, (int32_t)(2 * f)});
+    ce.push_back({j, (int32_t)(2 * f + 1)});
+  }
+  for (int64_t k = 0; k < 3; ++k)
+    for (int64_t i = 0; i < nc; ++i) {
+      const size_t sl = (size_t)k * nc + i;
+      ne.push_back({ea[sl], (int32_t)sl});
+      ne.push_back({eb[sl], (int32_t)sl});
+    }
+  struct Memb { int32_t node; double w, c; };
+  std::vector<std::pair<int64_t, Memb>> cm;
+  cm.reserve((size_t)v.ncl * nn);
+  for (int64_t J = 0; J < nn; ++J) {  // the forward node means' weights, summed in slot order
+    double det = 0.0;
+    int cnt = 0;
+    for (int q = 0; q < v.ncl; ++q)
+      if (cl[(size_t)q * nn + J] >= 0) { det = det + w[(size_t)q * nn + J]; ++cnt; }
+    for (int q = 0; q < v.ncl; ++q) {
+      const int32_t c = cl[(size_t)q * nn + J];
+      if (c < 0) continue;
+      const double wi = w[(size_t)q * nn + J];
+      cm.push_back({c, Memb{(int32_t)J, det != 0 ? wi / det : wi, 1.0 / (double)cnt}});
+    }
+  }
+  std::vector<int64_t> nq_off, ne_off, ce_off, cm_off;
+  std::vector<int32_t> nq_ref, ne_ref, ce_ref;
+  std::vector<Memb> cmr;
+  csr(nn, nq, nq_off, nq_ref);
+  csr(nn, ne, ne_off, ne_ref);
+  csr(nc, ce, ce_off, ce_ref);
+  csr(nc, cm, cm_off, cmr);
+  std::vector<int32_t> cm_node(cmr.size());
+  std::vector<double> cm_w(cmr.size()), cm_c(cmr.size());
+  for (size_t q = 0; q < cmr.size(); ++q) { cm_node[q] = cmr[q].node; cm_w[q] = cmr[q].w; cm_c[q] = cmr[q].c; }
+  int64_t* B = &M->device_bytes;
+  auto& al = M->allocs;
+  if ((rc = upload(al, &a.nq_off, nq_off, B)) || (rc = upload(al, &a.nq_ref, nq_ref, B)) ||
+      (rc = upload(al, &a.ne_off, ne_off, B)) || (rc = upload(al, &a.ne_ref, ne_ref, B)) ||
+      (rc = upload(al, &a.ce_off, ce_off, B)) || (rc = upload(al, &a.ce_ref, ce_ref, B)) ||
+      (rc = upload(al, &a.cm_off, cm_off, B)) || (rc = upload(al, &a.cm_node, cm_node, B)) ||
+      (rc = upload(al, &a.cm_w, cm_w, B)) || (rc = upload(al, &a.cm_c, cm_c, B))) {
+    a.built = false;
+    return rc;
+  }
+  return GD_OK;
+}
+
+// gd_adjoint_run for the reference scheme: J = <dJ/dU_N, U_N> over N steps of
+// the reference loop at a fixed dt (betta 1), from the state's U with the
+// loop's primitives and node primitives recomputed from it.
+static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dUN, double* dJ_dU0,
+                           double* dJ_dbc, gd_adjoint_stats* stats) {
+  const gd_params& P = S->prm;
+  if (P.flux != GD_FLUX_AUSM_PLUS) return fail(GD_ERR_UNSUPPORTED, "the reference-scheme adjoint covers AUSM+");
+  if (P.limiter != GD_LIM_REF_HARMONIC && P.limiter != GD_LIM_VAN_ALBADA_A && P.limiter != GD_LIM_MINMOD)
+    return fail(GD_ERR_UNSUPPORTED, "the reference-scheme adjoint covers the odd, symmetric limiters");
+  if (P.viscous && P.energy_law != GD_ENERGY_CONST)
+    return fail(GD_ERR_UNSUPPORTED, "the reference-scheme adjoint covers constant mu and k");
+  if (ap->wall_code != 0) return fail(GD_ERR_UNSUPPORTED, "the reference-scheme adjoint has no wall-force objective");
+  if (ap->n_steps < 1 || ap->checkpoint_every < 1) return fail(GD_ERR_INVALID, "n_steps and checkpoint_every >= 1");
+  if (!(ap->dt > 0.0)) return fail(GD_ERR_INVALID, "the adjoint needs a fixed dt > 0");
+  GD_MATERIALIZE(S);
+  int rc = 0;
+  if ((rc = ref_adj_maps(S->mesh))) return rc;
+  const RefAdjMaps& maps = S->mesh->radj;
+  if (!maps.supported)
+    return fail(GD_ERR_UNSUPPORTED, "the reference-scheme adjoint covers triangle meshes without boundary faces");
+  const MeshView& v = S->mesh->v;
+  const int64_t nc = v.nc, nn = v.nn;
+  const bool visc = P.viscous != 0;
+  const int32_t N = ap->n_steps;
+  const int32_t c = std::min(ap->checkpoint_every, N);
+  const int32_t nseg = (N + c - 1) / c;
+  const double bdt = 1.0 * ap->dt;  // betta 1, the explicit Euler step of the reference drivers
+  const double mu[3] = {P.mu_const, P.mu_const, P.mu_const * P.k_const};
+
+  // workspace: checkpoints, one segment's states, two adjoint states, the
+  // entries' adjoints, node- and cell-level adjoints
+  const size_t U4 = 4 * (size_t)nc * sizeof(double);
+  const size_t parts[] = {(size_t)nseg * U4, (size_t)(c + 1) * U4, U4, U4,
+                          16 * (size_t)std::max<int64_t>(v.n_rfaces, 1) * sizeof(double), 4 * (size_t)nn * sizeof(double),
+                          6 * (size_t)nn * sizeof(double), 6 * (size_t)nc * sizeof(double),
+                          4 * (size_t)nn * sizeof(double)};
+  constexpr int kParts = (int)(sizeof(parts) / sizeof(parts[0]));
+  size_t offs[kParts], need = 0;
+  for (int q = 0; q < kParts; ++q) {
+    offs[q] = need;
+    need += (parts[q] + 255) & ~size_t(255);
+  }
+  if (need > S->adj_ws_bytes) {
+    if (S->adj_ws) (void)hipFree(S->adj_ws);
+    S->adj_ws = nullptr;
+    S->adj_ws_bytes = 0;
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, need);
+    if (e != hipSuccess) return fail(GD_ERR_NOMEM, std::string("hipMalloc (adjoint workspace): ") + hipGetErrorString(e));
+    S->adj_ws = p;
+    S->adj_ws_bytes = need;
+  }
+  S->adj_geo_precision = -1;  // the ns adjoint's slot geometry at the front of the workspace is overwritten
+  char* W = static_cast<char*>(S->adj_ws);
+  double* ck = reinterpret_cast<double*>(W + offs[0]);
+  double* seg = reinterpret_cast<double*>(W + offs[1]);
+  double* lU[2] = {reinterpret_cast<double*>(W + offs[2]), reinterpret_cast<double*>(W + offs[3])};
+  double* lamE = reinterpret_cast<double*>(W + offs[4]);
+  double* lamQn = reinterpret_cast<double*>(W + offs[5]);
+  double* lamG6N = reinterpret_cast<double*>(W + offs[6]);
+  double* tG = reinterpret_cast<double*>(W + offs[7]);
+  double* lamPhiN = reinterpret_cast<double*>(W + offs[8]);
+  double* rflux = static_cast<double*>(S->fflux);
+  const int64_t bytes = (int64_t)(parts[0] + parts[1]);
+
+  // the loop's start from U: primitives, node primitives, cluster means
+  auto start = [&](const double* U, double* Q) -> int {
+    GD_HIP(launch_update_variables(v, P.Gamma, U, const_cast<double*>(U), Q, S->aS, S->vmax, nullptr, S->stream));
+    GD_HIP(launch_ref_nodes(v, Q, U, S->Qn, nullptr, visc ? S->phiN : nullptr, S->stream));
+    return GD_OK;
+  };
+  // one forward step as gd_run's reference loop takes it, at the fixed dt
+  auto fstep = [&](int32_t s, const double* Uo, double* Un, const double* Qo, double* Qw) -> int {
+    if (visc) {
+      GD_HIP(launch_ref_gg1(v, S->phiN, S->g6, S->stream));
+      GD_HIP(launch_ref_gg_nodes(v, S->g6, S->g6N, S->stream));
+    }
+    RefStepArgs a{};
+    a.m = v;
+    a.prm = P;
+    a.Q = Qo;
+    a.Qn = S->Qn;
+    a.Uo = Uo;
+    a.g6N = visc ? S->g6N : nullptr;
+    a.Un = Un;
+    a.Qw = Qw;
+    a.acc = S->acc;
+    a.part = S->part;
+    a.step = s;
+    a.dt_policy = GD_DT_FIXED;
+    a.dt = ap->dt;
+    a.cfl = 0.0;
+    a.betta = 1.0;
+    a.include_udiff = 1;
+    a.wall_code = 0;
+    GD_HIP(launch_ref_faces(a, rflux, S->stream));
+    a.rflux = rflux;
+    GD_HIP(launch_ref_step(a, S->stream));
+    GD_HIP(launch_ref_nodes(v, Qw, Un, S->Qn, nullptr, visc ? S->phiN : nullptr, S->stream));
+    return GD_OK;
+  };
+
+  hipEvent_t ev[3];
+  for (auto& e : ev) GD_HIP(hipEventCreate(&e));
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int q = 0; q < 3; ++q) (void)hipEventDestroy(e[q]);
+    }
+  } evg{ev};
+
+  // ---- forward sweep from the state's U, checkpoints every c steps
+  GD_HIP(hipEventRecord(ev[0], S->stream));
+  if ((rc = start(S->U[S->cur], S->Q[S->qc]))) return rc;
+  GD_HIP(launch_fields_derived(nc, P.Gamma, S->Q[S->qc], S->aS, S->vmax, S->stream));
+  GD_HIP(launch_prime_acc(v, P, S->Q[S->qc], S->vmax, S->acc, S->part, ap->flow_time, GD_DT_FIXED, 0, S->stream));
+  for (int32_t s = 0; s < N; ++s) {
+    if (s % c == 0) GD_HIP(launch_copy(ck + (size_t)(s / c) * 4 * nc, S->U[S->cur], 4 * nc, S->stream));
+    if ((rc = fstep(s, S->U[S->cur], S->U[1 - S->cur], S->Q[S->qc], S->Q[1 - S->qc]))) return rc;
+    S->cur ^= 1;
+    S->qc ^= 1;
+  }
+  GD_HIP(hipEventRecord(ev[1], S->stream));
+  double J = 0.0;
+  if (dJ_dUN) {
+    std::vector<double> UN((size_t)4 * nc);
+    if ((rc = get_cols(S, S->U[S->cur], UN.data(), 4, false))) return rc;
+    for (size_t q = 0; q < UN.size(); ++q) J += dJ_dUN[q] * UN[q];
+  }
+
+  // ---- reverse sweep: each segment recomputed from its checkpoint into seg,
+  // then its steps transposed from the last
+  int li = 0;
+  if (dJ_dUN) {
+    if ((rc = put_cols(S, lU[0], dJ_dUN, 4, false))) return rc;
+  } else {
+    GD_HIP(launch_fill(lU[0], 4 * nc, 0.0, S->stream));
+  }
+  double* Qs = S->Q[1 - S->qc];  // scratch: the primitives of the step being transposed
+  int32_t recomputed = 0;
+  for (int32_t g = nseg - 1; g >= 0; --g) {
+    const int32_t s0 = g * c, s1 = std::min(N, s0 + c);
+    GD_HIP(launch_copy(seg, ck + (size_t)g * 4 * nc, 4 * nc, S->stream));
+    if (s1 - s0 > 1) {
+      double* Qa = Qs;
+      double* Qb = S->Q[S->qc];  // the state's primitives are recomputed at the end
+      if ((rc = start(seg, Qa))) return rc;
+      for (int32_t s = s0; s < s1 - 1; ++s) {
+        if ((rc = fstep(s, seg + (size_t)(s - s0) * 4 * nc, seg + (size_t)(s - s0 + 1) * 4 * nc, Qa, Qb))) return rc;
+        std::swap(Qa, Qb);
+        ++recomputed;
+      }
+    }
+    for (int32_t s = s1 - 1; s >= s0; --s) {
+      const double* Us = seg + (size_t)(s - s0) * 4 * nc;
+      if ((rc = start(Us, Qs))) return rc;  // Q_s, its node means and cluster means
+      GD_HIP(launch_radj_face(v, P, Qs, S->Qn, lU[li], bdt, lamE, S->stream));
+      GD_HIP(launch_radj_nodes(v, maps, lamE, lU[li], bdt, mu, visc, lamQn, lamG6N, S->stream));
+      if (visc) {
+        GD_HIP(launch_radj_g6(v, maps, lamG6N, tG, S->stream));
+        GD_HIP(launch_radj_phi(v, maps, tG, lamPhiN, S->stream));
+      }
+      GD_HIP(launch_radj_cell(v, maps, P.Gamma, lamE, lamQn, visc ? lamPhiN : nullptr, Us, lU[li], lU[1 - li],
+                              S->stream));
+      li ^= 1;
+    }
+  }
+  GD_HIP(hipEventRecord(ev[2], S->stream));
+  if (!S->grad0 && (rc = dev_alloc(S->allocs, &S->grad0, 4 * nc))) return rc;
+  GD_HIP(launch_copy(S->grad0, lU[li], 4 * nc, S->stream));
+  if (dJ_dU0 && (rc = get_cols(S, lU[li], dJ_dU0, 4, false))) return rc;
+  if (dJ_dbc)
+    for (int q = 0; q < 4 * (GD_MAX_BC + 1); ++q) dJ_dbc[q] = 0.0;
+  // the state holds U_N as after gd_run: its primitives and node primitives
+  // recomputed (the reverse sweep used their buffers), the bookkeeping pending
+  if ((rc = start(S->U[S->cur], S->Q[S->qc]))) return rc;
+  S->pend_delta = true;
+  S->pend_derived = true;
+  Acc acc[3];
+  GD_HIP(hipMemcpyAsync(acc, S->acc, sizeof(acc), hipMemcpyDeviceToHost, S->stream));
+  GD_HIP(hipStreamSynchronize(S->stream));
+  if (stats) {
+    float f_ms = 0.f, r_ms = 0.f;
+    (void)hipEventElapsedTime(&f_ms, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&r_ms, ev[1], ev[2]);
+    stats->J = J;
+    stats->force_sum[0] = stats->force_sum[1] = 0.0;
+    stats->recomputed_steps = recomputed;
+    stats->n_checkpoints = nseg;
+    stats->storage_bytes = bytes;
+    stats->ms_forward = f_ms;
+    stats->ms_reverse = r_ms;
+    stats->flow_time = ap->flow_time + N * ap->dt;
+  }
+  return GD_OK;
+}
+
 int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dUN, double* dJ_dU0, double* dJ_dbc,
                    gd_adjoint_stats* stats) {
   if (!S || !ap) return fail(GD_ERR_INVALID, "null argument");
   const gd_params& P = S->prm;
-  if (P.scheme != GD_SCHEME_NS) return fail(GD_ERR_UNSUPPORTED, "the adjoint covers the ns scheme");
+  if (P.scheme == GD_SCHEME_REF) return ref_adjoint_run(S, ap, dJ_dUN, dJ_dU0, dJ_dbc, stats);
+  if (P.scheme != GD_SCHEME_NS) return fail(GD_ERR_UNSUPPORTED, "unknown scheme");
   if (P.flux != GD_FLUX_AUSM_PLUS && P.flux != GD_FLUX_ROE)
     return fail(GD_ERR_UNSUPPORTED, "the adjoint covers the AUSM+ and Roe fluxes");
   if (ap->n_steps < 1 || ap->checkpoint_every < 1) return fail(GD_ERR_INVALID, "n_steps and checkpoint_every >= 1");

@@ -112,8 +112,27 @@ struct MeshView {
   double max_hx;
 };
 
+// Incidence lists of the reference-scheme adjoint, built from the mesh on its
+// first use (compressed rows: row r's items are off[r] .. off[r+1]-1, in a
+// fixed order, so every transposed sum is deterministic).
+struct RefAdjMaps {
+  bool built = false;
+  bool supported = false;      // triangles only, every face interior
+  const int64_t* nq_off;       // per node: the face-list entries that read it as a far node
+  const int32_t* nq_ref;       //   2 f + (1 for the neighbour side's far node "up", 0 for "down")
+  const int64_t* ne_off;       // per node: the cell face slots s = k nc + i it ends
+  const int32_t* ne_ref;
+  const int64_t* ce_off;       // per cell: the face-list entries that read its primitives
+  const int32_t* ce_ref;       //   2 f + (0 as the entry's own cell, 1 as its neighbour)
+  const int64_t* cm_off;       // per cell: the node clusters it belongs to
+  const int32_t* cm_node;
+  const double* cm_w;          //   weight / the node's weight sum (node primitives)
+  const double* cm_c;          //   1 / the node's cluster size (cluster means)
+};
+
 struct Mesh {
   MeshView v{};
+  RefAdjMaps radj{};
   std::vector<void*> allocs;
   std::vector<int64_t> cperm, ciperm;  // cells new->old, old->new
   std::vector<int64_t> nperm, niperm;  // nodes new->old, old->new
@@ -315,5 +334,27 @@ hipError_t launch_ns_face_f64(const NsStepArgs& a, hipStream_t s);
 hipError_t launch_ns_update_f64(const NsStepArgs& a, hipStream_t s);
 hipError_t launch_ns_face_f32(const NsStepArgs& a, hipStream_t s);
 hipError_t launch_ns_update_f32(const NsStepArgs& a, hipStream_t s);
+
+// the reference-scheme adjoint (gd_ref_adj.hip), one reversed loop step from
+// the adjoint lamU of U_{s+1} and the primitives Q / node primitives Qn of
+// step s (records [n][4] for the node-level adjoints, [n][6] for gradients):
+//   face:  per face-list entry, the adjoint of its flux pushed through AUSM+
+//          and the limiters to its two cells and two far nodes -> lamE [n_rfaces][16]
+//   nodes: per node, those far-node parts -> lamQn; with the viscous term the
+//          second Green-Gauss level transposed -> lamG6N
+//   g6:    per cell, the node means of gradients and the first level's 1/A
+//          transposed -> tG
+//   phi:   per node, the first Green-Gauss level transposed -> lamPhiN
+//   cell:  per cell, the entries' cell parts, the node means and cluster means
+//          transposed, the primitives' Jacobian at U_s and the identity -> lamU of U_s
+hipError_t launch_radj_face(const MeshView& m, const gd_params& p, const double* Q, const double* Qn,
+                            const double* lamU, double bdt, double* lamE, hipStream_t s);
+hipError_t launch_radj_nodes(const MeshView& m, const RefAdjMaps& a, const double* lamE, const double* lamU, double bdt,
+                             const double mu[3], bool visc, double* lamQn, double* lamG6N, hipStream_t s);
+hipError_t launch_radj_g6(const MeshView& m, const RefAdjMaps& a, const double* lamG6N, double* tG, hipStream_t s);
+hipError_t launch_radj_phi(const MeshView& m, const RefAdjMaps& a, const double* tG, double* lamPhiN, hipStream_t s);
+hipError_t launch_radj_cell(const MeshView& m, const RefAdjMaps& a, double gamma, const double* lamE,
+                            const double* lamQn, const double* lamPhiN, const double* U, const double* lamU,
+                            double* lamOut, hipStream_t s);
 
 }  // namespace gd

@@ -23,8 +23,14 @@ __device__ __forceinline__ double jmax(double x, double y) {
   return (x != x || y != y) ? x + y : (y > x ? y : x);
 }
 
+// The limiters, splittings and AUSM+ below are templates on the scalar: with
+// double they are the forward kernels' arithmetic, with the dual number of
+// gd_ad.hpp (found by argument-dependent lookup: fabs, sqrt, fma, jsign, jmin)
+// they also carry one directional derivative, for the reference-scheme adjoint.
+
 // limiters.jl:33-56 and the commented true minmod (limiters.jl:45-47)
-__device__ __forceinline__ double limiter(int kind, double x, double y, double c) {
+template <class T>
+__device__ __forceinline__ T limiter(int kind, T x, T y, double c) {
   if (kind == GD_LIM_VAN_LEER_A) return (x * y + fabs(x * y)) / (x + y + c);
   if (kind == GD_LIM_VAN_ALBADA_A) return (x * (y * y + c) + y * (x * x + c)) / (x * x + y * y + 2.0 * c);
   if (kind == GD_LIM_MINMOD) return 0.5 * (jsign(x) + jsign(y)) * jmin(fabs(x), fabs(y));
@@ -34,44 +40,48 @@ __device__ __forceinline__ double limiter(int kind, double x, double y, double c
 
 // AUSM+ Mach and pressure splittings (Liou 1996, beta = 1/8, alpha = 3/16,
 // 0.25 in M_beta): AUSMflux2dFast.jl:23-53
-__device__ __forceinline__ double mach_plus(double M) {
-  const double d = M * M - 1.0;
+template <class T>
+__device__ __forceinline__ T mach_plus(T M) {
+  const T d = M * M - 1.0;
   return fabs(M) >= 1.0 ? 0.5 * (M + fabs(M)) : 0.25 * (M + 1.0) * (M + 1.0) + (1.0 / 8.0) * d * d;
 }
-__device__ __forceinline__ double mach_minus(double M) {
-  const double d = M * M - 1.0;
+template <class T>
+__device__ __forceinline__ T mach_minus(T M) {
+  const T d = M * M - 1.0;
   return fabs(M) >= 1.0 ? 0.5 * (M - fabs(M)) : -0.25 * (M - 1.0) * (M - 1.0) - (1.0 / 8.0) * d * d;
 }
-__device__ __forceinline__ double pres_plus(double M) {
-  const double d = M * M - 1.0;
+template <class T>
+__device__ __forceinline__ T pres_plus(T M) {
+  const T d = M * M - 1.0;
   return fabs(M) >= 1.0 ? 0.5 * (1.0 + jsign(M)) : 0.25 * (M + 1.0) * (M + 1.0) * (2.0 - M) + (3.0 / 16.0) * M * d * d;
 }
-__device__ __forceinline__ double pres_minus(double M) {
-  const double d = M * M - 1.0;
+template <class T>
+__device__ __forceinline__ T pres_minus(T M) {
+  const T d = M * M - 1.0;
   return fabs(M) >= 1.0 ? 0.5 * (1.0 - jsign(M)) : 0.25 * (M - 1.0) * (M - 1.0) * (2.0 + M) - (3.0 / 16.0) * M * d * d;
 }
 
 // AUSM+ face flux (AUSMflux2dFast.jl:87-175).  "L" is the state upstream of
 // the normal; the result is -(F . n) * len.
-__device__ __forceinline__ void ausm_plus(const double* L, const double* R, double nx, double ny, double len,
-                                          double g, double* out) {
-  const double rL = L[0], uL = L[1], vL = L[2], pL = L[3];
-  const double rR = R[0], uR = R[1], vR = R[2], pR = R[3];
-  const double htL = fma(0.5, (uL * uL + vL * vL), pL / rL + pL / rL / (g - 1.0));
-  const double htR = fma(0.5, (uR * uR + vR * vR), pR / rR + pR / rR / (g - 1.0));
+template <class T>
+__device__ __forceinline__ void ausm_plus(const T* L, const T* R, double nx, double ny, double len, double g, T* out) {
+  const T rL = L[0], uL = L[1], vL = L[2], pL = L[3];
+  const T rR = R[0], uR = R[1], vR = R[2], pR = R[3];
+  const T htL = fma(0.5, (uL * uL + vL * vL), pL / rL + pL / rL / (g - 1.0));
+  const T htR = fma(0.5, (uR * uR + vR * vR), pR / rR + pR / rR / (g - 1.0));
   const double cst = 2.0 * (g - 1.0) / (g + 1.0);
-  const double unL = uL * nx + vL * ny;
-  const double unR = uR * nx + vR * ny;
-  const double asL = sqrt(cst * htL);
-  const double asR = sqrt(cst * htR);
-  const double aL = asL * jmin(1.0, asL / fabs(unL));
-  const double aR = asR * jmin(1.0, asR / fabs(unR));
-  const double a12 = jmin(aL, aR);
-  const double ML = unL / a12, MR = unR / a12;
-  const double m = mach_plus(ML) + mach_minus(MR);
-  const double p12 = pres_plus(ML) * pL + pres_minus(MR) * pR;
-  const double mp = 0.5 * (m + fabs(m));
-  const double mm = 0.5 * (m - fabs(m));
+  const T unL = uL * nx + vL * ny;
+  const T unR = uR * nx + vR * ny;
+  const T asL = sqrt(cst * htL);
+  const T asR = sqrt(cst * htR);
+  const T aL = asL * jmin(1.0, asL / fabs(unL));
+  const T aR = asR * jmin(1.0, asR / fabs(unR));
+  const T a12 = jmin(aL, aR);
+  const T ML = unL / a12, MR = unR / a12;
+  const T m = mach_plus(ML) + mach_minus(MR);
+  const T p12 = pres_plus(ML) * pL + pres_minus(MR) * pR;
+  const T mp = 0.5 * (m + fabs(m));
+  const T mm = 0.5 * (m - fabs(m));
   out[0] = -(a12 * (mp * rL + mm * rR) + 0.0) * len;
   out[1] = -(a12 * (mp * rL * uL + mm * rR * uR) + p12 * nx) * len;
   out[2] = -(a12 * (mp * rL * vL + mm * rR * vR) + p12 * ny) * len;

@@ -398,10 +398,11 @@ class GodunovSolver:
 
     def adjoint(self, n_steps, dt, checkpoint_every=20, wall_code=0, force_weight=(1.0, 0.0), dJ_dUN=None,
                 record_budget=0, to_host=True):
-        """Discrete adjoint of ``n_steps`` fixed-dt ns steps from the current
-        state (which ends at U_N, as after ``run``) for
-        J = sum_n w . F_n + <dJ_dUN, U_N>, F_n the wall force of step n on
-        ``wall_code``.  ``record_budget`` (bytes) keeps the cell records of as
+        """Discrete adjoint of ``n_steps`` fixed-dt steps of the solver's
+        scheme from the current state (which ends at U_N, as after ``run``)
+        for J = sum_n w . F_n + <dJ_dUN, U_N>, F_n the wall force of step n on
+        ``wall_code`` (the reference scheme: periodic triangle meshes, AUSM+,
+        constant mu and k, no wall term).  ``record_budget`` (bytes) keeps the cell records of as
         many trailing checkpoint segments as fit instead of recomputing them.
         ``to_host=False`` leaves dJ/dU0 on the device (gd_state_device_adjoint).
         Returns (dJ/dU0 (n_cells, 4) or None, dJ/dbc (MAX_BC + 1, 4),

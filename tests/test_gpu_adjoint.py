@@ -153,13 +153,15 @@ def test_adjoint_mixed_close_to_fp64(flux):
     assert abs(a - b) <= 1e-4 * abs(a)
 
 
-def test_adjoint_rejects_ref_scheme():
+def test_adjoint_ref_scheme_on_a_box_mesh_is_refused():
+    """The reference scheme's adjoint covers periodic triangle meshes with
+    constant mu and k; a box with walls and k(T) is refused, not approximated."""
     S = pkg().stage
     L = pkg()._lib
     mesh, U = _box()
-    s = S.GodunovSolver(mesh, S.make_params(bcs={}))  # the reference scheme
+    s = S.GodunovSolver(mesh, S.make_params(bcs={}))  # the reference scheme, k(T) of air
     s.state.set_array(L.GD_ARR_U_OLD, U)
-    with pytest.raises(L.GodunovError, match="ns scheme"):
+    with pytest.raises(L.GodunovError, match="reference-scheme adjoint covers"):
         s.adjoint(2, DT)
     s.close()
 

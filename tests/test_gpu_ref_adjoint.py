@@ -1,0 +1,137 @@
+"""Discrete adjoint of the reference scheme's loop (gd_adjoint_run with the
+reference scheme) on the GPU.
+
+J = <w, U_N> after N fixed-dt steps of the reference loop (calcDiffTerm with
+constant mu and k, the fast MUSCL stage with AUSM+, updateVariablesSA, the
+node interpolation) on the periodic Taylor-Green mesh.  For any direction d,
+<dJ/dU0, d> must equal the derivative of J along d that the oracle's
+tangent-linear reference loop carries in dual numbers (itself pinned to the
+oracle loop bit for bit and to central differences): FP64, 1e-9 relative.
+The state gets seeded noise on every primitive so that no limiter argument
+sits exactly on a kink (the symmetric TGV has exact ties along its symmetry
+lines, where one-sided derivatives differ).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import pkg
+from helpers import SEED, tgv_fields
+
+pytestmark = pytest.mark.gpu
+
+DT = 2e-3
+MU, KC = 1.0 / 1600, 1.0 / 0.72
+
+
+def _case(n=100, noise=1e-2):
+    M = pkg().mesh
+    mesh = M.make_tgv_periodic_mesh(n)
+    th = M.THERMOPHYSICS(RGAS=1.0, Gamma=1.4)
+    f = tgv_fields(mesh, noise=noise)
+    r = M.uniform01(SEED + 3, 3 * mesh.nCells).reshape(3, -1) - 0.5
+    f.UxCells[:] += noise * r[0]
+    f.UyCells[:] += noise * r[1]
+    f.pressureCells[:] *= 1.0 + noise * r[2]
+    U = O.phs2dcns(f, th.Gamma, mesh.nCells)
+    return mesh, O.MeshView(mesh), th, U
+
+
+def _solver(mesh, limiter, viscous, U, flux=None, energy=None):
+    S = pkg().stage
+    L = pkg()._lib
+    p = S.make_params(scheme=L.GD_SCHEME_REF, flux=L.GD_FLUX_AUSM_PLUS if flux is None else flux, limiter=limiter,
+                      viscous=viscous, energy_law=L.GD_ENERGY_CONST if energy is None else energy, mu_const=MU,
+                      k_const=KC, RGAS=1.0, Gamma=1.4, bcs={})
+    s = S.GodunovSolver(mesh, p)
+    s.state.set_array(L.GD_ARR_U_OLD, U)
+    return s
+
+
+def _direction(mesh, U, seed):
+    M = pkg().mesh
+    d = M.normal(seed, 4 * mesh.nCells).reshape(mesh.nCells, 4, order="F") * np.abs(U).max(axis=0)
+    return np.asfortranarray(d)
+
+
+LIMS = {"harmonic": O.LIM_REF_HARMONIC, "vanAlbada": O.LIM_VAN_ALBADA_A, "minmod": O.LIM_MINMOD}
+
+
+@pytest.mark.parametrize("viscous", [True, False])
+@pytest.mark.parametrize("lim", list(LIMS))
+def test_ref_adjoint_dot_product_with_tangent_linear_oracle(lim, viscous):
+    M = pkg().mesh
+    mesh, mv, th, U = _case()
+    n = mesh.nCells
+    N = 6
+    w = np.asfortranarray(M.normal(SEED + 7, 4 * n).reshape(n, 4, order="F"))
+    s = _solver(mesh, LIMS[lim], viscous, U)
+    g0, gbc, st = s.adjoint(N, DT, checkpoint_every=4, dJ_dUN=w)
+    UN_dev = s.conservative()
+    s.close()
+    prm = O.params(limiter=LIMS[lim], include_udiff=True, energy_law=O.ENERGY_CONST, mu_const=MU, k_const=KC)
+    errs = []
+    for k in range(2):
+        d = _direction(mesh, U, SEED + 20 + k)
+        UN, dUN = O.ref_loop_tl(mv, th, prm, U, d, DT, N, viscous=viscous)
+        tl = float((w * dUN).sum())
+        adj = float((g0 * d).sum())
+        errs.append(abs(adj - tl) / abs(tl))
+        print(lim, viscous, "<dJ/dU0, d>", adj, "tangent", tl, "rel", errs[-1])
+    J = float((w * UN).sum())
+    print("J", st.J, J, "recomputed", st.recomputed_steps, "checkpoints", st.n_checkpoints,
+          "ms fwd/rev", st.ms_forward, st.ms_reverse)
+    assert max(errs) <= 1e-9
+    assert abs(st.J - J) <= 1e-12 * abs(J)
+    assert np.abs(UN_dev - UN).max() <= 1e-12 * np.abs(UN).max()
+    assert st.recomputed_steps == 3 + 1 and st.n_checkpoints == 2
+    assert not gbc.any()
+
+
+def test_ref_adjoint_independent_of_checkpointing():
+    """Every checkpoint interval recomputes the same states and transposes the
+    same steps in the same order: identical bits."""
+    mesh, _, _, U = _case(60)
+    out = []
+    for c in (1, 3, 7):
+        s = _solver(mesh, O.LIM_REF_HARMONIC, True, U)
+        g0, _, st = s.adjoint(7, DT, checkpoint_every=c, dJ_dUN=U)
+        out.append((g0, st.J, s.conservative()))
+        s.close()
+    for g0, J, UN in out[1:]:
+        assert np.array_equal(g0, out[0][0]) and J == out[0][1] and np.array_equal(UN, out[0][2])
+
+
+def test_ref_adjoint_leaves_the_loop_state():
+    """After the adjoint the state holds U_N with its primitives and node
+    primitives, as after gd_run: continuing with gd_run gives the oracle
+    loop's U_{N+K}."""
+    mesh, mv, th, U = _case(60)
+    s = _solver(mesh, O.LIM_VAN_ALBADA_A, True, U)
+    s.adjoint(4, DT, checkpoint_every=2)
+    s.run(3, dt=DT)
+    UN = s.conservative()
+    s.close()
+    prm = O.params(limiter=O.LIM_VAN_ALBADA_A, include_udiff=True, energy_law=O.ENERGY_CONST, mu_const=MU, k_const=KC)
+    ref, _ = O.ref_loop_tl(mv, th, prm, U, None, DT, 7)
+    assert np.abs(UN - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_ref_adjoint_refuses_what_it_does_not_cover():
+    L = pkg()._lib
+    M = pkg().mesh
+    mesh, _, _, U = _case(20)
+    for kw, msg in ((dict(limiter=O.LIM_VAN_LEER_A), "limiters"), (dict(limiter=0, flux=L.GD_FLUX_ROE), "AUSM"),
+                    (dict(limiter=0, energy=L.GD_ENERGY_AIR_POLY), "constant mu")):
+        lim = kw.pop("limiter")
+        s = _solver(mesh, lim, True, U, **kw)
+        with pytest.raises(L.GodunovError, match=msg):
+            s.adjoint(2, DT)
+        s.close()
+    box = M.make_box_mesh(8, 6, 2.0, 1.0, quads="none")
+    f = tgv_fields(box)
+    Ub = O.phs2dcns(f, 1.4, box.nCells)
+    s = _solver(box, O.LIM_REF_HARMONIC, True, Ub)
+    with pytest.raises(L.GodunovError, match="boundary faces"):
+        s.adjoint(2, DT)
+    s.close()
