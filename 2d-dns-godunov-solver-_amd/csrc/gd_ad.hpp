@@ -44,6 +44,34 @@ __device__ __forceinline__ AD fma(AD a, AD b, AD c) { return {::fma(a.v, b.v, c.
 __device__ __forceinline__ AD jsign(AD x) { return AD(x.v > 0.0 ? 1.0 : (x.v < 0.0 ? -1.0 : x.v)); }
 __device__ __forceinline__ AD jmin(AD x, AD y) { return (x.v != x.v || y.v != y.v) ? x + y : (y.v < x.v ? y : x); }
 
+// Two directional derivatives at once (the limiter's partials in x and y):
+// the value arithmetic is the double arithmetic, each division rounds its
+// value once and scales the derivatives by one reciprocal.
+struct AD2 {
+  double v, dx, dy;
+  __device__ AD2() : v(0.0), dx(0.0), dy(0.0) {}
+  __device__ AD2(double x) : v(x), dx(0.0), dy(0.0) {}  // NOLINT
+  __device__ AD2(double x, double a, double b) : v(x), dx(a), dy(b) {}
+};
+__device__ __forceinline__ AD2 operator+(AD2 a, AD2 b) { return {a.v + b.v, a.dx + b.dx, a.dy + b.dy}; }
+__device__ __forceinline__ AD2 operator-(AD2 a, AD2 b) { return {a.v - b.v, a.dx - b.dx, a.dy - b.dy}; }
+__device__ __forceinline__ AD2 operator-(AD2 a) { return {-a.v, -a.dx, -a.dy}; }
+__device__ __forceinline__ AD2 operator*(AD2 a, AD2 b) {
+  return {a.v * b.v, a.dx * b.v + a.v * b.dx, a.dy * b.v + a.v * b.dy};
+}
+__device__ __forceinline__ AD2 operator/(AD2 a, AD2 b) {
+  const double q = a.v / b.v, r = 1.0 / b.v;
+  return {q, (a.dx - q * b.dx) * r, (a.dy - q * b.dy) * r};
+}
+__device__ __forceinline__ bool operator<(AD2 a, AD2 b) { return a.v < b.v; }
+__device__ __forceinline__ bool operator>(AD2 a, AD2 b) { return a.v > b.v; }
+__device__ __forceinline__ bool operator<=(AD2 a, AD2 b) { return a.v <= b.v; }
+__device__ __forceinline__ bool operator>=(AD2 a, AD2 b) { return a.v >= b.v; }
+__device__ __forceinline__ AD2 fabs(AD2 a) { return a.v < 0.0 ? -a : a; }
+__device__ __forceinline__ AD2 jsign(AD2 x) { return AD2(x.v > 0.0 ? 1.0 : (x.v < 0.0 ? -1.0 : x.v)); }
+__device__ __forceinline__ AD2 jmin(AD2 x, AD2 y) { return (x.v != x.v || y.v != y.v) ? x + y : (y.v < x.v ? y : x); }
+
 }  // namespace ad
 using ad::AD;
+using ad::AD2;
 }  // namespace gd

@@ -1324,13 +1324,15 @@ static int ref_adj_maps(Mesh* M) {
   const MeshView& v = M->v;
   const int64_t nc = v.nc, nn = v.nn, nf = v.nf, nr = v.n_rfaces, nm = v.n_rmirror;
   std::vector<int32_t> nb, dn1, up1, ea, eb, rfc, rfs, rfnb, rfdn, rfup, cl;
-  std::vector<double> w;
+  std::vector<double> w, fnx, fny, flen, area;
   int rc = 0;
   if ((rc = download(v.nb, nf * nc, nb)) || (rc = download(v.dn1, nf * nc, dn1)) || (rc = download(v.up1, nf * nc, up1)) ||
       (rc = download(v.ea, nf * nc, ea)) || (rc = download(v.eb, nf * nc, eb)) || (rc = download(v.rf_cell, nr, rfc)) ||
       (rc = download(v.rf_slot, nr, rfs)) || (rc = download(v.rf_nb, nm, rfnb)) || (rc = download(v.rf_dn1, nm, rfdn)) ||
       (rc = download(v.rf_up1, nm, rfup)) || (rc = download(v.cl, (int64_t)v.ncl * nn, cl)) ||
-      (rc = download(v.w, (int64_t)v.ncl * nn, w)))
+      (rc = download(v.w, (int64_t)v.ncl * nn, w)) || (rc = download(v.fnx, nf * nc, fnx)) ||
+      (rc = download(v.fny, nf * nc, fny)) || (rc = download(v.flen, nf * nc, flen)) ||
+      (rc = download(v.area, nc, area)))
     return rc;
   // every list index below is checked here, so the lists address only rows
   // that exist
@@ -1343,10 +1345,29 @@ static int ref_adj_maps(Mesh* M) {
     a.supported = in(rfnb[(size_t)f], nc) && in(rfdn[(size_t)f], nn) && in(rfup[(size_t)f], nn);
   for (int64_t f = 0; f < nr && a.supported; ++f) a.supported = in(rfc[(size_t)f], nc) && in(rfs[(size_t)f], 3);
   for (size_t q = 0; q < cl.size() && a.supported; ++q) a.supported = cl[q] >= -1 && cl[q] < nc;
+  // the Green-Gauss coefficient pair of node J in cell i, in the cluster slot
+  // of J that holds i: sum over the cell's edges ending at J of
+  // 0.5 (-ny, -nx) len, over A_i; a vertex missing from its cluster is outside
+  // what the adjoint covers
+  std::vector<double> ccx((size_t)v.ncl * nn, 0.0), ccy((size_t)v.ncl * nn, 0.0);
+  for (int64_t i = 0; i < nc && a.supported; ++i)
+    for (int64_t k = 0; k < 3 && a.supported; ++k) {
+      const size_t sl = (size_t)k * nc + i;
+      const double fx = 0.5 * -fny[sl] * flen[sl], fy = 0.5 * -fnx[sl] * flen[sl];
+      for (const int64_t J : {(int64_t)ea[sl], (int64_t)eb[sl]}) {
+        int q = 0;
+        while (q < v.ncl && cl[(size_t)q * nn + J] != i) ++q;
+        if (q == v.ncl) { a.supported = false; break; }
+        ccx[(size_t)q * nn + J] += fx;
+        ccy[(size_t)q * nn + J] += fy;
+      }
+    }
   a.built = true;
   if (!a.supported) return GD_OK;
-  std::vector<std::pair<int64_t, int32_t>> nq, ne, ce;
-  nq.reserve((size_t)2 * nr); ne.reserve((size_t)6 * nc); ce.reserve((size_t)2 * nr);
+  for (size_t q = 0; q < cl.size(); ++q)
+    if (cl[q] >= 0) { ccx[q] /= area[(size_t)cl[q]]; ccy[q] /= area[(size_t)cl[q]]; }
+  std::vector<std::pair<int64_t, int32_t>> nq, ce;
+  nq.reserve((size_t)2 * nr); ce.reserve((size_t)2 * nr);
   for (int64_t f = 0; f < nr; ++f) {
     const int64_t i = rfc[(size_t)f];
     int64_t j, d, u;
@@ -1361,44 +1382,36 @@ static int ref_adj_maps(Mesh* M) {
     ce.push_back({i, (int32_t)(2 * f)});
     ce.push_back({j, (int32_t)(2 * f + 1)});
   }
-  for (int64_t k = 0; k < 3; ++k)
-    for (int64_t i = 0; i < nc; ++i) {
-      const size_t sl = (size_t)k * nc + i;
-      ne.push_back({ea[sl], (int32_t)sl});
-      ne.push_back({eb[sl], (int32_t)sl});
-    }
-  struct Memb { int32_t node; double w, c; };
+  struct Memb { int32_t node; double w; };
   std::vector<std::pair<int64_t, Memb>> cm;
   cm.reserve((size_t)v.ncl * nn);
   for (int64_t J = 0; J < nn; ++J) {  // the forward node means' weights, summed in slot order
     double det = 0.0;
-    int cnt = 0;
     for (int q = 0; q < v.ncl; ++q)
-      if (cl[(size_t)q * nn + J] >= 0) { det = det + w[(size_t)q * nn + J]; ++cnt; }
+      if (cl[(size_t)q * nn + J] >= 0) det = det + w[(size_t)q * nn + J];
     for (int q = 0; q < v.ncl; ++q) {
       const int32_t c = cl[(size_t)q * nn + J];
       if (c < 0) continue;
       const double wi = w[(size_t)q * nn + J];
-      cm.push_back({c, Memb{(int32_t)J, det != 0 ? wi / det : wi, 1.0 / (double)cnt}});
+      cm.push_back({c, Memb{(int32_t)J, det != 0 ? wi / det : wi}});
     }
   }
-  std::vector<int64_t> nq_off, ne_off, ce_off, cm_off;
-  std::vector<int32_t> nq_ref, ne_ref, ce_ref;
+  std::vector<int64_t> nq_off, ce_off, cm_off;
+  std::vector<int32_t> nq_ref, ce_ref;
   std::vector<Memb> cmr;
   csr(nn, nq, nq_off, nq_ref);
-  csr(nn, ne, ne_off, ne_ref);
   csr(nc, ce, ce_off, ce_ref);
   csr(nc, cm, cm_off, cmr);
   std::vector<int32_t> cm_node(cmr.size());
-  std::vector<double> cm_w(cmr.size()), cm_c(cmr.size());
-  for (size_t q = 0; q < cmr.size(); ++q) { cm_node[q] = cmr[q].node; cm_w[q] = cmr[q].w; cm_c[q] = cmr[q].c; }
+  std::vector<double> cm_w(cmr.size());
+  for (size_t q = 0; q < cmr.size(); ++q) { cm_node[q] = cmr[q].node; cm_w[q] = cmr[q].w; }
   int64_t* B = &M->device_bytes;
   auto& al = M->allocs;
   if ((rc = upload(al, &a.nq_off, nq_off, B)) || (rc = upload(al, &a.nq_ref, nq_ref, B)) ||
-      (rc = upload(al, &a.ne_off, ne_off, B)) || (rc = upload(al, &a.ne_ref, ne_ref, B)) ||
+      (rc = upload(al, &a.ccx, ccx, B)) || (rc = upload(al, &a.ccy, ccy, B)) ||
       (rc = upload(al, &a.ce_off, ce_off, B)) || (rc = upload(al, &a.ce_ref, ce_ref, B)) ||
       (rc = upload(al, &a.cm_off, cm_off, B)) || (rc = upload(al, &a.cm_node, cm_node, B)) ||
-      (rc = upload(al, &a.cm_w, cm_w, B)) || (rc = upload(al, &a.cm_c, cm_c, B))) {
+      (rc = upload(al, &a.cm_w, cm_w, B))) {
     a.built = false;
     return rc;
   }
@@ -1434,13 +1447,25 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
   const double bdt = 1.0 * ap->dt;  // betta 1, the explicit Euler step of the reference drivers
   const double mu[3] = {P.mu_const, P.mu_const, P.mu_const * P.k_const};
 
-  // workspace: checkpoints, one segment's states, two adjoint states, the
-  // entries' adjoints, node- and cell-level adjoints
-  const size_t U4 = 4 * (size_t)nc * sizeof(double);
-  const size_t parts[] = {(size_t)nseg * U4, (size_t)(c + 1) * U4, U4, U4,
-                          16 * (size_t)std::max<int64_t>(v.n_rfaces, 1) * sizeof(double), 4 * (size_t)nn * sizeof(double),
-                          6 * (size_t)nn * sizeof(double), 6 * (size_t)nc * sizeof(double),
-                          4 * (size_t)nn * sizeof(double)};
+  // What the transposed step s reads of the forward loop is Q_s and its node
+  // means Qn_s.  They are kept from the forward sweep for every step of the K
+  // trailing segments that record_budget holds (steps sK .. N-1); the segments
+  // before them keep U at their first step and are recomputed on the way
+  // back, writing their steps' Q and Qn into one segment's store.
+  const size_t U4 = 4 * (size_t)nc * sizeof(double), N4 = 4 * (size_t)nn * sizeof(double);
+  const int32_t K = ap->record_budget > 0
+                        ? (int32_t)std::min<int64_t>(nseg, ap->record_budget / (int64_t)((size_t)c * (U4 + N4)))
+                        : 0;
+  const int32_t sK = std::min(N, (nseg - K) * c);
+  const int32_t n_ck = nseg - K;
+  const size_t nr4 = 4 * (size_t)std::max<int64_t>(v.n_rfaces, 1) * sizeof(double);
+  // workspace: checkpoints, one recomputed segment's Q and Qn, the kept Q and
+  // Qn, a scratch U, two adjoint states ([cell][4] records), the entries'
+  // adjoints, node- and cell-level adjoints
+  const size_t nseg_st = (size_t)(n_ck > 0 ? c : 0), nkeep = (size_t)(N - sK);
+  const size_t parts[] = {(size_t)n_ck * U4, nseg_st * U4, nseg_st * N4, nkeep * U4, nkeep * N4, U4,
+                          U4, U4, nr4, nr4, nr4, nr4, N4, 6 * (size_t)nn * sizeof(double),
+                          6 * (size_t)nc * sizeof(double), N4};
   constexpr int kParts = (int)(sizeof(parts) / sizeof(parts[0]));
   size_t offs[kParts], need = 0;
   for (int q = 0; q < kParts; ++q) {
@@ -1458,26 +1483,32 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
     S->adj_ws_bytes = need;
   }
   S->adj_geo_precision = -1;  // the ns adjoint's slot geometry at the front of the workspace is overwritten
+  if (!S->grad0 && (rc = dev_alloc(S->allocs, &S->grad0, 4 * nc))) return rc;
   char* W = static_cast<char*>(S->adj_ws);
-  double* ck = reinterpret_cast<double*>(W + offs[0]);
-  double* seg = reinterpret_cast<double*>(W + offs[1]);
-  double* lU[2] = {reinterpret_cast<double*>(W + offs[2]), reinterpret_cast<double*>(W + offs[3])};
-  double* lamE = reinterpret_cast<double*>(W + offs[4]);
-  double* lamQn = reinterpret_cast<double*>(W + offs[5]);
-  double* lamG6N = reinterpret_cast<double*>(W + offs[6]);
-  double* tG = reinterpret_cast<double*>(W + offs[7]);
-  double* lamPhiN = reinterpret_cast<double*>(W + offs[8]);
+  auto at = [&](int q) { return reinterpret_cast<double*>(W + offs[q]); };
+  double* ck = at(0);
+  double *segQ = at(1), *segQn = at(2), *keepQ = at(3), *keepQn = at(4);
+  double* Uscr = at(5);
+  double* lam[2] = {at(6), at(7)};
+  double *lqo = at(8), *lqn = at(9), *ldn = at(10), *lup = at(11);
+  double* lamQn = at(12);
+  double* lamG6N = at(13);
+  double* tG = at(14);
+  double* lamPhiN = at(15);
   double* rflux = static_cast<double*>(S->fflux);
-  const int64_t bytes = (int64_t)(parts[0] + parts[1]);
+  const int64_t bytes = (int64_t)(parts[0] + parts[1] + parts[2] + parts[3] + parts[4]);
+  const size_t q4 = 4 * (size_t)nc, n4 = 4 * (size_t)nn;  // doubles per Q, per Qn
 
   // the loop's start from U: primitives, node primitives, cluster means
-  auto start = [&](const double* U, double* Q) -> int {
+  auto start = [&](const double* U, double* Q, double* Qn) -> int {
     GD_HIP(launch_update_variables(v, P.Gamma, U, const_cast<double*>(U), Q, S->aS, S->vmax, nullptr, S->stream));
-    GD_HIP(launch_ref_nodes(v, Q, U, S->Qn, nullptr, visc ? S->phiN : nullptr, S->stream));
+    GD_HIP(launch_ref_nodes(v, Q, U, Qn, nullptr, visc ? S->phiN : nullptr, S->stream));
     return GD_OK;
   };
-  // one forward step as gd_run's reference loop takes it, at the fixed dt
-  auto fstep = [&](int32_t s, const double* Uo, double* Un, const double* Qo, double* Qw) -> int {
+  // one forward step as gd_run's reference loop takes it, at the fixed dt:
+  // (U, Q, Qn) of step s -> (Un, Qw, Qnw) of step s + 1
+  auto fstep = [&](int32_t s, const double* Uo, const double* Qo, const double* Qno, double* Un, double* Qw,
+                   double* Qnw) -> int {
     if (visc) {
       GD_HIP(launch_ref_gg1(v, S->phiN, S->g6, S->stream));
       GD_HIP(launch_ref_gg_nodes(v, S->g6, S->g6N, S->stream));
@@ -1486,7 +1517,7 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
     a.m = v;
     a.prm = P;
     a.Q = Qo;
-    a.Qn = S->Qn;
+    a.Qn = Qno;
     a.Uo = Uo;
     a.g6N = visc ? S->g6N : nullptr;
     a.Un = Un;
@@ -1503,7 +1534,7 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
     GD_HIP(launch_ref_faces(a, rflux, S->stream));
     a.rflux = rflux;
     GD_HIP(launch_ref_step(a, S->stream));
-    GD_HIP(launch_ref_nodes(v, Qw, Un, S->Qn, nullptr, visc ? S->phiN : nullptr, S->stream));
+    GD_HIP(launch_ref_nodes(v, Qw, Un, Qnw, nullptr, visc ? S->phiN : nullptr, S->stream));
     return GD_OK;
   };
 
@@ -1516,17 +1547,34 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
     }
   } evg{ev};
 
-  // ---- forward sweep from the state's U, checkpoints every c steps
+  // ---- forward sweep from the state's U: U in the state's two buffers, Q
+  // and Qn in the state's buffers before sK and where they are kept from sK
+  // on; U_N, Q_N and Qn_N end in the state's buffers
   GD_HIP(hipEventRecord(ev[0], S->stream));
-  if ((rc = start(S->U[S->cur], S->Q[S->qc]))) return rc;
-  GD_HIP(launch_fields_derived(nc, P.Gamma, S->Q[S->qc], S->aS, S->vmax, S->stream));
-  GD_HIP(launch_prime_acc(v, P, S->Q[S->qc], S->vmax, S->acc, S->part, ap->flow_time, GD_DT_FIXED, 0, S->stream));
+  double* Ucur = S->U[S->cur];
+  double* Qcur = sK == 0 ? keepQ : S->Q[S->qc];
+  double* Qncur = sK == 0 ? keepQn : S->Qn;
+  if ((rc = start(Ucur, Qcur, Qncur))) return rc;
+  GD_HIP(launch_fields_derived(nc, P.Gamma, Qcur, S->aS, S->vmax, S->stream));
+  GD_HIP(launch_prime_acc(v, P, Qcur, S->vmax, S->acc, S->part, ap->flow_time, GD_DT_FIXED, 0, S->stream));
   for (int32_t s = 0; s < N; ++s) {
-    if (s % c == 0) GD_HIP(launch_copy(ck + (size_t)(s / c) * 4 * nc, S->U[S->cur], 4 * nc, S->stream));
-    if ((rc = fstep(s, S->U[S->cur], S->U[1 - S->cur], S->Q[S->qc], S->Q[1 - S->qc]))) return rc;
-    S->cur ^= 1;
-    S->qc ^= 1;
+    if (s < sK && s % c == 0) GD_HIP(launch_copy(ck + (size_t)(s / c) * q4, Ucur, 4 * nc, S->stream));
+    double* Unext = Ucur == S->U[0] ? S->U[1] : S->U[0];
+    double *Qnext, *Qnnext;
+    if (s + 1 < N && s + 1 >= sK) {
+      Qnext = keepQ + (size_t)(s + 1 - sK) * q4;
+      Qnnext = keepQn + (size_t)(s + 1 - sK) * n4;
+    } else {
+      Qnext = Qcur == S->Q[0] ? S->Q[1] : S->Q[0];
+      Qnnext = S->Qn;  // read by the step, then rewritten by its node pass
+    }
+    if ((rc = fstep(s, Ucur, Qcur, Qncur, Unext, Qnext, Qnnext))) return rc;
+    Ucur = Unext;
+    Qcur = Qnext;
+    Qncur = Qnnext;
   }
+  S->cur = Ucur == S->U[0] ? 0 : 1;
+  S->qc = Qcur == S->Q[0] ? 0 : 1;
   GD_HIP(hipEventRecord(ev[1], S->stream));
   double J = 0.0;
   if (dJ_dUN) {
@@ -1535,52 +1583,58 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
     for (size_t q = 0; q < UN.size(); ++q) J += dJ_dUN[q] * UN[q];
   }
 
-  // ---- reverse sweep: each segment recomputed from its checkpoint into seg,
-  // then its steps transposed from the last
+  // ---- reverse sweep: each segment's Q and Qn (kept, or recomputed from its
+  // checkpoint), then its steps transposed from the last
   int li = 0;
   if (dJ_dUN) {
-    if ((rc = put_cols(S, lU[0], dJ_dUN, 4, false))) return rc;
+    if ((rc = put_cols(S, S->grad0, dJ_dUN, 4, false))) return rc;
+    GD_HIP(launch_radj_to_rec(nc, S->grad0, lam[0], S->stream));
   } else {
-    GD_HIP(launch_fill(lU[0], 4 * nc, 0.0, S->stream));
+    GD_HIP(launch_fill(lam[0], 4 * nc, 0.0, S->stream));
   }
-  double* Qs = S->Q[1 - S->qc];  // scratch: the primitives of the step being transposed
   int32_t recomputed = 0;
   for (int32_t g = nseg - 1; g >= 0; --g) {
     const int32_t s0 = g * c, s1 = std::min(N, s0 + c);
-    GD_HIP(launch_copy(seg, ck + (size_t)g * 4 * nc, 4 * nc, S->stream));
-    if (s1 - s0 > 1) {
-      double* Qa = Qs;
-      double* Qb = S->Q[S->qc];  // the state's primitives are recomputed at the end
-      if ((rc = start(seg, Qa))) return rc;
+    const bool kept = s0 >= sK;
+    const double* Qb = kept ? keepQ + (size_t)(s0 - sK) * q4 : segQ;
+    const double* Qnb = kept ? keepQn + (size_t)(s0 - sK) * n4 : segQn;
+    if (!kept) {
+      // U ping-pongs between the scratch and the state's free buffer (the
+      // state's other buffer holds U_N)
+      double* Ua = Uscr;
+      double* Ub = S->U[1 - S->cur];
+      GD_HIP(launch_copy(Ua, ck + (size_t)g * q4, 4 * nc, S->stream));
+      if ((rc = start(Ua, segQ, segQn))) return rc;
       for (int32_t s = s0; s < s1 - 1; ++s) {
-        if ((rc = fstep(s, seg + (size_t)(s - s0) * 4 * nc, seg + (size_t)(s - s0 + 1) * 4 * nc, Qa, Qb))) return rc;
-        std::swap(Qa, Qb);
+        const size_t k = (size_t)(s - s0);
+        if ((rc = fstep(s, Ua, segQ + k * q4, segQn + k * n4, Ub, segQ + (k + 1) * q4, segQn + (k + 1) * n4)))
+          return rc;
+        std::swap(Ua, Ub);
         ++recomputed;
       }
     }
     for (int32_t s = s1 - 1; s >= s0; --s) {
-      const double* Us = seg + (size_t)(s - s0) * 4 * nc;
-      if ((rc = start(Us, Qs))) return rc;  // Q_s, its node means and cluster means
-      GD_HIP(launch_radj_face(v, P, Qs, S->Qn, lU[li], bdt, lamE, S->stream));
-      GD_HIP(launch_radj_nodes(v, maps, lamE, lU[li], bdt, mu, visc, lamQn, lamG6N, S->stream));
+      const double* Qs = Qb + (size_t)(s - s0) * q4;
+      const double* Qns = Qnb + (size_t)(s - s0) * n4;
+      GD_HIP(launch_radj_face(v, P, Qs, Qns, lam[li], bdt, lqo, lqn, ldn, lup, S->stream));
+      GD_HIP(launch_radj_nodes(v, maps, ldn, lup, lam[li], bdt, mu, visc, lamQn, lamG6N, S->stream));
       if (visc) {
         GD_HIP(launch_radj_g6(v, maps, lamG6N, tG, S->stream));
         GD_HIP(launch_radj_phi(v, maps, tG, lamPhiN, S->stream));
       }
-      GD_HIP(launch_radj_cell(v, maps, P.Gamma, lamE, lamQn, visc ? lamPhiN : nullptr, Us, lU[li], lU[1 - li],
+      GD_HIP(launch_radj_cell(v, maps, P.Gamma, lqo, lqn, lamQn, visc ? lamPhiN : nullptr, Qs, lam[li], lam[1 - li],
                               S->stream));
       li ^= 1;
     }
   }
   GD_HIP(hipEventRecord(ev[2], S->stream));
-  if (!S->grad0 && (rc = dev_alloc(S->allocs, &S->grad0, 4 * nc))) return rc;
-  GD_HIP(launch_copy(S->grad0, lU[li], 4 * nc, S->stream));
-  if (dJ_dU0 && (rc = get_cols(S, lU[li], dJ_dU0, 4, false))) return rc;
+  GD_HIP(launch_radj_from_rec(nc, lam[li], S->grad0, S->stream));
+  if (dJ_dU0 && (rc = get_cols(S, S->grad0, dJ_dU0, 4, false))) return rc;
   if (dJ_dbc)
     for (int q = 0; q < 4 * (GD_MAX_BC + 1); ++q) dJ_dbc[q] = 0.0;
-  // the state holds U_N as after gd_run: its primitives and node primitives
-  // recomputed (the reverse sweep used their buffers), the bookkeeping pending
-  if ((rc = start(S->U[S->cur], S->Q[S->qc]))) return rc;
+  // the state holds U_N, Q_N and Qn_N as after gd_run (the cluster means,
+  // which recomputed segments overwrite, are gd_run's first pass), the
+  // bookkeeping pending
   S->pend_delta = true;
   S->pend_derived = true;
   Acc acc[3];

@@ -120,14 +120,13 @@ struct RefAdjMaps {
   bool supported = false;      // triangles only, every face interior
   const int64_t* nq_off;       // per node: the face-list entries that read it as a far node
   const int32_t* nq_ref;       //   2 f + (1 for the neighbour side's far node "up", 0 for "down")
-  const int64_t* ne_off;       // per node: the cell face slots s = k nc + i it ends
-  const int32_t* ne_ref;
+  const double* ccx;           // per cluster slot [ncl][nn] (cell cl[q nn + J]): the Green-Gauss
+  const double* ccy;           //   coefficient pair of the node in that cell (0 if not a vertex)
   const int64_t* ce_off;       // per cell: the face-list entries that read its primitives
   const int32_t* ce_ref;       //   2 f + (0 as the entry's own cell, 1 as its neighbour)
   const int64_t* cm_off;       // per cell: the node clusters it belongs to
   const int32_t* cm_node;
   const double* cm_w;          //   weight / the node's weight sum (node primitives)
-  const double* cm_c;          //   1 / the node's cluster size (cluster means)
 };
 
 struct Mesh {
@@ -348,13 +347,18 @@ hipError_t launch_ns_update_f32(const NsStepArgs& a, hipStream_t s);
 //   cell:  per cell, the entries' cell parts, the node means and cluster means
 //          transposed, the primitives' Jacobian at U_s and the identity -> lamU of U_s
 hipError_t launch_radj_face(const MeshView& m, const gd_params& p, const double* Q, const double* Qn,
-                            const double* lamU, double bdt, double* lamE, hipStream_t s);
-hipError_t launch_radj_nodes(const MeshView& m, const RefAdjMaps& a, const double* lamE, const double* lamU, double bdt,
-                             const double mu[3], bool visc, double* lamQn, double* lamG6N, hipStream_t s);
+                            const double* lam, double bdt, double* lqo, double* lqn, double* ldn, double* lup,
+                            hipStream_t s);
+hipError_t launch_radj_nodes(const MeshView& m, const RefAdjMaps& a, const double* ldn, const double* lup,
+                             const double* lam, double bdt, const double mu[3], bool visc, double* lamQn,
+                             double* lamG6N, hipStream_t s);
 hipError_t launch_radj_g6(const MeshView& m, const RefAdjMaps& a, const double* lamG6N, double* tG, hipStream_t s);
 hipError_t launch_radj_phi(const MeshView& m, const RefAdjMaps& a, const double* tG, double* lamPhiN, hipStream_t s);
-hipError_t launch_radj_cell(const MeshView& m, const RefAdjMaps& a, double gamma, const double* lamE,
-                            const double* lamQn, const double* lamPhiN, const double* U, const double* lamU,
-                            double* lamOut, hipStream_t s);
+hipError_t launch_radj_cell(const MeshView& m, const RefAdjMaps& a, double gamma, const double* lqo,
+                            const double* lqn, const double* lamQn, const double* lamPhiN, const double* Q,
+                            const double* lam, double* lamOut, hipStream_t s);
+// [4][nc] <-> [nc][4]
+hipError_t launch_radj_to_rec(int64_t nc, const double* soa, double* rec, hipStream_t s);
+hipError_t launch_radj_from_rec(int64_t nc, const double* rec, double* soa, hipStream_t s);
 
 }  // namespace gd

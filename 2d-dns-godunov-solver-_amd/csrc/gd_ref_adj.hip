@@ -8,12 +8,14 @@
 //
 // The nonlinear part is per face-list entry (the face pass's list: a mirrored
 // face once from its owner, every other face slot from its cell): the entry's
-// reconstruction and AUSM+ are recomputed from Q and Qn and differentiated
-// with dual numbers, seeding each input in turn (the templated reference
-// arithmetic of gd_ref_math.hpp, so the values are the forward kernels' own).
-// Everything else is linear and transposed as gathers over the incidence lists
-// of RefAdjMaps, each summed in a fixed order.  Built without contraction, like
-// the forward reference kernels.
+// reconstruction and AUSM+ are recomputed from Q and Qn with the forward
+// kernels' arithmetic (so every branch is taken the same way), the limiter's
+// two partials carried by a two-direction dual number, and AUSM+ transposed in
+// one reverse sweep.  Everything else is linear and transposed as gathers,
+// each summed in a fixed order: face-list entries per node and per cell
+// (RefAdjMaps' CSR lists), and the node clusters slot by slot.  The adjoint
+// state is kept as [cell][4] records, the form every gather reads.  Built
+// without contraction, like the forward reference kernels.
 #include "gd_ad.hpp"
 #include "gd_fold.hpp"
 #include "gd_internal.hpp"
@@ -38,13 +40,121 @@ __device__ __forceinline__ void st2(double* p, const double* o) {
   for (int q = 0; q < K; ++q) *reinterpret_cast<double2*>(p + 2 * q) = make_double2(o[2 * q], o[2 * q + 1]);
 }
 
+// Adds, in row order, the 4-double records rec(ref[r]) of the CSR row
+// [r0, r1) to acc; refs and records are loaded four at a time before they are
+// added, so the gathers are in flight together.
+template <class F>
+__device__ __forceinline__ void row_sum4(const int32_t* ref, int64_t r0, int64_t r1, F rec, double* acc) {
+  for (int64_t r = r0; r < r1; r += 4) {
+    int32_t c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = r + u < r1 ? ref[r + u] : -1;
+    double x[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (c[u] >= 0) ld2<2>(rec(c[u]), x[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (c[u] >= 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] = acc[v] + x[u][v];
+      }
+  }
+}
+
+// derivatives of the AUSM+ splittings of gd_ref_math.hpp (beta 1/8, alpha
+// 3/16); at |M| >= 1 the branch's own derivative, with fabs'(M) = +1 at M = 0
+__device__ __forceinline__ double dmach_plus(double M) {
+  const double s = M < 0.0 ? -1.0 : 1.0;
+  return fabs(M) >= 1.0 ? 0.5 * (1.0 + s) : 0.5 * (M + 1.0) + 0.5 * M * (M * M - 1.0);
+}
+__device__ __forceinline__ double dmach_minus(double M) {
+  const double s = M < 0.0 ? -1.0 : 1.0;
+  return fabs(M) >= 1.0 ? 0.5 * (1.0 - s) : -0.5 * (M - 1.0) - 0.5 * M * (M * M - 1.0);
+}
+__device__ __forceinline__ double dpres_plus(double M) {
+  return fabs(M) >= 1.0 ? 0.0 : 0.75 * (1.0 - M * M) + (3.0 / 16.0) * (M * M - 1.0) * (5.0 * M * M - 1.0);
+}
+__device__ __forceinline__ double dpres_minus(double M) {
+  return fabs(M) >= 1.0 ? 0.0 : 0.75 * (M * M - 1.0) - (3.0 / 16.0) * (M * M - 1.0) * (5.0 * M * M - 1.0);
+}
+
+// The reverse sweep of refm::ausm_plus (AUSMflux2dFast.jl:87-175): with the
+// outputs' adjoints w, the gradients of w . out with respect to the states L
+// and R.  The forward values are recomputed with the same expressions as
+// refm::ausm_plus, so the branches (min(1, a*/|un|), min(aL, aR), |M| >= 1,
+// the sign of m) are the forward kernels'.
+__device__ __forceinline__ void ausm_plus_adj(const double* L, const double* R, double nx, double ny, double len,
+                                              double g, const double* w, double* gL, double* gR) {
+  const double rL = L[0], uL = L[1], vL = L[2], pL = L[3];
+  const double rR = R[0], uR = R[1], vR = R[2], pR = R[3];
+  const double htL = fma(0.5, (uL * uL + vL * vL), pL / rL + pL / rL / (g - 1.0));
+  const double htR = fma(0.5, (uR * uR + vR * vR), pR / rR + pR / rR / (g - 1.0));
+  const double cst = 2.0 * (g - 1.0) / (g + 1.0);
+  const double unL = uL * nx + vL * ny;
+  const double unR = uR * nx + vR * ny;
+  const double asL = sqrt(cst * htL);
+  const double asR = sqrt(cst * htR);
+  const double tL = asL / fabs(unL), tR = asR / fabs(unR);
+  const bool cL = tL < 1.0, cR = tR < 1.0;  // refm::jmin(1, t) keeps t
+  const double aL = asL * (cL ? tL : 1.0);
+  const double aR = asR * (cR ? tR : 1.0);
+  const bool takeR = aR < aL;  // refm::jmin(aL, aR)
+  const double a12 = takeR ? aR : aL;
+  const double ML = unL / a12, MR = unR / a12;
+  const double m = refm::mach_plus(ML) + refm::mach_minus(MR);
+  const double sm = m < 0.0 ? -1.0 : 1.0;
+  const double mp = 0.5 * (m + fabs(m)), mm = 0.5 * (m - fabs(m));
+  // out_k = -(a12 (mp rL phiL_k + mm rR phiR_k) + p12 n_k) len, phi = (1, u, v, ht)
+  const double W0 = -w[0] * len, W1 = -w[1] * len, W2 = -w[2] * len, W3 = -w[3] * len;
+  const double sL = W0 + W1 * uL + W2 * vL + W3 * htL;
+  const double sR = W0 + W1 * uR + W2 * vR + W3 * htR;
+  double a12b = mp * rL * sL + mm * rR * sR;
+  const double mb = (a12 * rL * sL) * 0.5 * (1.0 + sm) + (a12 * rR * sR) * 0.5 * (1.0 - sm);
+  double rLb = a12 * mp * sL, rRb = a12 * mm * sR;
+  const double fL = a12 * mp * rL, fR = a12 * mm * rR;
+  double uLb = fL * W1, vLb = fL * W2, htLb = fL * W3;
+  double uRb = fR * W1, vRb = fR * W2, htRb = fR * W3;
+  const double p12b = W1 * nx + W2 * ny;
+  double pLb = p12b * refm::pres_plus(ML), pRb = p12b * refm::pres_minus(MR);
+  const double MLb = p12b * pL * dpres_plus(ML) + mb * dmach_plus(ML);
+  const double MRb = p12b * pR * dpres_minus(MR) + mb * dmach_minus(MR);
+  const double ia = 1.0 / a12;
+  double unLb = MLb * ia, unRb = MRb * ia;
+  a12b = a12b - (MLb * ML + MRb * MR) * ia;
+  const double aLb = takeR ? 0.0 : a12b, aRb = takeR ? a12b : 0.0;
+  // a = as * min(1, as / |un|)
+  double asLb = aLb, asRb = aRb;
+  if (cL) {
+    asLb = aLb * 2.0 * tL;
+    unLb = unLb - aLb * asL * tL / fabs(unL) * (unL < 0.0 ? -1.0 : 1.0);
+  }
+  if (cR) {
+    asRb = aRb * 2.0 * tR;
+    unRb = unRb - aRb * asR * tR / fabs(unR) * (unR < 0.0 ? -1.0 : 1.0);
+  }
+  htLb = htLb + asLb * cst / (2.0 * asL);
+  htRb = htRb + asRb * cst / (2.0 * asR);
+  uLb = uLb + unLb * nx; vLb = vLb + unLb * ny;
+  uRb = uRb + unRb * nx; vRb = vRb + unRb * ny;
+  // ht = (u^2 + v^2) / 2 + (p / r) (1 + 1 / (g - 1))
+  const double k = 1.0 + 1.0 / (g - 1.0);
+  uLb = uLb + htLb * uL; vLb = vLb + htLb * vL;
+  pLb = pLb + htLb * k / rL; rLb = rLb - htLb * k * pL / (rL * rL);
+  uRb = uRb + htRb * uR; vRb = vRb + htRb * vR;
+  pRb = pRb + htRb * k / rR; rRb = rRb - htRb * k * pR / (rR * rR);
+  gL[0] = rLb; gL[1] = uLb; gL[2] = vLb; gL[3] = pLb;
+  gR[0] = rRb; gR[1] = uRb; gR[2] = vRb; gR[3] = pRb;
+}
+
 // One face-list entry f: the flux e = AUSM+(fR, fL) enters U' of its cell i
 // with -b dt / A_i and, for a mirrored face, of the neighbour j with
 // +b dt / A_j; its inputs are qL = Q_i, qR = Q_j and the far nodes' dn, up:
 //   fL = qL + 0.5 lim(qL - dn, qR - qL),  fR = qR - 0.5 lim(qR - qL, up - qR).
-// Writes (lam qL, lam qR, lam dn, lam up), 16 values.
+// Writes the adjoints of qL, qR, dn and up to four [entry][4] arrays.
 __global__ __launch_bounds__(kB) void k_radj_face(MeshView m, gd_params p, const double* Q, const double* Qn,
-                                                   const double* lamU, double bdt, double* lamE) {
+                                                   const double* lam, double bdt, double* lqo, double* lqn,
+                                                   double* ldn, double* lup) {
   const int64_t f = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
   if (f >= m.n_rfaces) return;
   const int64_t nc = m.nc, nn = m.nn;
@@ -61,14 +171,17 @@ __global__ __launch_bounds__(kB) void k_radj_face(MeshView m, gd_params p, const
     j = m.nb[s]; d1 = m.dn1[s]; u1 = m.up1[s];
     nx = m.fnx[s]; ny = m.fny[s]; len = m.flen[s];
   }
-  double lam[4];
+  double w[4], li[4];
+  ld2<2>(lam + 4 * i, li);
   const double ci = -bdt * m.rarea[i];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) lam[v] = ci * lamU[v * nc + i];
+  for (int v = 0; v < 4; ++v) w[v] = ci * li[v];
   if (mir) {
+    double lj[4];
+    ld2<2>(lam + 4 * j, lj);
     const double cj = bdt * m.rarea[j];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) lam[v] = lam[v] + cj * lamU[v * nc + j];
+    for (int v = 0; v < 4; ++v) w[v] = w[v] + cj * lj[v];
   }
   double qL[4], qR[4], dn[4], up[4];
 #pragma unroll
@@ -81,164 +194,234 @@ __global__ __launch_bounds__(kB) void k_radj_face(MeshView m, gd_params p, const
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const double a = qL[v] - dn[v], b = qR[v] - qL[v], c = up[v] - qR[v];
-    const AD l1x = refm::limiter(p.limiter, AD(a, 1.0), AD(b, 0.0), eps);
-    const AD l1y = refm::limiter(p.limiter, AD(a, 0.0), AD(b, 1.0), eps);
-    const AD l2x = refm::limiter(p.limiter, AD(b, 1.0), AD(c, 0.0), eps);
-    const AD l2y = refm::limiter(p.limiter, AD(b, 0.0), AD(c, 1.0), eps);
-    fL[v] = qL[v] + 0.5 * l1x.v;
-    fR[v] = qR[v] - 0.5 * l2x.v;
-    dLqL[v] = 1.0 + 0.5 * (l1x.d - l1y.d);
-    dLdn[v] = -0.5 * l1x.d;
-    dLqR[v] = 0.5 * l1y.d;
-    dRqR[v] = 1.0 - 0.5 * (l2x.d - l2y.d);
-    dRqL[v] = 0.5 * l2x.d;
-    dRup[v] = -0.5 * l2y.d;
+    const AD2 l1 = refm::limiter(p.limiter, AD2(a, 1.0, 0.0), AD2(b, 0.0, 1.0), eps);
+    const AD2 l2 = refm::limiter(p.limiter, AD2(b, 1.0, 0.0), AD2(c, 0.0, 1.0), eps);
+    fL[v] = qL[v] + 0.5 * l1.v;
+    fR[v] = qR[v] - 0.5 * l2.v;
+    dLqL[v] = 1.0 + 0.5 * (l1.dx - l1.dy);
+    dLdn[v] = -0.5 * l1.dx;
+    dLqR[v] = 0.5 * l1.dy;
+    dRqR[v] = 1.0 - 0.5 * (l2.dx - l2.dy);
+    dRqL[v] = 0.5 * l2.dx;
+    dRup[v] = -0.5 * l2.dy;
   }
-  // AUSM+ with the neighbour side fR as "L": the adjoint of each of its 8
-  // inputs from one dual evaluation seeded on that input
-  double lfR[4], lfL[4];
-  for (int k = 0; k < 8; ++k) {
-    AD L[4], R[4], e[4];
+  // AUSM+ takes the neighbour side fR as "L"
+  double gfR[4], gfL[4];
+  ausm_plus_adj(fR, fL, nx, ny, len, p.Gamma, w, gfR, gfL);
+  double o[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      L[v] = AD(fR[v], k == v ? 1.0 : 0.0);
-      R[v] = AD(fL[v], k == v + 4 ? 1.0 : 0.0);
-    }
-    refm::ausm_plus(L, R, nx, ny, len, p.Gamma, e);
-    const double g = lam[0] * e[0].d + lam[1] * e[1].d + lam[2] * e[2].d + lam[3] * e[3].d;
-    if (k < 4) lfR[k] = g;
-    else lfL[k - 4] = g;
-  }
-  double o[16];
+  for (int v = 0; v < 4; ++v) o[v] = gfL[v] * dLqL[v] + gfR[v] * dRqL[v];
+  st2<2>(lqo + 4 * f, o);
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    o[v] = lfL[v] * dLqL[v] + lfR[v] * dRqL[v];
-    o[4 + v] = lfL[v] * dLqR[v] + lfR[v] * dRqR[v];
-    o[8 + v] = lfL[v] * dLdn[v];
-    o[12 + v] = lfR[v] * dRup[v];
-  }
-  st2<8>(lamE + 16 * f, o);
+  for (int v = 0; v < 4; ++v) o[v] = gfL[v] * dLqR[v] + gfR[v] * dRqR[v];
+  st2<2>(lqn + 4 * f, o);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) o[v] = gfL[v] * dLdn[v];
+  st2<2>(ldn + 4 * f, o);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) o[v] = gfR[v] * dRup[v];
+  st2<2>(lup + 4 * f, o);
 }
 
+// The viscous term's linear chain, per cell i with vertices J:
+//   phiN_J = mean over the cluster of J of (rho u, rho v, E),
+//   G_i    = sum_J C_iJ phiN_J         (first Green-Gauss level),
+//   gN_J   = mean over the cluster of J of G,
+//   Lap_i  = sum_J C_iJ . gN_J          (second level, "gradX" of "gradX" + "gradY" of "gradY"),
+// C_iJ = sum over the cell's edges at J of 0.5 (-ny, -nx) len / A_i
+// (calcGradients.jl:36-139's swapped normals), and Udiff = mu_c Lap.
+
 // Per node: the far-node parts of the entries that read it -> lamQn [nn][4];
-// with the viscous term, the second Green-Gauss level transposed: a cell i's
-// Laplacian of field c is (sum over its edges of 0.5 (g_a + g_b) (-n) len) / A_i
-// with the x/y swap of calcGradients.jl, and enters U'[c+1] with b dt mu_c.
-__global__ __launch_bounds__(kB) void k_radj_nodes(MeshView m, RefAdjMaps a, const double* lamE, const double* lamU,
-                                                    double bdt, double mu0, double mu1, double mu2, int visc,
-                                                    double* lamQn, double* lamG6N) {
+// with the viscous term, the second level transposed over the cluster and
+// divided by the cluster size (the node mean's transpose, taken here once per
+// node) -> lamG6N [nn][6].  NCL > 0: cluster width known (every slot's loads
+// first).
+template <int NCL>
+__global__ __launch_bounds__(kB) void k_radj_nodes(MeshView m, RefAdjMaps a, const double* ldn, const double* lup,
+                                                    const double* lam, double bdt, double mu0, double mu1, double mu2,
+                                                    int visc, double* lamQn, double* lamG6N) {
   const int64_t J = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
   if (J >= m.nn) return;
+  const int64_t nn = m.nn;
   double q[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int64_t r = a.nq_off[J]; r < a.nq_off[J + 1]; ++r) {
-    const int32_t code = a.nq_ref[r];
-    double x[4];
-    ld2<2>(lamE + 16 * (int64_t)(code >> 1) + 8 + 4 * (code & 1), x);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) q[v] = q[v] + x[v];
-  }
+  row_sum4(a.nq_ref, a.nq_off[J], a.nq_off[J + 1],
+           [&](int32_t code) { return (code & 1 ? lup : ldn) + 4 * (int64_t)(code >> 1); }, q);
   st2<2>(lamQn + 4 * J, q);
   if (!visc) return;
-  const int64_t nc = m.nc;
-  const double mu[3] = {mu0, mu1, mu2};
+  const double mu[3] = {mu0 * bdt, mu1 * bdt, mu2 * bdt};
   double g[6] = {0, 0, 0, 0, 0, 0};
-  for (int64_t r = a.ne_off[J]; r < a.ne_off[J + 1]; ++r) {
-    const int64_t s = a.ne_ref[r];
-    const int64_t i = s % nc;
-    const double fx = 0.5 * -m.fny[s] * m.flen[s], fy = 0.5 * -m.fnx[s] * m.flen[s];
-    const double sc = bdt / m.area[i];
+  int cnt = 0;
+  auto add = [&](double cx, double cy, const double* l) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double l = mu[c] * lamU[(c + 1) * nc + i] * sc;
-      g[2 * c] = g[2 * c] + fx * l;
-      g[2 * c + 1] = g[2 * c + 1] + fy * l;
+      const double t = mu[c] * l[c + 1];
+      g[2 * c] = g[2 * c] + cx * t;
+      g[2 * c + 1] = g[2 * c + 1] + cy * t;
+    }
+    ++cnt;
+  };
+  if constexpr (NCL > 0) {
+    int32_t c[NCL];
+    double cx[NCL], cy[NCL], l[NCL][4];
+#pragma unroll
+    for (int s = 0; s < NCL; ++s) {
+      c[s] = m.cl[s * nn + J];
+      cx[s] = a.ccx[s * nn + J];
+      cy[s] = a.ccy[s * nn + J];
+      ld2<2>(lam + 4 * (int64_t)(c[s] >= 0 ? c[s] : 0), l[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < NCL; ++s)
+      if (c[s] >= 0) add(cx[s], cy[s], l[s]);
+  } else {
+    for (int s = 0; s < m.ncl; ++s) {
+      const int32_t c = m.cl[s * nn + J];
+      if (c < 0) continue;
+      double l[4];
+      ld2<2>(lam + 4 * (int64_t)c, l);
+      add(a.ccx[s * nn + J], a.ccy[s * nn + J], l);
     }
   }
+  const double ic = cnt ? 1.0 / (double)cnt : 0.0;
+#pragma unroll
+  for (int t = 0; t < 6; ++t) g[t] = g[t] * ic;
   st2<3>(lamG6N + 6 * J, g);
 }
 
-// Per cell: the node means of the gradients transposed (1 / cluster size),
-// then the first Green-Gauss level's 1/A -> tG [nc][6]
+// Per cell: the sum over the clusters it belongs to -> tG [nc][6]
 __global__ __launch_bounds__(kB) void k_radj_g6(MeshView m, RefAdjMaps a, const double* lamG6N, double* tG) {
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
   if (i >= m.nc) return;
   double g[6] = {0, 0, 0, 0, 0, 0};
-  for (int64_t r = a.cm_off[i]; r < a.cm_off[i + 1]; ++r) {
-    double x[6];
-    ld2<3>(lamG6N + 6 * (int64_t)a.cm_node[r], x);
-    const double c = a.cm_c[r];
+  const int64_t r1 = a.cm_off[i + 1];
+  for (int64_t r = a.cm_off[i]; r < r1; r += 4) {
+    int32_t n[4];
 #pragma unroll
-    for (int t = 0; t < 6; ++t) g[t] = g[t] + c * x[t];
+    for (int u = 0; u < 4; ++u) n[u] = r + u < r1 ? a.cm_node[r + u] : -1;
+    double x[4][6];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (n[u] >= 0) ld2<3>(lamG6N + 6 * (int64_t)n[u], x[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (n[u] >= 0) {
+#pragma unroll
+        for (int t = 0; t < 6; ++t) g[t] = g[t] + x[u][t];
+      }
   }
-  const double A = m.area[i];
-#pragma unroll
-  for (int t = 0; t < 6; ++t) g[t] = g[t] / A;
   st2<3>(tG + 6 * i, g);
 }
 
-// Per node: the first Green-Gauss level transposed -> lamPhiN [nn][4]
+// Per node: the first Green-Gauss level transposed over the cluster, divided
+// by the cluster size -> lamPhiN [nn][4]
+template <int NCL>
 __global__ __launch_bounds__(kB) void k_radj_phi(MeshView m, RefAdjMaps a, const double* tG, double* lamPhiN) {
   const int64_t J = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
   if (J >= m.nn) return;
-  const int64_t nc = m.nc;
+  const int64_t nn = m.nn;
   double ph[4] = {0, 0, 0, 0};
-  for (int64_t r = a.ne_off[J]; r < a.ne_off[J + 1]; ++r) {
-    const int64_t s = a.ne_ref[r];
-    const int64_t i = s % nc;
-    const double fx = 0.5 * -m.fny[s] * m.flen[s], fy = 0.5 * -m.fnx[s] * m.flen[s];
-    double t[6];
-    ld2<3>(tG + 6 * i, t);
+  int cnt = 0;
+  auto add = [&](double cx, double cy, const double* t) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) ph[c] = ph[c] + (fx * t[2 * c] + fy * t[2 * c + 1]);
+    for (int c = 0; c < 3; ++c) ph[c] = ph[c] + (cx * t[2 * c] + cy * t[2 * c + 1]);
+    ++cnt;
+  };
+  if constexpr (NCL > 0) {
+    int32_t c[NCL];
+    double cx[NCL], cy[NCL], t[NCL][6];
+#pragma unroll
+    for (int s = 0; s < NCL; ++s) {
+      c[s] = m.cl[s * nn + J];
+      cx[s] = a.ccx[s * nn + J];
+      cy[s] = a.ccy[s * nn + J];
+      ld2<3>(tG + 6 * (int64_t)(c[s] >= 0 ? c[s] : 0), t[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < NCL; ++s)
+      if (c[s] >= 0) add(cx[s], cy[s], t[s]);
+  } else {
+    for (int s = 0; s < m.ncl; ++s) {
+      const int32_t c = m.cl[s * nn + J];
+      if (c < 0) continue;
+      double t[6];
+      ld2<3>(tG + 6 * (int64_t)c, t);
+      add(a.ccx[s * nn + J], a.ccy[s * nn + J], t);
+    }
   }
+  const double ic = cnt ? 1.0 / (double)cnt : 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) ph[c] = ph[c] * ic;
   st2<2>(lamPhiN + 4 * J, ph);
 }
 
 // Per cell: lamQ from the entries that read Q_i and from the node primitives
-// (weight / weight sum), then lamU of U_s = lamU' + P'(U_s)^T lamQ, plus the
-// cluster means of rho*u, rho*v, E transposed (1 / cluster size).
-__global__ __launch_bounds__(kB) void k_radj_cell(MeshView m, RefAdjMaps a, double gamma, const double* lamE,
-                                                   const double* lamQn, const double* lamPhiN, const double* U,
-                                                   const double* lamU, double* lamOut) {
+// (weight / weight sum), then the adjoint of U_s = lam' + P'(U_s)^T lamQ, with
+// P' written in the primitives Q_s = P(U_s), plus the cluster means of rho u,
+// rho v, E transposed (lamPhiN already carries 1 / cluster size).
+__global__ __launch_bounds__(kB) void k_radj_cell(MeshView m, RefAdjMaps a, double gamma, const double* lqo,
+                                                   const double* lqn, const double* lamQn, const double* lamPhiN,
+                                                   const double* Q, const double* lam, double* lamOut) {
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
   if (i >= m.nc) return;
   const int64_t nc = m.nc;
   double lq[4] = {0, 0, 0, 0};
-  for (int64_t r = a.ce_off[i]; r < a.ce_off[i + 1]; ++r) {
-    const int32_t code = a.ce_ref[r];
-    double x[4];
-    ld2<2>(lamE + 16 * (int64_t)(code >> 1) + 4 * (code & 1), x);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) lq[v] = lq[v] + x[v];
-  }
+  row_sum4(a.ce_ref, a.ce_off[i], a.ce_off[i + 1],
+           [&](int32_t code) { return (code & 1 ? lqn : lqo) + 4 * (int64_t)(code >> 1); }, lq);
   double lph[3] = {0, 0, 0};
-  for (int64_t r = a.cm_off[i]; r < a.cm_off[i + 1]; ++r) {
-    const int64_t n = a.cm_node[r];
-    double x[4];
-    ld2<2>(lamQn + 4 * n, x);
-    const double w = a.cm_w[r];
+  const int64_t r1 = a.cm_off[i + 1];
+  for (int64_t r = a.cm_off[i]; r < r1; r += 4) {
+    int32_t n[4];
+    double w[4], x[4][4], y[4][4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) lq[v] = lq[v] + w * x[v];
-    if (lamPhiN) {
-      double y[4];
-      ld2<2>(lamPhiN + 4 * n, y);
-      const double c = a.cm_c[r];
-#pragma unroll
-      for (int t = 0; t < 3; ++t) lph[t] = lph[t] + c * y[t];
+    for (int u = 0; u < 4; ++u) {
+      n[u] = r + u < r1 ? a.cm_node[r + u] : -1;
+      w[u] = r + u < r1 ? a.cm_w[r + u] : 0.0;
     }
-  }
-  const double u0 = U[i], u1 = U[nc + i], u2 = U[2 * nc + i];
-  const double g1 = gamma - 1.0;
-  const double r0 = 1.0 / u0;
-  // Q = (u0, u1/u0, u2/u0, (gamma-1)(u3 - (u1^2 + u2^2)/(2 u0)))
-  double o[4];
-  o[0] = lamU[i] + lq[0] - lq[1] * u1 * r0 * r0 - lq[2] * u2 * r0 * r0 +
-         lq[3] * g1 * 0.5 * (u1 * u1 + u2 * u2) * r0 * r0;
-  o[1] = lamU[nc + i] + lq[1] * r0 - lq[3] * g1 * u1 * r0 + lph[0];
-  o[2] = lamU[2 * nc + i] + lq[2] * r0 - lq[3] * g1 * u2 * r0 + lph[1];
-  o[3] = lamU[3 * nc + i] + lq[3] * g1 + lph[2];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) lamOut[v * nc + i] = o[v];
+    for (int u = 0; u < 4; ++u)
+      if (n[u] >= 0) {
+        ld2<2>(lamQn + 4 * (int64_t)n[u], x[u]);
+        if (lamPhiN) ld2<2>(lamPhiN + 4 * (int64_t)n[u], y[u]);
+      }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (n[u] >= 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) lq[v] = lq[v] + w[u] * x[u][v];
+        if (lamPhiN) {
+#pragma unroll
+          for (int t = 0; t < 3; ++t) lph[t] = lph[t] + y[u][t];
+        }
+      }
+  }
+  const double q0 = Q[i], q1 = Q[nc + i], q2 = Q[2 * nc + i];
+  const double g1 = gamma - 1.0;
+  const double r0 = 1.0 / q0;
+  double l[4];
+  ld2<2>(lam + 4 * i, l);
+  // Q = (U0, U1/U0, U2/U0, (gamma-1)(U3 - (U1^2 + U2^2)/(2 U0)))
+  double o[4];
+  o[0] = l[0] + lq[0] - lq[1] * q1 * r0 - lq[2] * q2 * r0 + lq[3] * g1 * 0.5 * (q1 * q1 + q2 * q2);
+  o[1] = l[1] + lq[1] * r0 - lq[3] * g1 * q1 + lph[0];
+  o[2] = l[2] + lq[2] * r0 - lq[3] * g1 * q2 + lph[1];
+  o[3] = l[3] + lq[3] * g1 + lph[2];
+  st2<2>(lamOut + 4 * i, o);
+}
+
+// [4][nc] <-> [nc][4]
+__global__ __launch_bounds__(kB) void k_radj_to_rec(int64_t nc, const double* soa, double* rec) {
+  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i >= nc) return;
+  const double o[4] = {soa[i], soa[nc + i], soa[2 * nc + i], soa[3 * nc + i]};
+  st2<2>(rec + 4 * i, o);
+}
+__global__ __launch_bounds__(kB) void k_radj_from_rec(int64_t nc, const double* rec, double* soa) {
+  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i >= nc) return;
+  double o[4];
+  ld2<2>(rec + 4 * i, o);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) soa[v * nc + i] = o[v];
 }
 
 unsigned nb(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
@@ -246,15 +429,29 @@ unsigned nb(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
 }  // namespace
 
 hipError_t launch_radj_face(const MeshView& m, const gd_params& p, const double* Q, const double* Qn,
-                            const double* lamU, double bdt, double* lamE, hipStream_t s) {
+                            const double* lam, double bdt, double* lqo, double* lqn, double* ldn, double* lup,
+                            hipStream_t s) {
   if (m.n_rfaces == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_radj_face, dim3(nb(m.n_rfaces)), dim3(kB), 0, s, m, p, Q, Qn, lamU, bdt, lamE);
+  hipLaunchKernelGGL(k_radj_face, dim3(nb(m.n_rfaces)), dim3(kB), 0, s, m, p, Q, Qn, lam, bdt, lqo, lqn, ldn, lup);
   return hipGetLastError();
 }
-hipError_t launch_radj_nodes(const MeshView& m, const RefAdjMaps& a, const double* lamE, const double* lamU, double bdt,
-                             const double mu[3], bool visc, double* lamQn, double* lamG6N, hipStream_t s) {
-  hipLaunchKernelGGL(k_radj_nodes, dim3(nb(m.nn)), dim3(kB), 0, s, m, a, lamE, lamU, bdt, mu[0], mu[1], mu[2],
-                     visc ? 1 : 0, lamQn, lamG6N);
+
+#define GD_RADJ_BY_WIDTH(KER, GRID, S, ...)                                                  \
+  switch (m.ncl) {                                                                           \
+    case 3: hipLaunchKernelGGL(KER<3>, dim3(GRID), dim3(kB), 0, S, __VA_ARGS__); break;      \
+    case 4: hipLaunchKernelGGL(KER<4>, dim3(GRID), dim3(kB), 0, S, __VA_ARGS__); break;      \
+    case 5: hipLaunchKernelGGL(KER<5>, dim3(GRID), dim3(kB), 0, S, __VA_ARGS__); break;      \
+    case 6: hipLaunchKernelGGL(KER<6>, dim3(GRID), dim3(kB), 0, S, __VA_ARGS__); break;      \
+    case 7: hipLaunchKernelGGL(KER<7>, dim3(GRID), dim3(kB), 0, S, __VA_ARGS__); break;      \
+    case 8: hipLaunchKernelGGL(KER<8>, dim3(GRID), dim3(kB), 0, S, __VA_ARGS__); break;      \
+    default: hipLaunchKernelGGL(KER<0>, dim3(GRID), dim3(kB), 0, S, __VA_ARGS__); break;     \
+  }
+
+hipError_t launch_radj_nodes(const MeshView& m, const RefAdjMaps& a, const double* ldn, const double* lup,
+                             const double* lam, double bdt, const double mu[3], bool visc, double* lamQn,
+                             double* lamG6N, hipStream_t s) {
+  GD_RADJ_BY_WIDTH(k_radj_nodes, nb(m.nn), s, m, a, ldn, lup, lam, bdt, mu[0], mu[1], mu[2], visc ? 1 : 0, lamQn,
+                   lamG6N)
   return hipGetLastError();
 }
 hipError_t launch_radj_g6(const MeshView& m, const RefAdjMaps& a, const double* lamG6N, double* tG, hipStream_t s) {
@@ -262,14 +459,22 @@ hipError_t launch_radj_g6(const MeshView& m, const RefAdjMaps& a, const double* 
   return hipGetLastError();
 }
 hipError_t launch_radj_phi(const MeshView& m, const RefAdjMaps& a, const double* tG, double* lamPhiN, hipStream_t s) {
-  hipLaunchKernelGGL(k_radj_phi, dim3(nb(m.nn)), dim3(kB), 0, s, m, a, tG, lamPhiN);
+  GD_RADJ_BY_WIDTH(k_radj_phi, nb(m.nn), s, m, a, tG, lamPhiN)
   return hipGetLastError();
 }
-hipError_t launch_radj_cell(const MeshView& m, const RefAdjMaps& a, double gamma, const double* lamE,
-                            const double* lamQn, const double* lamPhiN, const double* U, const double* lamU,
-                            double* lamOut, hipStream_t s) {
-  hipLaunchKernelGGL(k_radj_cell, dim3(nb(m.nc)), dim3(kB), 0, s, m, a, gamma, lamE, lamQn, lamPhiN, U, lamU,
+hipError_t launch_radj_cell(const MeshView& m, const RefAdjMaps& a, double gamma, const double* lqo,
+                            const double* lqn, const double* lamQn, const double* lamPhiN, const double* Q,
+                            const double* lam, double* lamOut, hipStream_t s) {
+  hipLaunchKernelGGL(k_radj_cell, dim3(nb(m.nc)), dim3(kB), 0, s, m, a, gamma, lqo, lqn, lamQn, lamPhiN, Q, lam,
                      lamOut);
+  return hipGetLastError();
+}
+hipError_t launch_radj_to_rec(int64_t nc, const double* soa, double* rec, hipStream_t s) {
+  hipLaunchKernelGGL(k_radj_to_rec, dim3(nb(nc)), dim3(kB), 0, s, nc, soa, rec);
+  return hipGetLastError();
+}
+hipError_t launch_radj_from_rec(int64_t nc, const double* rec, double* soa, hipStream_t s) {
+  hipLaunchKernelGGL(k_radj_from_rec, dim3(nb(nc)), dim3(kB), 0, s, nc, rec, soa);
   return hipGetLastError();
 }
 

@@ -155,7 +155,38 @@ def adjoint_cost(mesh, case, order_dev, U0, steps, ckpt, cfl, precision=L.GD_FP3
             "all_records": variant(adj1, ast1, g1, 64)}
 
 
-def ref_tgv(n, steps, warmup):
+def ref_adjoint_cost(sol, mesh, steps, ckpt, dt, budget_gib=16):
+    """Discrete adjoint of J = <w, U_N> over ``steps`` fixed-dt steps of the
+    reference loop (w seeded N(0, 1)), against the same steps forward with
+    gd_run at that dt: with every state kept from the forward sweep
+    (``budget_gib`` of HBM; the headline) and checkpointed every ``ckpt``
+    steps with the rest recomputed.  Timed by the host clock around
+    synchronized calls; dJ/dU0 stays on the device, w is uploaded and U_N read
+    back for J inside the timed call."""
+    n = mesh.nCells
+    w = np.asfortranarray(M.normal(SEED + 5, 4 * n).reshape(n, 4, order="F"))
+    sol.adjoint(2, dt, checkpoint_every=1, dJ_dUN=w, to_host=False)  # incidence lists, workspace
+    sol.state.sync()
+    t0 = time.perf_counter()
+    sol.run(steps, dt=dt)
+    sol.state.sync()
+    fwd = time.perf_counter() - t0
+
+    def one(budget):
+        t0 = time.perf_counter()
+        _, _, st = sol.adjoint(steps, dt, checkpoint_every=ckpt, dJ_dUN=w, to_host=False, record_budget=budget)
+        sol.state.sync()
+        adj = time.perf_counter() - t0
+        return {"adjoint_ms": round(adj * 1e3, 3), "cost_vs_forward": round(adj / fwd, 3),
+                "adjoint_forward_sweep_ms": round(st.ms_forward, 3), "adjoint_reverse_ms": round(st.ms_reverse, 3),
+                "recomputed_steps": st.recomputed_steps, "storage_GB": round(st.storage_bytes / 1e9, 3), "J": st.J}
+    kept = one(int(budget_gib * 2**30))
+    ckpted = one(0)
+    return {"steps": steps, "checkpoint_every": ckpt, "dt": dt, "forward_ms": round(fwd * 1e3, 3),
+            "record_budget_GiB": budget_gib, **kept, "checkpointed": ckpted}
+
+
+def ref_tgv(n, steps, warmup, adj_steps=0, adj_ckpt=10):
     """The reference scheme, FP64 (vertex-based MUSCL with the reference's
     Minmod_Limiter, AUSM+, the calcDiffTerm viscous term, updateVariablesSA and
     the node interpolation, as calcOneStageMUSCLcpuFast and its driver run
@@ -183,8 +214,10 @@ def ref_tgv(n, steps, warmup):
     sol.run(steps, cfl=0.5, dt_policy=L.GD_DT_LOCAL)
     sol.state.sync()
     el = time.perf_counter() - t0
-    kt = L.kernel_table(sol.run(steps, cfl=0.5, dt_policy=L.GD_DT_LOCAL, time_kernels=True))
+    rs = sol.run(steps, cfl=0.5, dt_policy=L.GD_DT_LOCAL, time_kernels=True)
+    kt = L.kernel_table(rs)
     sol.state.sync()
+    adjoint = ref_adjoint_cost(sol, mesh, adj_steps, adj_ckpt, rs.last_dt) if adj_steps > 0 else None
     sol.close()
     loop = [k for k in kt if k["launches"] == steps]  # ref_phi runs once per call, before the loop
     ms = 1e3 * el / steps
@@ -195,6 +228,13 @@ def ref_tgv(n, steps, warmup):
             f"{k['bytes_per_launch'] / (k['ms_avg'] * 1e-3) / 1e9:8.1f} GB/s")
     log(f"reference scheme: {ms:.4f} ms/step, {mesh.nCells / (ms * 1e-3):.4e} cell-updates/s, "
         f"step {b / (ms * 1e-3) / 1e9:.0f} GB/s")
+    if adjoint:
+        log(f"reference-scheme adjoint: {adjoint['steps']} steps, checkpoints every {adjoint['checkpoint_every']}: "
+            f"{adjoint['adjoint_ms']:.1f} ms vs forward {adjoint['forward_ms']:.1f} ms -> "
+            f"{adjoint['cost_vs_forward']:.2f}x with every state kept ({adjoint['storage_GB']:.1f} GB; "
+            f"forward sweep {adjoint['adjoint_forward_sweep_ms']:.1f} ms, reverse "
+            f"{adjoint['adjoint_reverse_ms']:.1f} ms), {adjoint['checkpointed']['cost_vs_forward']:.2f}x "
+            f"checkpointed ({adjoint['checkpointed']['recomputed_steps']} steps recomputed)")
     return init, {"workload": f"periodic Taylor-Green square, {n}x{n}x2 = {mesh.nCells} triangles, reference scheme "
                         f"(MUSCL + Minmod_Limiter + AUSM+ + calcDiffTerm), FP64, CFL 0.5, Hilbert device order",
             "cells": mesh.nCells, "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 5),
@@ -204,7 +244,8 @@ def ref_tgv(n, steps, warmup):
             "step_hbm": {"bytes_per_step": b, "achieved_GBs": round(b / (ms * 1e-3) / 1e9, 1),
                          "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
             "roofline": {**roof, "kernel": dom["name"], "bytes_per_launch": dom["bytes_per_launch"],
-                         "avg_launch_us": round(dom["ms_avg"] * 1e3, 2)}}
+                         "avg_launch_us": round(dom["ms_avg"] * 1e3, 2)},
+            "adjoint": adjoint}
 
 
 def roofline(kt, steps):
@@ -331,6 +372,9 @@ def main():
     ap.add_argument("--ref-tgv", type=int, default=1000,
                     help="n of the n x n x 2 periodic TGV for the reference-scheme leg; 0 skips it")
     ap.add_argument("--ref-steps", type=int, default=200)
+    ap.add_argument("--ref-adjoint-steps", type=int, default=100,
+                    help="steps of the reference-scheme adjoint on the TGV square; 0 skips it")
+    ap.add_argument("--ref-adjoint-checkpoint", type=int, default=10)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU path is not built",
@@ -421,7 +465,8 @@ def main():
     # may run on)
     threads = a.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
     if a.ref_tgv > 0:
-        init, out["ref_scheme"] = ref_tgv(a.ref_tgv, a.ref_steps, min(a.warmup, 10))
+        init, out["ref_scheme"] = ref_tgv(a.ref_tgv, a.ref_steps, min(a.warmup, 10), a.ref_adjoint_steps,
+                                          a.ref_adjoint_checkpoint)
         if not a.no_cpu:
             try:
                 rc = ref_cpu_baseline(init, threads)

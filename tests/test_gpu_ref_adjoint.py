@@ -89,15 +89,21 @@ def test_ref_adjoint_dot_product_with_tangent_linear_oracle(lim, viscous):
 
 
 def test_ref_adjoint_independent_of_checkpointing():
-    """Every checkpoint interval recomputes the same states and transposes the
-    same steps in the same order: identical bits."""
+    """Every checkpoint interval recomputes the same states, and states kept
+    from the forward sweep (record_budget) are the same states: the steps are
+    transposed in the same order, with identical bits."""
     mesh, _, _, U = _case(60)
+    seg_bytes = 32 * (mesh.nCells + mesh.nNodes)  # one step's Q and node means Qn
     out = []
-    for c in (1, 3, 7):
+    # 7 steps; a budget of k segments keeps the trailing k segments' states
+    for c, budget, recomputed in ((1, 0, 0), (3, 0, 4), (7, 0, 6), (3, 6 * seg_bytes, 2), (3, 1 << 40, 0),
+                                  (2, 4 * seg_bytes, 2)):
         s = _solver(mesh, O.LIM_REF_HARMONIC, True, U)
-        g0, _, st = s.adjoint(7, DT, checkpoint_every=c, dJ_dUN=U)
+        g0, _, st = s.adjoint(7, DT, checkpoint_every=c, dJ_dUN=U, record_budget=budget)
         out.append((g0, st.J, s.conservative()))
         s.close()
+        print(c, budget, "recomputed", st.recomputed_steps, "storage", st.storage_bytes)
+        assert st.recomputed_steps == recomputed
     for g0, J, UN in out[1:]:
         assert np.array_equal(g0, out[0][0]) and J == out[0][1] and np.array_equal(UN, out[0][2])
 
