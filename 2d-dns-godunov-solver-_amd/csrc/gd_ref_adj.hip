@@ -123,26 +123,29 @@ __device__ __forceinline__ void ausm_plus_adj(const double* L, const double* R, 
   double unLb = MLb * ia, unRb = MRb * ia;
   a12b = a12b - (MLb * ML + MRb * MR) * ia;
   const double aLb = takeR ? 0.0 : a12b, aRb = takeR ? a12b : 0.0;
-  // a = as * min(1, as / |un|)
+  // a = as * min(1, as / |un|): with t = as / |un| < 1, da/das = 2 t and
+  // da/d|un| = -t^2
   double asLb = aLb, asRb = aRb;
   if (cL) {
     asLb = aLb * 2.0 * tL;
-    unLb = unLb - aLb * asL * tL / fabs(unL) * (unL < 0.0 ? -1.0 : 1.0);
+    unLb = unLb - aLb * tL * tL * (unL < 0.0 ? -1.0 : 1.0);
   }
   if (cR) {
     asRb = aRb * 2.0 * tR;
-    unRb = unRb - aRb * asR * tR / fabs(unR) * (unR < 0.0 ? -1.0 : 1.0);
+    unRb = unRb - aRb * tR * tR * (unR < 0.0 ? -1.0 : 1.0);
   }
-  htLb = htLb + asLb * cst / (2.0 * asL);
-  htRb = htRb + asRb * cst / (2.0 * asR);
+  // as = sqrt(cst ht)
+  htLb = htLb + asLb * (0.5 * cst) / asL;
+  htRb = htRb + asRb * (0.5 * cst) / asR;
   uLb = uLb + unLb * nx; vLb = vLb + unLb * ny;
   uRb = uRb + unRb * nx; vRb = vRb + unRb * ny;
   // ht = (u^2 + v^2) / 2 + (p / r) (1 + 1 / (g - 1))
   const double k = 1.0 + 1.0 / (g - 1.0);
+  const double kL = htLb * k / rL, kR = htRb * k / rR;
   uLb = uLb + htLb * uL; vLb = vLb + htLb * vL;
-  pLb = pLb + htLb * k / rL; rLb = rLb - htLb * k * pL / (rL * rL);
+  pLb = pLb + kL; rLb = rLb - kL * (pL / rL);
   uRb = uRb + htRb * uR; vRb = vRb + htRb * vR;
-  pRb = pRb + htRb * k / rR; rRb = rRb - htRb * k * pR / (rR * rR);
+  pRb = pRb + kR; rRb = rRb - kR * (pR / rR);
   gL[0] = rLb; gL[1] = uLb; gL[2] = vLb; gL[3] = pLb;
   gR[0] = rRb; gR[1] = uRb; gR[2] = vRb; gR[3] = pRb;
 }

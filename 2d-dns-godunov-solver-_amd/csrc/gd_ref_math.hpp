@@ -23,10 +23,11 @@ __device__ __forceinline__ double jmax(double x, double y) {
   return (x != x || y != y) ? x + y : (y > x ? y : x);
 }
 
-// The limiters, splittings and AUSM+ below are templates on the scalar: with
-// double they are the forward kernels' arithmetic, with the dual number of
-// gd_ad.hpp (found by argument-dependent lookup: fabs, sqrt, fma, jsign, jmin)
-// they also carry one directional derivative, for the reference-scheme adjoint.
+// The limiter, splittings and AUSM+ below are templates on the scalar: with
+// double they are the forward kernels' arithmetic; the reference-scheme
+// adjoint instantiates the limiter with the dual number of gd_ad.hpp (found by
+// argument-dependent lookup: fabs, jsign, jmin), which also carries its two
+// partial derivatives.
 
 // limiters.jl:33-56 and the commented true minmod (limiters.jl:45-47)
 template <class T>
