@@ -141,3 +141,51 @@ def test_ref_adjoint_refuses_what_it_does_not_cover():
     with pytest.raises(L.GodunovError, match="boundary faces"):
         s.adjoint(2, DT)
     s.close()
+
+
+def test_ref_adjoint_full_size_against_central_differences():
+    """The 2,000,000-triangle TGV square (noisy state), 20 steps at dt 5e-5,
+    J = <w, U_N>: <dJ/dU0, d> in three smooth seeded directions (a plane wave
+    per variable, |k| in [1, 4]) against central differences of J from the
+    GPU's own forward loop (the forward sweep of an adjoint call, which starts
+    from the primitives of U0 as the adjoint does) at h = 1e-8, and the
+    gradient unchanged between checkpoint intervals 5 and 20.
+
+    J(U0 + h d) - J(U0 - h d) is summed as <w, U_N+ - U_N-> (a sum of 8M
+    products accumulated one by one carries rounding of order 1e-8, as large
+    as the difference itself at h = 1e-9).  The noisy state has limiter
+    arguments near zero in many cells, where the limiter has a kink: at
+    h = 1e-6 some are crossed and the differences are 1e-4 to 1e-3 off (at
+    1e-8: 1.6e-6 to 6.3e-6)."""
+    M = pkg().mesh
+    L = pkg()._lib
+    mesh, _, _, U = _case(1000)
+    n = mesh.nCells
+    N, dt = 20, 5e-5
+    w = np.asfortranarray(M.normal(SEED + 9, 4 * n).reshape(n, 4, order="F"))
+    s = _solver(mesh, O.LIM_REF_HARMONIC, True, U)
+    g0, _, st = s.adjoint(N, dt, checkpoint_every=5, dJ_dUN=w)
+    s.state.set_array(L.GD_ARR_U_OLD, U)
+    g1, _, _ = s.adjoint(N, dt, checkpoint_every=20, dJ_dUN=w)
+    assert np.array_equal(g0, g1)
+
+    def UN(U0):
+        s.state.set_array(L.GD_ARR_U_OLD, U0)
+        s.adjoint(N, dt, checkpoint_every=N, to_host=False)
+        return s.conservative()
+
+    x, y = mesh.cell_mid_points[:, 0], mesh.cell_mid_points[:, 1]
+    scale = np.abs(U).max(axis=0)
+    h = 1e-8
+    errs = []
+    for k in range(3):
+        r = M.uniform01(SEED + 40 + k, 12).reshape(4, 3)
+        d = np.asfortranarray(np.stack(
+            [scale[v] * np.sin((1.0 + 3.0 * r[v, 0]) * (np.cos(6.3 * r[v, 1]) * x + np.sin(6.3 * r[v, 1]) * y)
+                               + 6.3 * r[v, 2]) for v in range(4)], 1))
+        adj = float((g0 * d).sum())
+        fd = float((w * (UN(np.asfortranarray(U + h * d)) - UN(np.asfortranarray(U - h * d)))).sum()) / (2 * h)
+        errs.append(abs(adj - fd) / abs(adj))
+        print("direction", k, "adjoint", adj, "central difference", fd, "rel", errs[-1])
+    s.close()
+    assert max(errs) <= 1e-4
