@@ -1294,11 +1294,7 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
 }
 
 
-// ---- discrete adjoint of the ns loop ---------------------------------------
-// Forward sweep from the state's U keeps U every checkpoint_every steps and the
-// cell records of the current segment; the reverse sweep walks the segments
-// backwards, recomputing each (except the last) from its checkpoint into the
-// segment's record slots, then applies the transposed steps.
+// ---- discrete adjoint of the reference loop --------------------------------
 // The reference-scheme adjoint's incidence lists (RefAdjMaps), built once per
 // mesh from its device arrays; flags meshes outside what the adjoint covers.
 template <class T>
@@ -1656,6 +1652,11 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
   return GD_OK;
 }
 
+// ---- discrete adjoint of the ns loop ---------------------------------------
+// Forward sweep from the state's U keeps U every checkpoint_every steps and the
+// cell records of the current segment; the reverse sweep walks the segments
+// backwards, recomputing each (except the last) from its checkpoint into the
+// segment's record slots, then applies the transposed steps.
 int gd_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dUN, double* dJ_dU0, double* dJ_dbc,
                    gd_adjoint_stats* stats) {
   if (!S || !ap) return fail(GD_ERR_INVALID, "null argument");
