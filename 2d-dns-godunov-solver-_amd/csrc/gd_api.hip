@@ -1333,7 +1333,7 @@ static int ref_adj_maps(Mesh* M) {
   // every list index below is checked here, so the lists address only rows
   // that exist
   auto in = [](int64_t x, int64_t n) { return x >= 0 && x < n; };
-  a.supported = !v.has_quads;
+  a.supported = !v.has_quads && 2 * nr + 1 <= INT32_MAX;  // entry codes 2 f + role are int32
   for (int64_t q = 0; q < 3 * nc && a.supported; ++q)
     a.supported = in(nb[(size_t)q], nc) && in(dn1[(size_t)q], nn) && in(up1[(size_t)q], nn) && in(ea[(size_t)q], nn) &&
                   in(eb[(size_t)q], nn);
