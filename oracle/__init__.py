@@ -123,7 +123,8 @@ def lib():
                                      dp, dp]
         L.orc_set_threads.restype = None
         L.orc_ref_loop_tl.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcThermo), C.POINTER(OrcParams), C.c_int32,
-                                      C.c_double, C.c_double, C.c_int32, dp, dp, dp, dp]
+                                      C.c_double, C.c_double, C.c_int32, C.c_double, C.c_int32, dp, dp, dp, dp, dp,
+                                      dp]
         L.orc_ns_dt.restype = C.c_double
         L.orc_ns_dt.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcNsParams), dp, C.c_double]
         _lib = L
@@ -411,20 +412,33 @@ def ns_step_tl(mv: MeshView, prm, dt, U, dU, dbc=None):
     return UN, dUN, f2, df2
 
 
-def ref_loop_tl(mv: MeshView, thermo, prm, U0, dU0, dt, n_steps, viscous=True, betta=1.0):
+def ref_loop_tl(mv: MeshView, thermo, prm, U0, dU0, dt, n_steps, viscous=True, betta=1.0, flow_time=0.0):
     """N steps of the reference loop (calcDiffTerm, calcOneStageMUSCLcpuFast,
     updateVariablesSA, node interpolation) from U0 at a fixed dt, starting from
-    the primitives of U0 and their node means, with the tangent dU0 carried in
-    dual numbers: returns (U_N, dU_N).  Periodic triangle meshes, AUSM+,
-    constant mu and k."""
+    the primitives of U0 and their node means, the flow time advancing from
+    flow_time by dt per step, with the tangent dU0 carried in dual numbers:
+    returns (U_N, dU_N).  AUSM+, constant mu and k; triangles and quads,
+    periodic or with the boundary ghosts."""
+    UN, dUN, _, _ = ref_loop_tl_force(mv, thermo, prm, U0, dU0, dt, n_steps, 0, viscous, betta, flow_time)
+    return UN, dUN
+
+
+def ref_loop_tl_force(mv: MeshView, thermo, prm, U0, dU0, dt, n_steps, wall_code, viscous=True, betta=1.0,
+                      flow_time=0.0):
+    """ref_loop_tl with the wall force of the faces of code wall_code (the
+    AUSM+ momentum flux through them) summed over the N steps: returns (U_N,
+    dU_N, force (2,), dforce (2,))."""
     n = mv.mesh.nCells
     UN = np.zeros((n, 4), order="F")
     dUN = np.zeros((n, 4), order="F")
+    F = np.zeros(2)
+    dF = np.zeros(2)
     th = thermo_struct(thermo)
     d = None if dU0 is None else _chk_f64(dU0)
     _rc(lib().orc_ref_loop_tl(C.byref(mv.c), C.byref(th), C.byref(prm), int(bool(viscous)), betta, dt, int(n_steps),
-                              _p(_chk_f64(U0)), _p(d), _p(UN), _p(dUN)), "ref_loop_tl")
-    return UN, dUN
+                              float(flow_time), int(wall_code), _p(_chk_f64(U0)), _p(d), _p(UN), _p(dUN), _p(F),
+                              _p(dF)), "ref_loop_tl")
+    return UN, dUN, F, dF
 
 
 def ns_dt(mv: MeshView, prm, U, cfl):

@@ -10,12 +10,15 @@
 //     AUSMflux2dFast.jl:87-175),
 //   updateVariablesSA (evaluate2d.jl:83-93) and the node interpolation
 //   (utilsFVM2dp.jl:299-365),
-// at a fixed dt.  The state is U alone: the loop starts from the primitives of
-// U0 and their node means.  Every function is templated on the scalar: with
-// double it is the oracle's loop operation for operation; with the dual
-// number of dual.hpp it carries one directional derivative dU through the same
-// arithmetic.  Triangle meshes whose faces are all interior (periodic) only —
-// the meshes the reference-scheme adjoint covers.
+// at a fixed dt, the flow time advancing by dt per step (the ghost states of
+// the TGV mirror depend on it).  The state is U alone: the loop starts from the
+// primitives of U0 and their node means.  Every function is templated on the
+// scalar: with double it is the oracle's loop operation for operation; with
+// the dual number of dual.hpp it carries one directional derivative dU through
+// the same arithmetic.  Triangles and quads, interior faces and the boundary
+// ghosts of oracle.cpp's bc_ghost (fixed state, zero gradient, slip wall, TGV
+// mirror, isothermal wall); the summed wall force of the faces of one code
+// (each step's AUSM+ momentum flux through them) is carried along.
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -90,6 +93,9 @@ struct RefLoop {
   double betta, dt;
   int64_t nc, nn;
   std::vector<T> U, Q, Qn, Ud, Un;
+  double t = 0.0;        // flow time of the step
+  int32_t wall_code = 0; // faces whose force is summed (0: none)
+  T force[2] = {T(0.0), T(0.0)};
 
   // updateVariablesSA's primitives, evaluate2d.jl:87-90 (oracle.cpp: orc_update_variables)
   void prims() {
@@ -134,9 +140,9 @@ struct RefLoop {
   void green_gauss(const T* phi, T* gx, T* gy) const {
 #pragma omp parallel for
     for (int64_t C = 1; C <= nc; ++C) {
+      const int nf = Ix(m->mesh_connectivity, nc, C, 3) == 4 ? 4 : 3;
       T fx[4], fy[4];
-      for (int k = 0; k < 4; ++k) { fx[k] = T(0.0); fy[k] = T(0.0); }
-      for (int k = 1; k <= 3; ++k) {
+      for (int k = 1; k <= nf; ++k) {
         const double side = Dx(m->cell_edges_length, nc, C, k);
         const double nx = Dx(m->cell_edges_Nx, nc, C, k), ny = Dx(m->cell_edges_Ny, nc, C, k);
         const T pl = phi_in_cluster(Ix(m->cells2nodes, nc, C, 2 * k - 1), phi);
@@ -145,8 +151,10 @@ struct RefLoop {
         fy[k - 1] = T(0.5) * (pl + pr) * T(-nx) * T(side);
       }
       // the 4th slot of a triangle: 0.5 (0 + 0) (-0) 0, as the oracle adds it
-      fx[3] = T(0.5) * (T(0.0) + T(0.0)) * T(-0.0) * T(0.0);
-      fy[3] = fx[3];
+      if (nf == 3) {
+        fx[3] = T(0.5) * (T(0.0) + T(0.0)) * T(-0.0) * T(0.0);
+        fy[3] = fx[3];
+      }
       const T A(m->cell_areas[C - 1]);
       gx[C - 1] = (fx[0] + fx[1] + fx[2] + fx[3]) / A;
       gy[C - 1] = (fy[0] + fy[1] + fy[2] + fy[3]) / A;
@@ -170,17 +178,71 @@ struct RefLoop {
       Ud[3 * nc + i] = mu * k * (dd[2][0][i] + dd[2][1][i]);
     }
   }
-  // computeInterfaceSlope on an interior triangle face (computeslope2d.jl:44-126)
+  // the ghost state of boundary face k of cell i (oracle.cpp: bc_ghost):
+  // ComputeUPhysFromBoundaries, boundaryConditions2d.jl:29-57, the wall
+  // reflection bcInviscidWall.jl:109-126, the TGV mirror
+  // boundaryConditions_TGV2d.jl:16-30 (a function of the geometry and the flow
+  // time only, taken from the oracle's own ghost), the isothermal no-slip wall
+  void ghost(int64_t i, int k, const T* cur, T* bnd) const {
+    const int32_t code = Ix(m->cell_stiffness, nc, i, k);
+    const int32_t idx = -code;
+    const orc_bc* b = (idx >= 0 && idx <= ORC_MAX_BC) ? &prm->bc[idx] : nullptr;
+    const int kind = b ? b->kind : ORC_BC_EXTRAPOLATE;
+    const double nx = Dx(m->cell_edges_Nx, nc, i, k), ny = Dx(m->cell_edges_Ny, nc, i, k);
+    switch (kind) {
+      case ORC_BC_FIXED:
+        for (int v = 0; v < 4; ++v) bnd[v] = T(b->data[v]);
+        break;
+      case ORC_BC_SLIP_WALL:
+        bnd[0] = cur[0];
+        bnd[1] = cur[1] * T(ny * ny - nx * nx) - T(2.0 * nx * ny) * cur[2];
+        bnd[2] = cur[2] * T(nx * nx - ny * ny) - T(2.0 * nx * ny) * cur[1];
+        bnd[3] = cur[3];
+        break;
+      case ORC_BC_TGV_MIRROR: {
+        double c[4], g[4];
+        for (int v = 0; v < 4; ++v) c[v] = orc_ad::val(cur[v]);
+        orc_ref_bc_ghost(m, th, prm, i, k, c, t, g);
+        for (int v = 0; v < 4; ++v) bnd[v] = T(g[v]);
+        break;
+      }
+      case ORC_BC_ISOTHERMAL_WALL: {
+        const T R(th->RGAS);
+        const T Tc = cur[3] / (cur[0] * R);
+        T Tg = T(2.0 * b->data[0]) - Tc;
+        if (orc_ad::val(Tg) < 0.5 * b->data[0]) Tg = T(0.5 * b->data[0]);
+        bnd[0] = cur[3] / (R * Tg); bnd[1] = -cur[1]; bnd[2] = -cur[2]; bnd[3] = cur[3];
+        break;
+      }
+      default:
+        for (int v = 0; v < 4; ++v) bnd[v] = cur[v];
+    }
+  }
+  // computeInterfaceSlope (computeslope2d.jl:44-126): a triangle's far nodes,
+  // a quad's two far-side nodes averaged, or at a boundary the ghost with
+  // uDown = uL and uUp = uR (a first-order face)
   void face(int64_t i, int k, T* e) const {
     const int32_t ek = Ix(m->cell_stiffness, nc, i, k);
+    const int32_t type = Ix(m->mesh_connectivity, nc, i, 2);
     const int index = 2 * k - 1;
-    const int32_t pDown1 = Ix(m->node2cellsL2down, nc, i, index), pUp1 = Ix(m->node2cellsL2up, nc, i, index);
     T qL[4], qR[4], up[4], dn[4], UpLeft[4], UpRight[4];
-    for (int v = 0; v < 4; ++v) {
-      qL[v] = Q[v * nc + i - 1];
-      qR[v] = Q[v * nc + ek - 1];
-      up[v] = Qn[v * nn + pUp1 - 1];
-      dn[v] = Qn[v * nn + pDown1 - 1];
+    for (int v = 0; v < 4; ++v) { qL[v] = Q[v * nc + i - 1]; up[v] = T(0.0); dn[v] = T(0.0); }
+    if (ek >= 1 && ek <= nc) {
+      if (type == 3) {
+        const int32_t pDown1 = Ix(m->node2cellsL2down, nc, i, index), pUp1 = Ix(m->node2cellsL2up, nc, i, index);
+        for (int v = 0; v < 4; ++v) { up[v] = Qn[v * nn + pUp1 - 1]; dn[v] = Qn[v * nn + pDown1 - 1]; }
+      } else if (type == 2) {
+        const int32_t pDown1 = Ix(m->node2cellsL2down, nc, i, index), pDown2 = Ix(m->node2cellsL2down, nc, i, index + 1);
+        const int32_t pUp1 = Ix(m->node2cellsL2up, nc, i, index), pUp2 = Ix(m->node2cellsL2up, nc, i, index + 1);
+        for (int v = 0; v < 4; ++v) {
+          up[v] = T(0.5) * (Qn[v * nn + pUp1 - 1] + Qn[v * nn + pUp2 - 1]);
+          dn[v] = T(0.5) * (Qn[v * nn + pDown1 - 1] + Qn[v * nn + pDown2 - 1]);
+        }
+      }
+      for (int v = 0; v < 4; ++v) qR[v] = Q[v * nc + ek - 1];
+    } else {
+      ghost(i, k, qL, qR);
+      for (int v = 0; v < 4; ++v) { dn[v] = qL[v]; up[v] = qR[v]; }
     }
     const double ksi = prm->limiter_eps;
     for (int v = 0; v < 4; ++v)
@@ -194,18 +256,29 @@ struct RefLoop {
   // calcOneStageMUSCLcpuFast, calcOneStageMUSCLcpuFast.jl:3-107 / SOUscheme.jl:56-111
   void stage() {
     const T b(betta), h(dt);
+    std::vector<T> wf(2 * nc, T(0.0));  // each cell's wall faces, summed in cell order below
 #pragma omp parallel for
     for (int64_t i = 1; i <= nc; ++i) {
-      T e1[4], e2[4], e3[4], F[4];
-      face(i, 1, e1);
-      face(i, 2, e2);
-      face(i, 3, e3);
-      for (int v = 0; v < 4; ++v) F[v] = e1[v] + e2[v] + e3[v];
+      const int nf = Ix(m->mesh_connectivity, nc, i, 3) == 4 ? 4 : 3;
+      T e[4][4], F[4];
+      for (int k = 1; k <= nf; ++k) {
+        face(i, k, e[k - 1]);
+        if (wall_code != 0 && Ix(m->cell_stiffness, nc, i, k) == wall_code) {
+          wf[2 * (i - 1)] += e[k - 1][1];
+          wf[2 * (i - 1) + 1] += e[k - 1][2];
+        }
+      }
+      for (int v = 0; v < 4; ++v)
+        F[v] = nf == 3 ? e[0][v] + e[1][v] + e[2][v] : e[0][v] + e[1][v] + e[2][v] + e[3][v];
       const T Rarea(1.0 / m->cell_areas[i - 1]);
       for (int v = 0; v < 4; ++v) {
         const int64_t s = v * nc + i - 1;
         Un[s] = viscous ? (U[s] - F[v] * b * h * Rarea + b * h * Ud[s]) : (U[s] - F[v] * b * h * Rarea);
       }
+    }
+    for (int64_t i = 0; i < nc; ++i) {
+      force[0] += wf[2 * i];
+      force[1] += wf[2 * i + 1];
     }
   }
   void step() {
@@ -214,18 +287,18 @@ struct RefLoop {
     U.swap(Un);
     prims();
     node_means();
+    t = t + dt;
   }
 };
 
-bool supported(const orc_mesh* m, const orc_params* prm) {
+bool supported(const orc_mesh* m, const orc_params* prm, int32_t viscous) {
   if (prm->flux != ORC_FLUX_AUSM_PLUS) return false;
-  if (prm->energy_law != ORC_ENERGY_CONST) return false;
+  if (viscous && prm->energy_law != ORC_ENERGY_CONST) return false;
   for (int64_t i = 1; i <= m->nCells; ++i) {
-    if (Ix(m->mesh_connectivity, m->nCells, i, 3) != 3) return false;
-    for (int k = 1; k <= 3; ++k) {
-      const int32_t ek = Ix(m->cell_stiffness, m->nCells, i, k);
-      if (ek < 1 || ek > m->nCells) return false;
-    }
+    const int32_t nv = Ix(m->mesh_connectivity, m->nCells, i, 3), type = Ix(m->mesh_connectivity, m->nCells, i, 2);
+    if (!((nv == 3 && type == 3) || (nv == 4 && type == 2))) return false;
+    for (int k = 1; k <= nv; ++k)
+      if (Ix(m->cell_stiffness, m->nCells, i, k) > m->nCells) return false;
   }
   return true;
 }
@@ -235,13 +308,19 @@ bool supported(const orc_mesh* m, const orc_params* prm) {
 extern "C" {
 
 // N steps of the reference loop from U0 (nCells x 4, column-major) at a fixed
-// dt, with the directional derivative dU0 carried along: UN = U_N and dUN =
-// dU_N / dU0 . dU0.  Returns -1 where the mesh or the parameters are outside
-// what it covers (quads, boundary faces, another flux, k(T)).
+// dt from flow time t0, with the directional derivative dU0 carried along:
+// UN = U_N and dUN = dU_N / dU0 . dU0; force / dforce (or NULL): the sum over
+// the N steps of the wall force (x, y) of the faces of code wall_code (0:
+// none) and its derivative.  Returns -1 where the mesh or the parameters are
+// outside what it covers (another flux, k(T), cells other than triangles and
+// quads).
 int orc_ref_loop_tl(const orc_mesh* m, const orc_thermo* th, const orc_params* prm, int32_t viscous, double betta,
-                    double dt, int32_t n_steps, const double* U0, const double* dU0, double* UN, double* dUN) {
-  if (!supported(m, prm)) return -1;
+                    double dt, int32_t n_steps, double t0, int32_t wall_code, const double* U0, const double* dU0,
+                    double* UN, double* dUN, double* force, double* dforce) {
+  if (!supported(m, prm, viscous)) return -1;
   RefLoop<Dual> L{m, th, prm, viscous, betta, dt, m->nCells, m->nNodes, {}, {}, {}, {}, {}};
+  L.t = t0;
+  L.wall_code = wall_code;
   const int64_t nc = m->nCells;
   L.U.resize(4 * nc);
   L.Q.resize(4 * nc);
@@ -255,6 +334,10 @@ int orc_ref_loop_tl(const orc_mesh* m, const orc_thermo* th, const orc_params* p
   for (int64_t s = 0; s < 4 * nc; ++s) {
     UN[s] = L.U[s].v;
     if (dUN) dUN[s] = L.U[s].d;
+  }
+  for (int q = 0; q < 2; ++q) {
+    if (force) force[q] = L.force[q].v;
+    if (dforce) dforce[q] = L.force[q].d;
   }
   return 0;
 }
