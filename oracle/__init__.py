@@ -124,7 +124,7 @@ def lib():
         L.orc_set_threads.restype = None
         L.orc_ref_loop_tl.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcThermo), C.POINTER(OrcParams), C.c_int32,
                                       C.c_double, C.c_double, C.c_int32, C.c_double, C.c_int32, dp, dp, dp, dp, dp,
-                                      dp]
+                                      dp, dp]
         L.orc_ns_dt.restype = C.c_double
         L.orc_ns_dt.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcNsParams), dp, C.c_double]
         _lib = L
@@ -424,10 +424,11 @@ def ref_loop_tl(mv: MeshView, thermo, prm, U0, dU0, dt, n_steps, viscous=True, b
 
 
 def ref_loop_tl_force(mv: MeshView, thermo, prm, U0, dU0, dt, n_steps, wall_code, viscous=True, betta=1.0,
-                      flow_time=0.0):
+                      flow_time=0.0, dbc=None):
     """ref_loop_tl with the wall force of the faces of code wall_code (the
-    AUSM+ momentum flux through them) summed over the N steps: returns (U_N,
-    dU_N, force (2,), dforce (2,))."""
+    AUSM+ momentum flux through them) summed over the N steps, and the
+    tangent dbc ((MAX_BC + 1, 4) by -code, or None) of the fixed states:
+    returns (U_N, dU_N, force (2,), dforce (2,))."""
     n = mv.mesh.nCells
     UN = np.zeros((n, 4), order="F")
     dUN = np.zeros((n, 4), order="F")
@@ -435,9 +436,10 @@ def ref_loop_tl_force(mv: MeshView, thermo, prm, U0, dU0, dt, n_steps, wall_code
     dF = np.zeros(2)
     th = thermo_struct(thermo)
     d = None if dU0 is None else _chk_f64(dU0)
+    b = None if dbc is None else np.ascontiguousarray(dbc, dtype=np.float64).reshape(MAX_BC + 1, 4)
     _rc(lib().orc_ref_loop_tl(C.byref(mv.c), C.byref(th), C.byref(prm), int(bool(viscous)), betta, dt, int(n_steps),
-                              float(flow_time), int(wall_code), _p(_chk_f64(U0)), _p(d), _p(UN), _p(dUN), _p(F),
-                              _p(dF)), "ref_loop_tl")
+                              float(flow_time), int(wall_code), _p(_chk_f64(U0)), _p(d), _p(b), _p(UN), _p(dUN),
+                              _p(F), _p(dF)), "ref_loop_tl")
     return UN, dUN, F, dF
 
 

@@ -21,6 +21,7 @@
 // (each step's AUSM+ momentum flux through them) is carried along.
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "dual.hpp"
@@ -95,6 +96,7 @@ struct RefLoop {
   std::vector<T> U, Q, Qn, Ud, Un;
   double t = 0.0;        // flow time of the step
   int32_t wall_code = 0; // faces whose force is summed (0: none)
+  const double* dbc = nullptr;  // [ORC_MAX_BC + 1][4]: tangent of the fixed states, or NULL
   T force[2] = {T(0.0), T(0.0)};
 
   // updateVariablesSA's primitives, evaluate2d.jl:87-90 (oracle.cpp: orc_update_variables)
@@ -191,7 +193,11 @@ struct RefLoop {
     const double nx = Dx(m->cell_edges_Nx, nc, i, k), ny = Dx(m->cell_edges_Ny, nc, i, k);
     switch (kind) {
       case ORC_BC_FIXED:
-        for (int v = 0; v < 4; ++v) bnd[v] = T(b->data[v]);
+        for (int v = 0; v < 4; ++v) {
+          bnd[v] = T(b->data[v]);
+          if constexpr (std::is_same_v<T, Dual>)
+            if (dbc) bnd[v].d = dbc[4 * idx + v];
+        }
         break;
       case ORC_BC_SLIP_WALL:
         bnd[0] = cur[0];
@@ -308,19 +314,21 @@ bool supported(const orc_mesh* m, const orc_params* prm, int32_t viscous) {
 extern "C" {
 
 // N steps of the reference loop from U0 (nCells x 4, column-major) at a fixed
-// dt from flow time t0, with the directional derivative dU0 carried along:
-// UN = U_N and dUN = dU_N / dU0 . dU0; force / dforce (or NULL): the sum over
-// the N steps of the wall force (x, y) of the faces of code wall_code (0:
-// none) and its derivative.  Returns -1 where the mesh or the parameters are
-// outside what it covers (another flux, k(T), cells other than triangles and
-// quads).
+// dt from flow time t0, with the directional derivative along dU0 and, for
+// the fixed-state codes, along dbc ([ORC_MAX_BC + 1][4] by -code, or NULL)
+// carried along: UN = U_N and dUN its derivative; force / dforce (or NULL):
+// the sum over the N steps of the wall force (x, y) of the faces of code
+// wall_code (0: none) and its derivative.  Returns -1 where the mesh or the
+// parameters are outside what it covers (another flux, k(T), cells other
+// than triangles and quads).
 int orc_ref_loop_tl(const orc_mesh* m, const orc_thermo* th, const orc_params* prm, int32_t viscous, double betta,
                     double dt, int32_t n_steps, double t0, int32_t wall_code, const double* U0, const double* dU0,
-                    double* UN, double* dUN, double* force, double* dforce) {
+                    const double* dbc, double* UN, double* dUN, double* force, double* dforce) {
   if (!supported(m, prm, viscous)) return -1;
   RefLoop<Dual> L{m, th, prm, viscous, betta, dt, m->nCells, m->nNodes, {}, {}, {}, {}, {}};
   L.t = t0;
   L.wall_code = wall_code;
+  L.dbc = dbc;
   const int64_t nc = m->nCells;
   L.U.resize(4 * nc);
   L.Q.resize(4 * nc);

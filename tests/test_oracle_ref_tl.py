@@ -241,3 +241,34 @@ def test_box_derivative_matches_central_differences(quads, tgv, limiter, viscous
         errs.append(abs(tl - fd) / scale)
         print(quads, tgv, limiter, viscous, k, tl, fd, errs[-1], dF)
     assert max(errs) < 1e-7
+
+
+@pytest.mark.parametrize("viscous", [True, False])
+def test_fixed_state_derivative_matches_central_differences(viscous):
+    """The derivative along the fixed inflow state (code -1's rho, u, v, p)
+    against central differences of the loop with that state perturbed, at
+    h = 1e-4 (J is about -36 and its derivative 0.013, so smaller steps
+    drown in rounding: 4e-11 off at 1e-4, 5e-8 at 1e-7)."""
+    M = pkg().mesh
+    mesh, mv, th, U0 = _box("mixed")
+    n = mesh.nCells
+    w = np.asfortranarray(M.uniform01(SEED + 9, 4 * n).reshape(n, 4, order="F") - 0.5)
+    fw = np.array([0.3, 0.8])
+    base = np.array(BOX_BCS[-1][1])
+    db = np.zeros((O.MAX_BC + 1, 4))
+    db[1] = (0.4, -0.6, 0.3, 0.9)
+
+    def J(state):
+        bcs = dict(BOX_BCS)
+        bcs[-1] = (O.BC_FIXED, tuple(state))
+        prm = O.params(limiter=O.LIM_REF_HARMONIC, include_udiff=True, energy_law=O.ENERGY_CONST,
+                       mu_const=1.0 / 1600, k_const=1.0 / 0.72, bcs=bcs)
+        UN, _, F, _ = O.ref_loop_tl_force(mv, th, prm, U0, None, DT, 4, -3, viscous)
+        return float((w * UN).sum()) + float(fw @ F)
+    _, dUN, _, dF = O.ref_loop_tl_force(mv, th, _bprm(), U0, np.zeros_like(U0), DT, 4, -3, viscous, dbc=db)
+    tl = float((w * dUN).sum()) + float(fw @ dF)
+    h = 1e-4
+    fd = (J(base + h * db[1]) - J(base - h * db[1])) / (2 * h)
+    print(viscous, tl, fd, abs(tl - fd) / abs(tl))
+    assert abs(tl) > 0
+    assert abs(tl - fd) <= 1e-7 * abs(tl)
