@@ -1319,7 +1319,7 @@ static int ref_adj_maps(Mesh* M) {
   if (a.built) return GD_OK;
   const MeshView& v = M->v;
   const int64_t nc = v.nc, nn = v.nn, nf = v.nf, nr = v.n_rfaces, nm = v.n_rmirror;
-  std::vector<int32_t> nb, dn1, up1, ea, eb, rfc, rfs, rfnb, rfdn, rfup, cl;
+  std::vector<int32_t> nb, dn1, up1, dn2, up2, ea, eb, rfc, rfs, rfnb, rfdn, rfup, cl, nv;
   std::vector<double> w, fnx, fny, flen, area;
   int rc = 0;
   if ((rc = download(v.nb, nf * nc, nb)) || (rc = download(v.dn1, nf * nc, dn1)) || (rc = download(v.up1, nf * nc, up1)) ||
@@ -1330,16 +1330,31 @@ static int ref_adj_maps(Mesh* M) {
       (rc = download(v.fny, nf * nc, fny)) || (rc = download(v.flen, nf * nc, flen)) ||
       (rc = download(v.area, nc, area)))
     return rc;
+  if (v.has_quads) {
+    if ((rc = download(v.nv, nc, nv)) || (rc = download(v.dn2, nf * nc, dn2)) || (rc = download(v.up2, nf * nc, up2)))
+      return rc;
+  } else {
+    nv.assign((size_t)nc, 3);
+  }
   // every list index below is checked here, so the lists address only rows
   // that exist
   auto in = [](int64_t x, int64_t n) { return x >= 0 && x < n; };
-  a.supported = !v.has_quads && 2 * nr + 1 <= INT32_MAX;  // entry codes 2 f + role are int32
-  for (int64_t q = 0; q < 3 * nc && a.supported; ++q)
-    a.supported = in(nb[(size_t)q], nc) && in(dn1[(size_t)q], nn) && in(up1[(size_t)q], nn) && in(ea[(size_t)q], nn) &&
-                  in(eb[(size_t)q], nn);
+  a.supported = 2 * nr + 1 <= INT32_MAX;  // entry codes 2 f + role are int32
+  for (int64_t i = 0; i < nc && a.supported; ++i) {
+    a.supported = nv[(size_t)i] == 3 || (nv[(size_t)i] == 4 && v.has_quads && nf == 4);
+    for (int64_t k = 0; k < nv[(size_t)i] && a.supported; ++k) {
+      const size_t sl = (size_t)k * nc + i;
+      a.supported = nb[sl] < nc && in(ea[sl], nn) && in(eb[sl], nn);
+      if (a.supported && nb[sl] >= 0) {
+        a.supported = in(dn1[sl], nn) && in(up1[sl], nn);
+        if (a.supported && nv[(size_t)i] == 4) a.supported = in(dn2[sl], nn) && in(up2[sl], nn);
+      }
+    }
+  }
   for (int64_t f = 0; f < nm && a.supported; ++f)
     a.supported = in(rfnb[(size_t)f], nc) && in(rfdn[(size_t)f], nn) && in(rfup[(size_t)f], nn);
-  for (int64_t f = 0; f < nr && a.supported; ++f) a.supported = in(rfc[(size_t)f], nc) && in(rfs[(size_t)f], 3);
+  for (int64_t f = 0; f < nr && a.supported; ++f)
+    a.supported = in(rfc[(size_t)f], nc) && rfs[(size_t)f] >= 0 && rfs[(size_t)f] < nv[(size_t)rfc[(size_t)f]];
   for (size_t q = 0; q < cl.size() && a.supported; ++q) a.supported = cl[q] >= -1 && cl[q] < nc;
   // the Green-Gauss coefficient pair of node J in cell i, in the cluster slot
   // of J that holds i: sum over the cell's edges ending at J of
@@ -1347,7 +1362,7 @@ static int ref_adj_maps(Mesh* M) {
   // what the adjoint covers
   std::vector<double> ccx((size_t)v.ncl * nn, 0.0), ccy((size_t)v.ncl * nn, 0.0);
   for (int64_t i = 0; i < nc && a.supported; ++i)
-    for (int64_t k = 0; k < 3 && a.supported; ++k) {
+    for (int64_t k = 0; k < nv[(size_t)i] && a.supported; ++k) {
       const size_t sl = (size_t)k * nc + i;
       const double fx = 0.5 * -fny[sl] * flen[sl], fy = 0.5 * -fnx[sl] * flen[sl];
       for (const int64_t J : {(int64_t)ea[sl], (int64_t)eb[sl]}) {
@@ -1362,21 +1377,28 @@ static int ref_adj_maps(Mesh* M) {
   if (!a.supported) return GD_OK;
   for (size_t q = 0; q < cl.size(); ++q)
     if (cl[q] >= 0) { ccx[q] /= area[(size_t)cl[q]]; ccy[q] /= area[(size_t)cl[q]]; }
+  // the entries' far nodes (a quad cell's one-sided entry: both far-side
+  // nodes, whose halves the face kernel writes) and cells
   std::vector<std::pair<int64_t, int32_t>> nq, ce;
   nq.reserve((size_t)2 * nr); ce.reserve((size_t)2 * nr);
+  M->radj_code.assign((size_t)(nr - nm), 0);
   for (int64_t f = 0; f < nr; ++f) {
     const int64_t i = rfc[(size_t)f];
-    int64_t j, d, u;
-    if (f < nm) {
-      j = rfnb[(size_t)f]; d = rfdn[(size_t)f]; u = rfup[(size_t)f];
-    } else {
-      const size_t sl = (size_t)rfs[(size_t)f] * nc + i;
-      j = nb[sl]; d = dn1[sl]; u = up1[sl];
-    }
-    nq.push_back({d, (int32_t)(2 * f)});
-    nq.push_back({u, (int32_t)(2 * f + 1)});
     ce.push_back({i, (int32_t)(2 * f)});
-    ce.push_back({j, (int32_t)(2 * f + 1)});
+    if (f < nm) {
+      nq.push_back({rfdn[(size_t)f], (int32_t)(2 * f)});
+      nq.push_back({rfup[(size_t)f], (int32_t)(2 * f + 1)});
+      ce.push_back({rfnb[(size_t)f], (int32_t)(2 * f + 1)});
+      continue;
+    }
+    const size_t sl = (size_t)rfs[(size_t)f] * nc + i;
+    M->radj_code[(size_t)(f - nm)] = nb[sl] < 0 ? nb[sl] + 1 : 0;
+    if (nb[sl] < 0) continue;  // a boundary face reads only its cell
+    nq.push_back({dn1[sl], (int32_t)(2 * f)});
+    if (nv[(size_t)i] == 4) nq.push_back({dn2[sl], (int32_t)(2 * f)});
+    nq.push_back({up1[sl], (int32_t)(2 * f + 1)});
+    if (nv[(size_t)i] == 4) nq.push_back({up2[sl], (int32_t)(2 * f + 1)});
+    ce.push_back({nb[sl], (int32_t)(2 * f + 1)});
   }
   struct Memb { int32_t node; double w; };
   std::vector<std::pair<int64_t, Memb>> cm;
@@ -1414,9 +1436,11 @@ static int ref_adj_maps(Mesh* M) {
   return GD_OK;
 }
 
-// gd_adjoint_run for the reference scheme: J = <dJ/dU_N, U_N> over N steps of
-// the reference loop at a fixed dt (betta 1), from the state's U with the
-// loop's primitives and node primitives recomputed from it.
+// gd_adjoint_run for the reference scheme: J = sum_n w . F_n + <dJ/dU_N, U_N>
+// over N steps of the reference loop at a fixed dt (betta 1), F_n the step's
+// force through the faces of wall_code, from the state's U with the loop's
+// primitives and node primitives recomputed from it; dJ/dU0 and dJ/dbc of the
+// fixed boundary states.
 static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const double* dJ_dUN, double* dJ_dU0,
                            double* dJ_dbc, gd_adjoint_stats* stats) {
   const gd_params& P = S->prm;
@@ -1425,7 +1449,6 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
     return fail(GD_ERR_UNSUPPORTED, "the reference-scheme adjoint covers the odd, symmetric limiters");
   if (P.viscous && P.energy_law != GD_ENERGY_CONST)
     return fail(GD_ERR_UNSUPPORTED, "the reference-scheme adjoint covers constant mu and k");
-  if (ap->wall_code != 0) return fail(GD_ERR_UNSUPPORTED, "the reference-scheme adjoint has no wall-force objective");
   if (ap->n_steps < 1 || ap->checkpoint_every < 1) return fail(GD_ERR_INVALID, "n_steps and checkpoint_every >= 1");
   if (!(ap->dt > 0.0)) return fail(GD_ERR_INVALID, "the adjoint needs a fixed dt > 0");
   GD_MATERIALIZE(S);
@@ -1433,10 +1456,13 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
   if ((rc = ref_adj_maps(S->mesh))) return rc;
   const RefAdjMaps& maps = S->mesh->radj;
   if (!maps.supported)
-    return fail(GD_ERR_UNSUPPORTED, "the reference-scheme adjoint covers triangle meshes without boundary faces");
+    return fail(GD_ERR_UNSUPPORTED,
+                "the reference-scheme adjoint covers cells of 3 or 4 vertices, each in its vertices' clusters");
   const MeshView& v = S->mesh->v;
   const int64_t nc = v.nc, nn = v.nn;
   const bool visc = P.viscous != 0;
+  const bool wall = ap->wall_code != 0 && v.n_wall > 0;
+  const int64_t nparts = (nc + kThreads - 1) / kThreads;
   const int32_t N = ap->n_steps;
   const int32_t c = std::min(ap->checkpoint_every, N);
   const int32_t nseg = (N + c - 1) / c;
@@ -1459,9 +1485,10 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
   // Qn, a scratch U, two adjoint states ([cell][4] records), the entries'
   // adjoints, node- and cell-level adjoints
   const size_t nseg_st = (size_t)(n_ck > 0 ? c : 0), nkeep = (size_t)(N - sK);
+  const size_t n1 = (size_t)std::max<int64_t>(v.n_rfaces - v.n_rmirror, 1);  // one-sided entries
   const size_t parts[] = {(size_t)n_ck * U4, nseg_st * U4, nseg_st * N4, nkeep * U4, nkeep * N4, U4,
                           U4, U4, nr4, nr4, nr4, nr4, N4, 6 * (size_t)nn * sizeof(double),
-                          6 * (size_t)nc * sizeof(double), N4};
+                          6 * (size_t)nc * sizeof(double), N4, 4 * n1 * sizeof(double)};
   constexpr int kParts = (int)(sizeof(parts) / sizeof(parts[0]));
   size_t offs[kParts], need = 0;
   for (int q = 0; q < kParts; ++q) {
@@ -1491,6 +1518,7 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
   double* lamG6N = at(13);
   double* tG = at(14);
   double* lamPhiN = at(15);
+  double* lbc = at(16);  // fixed states' adjoints per one-sided entry, summed over the steps
   double* rflux = static_cast<double*>(S->fflux);
   const int64_t bytes = (int64_t)(parts[0] + parts[1] + parts[2] + parts[3] + parts[4]);
   const size_t q4 = 4 * (size_t)nc, n4 = 4 * (size_t)nn;  // doubles per Q, per Qn
@@ -1502,9 +1530,10 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
     return GD_OK;
   };
   // one forward step as gd_run's reference loop takes it, at the fixed dt:
-  // (U, Q, Qn) of step s -> (Un, Qw, Qnw) of step s + 1
+  // (U, Q, Qn) of step s -> (Un, Qw, Qnw) of step s + 1; the forward sweep's
+  // steps (fwd) also sum the wall force of the objective's faces
   auto fstep = [&](int32_t s, const double* Uo, const double* Qo, const double* Qno, double* Un, double* Qw,
-                   double* Qnw) -> int {
+                   double* Qnw, bool fwd) -> int {
     if (visc) {
       GD_HIP(launch_ref_gg1(v, S->phiN, S->g6, S->stream));
       GD_HIP(launch_ref_gg_nodes(v, S->g6, S->g6N, S->stream));
@@ -1526,13 +1555,27 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
     a.cfl = 0.0;
     a.betta = 1.0;
     a.include_udiff = 1;
-    a.wall_code = 0;
+    a.wall_f = fwd && wall ? S->wall_f : nullptr;
+    a.wall_code = fwd && wall ? ap->wall_code : 0;
     GD_HIP(launch_ref_faces(a, rflux, S->stream));
     a.rflux = rflux;
     GD_HIP(launch_ref_step(a, S->stream));
-    GD_HIP(launch_ref_nodes(v, Qw, Un, Qnw, nullptr, visc ? S->phiN : nullptr, S->stream));
+    FoldArgs fa;  // the step's wall forces summed into drag by the node pass
+    if (fwd && wall) {
+      fa.part = S->part;
+      fa.nparts = nparts;
+      fa.slot = &S->acc[(s + 1) % 3];
+      fa.wall_f = S->wall_f;
+      fa.n_wall = v.n_wall;
+      fa.drag = S->drag;
+    }
+    GD_HIP(launch_ref_nodes(v, Qw, Un, Qnw, nullptr, visc ? S->phiN : nullptr, S->stream, fa));
     return GD_OK;
   };
+  // the flow time of each step, as the forward accumulates it
+  std::vector<double> ts((size_t)N + 1);
+  ts[0] = ap->flow_time;
+  for (int32_t s = 0; s < N; ++s) ts[(size_t)s + 1] = ts[(size_t)s] + ap->dt;
 
   hipEvent_t ev[3];
   for (auto& e : ev) GD_HIP(hipEventCreate(&e));
@@ -1547,6 +1590,10 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
   // and Qn in the state's buffers before sK and where they are kept from sK
   // on; U_N, Q_N and Qn_N end in the state's buffers
   GD_HIP(hipEventRecord(ev[0], S->stream));
+  if (wall) {
+    GD_HIP(launch_fill(S->wall_f, 2 * (int64_t)v.n_wall, 0.0, S->stream));
+    GD_HIP(launch_fill(S->drag, 4, 0.0, S->stream));
+  }
   double* Ucur = S->U[S->cur];
   double* Qcur = sK == 0 ? keepQ : S->Q[S->qc];
   double* Qncur = sK == 0 ? keepQn : S->Qn;
@@ -1564,7 +1611,7 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
       Qnext = Qcur == S->Q[0] ? S->Q[1] : S->Q[0];
       Qnnext = S->Qn;  // read by the step, then rewritten by its node pass
     }
-    if ((rc = fstep(s, Ucur, Qcur, Qncur, Unext, Qnext, Qnnext))) return rc;
+    if ((rc = fstep(s, Ucur, Qcur, Qncur, Unext, Qnext, Qnnext, true))) return rc;
     Ucur = Unext;
     Qcur = Qnext;
     Qncur = Qnnext;
@@ -1572,7 +1619,12 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
   S->cur = Ucur == S->U[0] ? 0 : 1;
   S->qc = Qcur == S->Q[0] ? 0 : 1;
   GD_HIP(hipEventRecord(ev[1], S->stream));
-  double J = 0.0;
+  double drag[4] = {0, 0, 0, 0};
+  if (wall) {
+    GD_HIP(hipMemcpyAsync(drag, S->drag, sizeof(drag), hipMemcpyDeviceToHost, S->stream));
+    GD_HIP(hipStreamSynchronize(S->stream));
+  }
+  double J = ap->force_weight[0] * drag[2] + ap->force_weight[1] * drag[3];
   if (dJ_dUN) {
     std::vector<double> UN((size_t)4 * nc);
     if ((rc = get_cols(S, S->U[S->cur], UN.data(), 4, false))) return rc;
@@ -1588,6 +1640,7 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
   } else {
     GD_HIP(launch_fill(lam[0], 4 * nc, 0.0, S->stream));
   }
+  GD_HIP(launch_fill(lbc, 4 * (int64_t)n1, 0.0, S->stream));
   int32_t recomputed = 0;
   for (int32_t g = nseg - 1; g >= 0; --g) {
     const int32_t s0 = g * c, s1 = std::min(N, s0 + c);
@@ -1601,9 +1654,11 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
       double* Ub = S->U[1 - S->cur];
       GD_HIP(launch_copy(Ua, ck + (size_t)g * q4, 4 * nc, S->stream));
       if ((rc = start(Ua, segQ, segQn))) return rc;
+      // the step's flow time, which the boundary ghosts read, as the forward had it
+      GD_HIP(launch_fill(&S->acc[s0 % 3].time, 1, ts[(size_t)s0], S->stream));
       for (int32_t s = s0; s < s1 - 1; ++s) {
         const size_t k = (size_t)(s - s0);
-        if ((rc = fstep(s, Ua, segQ + k * q4, segQn + k * n4, Ub, segQ + (k + 1) * q4, segQn + (k + 1) * n4)))
+        if ((rc = fstep(s, Ua, segQ + k * q4, segQn + k * n4, Ub, segQ + (k + 1) * q4, segQn + (k + 1) * n4, false)))
           return rc;
         std::swap(Ua, Ub);
         ++recomputed;
@@ -1612,7 +1667,8 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
     for (int32_t s = s1 - 1; s >= s0; --s) {
       const double* Qs = Qb + (size_t)(s - s0) * q4;
       const double* Qns = Qnb + (size_t)(s - s0) * n4;
-      GD_HIP(launch_radj_face(v, P, Qs, Qns, lam[li], bdt, lqo, lqn, ldn, lup, S->stream));
+      GD_HIP(launch_radj_face(v, P, Qs, Qns, lam[li], bdt, ts[(size_t)s], wall ? ap->wall_code : 0, ap->force_weight,
+                              lqo, lqn, ldn, lup, lbc, S->stream));
       GD_HIP(launch_radj_nodes(v, maps, ldn, lup, lam[li], bdt, mu, visc, lamQn, lamG6N, S->stream));
       if (visc) {
         GD_HIP(launch_radj_g6(v, maps, lamG6N, tG, S->stream));
@@ -1626,8 +1682,23 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
   GD_HIP(hipEventRecord(ev[2], S->stream));
   GD_HIP(launch_radj_from_rec(nc, lam[li], S->grad0, S->stream));
   if (dJ_dU0 && (rc = get_cols(S, S->grad0, dJ_dU0, 4, false))) return rc;
-  if (dJ_dbc)
+  if (dJ_dbc) {  // the fixed states' adjoints, summed per code in entry order
     for (int q = 0; q < 4 * (GD_MAX_BC + 1); ++q) dJ_dbc[q] = 0.0;
+    const std::vector<int32_t>& code = S->mesh->radj_code;
+    bool any = false;
+    for (const int32_t c0 : code)
+      if (c0 < 0 && -c0 <= GD_MAX_BC && P.bc[-c0].kind == GD_BC_FIXED) any = true;
+    if (any) {
+      std::vector<double> h(4 * code.size());
+      GD_HIP(hipMemcpyAsync(h.data(), lbc, h.size() * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+      GD_HIP(hipStreamSynchronize(S->stream));
+      for (size_t e = 0; e < code.size(); ++e) {
+        const int32_t c0 = code[e];
+        if (c0 < 0 && -c0 <= GD_MAX_BC && P.bc[-c0].kind == GD_BC_FIXED)
+          for (int q = 0; q < 4; ++q) dJ_dbc[4 * (-c0) + q] += h[4 * e + q];
+      }
+    }
+  }
   // the state holds U_N, Q_N and Qn_N as after gd_run (the cluster means,
   // which recomputed segments overwrite, are gd_run's first pass), the
   // bookkeeping pending
@@ -1641,7 +1712,8 @@ static int ref_adjoint_run(gd_state* S, const gd_adjoint_params* ap, const doubl
     (void)hipEventElapsedTime(&f_ms, ev[0], ev[1]);
     (void)hipEventElapsedTime(&r_ms, ev[1], ev[2]);
     stats->J = J;
-    stats->force_sum[0] = stats->force_sum[1] = 0.0;
+    stats->force_sum[0] = drag[2];
+    stats->force_sum[1] = drag[3];
     stats->recomputed_steps = recomputed;
     stats->n_checkpoints = nseg;
     stats->storage_bytes = bytes;

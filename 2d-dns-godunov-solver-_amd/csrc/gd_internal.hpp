@@ -117,7 +117,7 @@ struct MeshView {
 // fixed order, so every transposed sum is deterministic).
 struct RefAdjMaps {
   bool built = false;
-  bool supported = false;      // triangles only, every face interior
+  bool supported = false;      // every index in range
   const int64_t* nq_off;       // per node: the face-list entries that read it as a far node
   const int32_t* nq_ref;       //   2 f + (1 for the neighbour side's far node "up", 0 for "down")
   const double* ccx;           // per cluster slot [ncl][nn] (cell cl[q nn + J]): the Green-Gauss
@@ -132,6 +132,7 @@ struct RefAdjMaps {
 struct Mesh {
   MeshView v{};
   RefAdjMaps radj{};
+  std::vector<int32_t> radj_code;  // per one-sided face-list entry: its boundary code (< 0), else 0
   std::vector<void*> allocs;
   std::vector<int64_t> cperm, ciperm;  // cells new->old, old->new
   std::vector<int64_t> nperm, niperm;  // nodes new->old, old->new
@@ -347,8 +348,8 @@ hipError_t launch_ns_update_f32(const NsStepArgs& a, hipStream_t s);
 //   cell:  per cell, the entries' cell parts, the node means and cluster means
 //          transposed, the primitives' Jacobian at U_s and the identity -> lamU of U_s
 hipError_t launch_radj_face(const MeshView& m, const gd_params& p, const double* Q, const double* Qn,
-                            const double* lam, double bdt, double* lqo, double* lqn, double* ldn, double* lup,
-                            hipStream_t s);
+                            const double* lam, double bdt, double t, int32_t wall_code, const double fw[2],
+                            double* lqo, double* lqn, double* ldn, double* lup, double* lbc, hipStream_t s);
 hipError_t launch_radj_nodes(const MeshView& m, const RefAdjMaps& a, const double* ldn, const double* lup,
                              const double* lam, double bdt, const double mu[3], bool visc, double* lamQn,
                              double* lamG6N, hipStream_t s);

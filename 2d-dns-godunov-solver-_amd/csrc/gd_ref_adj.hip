@@ -4,7 +4,8 @@
 // with the face fluxes from vertex MUSCL + limiter + AUSM+ of the primitives Q
 // = P(U) and their node means Qn = N(Q), and Udiff = mu Laplacian of the
 // cluster means of U through two Green-Gauss levels (calcDiffTerm with
-// constant mu and k), on triangle meshes whose faces are all interior.
+// constant mu and k), on triangles and quads with the boundary ghosts, and
+// the wall force of one boundary code summed over the steps.
 //
 // The nonlinear part is per face-list entry (the face pass's list: a mirrored
 // face once from its owner, every other face slot from its cell): the entry's
@@ -150,27 +151,70 @@ __device__ __forceinline__ void ausm_plus_adj(const double* L, const double* R, 
   gR[0] = rRb; gR[1] = uRb; gR[2] = vRb; gR[3] = pRb;
 }
 
+// The Jacobian dg/dq (row-major 4 x 4) of refm::bc_ghost for a boundary
+// code: zero for a fixed state and the TGV mirror (functions of the geometry
+// and the time only), the identity for zero gradient, the reflection of the
+// slip wall, and the isothermal wall's (T_g = 2 T_w - T_c unless floored at
+// T_w / 2, rho_g = p / (R T_g)).
+__device__ __forceinline__ void ghost_jac(const gd_bc* table, int code, const double* q, double nx, double ny,
+                                          double R, double* G) {
+  const int idx = -code;
+  const gd_bc b = (idx >= 0 && idx <= GD_MAX_BC) ? table[idx] : gd_bc{GD_BC_EXTRAPOLATE, 0, {0, 0, 0, 0}};
+#pragma unroll
+  for (int t = 0; t < 16; ++t) G[t] = 0.0;
+  if (b.kind == GD_BC_FIXED || b.kind == GD_BC_TGV_MIRROR) return;
+  if (b.kind == GD_BC_SLIP_WALL) {
+    // g1 = q1 (ny^2 - nx^2) - 2 nx ny q2,  g2 = q2 (nx^2 - ny^2) - 2 nx ny q1
+    const double a = ny * ny - nx * nx, c = 2.0 * nx * ny;
+    G[0] = 1.0; G[5] = a; G[6] = -c; G[9] = -c; G[10] = nx * nx - ny * ny; G[15] = 1.0;
+    return;
+  }
+  if (b.kind == GD_BC_ISOTHERMAL_WALL) {
+    const double Tw = b.data[0];
+    const double Tc = q[3] / (q[0] * R);
+    double Tg = 2.0 * Tw - Tc;
+    const bool floored = Tg < 0.5 * Tw;
+    if (floored) Tg = 0.5 * Tw;
+    const double dT0 = floored ? 0.0 : Tc / q[0];          // dTg/drho
+    const double dT3 = floored ? 0.0 : -1.0 / (q[0] * R);  // dTg/dp
+    const double g0 = q[3] / (R * Tg);
+    G[0] = -g0 / Tg * dT0;
+    G[3] = 1.0 / (R * Tg) - g0 / Tg * dT3;
+    G[5] = -1.0; G[10] = -1.0; G[15] = 1.0;
+    return;
+  }
+  G[0] = 1.0; G[5] = 1.0; G[10] = 1.0; G[15] = 1.0;
+}
+
 // One face-list entry f: the flux e = AUSM+(fR, fL) enters U' of its cell i
 // with -b dt / A_i and, for a mirrored face, of the neighbour j with
-// +b dt / A_j; its inputs are qL = Q_i, qR = Q_j and the far nodes' dn, up:
+// +b dt / A_j; its inputs are qL = Q_i, qR = Q_j and the far nodes' dn, up
+// (a quad's: the mean of its two far-side nodes):
 //   fL = qL + 0.5 lim(qL - dn, qR - qL),  fR = qR - 0.5 lim(qR - qL, up - qR).
-// Writes the adjoints of qL, qR, dn and up to four [entry][4] arrays.
+// At a boundary face qR is the ghost of qL at the step's flow time t, dn = qL
+// and up = qR, so everything enters through qL; a face of code wall_code adds
+// (fw0, fw1) to the adjoints of its momentum fluxes (the wall force's share
+// of the objective).  Writes the adjoints of qL, qR, dn and up to four
+// [entry][4] arrays (a quad's dn and up halved, for each of its two nodes),
+// and adds a fixed boundary state's adjoint to the entry's record of lbc
+// ([one-sided entry][4], or nullptr).
 __global__ __launch_bounds__(kB) void k_radj_face(MeshView m, gd_params p, const double* Q, const double* Qn,
-                                                   const double* lam, double bdt, double* lqo, double* lqn,
-                                                   double* ldn, double* lup) {
+                                                   const double* lam, double bdt, double t, int32_t wall_code,
+                                                   double fw0, double fw1, double* lqo, double* lqn, double* ldn,
+                                                   double* lup, double* lbc) {
   const int64_t f = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
   if (f >= m.n_rfaces) return;
   const int64_t nc = m.nc, nn = m.nn;
   const int64_t i = m.rf_cell[f];
   const bool mir = f < m.n_rmirror;
-  int64_t j;
+  int64_t j, s = -1;
   int32_t d1, u1;
   double nx, ny, len;
   if (mir) {
     j = m.rf_nb[f]; d1 = m.rf_dn1[f]; u1 = m.rf_up1[f];
     nx = m.rf_nx[f]; ny = m.rf_ny[f]; len = m.rf_len[f];
   } else {
-    const int64_t s = (int64_t)m.rf_slot[f] * nc + i;
+    s = (int64_t)m.rf_slot[f] * nc + i;
     j = m.nb[s]; d1 = m.dn1[s]; u1 = m.up1[s];
     nx = m.fnx[s]; ny = m.fny[s]; len = m.flen[s];
   }
@@ -186,13 +230,72 @@ __global__ __launch_bounds__(kB) void k_radj_face(MeshView m, gd_params p, const
 #pragma unroll
     for (int v = 0; v < 4; ++v) w[v] = w[v] + cj * lj[v];
   }
+  const double eps = p.limiter_eps;
   double qL[4], qR[4], dn[4], up[4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    qL[v] = Q[v * nc + i]; qR[v] = Q[v * nc + j];
-    dn[v] = Qn[v * nn + d1]; up[v] = Qn[v * nn + u1];
+  for (int v = 0; v < 4; ++v) qL[v] = Q[v * nc + i];
+  if (j < 0) {
+    // a boundary face: first order, everything through qL
+    if (wall_code != 0 && j + 1 == wall_code && m.wall_id[s] >= 0) {
+      w[1] = w[1] + fw0;
+      w[2] = w[2] + fw1;
+    }
+    const int32_t na = m.ea[s], nb2 = m.eb[s];
+    refm::bc_ghost(p.bc, (int)j + 1, qL, nx, ny, m.xc[i], m.yc[i], m.xn[na], m.yn[na], m.xn[nb2], m.yn[nb2], t,
+                   p.RGAS, qR);
+    double G[16];
+    ghost_jac(p.bc, (int)j + 1, qL, nx, ny, p.RGAS, G);
+    double fL[4], fR[4], l1y[4], l2x[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double a = qL[v] - qL[v], b = qR[v] - qL[v], c = qR[v] - qR[v];
+      const AD2 l1 = refm::limiter(p.limiter, AD2(a, 1.0, 0.0), AD2(b, 0.0, 1.0), eps);
+      const AD2 l2 = refm::limiter(p.limiter, AD2(b, 1.0, 0.0), AD2(c, 0.0, 1.0), eps);
+      fL[v] = qL[v] + 0.5 * l1.v;
+      fR[v] = qR[v] - 0.5 * l2.v;
+      l1y[v] = l1.dy;
+      l2x[v] = l2.dx;
+    }
+    double gfR[4], gfL[4];
+    ausm_plus_adj(fR, fL, nx, ny, len, p.Gamma, w, gfR, gfL);
+    // dfL_v/dqL_u = d_vu + 0.5 l1y_v (G_vu - d_vu),  dfR_v/dqL_u = G_vu - 0.5 l2x_v (G_vu - d_vu)
+    double o[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double dvu = v == u ? 1.0 : 0.0, gm = G[4 * v + u] - dvu;
+        o[u] = o[u] + gfL[v] * (dvu + 0.5 * l1y[v] * gm) + gfR[v] * (G[4 * v + u] - 0.5 * l2x[v] * gm);
+      }
+    st2<2>(lqo + 4 * f, o);
+    // a fixed state's adjoint (qR = data, G = 0), summed over the steps per
+    // one-sided entry (each entry's own record: no two threads share one)
+    const int bidx = -((int)j + 1);
+    if (lbc && bidx >= 0 && bidx <= GD_MAX_BC && p.bc[bidx].kind == GD_BC_FIXED) {
+      double acc[4];
+      double* r = lbc + 4 * (f - m.n_rmirror);
+      ld2<2>(r, acc);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] = acc[v] + (gfL[v] * (0.5 * l1y[v]) + gfR[v] * (1.0 - 0.5 * l2x[v]));
+      st2<2>(r, acc);
+    }
+    return;
   }
-  const double eps = p.limiter_eps;
+  // a quad's one-sided entry reads the mean of its two far-side nodes
+  const bool quad = !mir && m.has_quads && m.nv[i] == 4;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) qR[v] = Q[v * nc + j];
+  if (quad) {
+    const int32_t d2 = m.dn2[s], u2 = m.up2[s];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      up[v] = 0.5 * (Qn[v * nn + u1] + Qn[v * nn + u2]);
+      dn[v] = 0.5 * (Qn[v * nn + d1] + Qn[v * nn + d2]);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { dn[v] = Qn[v * nn + d1]; up[v] = Qn[v * nn + u1]; }
+  }
   double fL[4], fR[4], dLqL[4], dLqR[4], dLdn[4], dRqR[4], dRqL[4], dRup[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -218,11 +321,12 @@ __global__ __launch_bounds__(kB) void k_radj_face(MeshView m, gd_params p, const
 #pragma unroll
   for (int v = 0; v < 4; ++v) o[v] = gfL[v] * dLqR[v] + gfR[v] * dRqR[v];
   st2<2>(lqn + 4 * f, o);
+  const double hq = quad ? 0.5 : 1.0;
 #pragma unroll
-  for (int v = 0; v < 4; ++v) o[v] = gfL[v] * dLdn[v];
+  for (int v = 0; v < 4; ++v) o[v] = hq * (gfL[v] * dLdn[v]);
   st2<2>(ldn + 4 * f, o);
 #pragma unroll
-  for (int v = 0; v < 4; ++v) o[v] = gfR[v] * dRup[v];
+  for (int v = 0; v < 4; ++v) o[v] = hq * (gfR[v] * dRup[v]);
   st2<2>(lup + 4 * f, o);
 }
 
@@ -432,10 +536,11 @@ unsigned nb(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
 }  // namespace
 
 hipError_t launch_radj_face(const MeshView& m, const gd_params& p, const double* Q, const double* Qn,
-                            const double* lam, double bdt, double* lqo, double* lqn, double* ldn, double* lup,
-                            hipStream_t s) {
+                            const double* lam, double bdt, double t, int32_t wall_code, const double fw[2],
+                            double* lqo, double* lqn, double* ldn, double* lup, double* lbc, hipStream_t s) {
   if (m.n_rfaces == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_radj_face, dim3(nb(m.n_rfaces)), dim3(kB), 0, s, m, p, Q, Qn, lam, bdt, lqo, lqn, ldn, lup);
+  hipLaunchKernelGGL(k_radj_face, dim3(nb(m.n_rfaces)), dim3(kB), 0, s, m, p, Q, Qn, lam, bdt, t, wall_code, fw[0],
+                     fw[1], lqo, lqn, ldn, lup, lbc);
   return hipGetLastError();
 }
 

@@ -1,9 +1,11 @@
 """Discrete adjoint of the reference scheme's loop (gd_adjoint_run with the
 reference scheme) on the GPU.
 
-J = <w, U_N> after N fixed-dt steps of the reference loop (calcDiffTerm with
-constant mu and k, the fast MUSCL stage with AUSM+, updateVariablesSA, the
-node interpolation) on the periodic Taylor-Green mesh.  For any direction d,
+J = <w, U_N> (+ the summed wall force of one boundary code) after N fixed-dt
+steps of the reference loop (calcDiffTerm with constant mu and k, the fast
+MUSCL stage with AUSM+, updateVariablesSA, the node interpolation) on the
+periodic Taylor-Green mesh and on boxes of triangles and quads with every
+boundary ghost.  For any direction d,
 <dJ/dU0, d> must equal the derivative of J along d that the oracle's
 tangent-linear reference loop carries in dual numbers (itself pinned to the
 oracle loop bit for bit and to central differences): FP64, 1e-9 relative.
@@ -125,7 +127,6 @@ def test_ref_adjoint_leaves_the_loop_state():
 
 def test_ref_adjoint_refuses_what_it_does_not_cover():
     L = pkg()._lib
-    M = pkg().mesh
     mesh, _, _, U = _case(20)
     for kw, msg in ((dict(limiter=O.LIM_VAN_LEER_A), "limiters"), (dict(limiter=0, flux=L.GD_FLUX_ROE), "AUSM"),
                     (dict(limiter=0, energy=L.GD_ENERGY_AIR_POLY), "constant mu")):
@@ -134,13 +135,6 @@ def test_ref_adjoint_refuses_what_it_does_not_cover():
         with pytest.raises(L.GodunovError, match=msg):
             s.adjoint(2, DT)
         s.close()
-    box = M.make_box_mesh(8, 6, 2.0, 1.0, quads="none")
-    f = tgv_fields(box)
-    Ub = O.phs2dcns(f, 1.4, box.nCells)
-    s = _solver(box, O.LIM_REF_HARMONIC, True, Ub)
-    with pytest.raises(L.GodunovError, match="boundary faces"):
-        s.adjoint(2, DT)
-    s.close()
 
 
 def test_ref_adjoint_full_size_against_central_differences():
@@ -189,3 +183,78 @@ def test_ref_adjoint_full_size_against_central_differences():
         print("direction", k, "adjoint", adj, "central difference", fd, "rel", errs[-1])
     s.close()
     assert max(errs) <= 1e-4
+
+
+# boxes of triangles and quads with a fixed inflow (-1), zero gradient (-2), a
+# slip wall (-3) and an isothermal wall (-4); a TGV box with mirror ghosts
+BOX = {-1: ("FIXED", (1.3, 0.7, -0.2, 1.9)), -2: ("EXTRAPOLATE", ()), -3: ("SLIP_WALL", ()),
+       -4: ("ISOTHERMAL_WALL", (1.1,))}
+TGV_BOX = {c: ("TGV_MIRROR", (71.43, 1.0, 1.0, 0.05)) for c in (-1, -2, -3, -4)}
+
+
+def _box(quads, tgv):
+    from helpers import random_fields
+    M = pkg().mesh
+    if tgv:
+        mesh = M.make_box_mesh(40, 34, 6.283185307179586, 6.283185307179586, quads=quads)
+        f = tgv_fields(mesh, noise=1e-2)
+        r = M.uniform01(SEED + 5, 3 * mesh.nCells).reshape(3, -1) - 0.5
+        f.UxCells[:] += 1e-2 * r[0]
+        f.UyCells[:] += 1e-2 * r[1]
+        f.pressureCells[:] *= 1.0 + 1e-2 * r[2]
+    else:
+        mesh = M.make_box_mesh(40, 26, 2.0, 1.0, quads=quads)
+        f = random_fields(mesh, seed=SEED + 6)
+    th = M.THERMOPHYSICS(RGAS=1.0, Gamma=1.4)
+    return mesh, O.MeshView(mesh), th, O.phs2dcns(f, th.Gamma, mesh.nCells)
+
+
+@pytest.mark.parametrize("viscous", [True, False])
+@pytest.mark.parametrize("quads,tgv,lim", [("mixed", False, "harmonic"), ("all", False, "vanAlbada"),
+                                           ("none", False, "minmod"), ("mixed", True, "harmonic")])
+def test_ref_adjoint_boxes_against_tangent_linear_oracle(quads, tgv, lim, viscous):
+    """<dJ/dU0, d> + <dJ/dbc, dbc> against the tangent-linear loop's
+    derivative along (d, dbc), J = <w, U_N> + 0.7 Fx - 0.4 Fy over 5 steps
+    from flow time 0.1 (F the force through the slip wall, or through code -2
+    of the TGV box), checkpoints every 2."""
+    S = pkg().stage
+    L = pkg()._lib
+    M = pkg().mesh
+    mesh, mv, th, U = _box(quads, tgv)
+    n = mesh.nCells
+    bcs = TGV_BOX if tgv else BOX
+    wc = -2 if tgv else -3
+    N, dt, t0 = 5, 1e-3, 0.1
+    fw = (0.7, -0.4)
+    p = S.make_params(scheme=L.GD_SCHEME_REF, limiter=LIMS[lim], viscous=viscous, energy_law=L.GD_ENERGY_CONST,
+                      mu_const=MU, k_const=KC, RGAS=1.0, Gamma=1.4,
+                      bcs={c: (getattr(L, "GD_BC_" + k), d) for c, (k, d) in bcs.items()})
+    w = np.asfortranarray(M.normal(SEED + 7, 4 * n).reshape(n, 4, order="F"))
+    s = S.GodunovSolver(mesh, p, order=L.GD_ORDER_HILBERT)
+    s.state.set_array(L.GD_ARR_U_OLD, U)
+    s.flow_time = t0
+    g0, gbc, st = s.adjoint(N, dt, checkpoint_every=2, wall_code=wc, force_weight=fw, dJ_dUN=w)
+    s.close()
+    oprm = O.params(limiter=LIMS[lim], include_udiff=True, energy_law=O.ENERGY_CONST, mu_const=MU, k_const=KC,
+                    bcs={c: (getattr(O, "BC_" + k), d) for c, (k, d) in bcs.items()})
+    errs = []
+    for k in range(2):
+        d = np.asfortranarray(M.normal(SEED + 30 + k, 4 * n).reshape(n, 4, order="F") * np.abs(U).max(axis=0))
+        db = np.zeros((O.MAX_BC + 1, 4))
+        if not tgv:
+            db[1] = M.normal(SEED + 40 + k, 4)
+        UN, dUN, F, dF = O.ref_loop_tl_force(mv, th, oprm, U, d, dt, N, wc, viscous, flow_time=t0, dbc=db)
+        tl = float((w * dUN).sum()) + fw[0] * dF[0] + fw[1] * dF[1]
+        adj = float((g0 * d).sum()) + float((gbc * db).sum())
+        errs.append(abs(adj - tl) / abs(tl))
+        print(quads, tgv, lim, viscous, "adjoint", adj, "tangent", tl, "rel", errs[-1])
+    J = float((w * UN).sum()) + fw[0] * F[0] + fw[1] * F[1]
+    print("J", st.J, J, "force", st.force_sum[0], st.force_sum[1], F)
+    assert max(errs) <= 1e-9
+    assert abs(st.J - J) <= 1e-11 * max(abs(J), float(np.abs(w * UN).sum()))
+    assert np.allclose([st.force_sum[0], st.force_sum[1]], F, rtol=1e-11, atol=1e-13 * np.abs(F).max())
+    if tgv:
+        assert not gbc.any()
+    else:
+        assert np.abs(gbc[1]).max() > 0 and not gbc[2:].any()
+
