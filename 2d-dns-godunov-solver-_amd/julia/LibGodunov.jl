@@ -410,9 +410,10 @@ end
 """drag_gradient(r, N, dt; every, wall, w, dJdUN, budget): the discrete adjoint
 (gd_adjoint_run) of J = sum over N fixed-dt steps of w . F_wall (+ <dJdUN, U_N>)
 from the run's current state, which ends at U_N.  Returns (dJ/dU0 (n_cells,
-4), dJ/dbc (4, 17): column 1 - code for a fixed-state code, stats).  A
-reference-scheme run takes wall = 0 (J = <dJdUN, U_N>; periodic triangle
-meshes, AUSM+, constant mu and k)."""
+4), dJ/dbc (4, 17): column 1 - code for a fixed-state code, stats).  On a
+reference-scheme run the wall force is the AUSM+ momentum flux through the
+faces of code `wall` (AUSM+ with an odd, symmetric limiter, constant mu and
+k)."""
 function drag_gradient(r::GodunovRun, N::Integer, dt::Real; every = 20, wall = -3, w = (1.0, 0.0),
                        dJdUN = nothing, budget = 0)
     n = r.state.mesh.n_cells

@@ -401,8 +401,8 @@ class GodunovSolver:
         """Discrete adjoint of ``n_steps`` fixed-dt steps of the solver's
         scheme from the current state (which ends at U_N, as after ``run``)
         for J = sum_n w . F_n + <dJ_dUN, U_N>, F_n the wall force of step n on
-        ``wall_code`` (the reference scheme: periodic triangle meshes, AUSM+,
-        constant mu and k, no wall term).  ``record_budget`` (bytes) keeps the cell records of as
+        ``wall_code`` (the reference scheme: AUSM+ with an odd, symmetric
+        limiter, constant mu and k).  ``record_budget`` (bytes) keeps the cell records of as
         many trailing checkpoint segments as fit instead of recomputing them.
         ``to_host=False`` leaves dJ/dU0 on the device (gd_state_device_adjoint).
         Returns (dJ/dU0 (n_cells, 4) or None, dJ/dbc (MAX_BC + 1, 4),

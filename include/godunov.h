@@ -237,10 +237,10 @@ int gd_run(gd_state* st, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats
    faces of code wall_code, at a fixed dt.  With the reference scheme the step
    is gd_run's reference loop at betta 1 (calcDiffTerm, the fast MUSCL stage,
    updateVariablesSA, the node interpolation), started from the primitives of
-   the state's U and their node means; it covers AUSM+ with the harmonic, van
-   Albada or minmod limiter, constant mu and k, triangle meshes whose faces are
-   all interior (periodic), and wall_code 0 (J = <dJ_dUN, U_N>); anything else
-   returns GD_ERR_UNSUPPORTED.  Runs N steps forward from the
+   the state's U and their node means, F_n the AUSM+ momentum flux through the
+   faces of wall_code; it covers AUSM+ with the harmonic, van Albada or minmod
+   limiter, constant mu and k (or inviscid), triangles and quads, and every
+   boundary code; anything else returns GD_ERR_UNSUPPORTED.  Runs N steps forward from the
    state's U (which ends at U_N, as after gd_run), keeping U every
    checkpoint_every steps and recomputing the steps between checkpoints on the
    way back (unless record_budget holds their records).  dJ_dU0 (NULL: kept
