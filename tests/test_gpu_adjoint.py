@@ -153,9 +153,9 @@ def test_adjoint_mixed_close_to_fp64(flux):
     assert abs(a - b) <= 1e-4 * abs(a)
 
 
-def test_adjoint_ref_scheme_on_a_box_mesh_is_refused():
-    """The reference scheme's adjoint covers periodic triangle meshes with
-    constant mu and k; a box with walls and k(T) is refused, not approximated."""
+def test_adjoint_ref_scheme_with_air_conductivity_is_refused():
+    """The reference scheme's adjoint covers constant mu and k; the viscous
+    term with k(T) of air is refused, not approximated."""
     S = pkg().stage
     L = pkg()._lib
     mesh, U = _box()
