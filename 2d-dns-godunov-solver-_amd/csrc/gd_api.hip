@@ -688,10 +688,17 @@ int gd_state_create(gd_mesh* m, const gd_params* p, gd_state** out) {
   S->Qt[0] = qt0;
   S->Qt[1] = qt1;
   rc |= dev_alloc(al, &S->wall_f, 2 * (size_t)std::max<int32_t>(m->v.n_wall, 1));
-  rc |= dev_alloc(al, &S->drag, 4);
-  rc |= dev_alloc(al, &S->acc, 3);
+  rc |= dev_alloc(al, &S->hdr, 1);
   rc |= dev_alloc(al, &S->part, 3 * ((nc + kThreads - 1) / kThreads) + 3);
-  rc |= dev_alloc(al, &S->red, 8 + 4 * ((nc + kThreads - 1) / kThreads) + 8);
+  rc |= dev_alloc(al, &S->red, 4 * ((nc + kThreads - 1) / kThreads) + 4);
+  if (!rc) {
+    S->acc = S->hdr->acc;
+    S->drag = S->hdr->drag;
+    if (hipHostMalloc((void**)&S->hdr_host, sizeof(RunHdr), hipHostMallocDefault) != hipSuccess) {
+      S->hdr_host = nullptr;
+      rc = fail(GD_ERR_NOMEM, "hipHostMalloc failed");
+    }
+  }
   rc |= dev_alloc(al, &S->xfer, 4 * (size_t)std::max(nc, nn));
   if (rc) {
     std::string msg = g_err;
@@ -711,7 +718,7 @@ int gd_state_create(gd_mesh* m, const gd_params* p, gd_state** out) {
   (void)hipMemsetAsync(S->F, 0, 4 * nc * sizeof(double), S->stream);
   (void)hipMemsetAsync(S->Dl, 0, 4 * nc * sizeof(double), S->stream);
   (void)hipMemsetAsync(S->wall_f, 0, 2 * std::max<int32_t>(m->v.n_wall, 1) * sizeof(double), S->stream);
-  (void)hipMemsetAsync(S->drag, 0, 4 * sizeof(double), S->stream);
+  (void)hipMemsetAsync(S->hdr, 0, sizeof(RunHdr), S->stream);
   e = hipStreamSynchronize(S->stream);
   if (e != hipSuccess) { gd_state_destroy(S); return fail(GD_ERR_HIP, "state init failed"); }
   *out = S;
@@ -724,6 +731,7 @@ int gd_state_destroy(gd_state* S) {
   for (hipEvent_t ev : S->ev_pool) (void)hipEventDestroy(ev);
   for (void* p : S->allocs) (void)hipFree(p);
   if (S->adj_ws) (void)hipFree(S->adj_ws);
+  if (S->hdr_host) (void)hipHostFree(S->hdr_host);
   if (S->stream) (void)hipStreamDestroy(S->stream);
   delete S;
   return GD_OK;
@@ -775,6 +783,7 @@ static int get_cols(gd_state* S, const double* dev, double* host, int ncols, boo
 // of U (evaluate2d.jl:87-93).  Every entry point that reads or writes those
 // arrays calls this first; the time loop itself never needs them.
 static int materialize(gd_state* S) {
+  S->ns_primed = false;  // every entry point but gd_run comes here first
   if (!S->pend_delta && !S->pend_derived) return GD_OK;
   const MeshView& v = S->mesh->v;
   if (S->pend_delta) {
@@ -867,6 +876,8 @@ int gd_state_get_array(gd_state* S, int32_t which, double* host) {
 
 int gd_state_device_u(gd_state* S, double** dev) {
   if (!S || !dev) return fail(GD_ERR_INVALID, "null argument");
+  S->u_exposed = true;  // the caller may write U: every later run starts from it afresh
+  S->ns_primed = false;
   *dev = S->U[S->cur];
   return GD_OK;
 }
@@ -1144,15 +1155,19 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
   const int64_t nparts = (v.nc + kThreads - 1) / kThreads;
   // a run that steps supersedes the previous run's pending Delta / U_NEW
   if (n_steps > 0) S->pend_delta = false;
-  if (wall) {
-    GD_HIP(launch_fill(S->wall_f, 2 * (int64_t)v.n_wall, 0.0, S->stream));
-    GD_HIP(launch_fill(S->drag, 4, 0.0, S->stream));
-  }
+  // a run that follows an ns run on the same state, with nothing between
+  // them, continues from where that one ended (see State::ns_primed)
+  const bool warm = ns && S->ns_primed && !S->u_exposed;
+  S->ns_primed = false;
+  int tq = warm ? S->ns_tq : 0;
+  GD_HIP(launch_run_begin(S->acc, rp->flow_time, wall ? S->wall_f : nullptr, v.n_wall, S->drag, S->stream));
   if (ns) {
     // the ns scheme advances U: its compute-type primitives and the first dt
-    // inputs come from U in one pass (folded by the first gradient launch)
-    GD_HIP(launch_ns_prime(v, P, S->U[S->cur], S->Qt[0], mixed, S->acc, S->part, rp->flow_time, P.viscous,
-                           S->stream));
+    // inputs come from U in one pass (folded by the first gradient launch),
+    // unless the last update left both
+    if (!warm)
+      GD_HIP(launch_ns_prime(v, P, S->U[S->cur], S->Qt[0], mixed, S->acc, S->part, rp->flow_time, P.viscous,
+                             S->stream, false));
     if (n_steps == 0) GD_HIP(launch_acc_fold(S->part, nparts, &S->acc[0], S->stream));
   } else {
     GD_HIP(launch_fields_derived(v.nc, P.Gamma, S->Q[S->qc], S->aS, S->vmax, S->stream));
@@ -1223,7 +1238,6 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
     auto grad = mixed ? launch_ns_grad_f32 : launch_ns_grad_f64;
     auto face = mixed ? launch_ns_face_f32 : launch_ns_face_f64;
     auto update = mixed ? launch_ns_update_f32 : launch_ns_update_f64;
-    int tq = 0;
     for (int32_t s = 0; s < n_steps; ++s) {
       NsStepArgs a{};
       a.m = v;
@@ -1252,25 +1266,32 @@ int gd_run(gd_state* S, int32_t n_steps, gd_run_params* rp, gd_run_stats* stats)
       S->cur ^= 1;
       tq ^= 1;
     }
-    if (n_steps > 0) {  // the last step's reductions (earlier ones ride on the next gradient launch)
-      GD_HIP(launch_acc_fold(S->part, nparts, &S->acc[n_steps % 3], S->stream));
-      if (wall) GD_HIP(launch_wall_sum(S->wall_f, v.n_wall, S->drag, S->stream));
-    }
+    if (n_steps > 0)  // the last step's reductions (earlier ones ride on the next gradient launch)
+      GD_HIP(launch_ns_close(S->part, nparts, &S->acc[n_steps % 3], wall ? S->wall_f : nullptr, v.n_wall, S->drag,
+                             S->stream));
     S->pend_derived = true;  // fields2d's FP64 primitives of U, when someone asks
   }
   // the reference's bookkeeping after the step (Delta of the last step,
   // U_NEW == U_OLD) is left pending for the caller that reads it
   if (n_steps > 0) {
-    if (rp->residuals) GD_HIP(launch_residuals(v, S->U[S->cur], S->U[1 - S->cur], S->red, S->stream));
+    if (rp->residuals)
+      GD_HIP(launch_residuals(v, S->U[S->cur], S->U[1 - S->cur], S->red, S->hdr->red, S->stream));
     S->pend_delta = true;
   }
-  Acc acc[3];
-  GD_HIP(hipMemcpyAsync(acc, S->acc, sizeof(acc), hipMemcpyDeviceToHost, S->stream));
-  double red[4] = {0, 0, 0, 0}, drag[4] = {0, 0, 0, 0};
-  if (rp->residuals && n_steps > 0) GD_HIP(hipMemcpyAsync(red, S->red, sizeof(red), hipMemcpyDeviceToHost, S->stream));
-  if (wall) GD_HIP(hipMemcpyAsync(drag, S->drag, sizeof(drag), hipMemcpyDeviceToHost, S->stream));
+  // one copy brings back the accumulators, the wall force and the residuals
+  GD_HIP(hipMemcpyAsync(S->hdr_host, S->hdr, sizeof(RunHdr), hipMemcpyDeviceToHost, S->stream));
   GD_HIP(hipStreamSynchronize(S->stream));
-  const Acc& last = acc[n_steps % 3];
+  if (ns) {
+    S->ns_primed = true;
+    S->ns_tq = tq;
+  }
+  const RunHdr& H = *S->hdr_host;
+  double red[4] = {0, 0, 0, 0}, drag[4] = {0, 0, 0, 0};
+  if (rp->residuals && n_steps > 0)
+    for (int q = 0; q < 4; ++q) red[q] = H.red[q];
+  if (wall)
+    for (int q = 0; q < 4; ++q) drag[q] = H.drag[q];
+  const Acc& last = H.acc[n_steps % 3];
   rp->flow_time = last.time;
   if (stats) {
     stats->flow_time = last.time;

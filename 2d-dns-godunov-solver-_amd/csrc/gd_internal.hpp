@@ -160,6 +160,15 @@ struct Acc {
   double force[2];
 };
 
+// What a run reads back at its end, in one device block so that one copy into
+// pinned host memory brings it all: the accumulator slots, the wall force
+// (last step x, y; running sums x, y) and the residual norms.
+struct RunHdr {
+  Acc acc[3];
+  double drag[4];
+  double red[4];
+};
+
 struct State {
   Mesh* mesh = nullptr;
   gd_params prm{};
@@ -182,10 +191,12 @@ struct State {
   void* fflux = nullptr;                // ns: [n_faces][4] face fluxes, compute type
   void* Qt[2] = {nullptr, nullptr};     // ns: primitives [nc][4], compute type
   double* wall_f = nullptr;             // [n_wall][2]
-  double* drag = nullptr;               // [4]: last step force x,y; running sums
-  Acc* acc = nullptr;                   // [3]
+  RunHdr* hdr = nullptr;                // device: acc, drag, residual norms
+  RunHdr* hdr_host = nullptr;           // pinned host copy of hdr
+  double* drag = nullptr;               // hdr->drag: last step force x,y; running sums
+  Acc* acc = nullptr;                   // hdr->acc [3]
   double* part = nullptr;               // [3 x cell blocks] partial reductions
-  double* red = nullptr;                // reduction scratch
+  double* red = nullptr;                // reduction scratch (per-block residual partials)
   double* xfer = nullptr;               // [4][max(nc, nn)] staging of host transfers
   double* grad0 = nullptr;              // [4][nc] dJ/dU0 of the last adjoint run (allocated by the first)
   double* fstate = nullptr;             // [2][4 slots][4][nc] two-phase face states (allocated by the first request)
@@ -195,6 +206,14 @@ struct State {
   // bookkeeping a run leaves to whoever reads it (done by the next call that does):
   bool pend_delta = false;    // Dl = U[cur] - U[1-cur] of the last step, then U[1-cur] = U[cur]
   bool pend_derived = false;  // Q[qc], aS, vmax from U[cur] (updateVariablesSA)
+  // The ns loop may start where the last one ended: Qt[ns_tq] holds the
+  // compute-type primitives of U[cur] and part the dt partials of U[cur], as
+  // the last k_ns_update wrote them.  Set by an ns gd_run, cleared by every
+  // other entry point on the state (materialize) and never trusted once
+  // gd_state_device_u has handed U to the caller, who may write it.
+  bool ns_primed = false;
+  int ns_tq = 0;
+  bool u_exposed = false;
   std::vector<void*> allocs;
 };
 
@@ -249,11 +268,18 @@ hipError_t launch_ref_cuda_stage(const MeshView& m, double gamma, bool x32, cons
                                  const double* Uo, double betta, double dt, double* F, double* Un, hipStream_t s);
 hipError_t launch_ref_nodes(const MeshView& m, const double* Q, const double* U, double* Qn, double* Un,
                             double* phiN, hipStream_t s, const FoldArgs& fold = FoldArgs{});
-// start of the ns loop: resets the accumulators to flow_time, writes the
-// compute-type primitives of U into Qt and the first step's dt partials into
-// part (folded by the first gradient launch)
+// start of the ns loop: resets the accumulators to flow_time (unless
+// reset_acc is false), writes the compute-type primitives of U into Qt and the
+// first step's dt partials into part (folded by the first gradient launch)
 hipError_t launch_ns_prime(const MeshView& m, const gd_params& prm, const double* U, void* Qt, bool f32, Acc* acc,
-                           double* part, double flow_time, int32_t viscous_ns, hipStream_t s);
+                           double* part, double flow_time, int32_t viscous_ns, hipStream_t s, bool reset_acc = true);
+// start of a run, one block: the accumulator slots reset to flow_time, the
+// wall-face forces (2 n_wall values, when wall_f is given) and drag zeroed
+hipError_t launch_run_begin(Acc* acc, double flow_time, double* wall_f, int32_t n_wall, double* drag, hipStream_t s);
+// end of the ns loop, one launch: the last update's dt partials into slot and,
+// when wall_f is given, the last face pass's wall forces into drag
+hipError_t launch_ns_close(const double* part, int64_t nparts, Acc* slot, const double* wall_f, int32_t n_wall,
+                           double* drag, hipStream_t s);
 hipError_t launch_ref_phi(const MeshView& m, const double* U, double* phiN, hipStream_t s);
 hipError_t launch_ref_gg1(const MeshView& m, const double* phiN, double* g6, hipStream_t s);
 hipError_t launch_ref_gg_nodes(const MeshView& m, const double* g6, double* g6N, hipStream_t s);
@@ -264,7 +290,9 @@ hipError_t launch_update_variables(const MeshView& m, double gamma, const double
 hipError_t launch_prime_acc(const MeshView& m, const gd_params& prm, const double* Q, const double* vmax, Acc* acc,
                             double* part, double flow_time, int32_t dt_policy, int32_t viscous_ns, hipStream_t s);
 hipError_t launch_acc_fold(const double* part, int64_t nparts, Acc* slot, hipStream_t s);
-hipError_t launch_residuals(const MeshView& m, const double* Unew, const double* Uold, double* red, hipStream_t s);
+// sums of (Unew - Uold)^2 per variable into out[0..3], through per-block partials in scratch
+hipError_t launch_residuals(const MeshView& m, const double* Unew, const double* Uold, double* scratch, double* out,
+                            hipStream_t s);
 hipError_t launch_wall_sum(const double* wall_f, int32_t n, double* out4, hipStream_t s);
 hipError_t launch_fields_derived(int64_t nc, double gamma, const double* Q, double* aS, double* vmax, hipStream_t s);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);

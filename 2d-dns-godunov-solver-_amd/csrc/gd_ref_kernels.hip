@@ -772,6 +772,32 @@ __global__ __launch_bounds__(kBlock) void k_wall_sum(const double* wf, int32_t n
   wall_sum_block(wf, n, out);
 }
 
+// start of a run: the three accumulator slots at flow_time, no wall force yet
+__global__ __launch_bounds__(kBlock) void k_run_begin(Acc* acc, double t, double* wall_f, int32_t n_wall,
+                                                      double* drag) {
+  if (threadIdx.x < 3) {
+    Acc* z = &acc[threadIdx.x];
+    z->vmax_bits = 0ull;
+    z->hdt_bits = 0x7ff0000000000000ull;
+    z->nonphysical = 0u;
+    z->time = t;
+    z->dt = 0.0;
+    z->force[0] = z->force[1] = 0.0;
+  }
+  if (threadIdx.x < 4) drag[threadIdx.x] = 0.0;
+  if (wall_f)
+    for (int64_t k = threadIdx.x; k < 2 * (int64_t)n_wall; k += kBlock) wall_f[k] = 0.0;
+}
+
+// end of the ns loop: block 0 folds the last update's dt partials, block 1
+// sums the last face pass's wall forces (the same block size as the gradient
+// launch's fold blocks, so the sums are the same as theirs)
+__global__ __launch_bounds__(kBlock) void k_ns_close(const double* part, int64_t nparts, Acc* slot,
+                                                     const double* wall_f, int32_t n_wall, double* drag) {
+  if (blockIdx.x == 0) fold_acc_block(part, nparts, slot);
+  else wall_sum_block(wall_f, n_wall, drag);
+}
+
 // aSoundCells and VMAXCells from the primitives (initfields_TGV2d.jl:47-48,
 // evaluate2d.jl:92-93)
 __global__ __launch_bounds__(kBlock) void k_fields_derived(int64_t nc, double G, const double* Q, double* aS,
@@ -938,8 +964,8 @@ hipError_t launch_ref_phi(const MeshView& m, const double* U, double* phiN, hipS
 }
 
 hipError_t launch_ns_prime(const MeshView& m, const gd_params& prm, const double* U, void* Qt, bool f32, Acc* acc,
-                           double* part, double flow_time, int32_t viscous_ns, hipStream_t s) {
-  hipLaunchKernelGGL(k_acc_reset, dim3(1), dim3(64), 0, s, acc, flow_time);
+                           double* part, double flow_time, int32_t viscous_ns, hipStream_t s, bool reset_acc) {
+  if (reset_acc) hipLaunchKernelGGL(k_acc_reset, dim3(1), dim3(64), 0, s, acc, flow_time);
   if (f32)
     hipLaunchKernelGGL(k_ns_prime<float>, dim3(nblocks(m.nc)), dim3(kBlock), 0, s, m, prm, U, static_cast<float*>(Qt),
                        part, viscous_ns);
@@ -990,10 +1016,22 @@ hipError_t launch_acc_fold(const double* part, int64_t nparts, Acc* slot, hipStr
   return hipGetLastError();
 }
 
-hipError_t launch_residuals(const MeshView& m, const double* Unew, const double* Uold, double* red, hipStream_t s) {
+hipError_t launch_residuals(const MeshView& m, const double* Unew, const double* Uold, double* scratch, double* out,
+                            hipStream_t s) {
   const unsigned nb = nblocks(m.nc);
-  hipLaunchKernelGGL(k_resid_partial, dim3(nb), dim3(kBlock), 0, s, m.nc, Unew, Uold, red + 8);
-  hipLaunchKernelGGL(k_fold, dim3(1), dim3(kBlock), 0, s, red + 8, (int64_t)nb, 4, red);
+  hipLaunchKernelGGL(k_resid_partial, dim3(nb), dim3(kBlock), 0, s, m.nc, Unew, Uold, scratch);
+  hipLaunchKernelGGL(k_fold, dim3(1), dim3(kBlock), 0, s, scratch, (int64_t)nb, 4, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_run_begin(Acc* acc, double flow_time, double* wall_f, int32_t n_wall, double* drag, hipStream_t s) {
+  hipLaunchKernelGGL(k_run_begin, dim3(1), dim3(kBlock), 0, s, acc, flow_time, wall_f, n_wall, drag);
+  return hipGetLastError();
+}
+
+hipError_t launch_ns_close(const double* part, int64_t nparts, Acc* slot, const double* wall_f, int32_t n_wall,
+                           double* drag, hipStream_t s) {
+  hipLaunchKernelGGL(k_ns_close, dim3(wall_f ? 2 : 1), dim3(kBlock), 0, s, part, nparts, slot, wall_f, n_wall, drag);
   return hipGetLastError();
 }
 

@@ -282,7 +282,9 @@ int gd_adjoint_run(gd_state* st, const gd_adjoint_params* ap, const double* dJ_d
 int gd_state_device_adjoint(gd_state* st, double** dev_grad_soa);
 int gd_sync(gd_state* st);
 
-/* device pointers for zero-copy callers; valid until the next gd_run/step */
+/* device pointers for zero-copy callers; valid until the next gd_run/step.
+   The caller may read and write U through it: once it has been handed out,
+   every later gd_run on the state starts from U as it then is. */
 int gd_state_device_u(gd_state* st, double** dev_u_soa /* 4 x n_cells, device order */);
 
 int gd_device_count(int32_t* n);

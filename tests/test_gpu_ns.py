@@ -251,3 +251,75 @@ def test_mesh_cell_order_reports_the_device_layout():
         assert np.array_equal(np.sort(got), np.arange(n))
         if want is not None:
             assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_ns_run_continues_where_the_last_ended(mixed):
+    """A run that follows another on the same state starts from the primitives
+    and dt inputs the last update left (no pass over U): run(3), run(0),
+    run(4) ends bit for bit where run(7) does, wall force and flow time
+    included, in FP64 and FP32-mixed.  Any other entry point in between makes
+    the next run start from U afresh: after the state is read and written back
+    the FP64 run is again bit-identical; after the caller writes U through
+    gd_state_device_u, the run starts from what was written."""
+    import ctypes as C
+    S = pkg().stage
+    L = pkg()._lib
+    mesh = _box("mixed", 30, 20)
+    U = _box_state(mesh)
+    prm = _dev_params(mixed=mixed)
+
+    def solver():
+        s = S.GodunovSolver(mesh, prm, order=L.GD_ORDER_HILBERT)
+        s.state.set_array(L.GD_ARR_U_OLD, U)
+        return s
+
+    kw = dict(cfl=0.4, dt_policy=L.GD_DT_LOCAL, wall_code=-3)
+    a = solver()
+    sa = a.run(7, **kw)
+    ua = a.conservative()
+    a.close()
+    b = solver()
+    b.run(3, **kw)
+    b.run(0, **kw)
+    sb = b.run(4, **kw)
+    ub = b.conservative()
+    b.close()
+    assert np.array_equal(ua, ub)
+    assert (sa.flow_time, sa.last_dt) == (sb.flow_time, sb.last_dt)
+    assert tuple(sa.wall_force) == tuple(sb.wall_force) and sa.wall_force[0] != 0.0
+    # the state read back and written again between the runs
+    c = solver()
+    c.run(3, **kw)
+    c.state.set_array(L.GD_ARR_U_OLD, c.conservative())
+    c.run(4, **kw)
+    uc = c.conservative()
+    c.close()
+    if mixed:
+        assert rel_linf(uc, ua) <= 1e-5
+    else:
+        assert np.array_equal(uc, ua)
+    # the caller writes U on the device between two runs (Hilbert order on
+    # the device: the caller's rows permuted by gd_mesh_cell_order)
+    d = solver()
+    d.run(3, **kw)
+    U3 = d.conservative()
+    W = np.asfortranarray(U3 * np.array([1.01, 1.0, 1.0, 1.02]))
+    lib = L.load()
+    Wd = np.asfortranarray(W[d.dmesh.cell_order()])
+    ptr = C.POINTER(C.c_double)()
+    assert lib.gd_state_device_u(d.state.handle, C.byref(ptr)) == 0
+    rt = C.CDLL("libamdhip64.so.7")
+    rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    d.state.sync()
+    assert rt.hipMemcpy(C.cast(ptr, C.c_void_p), Wd.ctypes.data, Wd.nbytes, 1) == 0  # host to device
+    d.run(4, **kw)
+    ud = d.conservative()
+    d.close()
+    e = S.GodunovSolver(mesh, prm, order=L.GD_ORDER_HILBERT)
+    e.state.set_array(L.GD_ARR_U_OLD, W)
+    e.flow_time = sb.flow_time  # not read by these boundaries; kept alike anyway
+    e.run(4, **kw)
+    ue = e.conservative()
+    e.close()
+    assert np.array_equal(ud, ue)
