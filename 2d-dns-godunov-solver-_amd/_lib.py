@@ -37,6 +37,7 @@ EXPORTS = (
     "gd_state_get_array", "gd_calc_one_stage", "gd_calc_one_stage_cuda", "gd_calc_face_states", "gd_calc_diff_term", "gd_update_variables", "gd_cells2nodes",
     "gd_run", "gd_sync", "gd_state_device_u", "gd_device_count", "gd_last_error", "gd_version",
     "gd_adjoint_run", "gd_state_device_adjoint", "gd_diff_term_scratch_bytes", "gd_calc_diff_term_noalloc",
+    "gd_hbm_stream_rates",
 )
 
 
@@ -138,6 +139,7 @@ def load():
     L.gd_diff_term_scratch_bytes.restype = C.c_size_t
     L.gd_diff_term_scratch_bytes.argtypes = [C.c_int64, C.c_int64]
     L.gd_calc_diff_term_noalloc.argtypes = [vp, C.POINTER(gd_params), vp, vp, vp, vp]
+    L.gd_hbm_stream_rates.argtypes = [C.c_int64, C.c_int32, dp, dp]
     L.gd_device_count.argtypes = [C.POINTER(C.c_int32)]
     L.gd_last_error.argtypes = [C.c_char_p, C.c_size_t]
     L.gd_version.restype = C.c_char_p
@@ -165,6 +167,14 @@ def device_count() -> int:
     n = C.c_int32(0)
     load().gd_device_count(C.byref(n))
     return int(n.value)
+
+
+def hbm_stream_rates(nbytes: int, iters: int = 20):
+    """(copy, read) GB/s of plain 16 B-vector streaming kernels on the current
+    device; copy counts bytes read plus bytes written."""
+    cp, rd = C.c_double(0.0), C.c_double(0.0)
+    check(load().gd_hbm_stream_rates(int(nbytes), int(iters), C.byref(cp), C.byref(rd)))
+    return cp.value, rd.value
 
 
 def ptr(a, ct=C.c_double):

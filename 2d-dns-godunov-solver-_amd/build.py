@@ -27,6 +27,7 @@ UNITS = {
     # forward kernels' to the bit, so every branch is taken the same way
     "gd_ref_adj.hip": ["-ffp-contract=off"],
     "gd_noalloc.hip": [],
+    "gd_stream.hip": [],
     "gd_ns_fp64.hip": ["-ffp-contract=off"],
     # FP32-mixed: hardware reciprocal / square root for float division and
     # sqrt; float denormals flushed, which drops the range-scaling steps from

@@ -449,6 +449,15 @@ def main():
         d["step_hbm"] = step_frac(r)
         out[var.replace("-", "_")] = d
     out["drag_fp64"] = r64["drag"]
+    # what plain streaming kernels reach on this device (a copy and a read of
+    # a 1 GiB buffer), beside the dominant kernel's and the step's rates
+    cp, rd = L.hbm_stream_rates(1 << 30, 20)
+    out["hbm_stream"] = {"copy_GBs": round(cp, 1), "read_GBs": round(rd, 1), "bytes": 1 << 30,
+                         "dominant_kernel_frac_of_copy": round(out["roofline"]["achieved"] / cp, 4) if cp else None,
+                         "step_frac_of_copy": round(out["step_hbm"]["achieved_GBs"] / cp, 4) if cp else None}
+    log(f"HBM streaming on this device: copy {cp:.0f} GB/s, read {rd:.0f} GB/s (1 GiB, 16 B vectors); "
+        f"{dom['name']} at {out['roofline']['achieved'] / cp:.3f} of the copy rate, "
+        f"the {head} step at {out['step_hbm']['achieved_GBs'] / cp:.3f}")
     if a.adjoint_steps > 0:
         out["adjoint"] = adjoint_cost(mesh, case, dev_order, U0, a.adjoint_steps, a.adjoint_checkpoint, a.cfl,
                                       L.GD_FP64 if a.adjoint_precision == "fp64" else L.GD_FP32_MIXED,

@@ -287,6 +287,15 @@ int gd_sync(gd_state* st);
    every later gd_run on the state starts from U as it then is. */
 int gd_state_device_u(gd_state* st, double** dev_u_soa /* 4 x n_cells, device order */);
 
+/* Measurement only, no counterpart in the reference: the current device's
+   HBM streaming rates, for reading the step's bandwidth against what a kernel
+   that only streams reaches.  A plain copy of `bytes` (a multiple of 16, at
+   least 16 KiB) between two buffers this call allocates and frees, and a plain
+   read of one; best of cached and non-temporal 16 B vector accesses, `iters`
+   timed launches each after one warm launch.  copy_gbs counts bytes read plus
+   bytes written. */
+int gd_hbm_stream_rates(int64_t bytes, int32_t iters, double* copy_gbs, double* read_gbs);
+
 int gd_device_count(int32_t* n);
 int gd_last_error(char* buf, size_t n);
 const char* gd_version(void);
