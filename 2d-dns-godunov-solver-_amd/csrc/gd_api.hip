@@ -653,6 +653,9 @@ int gd_state_create(gd_mesh* m, const gd_params* p, gd_state** out) {
   *out = nullptr;
   int rc = check_params(m, p);
   if (rc) return rc;
+  int dev = -1;
+  (void)hipGetDevice(&dev);
+  if (dev != m->device) return fail(GD_ERR_INVALID, "the current device is not the mesh's device");
   gd_state* S = new gd_state();
   S->mesh = m;
   S->prm = *p;
