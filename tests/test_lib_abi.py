@@ -77,3 +77,13 @@ def test_ctypes_layouts_match_the_c_compiler():
         assert ctypes.sizeof(st) == c[(st.__name__, "sizeof")], st.__name__
         for f in st._fields_:
             assert getattr(st, f[0]).offset == c[(st.__name__, f[0])], (st.__name__, f[0])
+
+
+def test_stream_rates_reject_bad_sizes_before_any_device_call():
+    L = pkg()._lib
+    lib = L.load()
+    cp, rd = ctypes.c_double(-1.0), ctypes.c_double(-1.0)
+    for nbytes, iters in [(1000, 1), (16 * 1024 + 8, 1), (1 << 20, 0)]:
+        assert lib.gd_hbm_stream_rates(nbytes, iters, ctypes.byref(cp), ctypes.byref(rd)) == L.GD_ERR_INVALID
+    assert lib.gd_hbm_stream_rates(1 << 20, 1, None, ctypes.byref(rd)) == L.GD_ERR_INVALID
+    assert cp.value == -1.0 and rd.value == -1.0
