@@ -574,7 +574,14 @@ __global__ __launch_bounds__(kBlock) void k_ref_nodes(MeshView m, const double* 
     r = r / det; u = u / det; v = v / det; p = p / det;
     s0 = s0 / det; s1 = s1 / det; s2 = s2 / det; s3 = s3 / det;
   }
-  if (QN) { Qn[J] = r; Qn[nn + J] = u; Qn[2 * nn + J] = v; Qn[3 * nn + J] = p; }
+  // the node primitives are read three launches later, by the face pass, after
+  // the viscous passes have moved several hundred MB: stored non-temporally
+  if (QN) {
+    __builtin_nontemporal_store(r, &Qn[J]);
+    __builtin_nontemporal_store(u, &Qn[nn + J]);
+    __builtin_nontemporal_store(v, &Qn[2 * nn + J]);
+    __builtin_nontemporal_store(p, &Qn[3 * nn + J]);
+  }
   if (UN) { Un[J] = s0; Un[nn + J] = s1; Un[2 * nn + J] = s2; Un[3 * nn + J] = s3; }
   if (PHI) {
     const double dc = (double)cnt;
