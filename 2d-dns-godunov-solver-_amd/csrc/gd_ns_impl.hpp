@@ -321,8 +321,8 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // Cache beside the step's other arrays it is read from there (2.08M cells,
 // FP32: 133 MB, cached stores are faster); larger, its dirty lines are
 // written back while the face pass runs and compete with its reads (FP64:
-// 266 MB, streamed stores take the face pass from 120.7 to 106.6 us and the
-// step from 0.2934 to 0.2745 ms).
+// 266 MB, streamed stores take the face pass from 123.9 to 106.8 us and the
+// step from 0.2857 to 0.2753 ms).
 constexpr int64_t kRecStreamBytes = int64_t(192) << 20;
 
 // blocks 0 and 1 of a folding launch close the previous step's reductions
