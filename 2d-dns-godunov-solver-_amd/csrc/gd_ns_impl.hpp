@@ -628,6 +628,7 @@ __global__ __launch_bounds__(kBlock) void k_ns_update(NsKArgs<T> a) {
   }
   double vmx = 0.0, hdt = __longlong_as_double(0x7ff0000000000000ll);
   unsigned bad = 0;
+  double qw[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < nc) {
     const int nf = QUADS ? m.nv[i] : 3;
     int32_t fc[NF];
@@ -667,7 +668,8 @@ __global__ __launch_bounds__(kBlock) void k_ns_update(NsKArgs<T> a) {
     const double Gm = P.Gamma;
     const double rho = Un[0], ux = Un[1] / rho, uy = Un[2] / rho;
     const double p = (Gm - 1.0) * (Un[3] - 0.5 * (Un[1] * Un[1] + Un[2] * Un[2]) / rho);
-    st4(a.Qw + 4 * i, (T)rho, (T)ux, (T)uy, (T)p);
+    if constexpr (sizeof(T) == 4) st4(a.Qw + 4 * i, (T)rho, (T)ux, (T)uy, (T)p);
+    else { qw[0] = rho; qw[1] = ux; qw[2] = uy; qw[3] = p; }
     const double as = sqrt(Gm * p / rho);
     const double vm = sqrt(ux * ux + uy * uy) + as;
     double lam = vm;
@@ -683,6 +685,23 @@ __global__ __launch_bounds__(kBlock) void k_ns_update(NsKArgs<T> a) {
     if (vm != vm || lam != lam) { vmx = 0.0; hdt = __longlong_as_double(0x7ff0000000000000ll); }
   }
   // block reduction of the next step's dt inputs into per-block partials
+  // FP64: the block's next primitives (32 B per cell) pass through LDS so
+  // that each store instruction writes 4 KB of whole lines rather than half
+  // of 64 lines (update 74.1 -> 70.8 us)
+  if constexpr (sizeof(T) == 8) {
+    __shared__ __align__(16) double sQ[kBlock * 4];
+    sQ[4 * threadIdx.x] = qw[0]; sQ[4 * threadIdx.x + 1] = qw[1];
+    sQ[4 * threadIdx.x + 2] = qw[2]; sQ[4 * threadIdx.x + 3] = qw[3];
+    __syncthreads();
+    const int64_t base = i - threadIdx.x;
+    const u32x4* src = reinterpret_cast<const u32x4*>(sQ);
+    u32x4* dst = reinterpret_cast<u32x4*>(a.Qw + 4 * base);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int p = threadIdx.x + k * kBlock;
+      if (base + p / 2 < nc) dst[p] = src[p];
+    }
+  }
   __shared__ double s_max[kBlock / 64], s_min[kBlock / 64];
   __shared__ unsigned s_bad[kBlock / 64];
   vmx = wmax64(vmx);
