@@ -30,6 +30,39 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
   return x * q + (x < r ? x : r) + k;
 }
 
+typedef unsigned int stage_u32x4 __attribute__((ext_vector_type(4)));
+// One output record of K 16 B pieces per lane, stored as the wave's contiguous
+// range through a per-wave LDS slab, so that each store instruction writes
+// 1 KB of whole lines.  item is the lane's record index, and a wave's lanes
+// hold consecutive items (block base + threadIdx.x).  Every lane of the wave
+// takes part; records at or past n (lanes standing in past the end) are not
+// stored.  A lane's own 32 or 48 B record stored directly splits lines between
+// store instructions; staged, the reference loop's 2M TGV step's face,
+// first-gradient and node-gradient passes go 72.2 -> 69.2, 68.5 -> 58.6 and
+// 42.1 -> 38.7 us (step 0.3501 -> 0.3350 ms), and its adjoint's reverse sweep
+// of 100 steps 65.7 -> 64.0 ms, same bits.
+template <int K>
+struct WaveStage {
+  stage_u32x4 slab[kThreads / 64][64 * K];
+  __device__ __forceinline__ void store(double* base, int64_t item, int64_t n, const double* o) {
+    typedef double f64x2 __attribute__((ext_vector_type(2)));
+    const int lane = threadIdx.x & 63;
+    stage_u32x4* S = slab[threadIdx.x >> 6];
+#pragma unroll
+    for (int q = 0; q < K; ++q) S[lane * K + q] = __builtin_bit_cast(stage_u32x4, f64x2{o[2 * q], o[2 * q + 1]});
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int64_t w0 = item - lane;
+    stage_u32x4* d = reinterpret_cast<stage_u32x4*>(base) + w0 * K;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int idx = q * 64 + lane;
+      if (w0 + idx / K < n) d[idx] = S[idx];
+    }
+  }
+};
+
 // Everything a kernel needs about the mesh, passed by value as a kernel argument.
 struct MeshView {
   int64_t nc, nn;

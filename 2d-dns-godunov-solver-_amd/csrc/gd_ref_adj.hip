@@ -347,13 +347,14 @@ template <int NCL>
 __global__ __launch_bounds__(kB) void k_radj_nodes(MeshView m, RefAdjMaps a, const double* ldn, const double* lup,
                                                     const double* lam, double bdt, double mu0, double mu1, double mu2,
                                                     int visc, double* lamQn, double* lamG6N) {
-  const int64_t J = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
-  if (J >= m.nn) return;
+  const int64_t J0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+  const int64_t J = J0 < m.nn ? J0 : m.nn - 1;
   const int64_t nn = m.nn;
   double q[4] = {0.0, 0.0, 0.0, 0.0};
   row_sum4(a.nq_ref, a.nq_off[J], a.nq_off[J + 1],
            [&](int32_t code) { return (code & 1 ? lup : ldn) + 4 * (int64_t)(code >> 1); }, q);
-  st2<2>(lamQn + 4 * J, q);
+  __shared__ WaveStage<2> sq;
+  sq.store(lamQn, J0, nn, q);
   if (!visc) return;
   const double mu[3] = {mu0 * bdt, mu1 * bdt, mu2 * bdt};
   double g[6] = {0, 0, 0, 0, 0, 0};
@@ -392,13 +393,14 @@ __global__ __launch_bounds__(kB) void k_radj_nodes(MeshView m, RefAdjMaps a, con
   const double ic = cnt ? 1.0 / (double)cnt : 0.0;
 #pragma unroll
   for (int t = 0; t < 6; ++t) g[t] = g[t] * ic;
-  st2<3>(lamG6N + 6 * J, g);
+  __shared__ WaveStage<3> sg;
+  sg.store(lamG6N, J0, nn, g);
 }
 
 // Per cell: the sum over the clusters it belongs to -> tG [nc][6]
 __global__ __launch_bounds__(kB) void k_radj_g6(MeshView m, RefAdjMaps a, const double* lamG6N, double* tG) {
-  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
-  if (i >= m.nc) return;
+  const int64_t i0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+  const int64_t i = i0 < m.nc ? i0 : m.nc - 1;
   double g[6] = {0, 0, 0, 0, 0, 0};
   const int64_t r1 = a.cm_off[i + 1];
   for (int64_t r = a.cm_off[i]; r < r1; r += 4) {
@@ -416,15 +418,16 @@ __global__ __launch_bounds__(kB) void k_radj_g6(MeshView m, RefAdjMaps a, const 
         for (int t = 0; t < 6; ++t) g[t] = g[t] + x[u][t];
       }
   }
-  st2<3>(tG + 6 * i, g);
+  __shared__ WaveStage<3> sg;
+  sg.store(tG, i0, m.nc, g);
 }
 
 // Per node: the first Green-Gauss level transposed over the cluster, divided
 // by the cluster size -> lamPhiN [nn][4]
 template <int NCL>
 __global__ __launch_bounds__(kB) void k_radj_phi(MeshView m, RefAdjMaps a, const double* tG, double* lamPhiN) {
-  const int64_t J = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
-  if (J >= m.nn) return;
+  const int64_t J0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+  const int64_t J = J0 < m.nn ? J0 : m.nn - 1;
   const int64_t nn = m.nn;
   double ph[4] = {0, 0, 0, 0};
   int cnt = 0;
@@ -458,7 +461,8 @@ __global__ __launch_bounds__(kB) void k_radj_phi(MeshView m, RefAdjMaps a, const
   const double ic = cnt ? 1.0 / (double)cnt : 0.0;
 #pragma unroll
   for (int c = 0; c < 3; ++c) ph[c] = ph[c] * ic;
-  st2<2>(lamPhiN + 4 * J, ph);
+  __shared__ WaveStage<2> sp;
+  sp.store(lamPhiN, J0, nn, ph);
 }
 
 // Per cell: lamQ from the entries that read Q_i and from the node primitives
@@ -468,8 +472,8 @@ __global__ __launch_bounds__(kB) void k_radj_phi(MeshView m, RefAdjMaps a, const
 __global__ __launch_bounds__(kB) void k_radj_cell(MeshView m, RefAdjMaps a, double gamma, const double* lqo,
                                                    const double* lqn, const double* lamQn, const double* lamPhiN,
                                                    const double* Q, const double* lam, double* lamOut) {
-  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
-  if (i >= m.nc) return;
+  const int64_t i0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+  const int64_t i = i0 < m.nc ? i0 : m.nc - 1;
   const int64_t nc = m.nc;
   double lq[4] = {0, 0, 0, 0};
   row_sum4(a.ce_ref, a.ce_off[i], a.ce_off[i + 1],
@@ -512,7 +516,8 @@ __global__ __launch_bounds__(kB) void k_radj_cell(MeshView m, RefAdjMaps a, doub
   o[1] = l[1] + lq[1] * r0 - lq[3] * g1 * q1 + lph[0];
   o[2] = l[2] + lq[2] * r0 - lq[3] * g1 * q2 + lph[1];
   o[3] = l[3] + lq[3] * g1 + lph[2];
-  st2<2>(lamOut + 4 * i, o);
+  __shared__ WaveStage<2> so;
+  so.store(lamOut, i0, nc, o);
 }
 
 // [4][nc] <-> [nc][4]

@@ -66,6 +66,7 @@ __device__ __forceinline__ void ld2k(const double* p, double* o) {
     o[2 * q] = v.x; o[2 * q + 1] = v.y;
   }
 }
+
 template <int K>
 __device__ __forceinline__ void st2k(double* p, const double* o) {
 #pragma unroll
@@ -240,8 +241,9 @@ __device__ __forceinline__ void ref_cell_flux_gathered(const RefStepArgs& a, int
 template <bool QUADS>
 __global__ __launch_bounds__(kBlock) void k_ref_faces(RefStepArgs a, double* rflux) {
   const MeshView& m = a.m;
-  const int64_t f = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (f >= m.n_rfaces) return;
+  const int64_t f0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const bool live = f0 < m.n_rfaces;
+  const int64_t f = live ? f0 : m.n_rfaces - 1;
   const int64_t nc = m.nc, nn = m.nn;
   // the face-ordered ids and geometry are read once per step: non-temporal,
   // leaving the caches to the gathered cell and node primitives
@@ -277,12 +279,13 @@ __global__ __launch_bounds__(kBlock) void k_ref_faces(RefStepArgs a, double* rfl
     const int32_t j = ref_face_states<QUADS>(a, i, k, nf, qL, t, fL, fR);
     const int64_t s = (int64_t)k * nc + i;
     refm::face_flux(a.prm.flux, fR, fL, m.fnx[s], m.fny[s], m.flen[s], a.prm.Gamma, e);  // neighbour side is "L"
-    if (j < 0 && a.wall_f && m.wall_id && a.wall_code == j + 1) {
+    if (live && j < 0 && a.wall_f && m.wall_id && a.wall_code == j + 1) {
       const int32_t wid = m.wall_id[s];
       if (wid >= 0) { a.wall_f[2 * wid] = e[1]; a.wall_f[2 * wid + 1] = e[2]; }
     }
   }
-  st2k<2>(rflux + 4 * f, e);
+  __shared__ WaveStage<2> stg;
+  stg.store(rflux, f0, m.n_rfaces, e);
 }
 
 // The reference's GPU stages, calcOneStageCUDA (calcOneStageCUDA.jl:4-135)
@@ -500,8 +503,9 @@ __global__ __launch_bounds__(kBlock) void k_ref_nodes(MeshView m, const double* 
     else if (fa.wall_f) wall_sum_block(fa.wall_f, fa.n_wall, fa.drag);
     return;
   }
-  const int64_t J = (int64_t)xcd_block(blockIdx.x - nfold, gridDim.x - nfold) * kBlock + threadIdx.x;
-  if (J >= m.nn) return;
+  const int64_t J0 = (int64_t)xcd_block(blockIdx.x - nfold, gridDim.x - nfold) * kBlock + threadIdx.x;
+  const bool live = J0 < m.nn;
+  const int64_t J = live ? J0 : m.nn - 1;
   const int64_t nc = m.nc, nn = m.nn;
   double det = 0.0, r = 0.0, u = 0.0, v = 0.0, p = 0.0;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -576,17 +580,18 @@ __global__ __launch_bounds__(kBlock) void k_ref_nodes(MeshView m, const double* 
   }
   // the node primitives are read three launches later, by the face pass, after
   // the viscous passes have moved several hundred MB: stored non-temporally
-  if (QN) {
+  if (QN && live) {
     __builtin_nontemporal_store(r, &Qn[J]);
     __builtin_nontemporal_store(u, &Qn[nn + J]);
     __builtin_nontemporal_store(v, &Qn[2 * nn + J]);
     __builtin_nontemporal_store(p, &Qn[3 * nn + J]);
   }
-  if (UN) { Un[J] = s0; Un[nn + J] = s1; Un[2 * nn + J] = s2; Un[3 * nn + J] = s3; }
+  if (UN && live) { Un[J] = s0; Un[nn + J] = s1; Un[2 * nn + J] = s2; Un[3 * nn + J] = s3; }
   if (PHI) {
     const double dc = (double)cnt;
     const double o[4] = {cnt ? f1 / dc : 0.0, cnt ? f2 / dc : 0.0, cnt ? f3 / dc : 0.0, 0.0};
-    st2k<2>(phiN + 4 * J, o);
+    __shared__ WaveStage<2> stg;
+    stg.store(phiN, J0, nn, o);
   }
 }
 
@@ -594,15 +599,16 @@ __global__ __launch_bounds__(kBlock) void k_ref_nodes(MeshView m, const double* 
 // rho*u, rho*v, E the pair ("gradX", "gradY") = (d/dy, d/dx).
 template <bool QUADS>
 __global__ __launch_bounds__(kBlock) void k_ref_gg1(MeshView m, const double* phiN, double* g6) {
-  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= m.nc) return;
+  const int64_t i0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int64_t i = i0 < m.nc ? i0 : m.nc - 1;
   const int nf = QUADS ? m.nv[i] : 3;
   const double A = __builtin_nontemporal_load(&m.area[i]);
   double ax[3], ay[3], o[6];
   gg_sums<QUADS, 4, 3>(m, i, nf, phiN, ax, ay);
 #pragma unroll
   for (int c = 0; c < 3; ++c) { o[2 * c] = ax[c] / A; o[2 * c + 1] = ay[c] / A; }
-  st2k<3>(g6 + 6 * i, o);
+  __shared__ WaveStage<3> stg;
+  stg.store(g6, i0, m.nc, o);
 }
 
 // Cluster means of the six gradient fields (calcPhiInCellCluster applied
@@ -611,8 +617,8 @@ __global__ __launch_bounds__(kBlock) void k_ref_gg1(MeshView m, const double* ph
 // slot order).
 template <int NCL>
 __global__ __launch_bounds__(kBlock) void k_ref_gg_nodes(MeshView m, const double* g6, double* g6N) {
-  const int64_t J = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (J >= m.nn) return;
+  const int64_t J0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int64_t J = J0 < m.nn ? J0 : m.nn - 1;
   const int64_t nn = m.nn;
   double s[6] = {0, 0, 0, 0, 0, 0};
   int cnt = 0;
@@ -646,7 +652,8 @@ __global__ __launch_bounds__(kBlock) void k_ref_gg_nodes(MeshView m, const doubl
   double o[6];
 #pragma unroll
   for (int t = 0; t < 6; ++t) o[t] = cnt ? s[t] / dc : 0.0;
-  st2k<3>(g6N + 6 * J, o);
+  __shared__ WaveStage<3> stg;
+  stg.store(g6N, J0, nn, o);
 }
 
 template <bool QUADS>
